@@ -1,0 +1,718 @@
+// dim2d_capi.hip -- C ABI of the DIM RK3-TVD library (include/dim2d_hip.h).
+//
+// A context owns one simulation: the HBM-resident state u^n, the stage
+// buffers, a step-control block and one HIP stream.  Every RK stage is one
+// fused kernel launch (plus a small launch for wall ghost cells when a wall
+// edge exists); a step never synchronises with the host.
+#include "dim2d_hip.h"
+#include "dim2d_internal.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace dim2d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(DIM2D_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));          \
+  } while (0)
+
+int validate(const dim2d_params* p) {
+  if (!p) return fail(DIM2D_EINVAL, "params is NULL");
+  if (p->bc_size != kBcSize)
+    return fail(DIM2D_EINVAL, "bc_size must be 2 (Mattsson operators), got %d", p->bc_size);
+  if (p->nx < 2 * kBcSize + 3 || p->ny < 2 * kBcSize + 3)
+    return fail(DIM2D_EINVAL, "grid %dx%d too small: need at least 3 interior points per axis",
+                p->nx, p->ny);
+  const int b[4] = {p->bc_w, p->bc_e, p->bc_s, p->bc_n};
+  for (int k = 0; k < 4; ++k)
+    if (b[k] < DIM2D_BC_REFLECTION || b[k] > DIM2D_BC_WALL)
+      return fail(DIM2D_EINVAL, "unknown boundary kind %d", b[k]);
+  if ((p->bc_w == DIM2D_BC_PERIODIC) != (p->bc_e == DIM2D_BC_PERIODIC) ||
+      (p->bc_s == DIM2D_BC_PERIODIC) != (p->bc_n == DIM2D_BC_PERIODIC))
+    return fail(DIM2D_EINVAL, "periodic boundaries must pair W/E and S/N");
+  const bool wall_x = p->bc_w == DIM2D_BC_WALL || p->bc_e == DIM2D_BC_WALL;
+  const bool wall_y = p->bc_s == DIM2D_BC_WALL || p->bc_n == DIM2D_BC_WALL;
+  if (wall_x && wall_y)
+    return fail(DIM2D_EINVAL,
+                "walls on both axes (wall_xy) are not supported, as in the reference's "
+                "library_config.py");
+  if (p->precision != DIM2D_PREC_FP64 && p->precision != DIM2D_PREC_MIXED)
+    return fail(DIM2D_EINVAL, "unknown precision mode %d", p->precision);
+  if (!(p->x_max > p->x_min) || !(p->y_max > p->y_min))
+    return fail(DIM2D_EINVAL, "empty coordinate range");
+  if (!(p->re > 0) || !(p->we > 0) || !(p->pr > 0) || !(p->cv_r > 0))
+    return fail(DIM2D_EINVAL, "Re, We, Pr and cv_r must be positive");
+  return DIM2D_OK;
+}
+
+// coordinates as field_abstract_class.f:286-315 and dx, dy as td_operators_class.f:156-157
+DevParams make_dev_params(const dim2d_params& q) {
+  DevParams p;
+  memset(&p, 0, sizeof p);
+  p.nx = q.nx;
+  p.ny = q.ny;
+  p.pitch = (q.nx + 31) / 32 * 32;
+  p.plane = (long long)p.pitch * q.ny;
+  p.bc_w = q.bc_w;
+  p.bc_e = q.bc_e;
+  p.bc_s = q.bc_s;
+  p.bc_n = q.bc_n;
+  p.gravity_ac = q.gravity_ac;
+  const double dxf = (q.x_max - q.x_min) / (q.nx - 1 - 2 * kBcSize);
+  const double dyf = (q.y_max - q.y_min) / (q.ny - 1 - 2 * kBcSize);
+  const double x1 = q.x_min + (0 - kBcSize) * dxf, x2 = q.x_min + (1 - kBcSize) * dxf;
+  const double y1 = q.y_min + (0 - kBcSize) * dyf, y2 = q.y_min + (1 - kBcSize) * dyf;
+  p.dx = x2 - x1;
+  p.dy = y2 - y1;
+  p.idx = 1.0 / p.dx;
+  p.idy = 1.0 / p.dy;
+  p.half_idx = 0.5 / p.dx;
+  p.half_idy = 0.5 / p.dy;
+  p.q_idx = 0.25 / p.dx;
+  p.q_idy = 0.25 / p.dy;
+  p.h2x = 0.5 / (p.dx * p.dx);
+  p.h2y = 0.5 / (p.dy * p.dy);
+  p.re = q.re;
+  p.we = q.we;
+  p.pr = q.pr;
+  p.vr = q.viscous_r;
+  p.cv = q.cv_r;
+  p.inv_re = 1.0 / q.re;
+  p.inv_we = 1.0 / q.we;
+  p.inv_pr = 1.0 / q.pr;
+  p.inv_repr = 1.0 / (q.re * q.pr);
+  p.c15cv = 1.5 / q.cv_r;
+  p.c3cv = 3.0 / q.cv_r;
+  p.c38cv = 3.0 / (8.0 * q.cv_r);
+  p.two_vr = 2.0 + q.viscous_r;
+  p.gravity = q.gravity;
+  p.wall_cos_angle = cos(q.wall_contact_angle * acos(-1.0) / 180.0);
+  p.wall_q = q.wall_heat_flux;
+  p.sigma_pref = 2.0 * sqrt(2.0) / (3.0 * sqrt(q.we));
+  return p;
+}
+
+bool has_wall(const DevParams& p) {
+  return p.bc_w == kBcWall || p.bc_e == kBcWall || p.bc_s == kBcWall || p.bc_n == kBcWall;
+}
+
+}  // namespace
+
+struct dim2d_ctx {
+  dim2d_params prm;
+  DevParams dp;
+  int device = 0;
+  int mixed = 0;
+  hipStream_t stream = nullptr;
+  double* U = nullptr;   // u^n
+  double* U2 = nullptr;  // u^{n+1}
+  double* A = nullptr;   // FP64 stage 1 output
+  double* B = nullptr;   // FP64 stage 2 output
+  float* D1 = nullptr;   // mixed stage increments
+  float* D2 = nullptr;
+  double* T = nullptr;   // time derivative scratch
+  StepCtl* ctl = nullptr;
+  int block_x = 64;
+  int rows = 64;
+  bool uploaded = false;
+  long long steps = 0;
+};
+
+extern "C" {
+
+const char* dim2d_last_error(void) { return g_err.c_str(); }
+
+const char* dim2d_version(void) { return "dim2d-mi355x 0.1 (gfx950)"; }
+
+int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
+  if (!out) return fail(DIM2D_EINVAL, "out is NULL");
+  *out = nullptr;
+  int st = validate(params);
+  if (st) return st;
+  dim2d_ctx* c = new (std::nothrow) dim2d_ctx();
+  if (!c) return fail(DIM2D_ENOMEM, "host allocation failed");
+  c->prm = *params;
+  c->dp = make_dev_params(*params);
+  c->mixed = params->precision == DIM2D_PREC_MIXED;
+  const size_t n = (size_t)c->dp.plane * 4;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->U, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->U2, n * sizeof(double));
+  if (e == hipSuccess && !c->mixed) e = hipMalloc(&c->A, n * sizeof(double));
+  if (e == hipSuccess && !c->mixed) e = hipMalloc(&c->B, n * sizeof(double));
+  if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D1, n * sizeof(float));
+  if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D2, n * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&c->ctl, sizeof(StepCtl));
+  if (e == hipSuccess) e = hipMemsetAsync(c->U, 0, n * sizeof(double), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->U2, 0, n * sizeof(double), c->stream);
+  if (e == hipSuccess && c->A) e = hipMemsetAsync(c->A, 0, n * sizeof(double), c->stream);
+  if (e == hipSuccess && c->B) e = hipMemsetAsync(c->B, 0, n * sizeof(double), c->stream);
+  if (e == hipSuccess && c->D1) e = hipMemsetAsync(c->D1, 0, n * sizeof(float), c->stream);
+  if (e == hipSuccess && c->D2) e = hipMemsetAsync(c->D2, 0, n * sizeof(float), c->stream);
+  if (e == hipSuccess) {
+    StepCtl h;
+    memset(&h, 0, sizeof h);
+    h.h = std::min(c->dp.dx, c->dp.dy);
+    h.re = params->re;
+    h.we = params->we;
+    e = hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    const int code = (e == hipErrorOutOfMemory) ? DIM2D_ENOMEM : DIM2D_EDEVICE;
+    dim2d_ctx_destroy(c);
+    return fail(code, "context allocation failed: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return DIM2D_OK;
+}
+
+int dim2d_ctx_destroy(dim2d_ctx* c) {
+  if (!c) return DIM2D_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->U);
+  (void)hipFree(c->U2);
+  (void)hipFree(c->A);
+  (void)hipFree(c->B);
+  (void)hipFree(c->D1);
+  (void)hipFree(c->D2);
+  (void)hipFree(c->T);
+  (void)hipFree(c->ctl);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return DIM2D_OK;
+}
+
+int dim2d_set_tiles(dim2d_ctx* c, int32_t block_x, int32_t rows) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (block_x != 0 && block_x != 64 && block_x != 128)
+    return fail(DIM2D_EINVAL, "block_x must be 64 or 128, got %d", block_x);
+  if (rows < 0 || rows > 4096) return fail(DIM2D_EINVAL, "rows_per_block out of range: %d", rows);
+  if (block_x) c->block_x = block_x;
+  if (rows) c->rows = rows;
+  return DIM2D_OK;
+}
+
+static int copy_in(dim2d_ctx* c, const double* src, hipMemcpyKind kind) {
+  const DevParams& p = c->dp;
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpy2DAsync(c->U + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
+                             src + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
+                             (size_t)p.nx * sizeof(double), p.ny, kind, c->stream));
+  c->uploaded = true;
+  return DIM2D_OK;
+}
+
+static int copy_out(dim2d_ctx* c, double* dst, hipMemcpyKind kind) {
+  const DevParams& p = c->dp;
+  HIP_TRY(launch_fill_ghosts(p, c->U, c->stream));
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpy2DAsync(dst + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
+                             c->U + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
+                             (size_t)p.nx * sizeof(double), p.ny, kind, c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_upload(dim2d_ctx* c, const double* nodes) {
+  if (!c || !nodes) return fail(DIM2D_EINVAL, "NULL argument");
+  int st = copy_in(c, nodes, hipMemcpyHostToDevice);
+  if (st) return st;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_upload_device(dim2d_ctx* c, const double* d_nodes) {
+  if (!c || !d_nodes) return fail(DIM2D_EINVAL, "NULL argument");
+  return copy_in(c, d_nodes, hipMemcpyDeviceToDevice);
+}
+
+int dim2d_download(dim2d_ctx* c, double* nodes) {
+  if (!c || !nodes) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "download before upload");
+  int st = copy_out(c, nodes, hipMemcpyDeviceToHost);
+  if (st) return st;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_download_device(dim2d_ctx* c, double* d_nodes) {
+  if (!c || !d_nodes) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "download before upload");
+  return copy_out(c, d_nodes, hipMemcpyDeviceToDevice);
+}
+
+int dim2d_apply_bc(dim2d_ctx* c, double t) {
+  (void)t;
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "apply_bc before upload");
+  HIP_TRY(launch_fill_ghosts(c->dp, c->U, c->stream));
+  if (has_wall(c->dp)) {
+    // bracket guess from the same state, as at initialisation (field_abstract_class.f:447-463)
+    const size_t n = (size_t)c->dp.plane * 4;
+    HIP_TRY(hipMemcpyAsync(c->U2, c->U, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(launch_wall(c->dp, 0, nullptr, c->U, nullptr, c->U, nullptr, c->U2, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_time_dev(dim2d_ctx* c, double t, double* time_dev) {
+  (void)t;
+  if (!c || !time_dev) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "time_dev before upload");
+  const DevParams& p = c->dp;
+  if (!c->T) HIP_TRY(hipMalloc(&c->T, (size_t)p.plane * 4 * sizeof(double)));
+  StageArgs a;
+  memset(&a, 0, sizeof a);
+  a.U = c->U;
+  a.S64 = c->U;
+  a.O64 = c->T;
+  HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpy2DAsync(time_dev + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
+                             c->T + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
+                             (size_t)p.nx * sizeof(double), p.ny, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // ghost cells hold debug_real (td_operators_class.f:150-154)
+  for (int k = 0; k < 4; ++k)
+    for (int j = 0; j < p.ny; ++j)
+      for (int i = 0; i < p.nx; ++i)
+        if (i < kBcSize || i >= p.nx - kBcSize || j < kBcSize || j >= p.ny - kBcSize)
+          time_dev[((size_t)k * p.ny + j) * p.nx + i] = kDebugReal;
+  return DIM2D_OK;
+}
+
+// the three stages of one step; dt from `dt_fixed` or from the CFL reductions
+static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
+  const DevParams& p = c->dp;
+  StageArgs a;
+  memset(&a, 0, sizeof a);
+  a.ctl = c->ctl;
+  a.dt_fixed = dt_fixed;
+  a.cfl_mode = cfl_mode;
+  a.parity = (int)(c->steps & 1);
+  const bool wall = has_wall(p);
+  if (!c->mixed) {
+    a.U = c->U;
+    a.S64 = c->U;
+    a.O64 = c->A;
+    HIP_TRY(launch_stage(p, 0, 1, a, c->block_x, c->rows, c->stream));
+    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->A, nullptr, c->A, nullptr, c->U, c->stream));
+    a.S64 = c->A;
+    a.O64 = c->B;
+    HIP_TRY(launch_stage(p, 0, 2, a, c->block_x, c->rows, c->stream));
+    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->B, nullptr, c->B, nullptr, c->U, c->stream));
+    a.S64 = c->B;
+    a.O64 = c->U2;
+    HIP_TRY(launch_stage(p, 0, 3, a, c->block_x, c->rows, c->stream));
+    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream));
+  } else {
+    a.U = c->U;
+    a.O32 = c->D1;
+    HIP_TRY(launch_stage(p, 1, 1, a, c->block_x, c->rows, c->stream));
+    if (wall) HIP_TRY(launch_wall(p, 1, c->U, nullptr, c->D1, nullptr, c->D1, c->U, c->stream));
+    a.S32 = c->D1;
+    a.O32 = c->D2;
+    HIP_TRY(launch_stage(p, 1, 2, a, c->block_x, c->rows, c->stream));
+    if (wall) HIP_TRY(launch_wall(p, 1, c->U, nullptr, c->D2, nullptr, c->D2, c->U, c->stream));
+    a.S32 = c->D2;
+    a.O32 = nullptr;
+    a.O64 = c->U2;
+    HIP_TRY(launch_stage(p, 1, 3, a, c->block_x, c->rows, c->stream));
+    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream));
+  }
+  std::swap(c->U, c->U2);
+  c->steps++;
+  return DIM2D_OK;
+}
+
+static int set_time(dim2d_ctx* c, double t) {
+  HIP_TRY(hipMemcpyAsync(&c->ctl->t, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_rk3_step(dim2d_ctx* c, double t, double dt) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "step before upload");
+  if (!(dt > 0) || !isfinite(dt)) return fail(DIM2D_EINVAL, "dt must be positive and finite");
+  int st = set_time(c, t);
+  if (st) return st;
+  st = run_step(c, dt, 0);
+  if (st) return st;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_t nsteps) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "integrate before upload");
+  if (nsteps < 0) return fail(DIM2D_EINVAL, "nsteps must be >= 0");
+  const int cfl_mode = cfl > 0;
+  if (!cfl_mode && (!(dt_fixed > 0) || !isfinite(dt_fixed)))
+    return fail(DIM2D_EINVAL, "dt must be positive and finite when cfl <= 0");
+  int st = set_time(c, t0);
+  if (st) return st;
+  if (cfl_mode) {
+    HIP_TRY(hipMemcpyAsync(&c->ctl->cfl, &cfl, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_reduce(c->dp, c->U, c->ctl, (int)(c->steps & 1), c->stream));
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    st = run_step(c, dt_fixed, cfl_mode);
+    if (st) return st;
+  }
+  return DIM2D_OK;
+}
+
+int dim2d_synchronize(dim2d_ctx* c) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_time(dim2d_ctx* c, double out[2]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  StepCtl h;
+  HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  out[0] = h.t;
+  out[1] = h.dt;
+  return DIM2D_OK;
+}
+
+static int reduce_now(dim2d_ctx* c, StepCtl* h) {
+  const int slot = (int)((c->steps + 1) & 1);  // the slot the next step does not read
+  HIP_TRY(launch_reduce(c->dp, c->U, c->ctl, slot, c->stream));
+  HIP_TRY(hipMemcpyAsync(h, c->ctl, sizeof *h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return slot;
+}
+
+int dim2d_cfl_dt(dim2d_ctx* c, double cfl, double out[4]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "cfl_dt before upload");
+  StepCtl h;
+  const int slot = reduce_now(c, &h);
+  if (slot < 0) return slot;
+  double amax, rmin, rmax;
+  memcpy(&amax, &h.amax[slot], 8);
+  memcpy(&rmin, &h.rmin[slot], 8);
+  memcpy(&rmax, &h.rmax[slot], 8);
+  const double hh = std::min(c->dp.dx, c->dp.dy);
+  double dt = hh / amax;
+  dt = std::min(dt, c->prm.re * rmin * hh * hh);
+  dt = std::min(dt, sqrt(c->prm.we / rmax) * hh * hh);
+  out[0] = cfl * dt;
+  out[1] = amax;
+  out[2] = rmin;
+  out[3] = rmax;
+  if (h.nonfinite[slot]) return fail(DIM2D_ENONFINITE, "%u non-finite cells", h.nonfinite[slot]);
+  return DIM2D_OK;
+}
+
+int dim2d_diagnostics(dim2d_ctx* c, double out[7]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "diagnostics before upload");
+  StepCtl h;
+  const int slot = reduce_now(c, &h);
+  if (slot < 0) return slot;
+  for (int k = 0; k < 4; ++k) out[k] = h.sum[slot][k];
+  memcpy(&out[4], &h.rmin[slot], 8);
+  memcpy(&out[5], &h.rmax[slot], 8);
+  out[6] = (double)h.nonfinite[slot];
+  return DIM2D_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// batches
+// ------------------------------------------------------------------------
+struct dim2d_batch {
+  int n = 0;
+  int mixed = 0;
+  int device = 0;
+  std::vector<dim2d_params> prm;
+  std::vector<DevParams> hp;
+  DevParams* dp = nullptr;
+  size_t total = 0;  // elements per variable-set buffer (sum of 4*plane)
+  double *U = nullptr, *U2 = nullptr, *A = nullptr, *B = nullptr, *tmp = nullptr;
+  float *D1 = nullptr, *D2 = nullptr;
+  double* d_dts = nullptr;
+  std::vector<double> h_dts;
+  WorkItem* d_items = nullptr;
+  int nitems = 0;
+  int block_x = 64, rows = 32, wg = 0;
+  int wg_eff = 1;  // persistent workgroups actually launched
+  bool any_wall = false;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  long long steps = 0;
+};
+
+static int batch_rebuild_items(dim2d_batch* b) {
+  std::vector<WorkItem> items;
+  const int ow = b->block_x - 2 * kBcSize;
+  for (int s = 0; s < b->n; ++s) {
+    const DevParams& p = b->hp[s];
+    const int nbx = (p.nx - 2 * kBcSize + ow - 1) / ow;
+    const int nby = (p.ny - 2 * kBcSize + b->rows - 1) / b->rows;
+    for (int by = 0; by < nby; ++by)
+      for (int bx = 0; bx < nbx; ++bx) items.push_back(WorkItem{s, bx, by, 0});
+  }
+  // largest simulations first so the persistent grid drains evenly
+  std::stable_sort(items.begin(), items.end(), [&](const WorkItem& x, const WorkItem& y) {
+    return b->hp[x.sim].nx * (long long)b->hp[x.sim].ny > b->hp[y.sim].nx * (long long)b->hp[y.sim].ny;
+  });
+  (void)hipFree(b->d_items);
+  b->d_items = nullptr;
+  b->nitems = (int)items.size();
+  HIP_TRY(hipMalloc(&b->d_items, items.size() * sizeof(WorkItem)));
+  HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem),
+                    hipMemcpyHostToDevice));
+  b->wg_eff = b->wg > 0 ? b->wg : std::min(b->nitems, max_resident_workgroups(b->block_x));
+  if (b->wg_eff < 1) b->wg_eff = 1;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return DIM2D_OK;
+}
+
+// launches of one step: U -> U2 (pointers as given)
+static int batch_step_launches(dim2d_batch* b, double* U, double* U2) {
+  StageArgs a;
+  memset(&a, 0, sizeof a);
+  a.dts = b->d_dts;
+  const int wg = b->wg_eff;
+  hipStream_t s = b->stream;
+  if (!b->mixed) {
+    a.U = U;
+    a.S64 = U;
+    a.O64 = b->A;
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 1, a, b->block_x, b->rows, wg, s));
+    if (b->any_wall)
+      HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, b->A, nullptr, b->A, nullptr, U, s));
+    a.S64 = b->A;
+    a.O64 = b->B;
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 2, a, b->block_x, b->rows, wg, s));
+    if (b->any_wall)
+      HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, b->B, nullptr, b->B, nullptr, U, s));
+    a.S64 = b->B;
+    a.O64 = U2;
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 3, a, b->block_x, b->rows, wg, s));
+    if (b->any_wall)
+      HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, U2, nullptr, U2, nullptr, U, s));
+  } else {
+    a.U = U;
+    a.O32 = b->D1;
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 1, a, b->block_x, b->rows, wg, s));
+    if (b->any_wall)
+      HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 1, U, nullptr, b->D1, nullptr, b->D1, U, s));
+    a.S32 = b->D1;
+    a.O32 = b->D2;
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 2, a, b->block_x, b->rows, wg, s));
+    if (b->any_wall)
+      HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 1, U, nullptr, b->D2, nullptr, b->D2, U, s));
+    a.S32 = b->D2;
+    a.O32 = nullptr;
+    a.O64 = U2;
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 3, a, b->block_x, b->rows, wg, s));
+    if (b->any_wall)
+      HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, U2, nullptr, U2, nullptr, U, s));
+  }
+  return DIM2D_OK;
+}
+
+extern "C" {
+
+int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out) {
+  if (!out || !params || n <= 0) return fail(DIM2D_EINVAL, "invalid batch arguments");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    int st = validate(&params[i]);
+    if (st) return fail(st, "simulation %d: %s", i, g_err.c_str());
+    if (params[i].precision != params[0].precision)
+      return fail(DIM2D_EINVAL, "all simulations of a batch must share the precision mode");
+  }
+  dim2d_batch* b = new (std::nothrow) dim2d_batch();
+  if (!b) return fail(DIM2D_ENOMEM, "host allocation failed");
+  b->n = n;
+  b->mixed = params[0].precision == DIM2D_PREC_MIXED;
+  b->prm.assign(params, params + n);
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    DevParams p = make_dev_params(params[i]);
+    p.offset = (long long)off;
+    off += (size_t)p.plane * 4;
+    off = (off + 63) / 64 * 64;
+    b->any_wall |= has_wall(p);
+    b->hp.push_back(p);
+  }
+  b->total = off;
+  b->h_dts.assign(n, 0.0);
+  hipError_t e = hipGetDevice(&b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&b->dp, n * sizeof(DevParams));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->dp, b->hp.data(), n * sizeof(DevParams), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&b->U, off * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&b->U2, off * sizeof(double));
+  if (e == hipSuccess && !b->mixed) e = hipMalloc(&b->A, off * sizeof(double));
+  if (e == hipSuccess && !b->mixed) e = hipMalloc(&b->B, off * sizeof(double));
+  if (e == hipSuccess && b->mixed) e = hipMalloc(&b->D1, off * sizeof(float));
+  if (e == hipSuccess && b->mixed) e = hipMalloc(&b->D2, off * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&b->d_dts, n * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(b->U, 0, off * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(b->U2, 0, off * sizeof(double));
+  if (e == hipSuccess && b->A) e = hipMemset(b->A, 0, off * sizeof(double));
+  if (e == hipSuccess && b->B) e = hipMemset(b->B, 0, off * sizeof(double));
+  if (e == hipSuccess && b->D1) e = hipMemset(b->D1, 0, off * sizeof(float));
+  if (e == hipSuccess && b->D2) e = hipMemset(b->D2, 0, off * sizeof(float));
+  if (e != hipSuccess) {
+    dim2d_batch_destroy(b);
+    return fail(e == hipErrorOutOfMemory ? DIM2D_ENOMEM : DIM2D_EDEVICE, "batch allocation: %s",
+                hipGetErrorString(e));
+  }
+  int st = batch_rebuild_items(b);
+  if (st) {
+    dim2d_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return DIM2D_OK;
+}
+
+int dim2d_batch_destroy(dim2d_batch* b) {
+  if (!b) return DIM2D_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipFree(b->dp);
+  (void)hipFree(b->U);
+  (void)hipFree(b->U2);
+  (void)hipFree(b->A);
+  (void)hipFree(b->B);
+  (void)hipFree(b->D1);
+  (void)hipFree(b->D2);
+  (void)hipFree(b->d_dts);
+  (void)hipFree(b->d_items);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return DIM2D_OK;
+}
+
+int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows, int32_t workgroups) {
+  if (!b) return fail(DIM2D_EINVAL, "batch is NULL");
+  if (block_x != 0 && block_x != 64 && block_x != 128)
+    return fail(DIM2D_EINVAL, "block_x must be 64 or 128, got %d", block_x);
+  if (rows < 0 || rows > 4096 || workgroups < 0)
+    return fail(DIM2D_EINVAL, "invalid tile arguments");
+  if (block_x) b->block_x = block_x;
+  if (rows) b->rows = rows;
+  b->wg = workgroups;
+  return batch_rebuild_items(b);
+}
+
+int dim2d_batch_upload(dim2d_batch* b, int32_t sim, const double* nodes) {
+  if (!b || !nodes || sim < 0 || sim >= b->n) return fail(DIM2D_EINVAL, "invalid argument");
+  const DevParams& p = b->hp[sim];
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpy2DAsync(b->U + p.offset + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
+                             nodes + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
+                             (size_t)p.nx * sizeof(double), p.ny, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_batch_download(dim2d_batch* b, int32_t sim, double* nodes) {
+  if (!b || !nodes || sim < 0 || sim >= b->n) return fail(DIM2D_EINVAL, "invalid argument");
+  DevParams p = b->hp[sim];
+  double* base = b->U + p.offset;
+  HIP_TRY(launch_fill_ghosts(p, base, b->stream));
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpy2DAsync(nodes + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
+                             base + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
+                             (size_t)p.nx * sizeof(double), p.ny, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_batch_integrate(dim2d_batch* b, const double* dts, int32_t nsteps) {
+  if (!b || !dts || nsteps < 0) return fail(DIM2D_EINVAL, "invalid argument");
+  for (int i = 0; i < b->n; ++i)
+    if (!(dts[i] > 0) || !isfinite(dts[i])) return fail(DIM2D_EINVAL, "dt of simulation %d", i);
+  HIP_TRY(hipMemcpyAsync(b->d_dts, dts, b->n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!b->exec) {
+    // capture two steps (U -> U2 -> U) so a replay leaves the state in U
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int st = batch_step_launches(b, b->U, b->U2);
+    if (!st) st = batch_step_launches(b, b->U2, b->U);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    HIP_TRY(e);
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  }
+  for (int s = 0; s + 1 < nsteps; s += 2) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  if (nsteps & 1) {
+    int st = batch_step_launches(b, b->U, b->U2);
+    if (st) return st;
+    std::swap(b->U, b->U2);
+    // the captured graph refers to the old buffer order
+    if (b->exec) {
+      (void)hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+    if (b->graph) {
+      (void)hipGraphDestroy(b->graph);
+      b->graph = nullptr;
+    }
+  }
+  b->steps += nsteps;
+  return DIM2D_OK;
+}
+
+int dim2d_batch_synchronize(dim2d_batch* b) {
+  if (!b) return fail(DIM2D_EINVAL, "batch is NULL");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return DIM2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// dim2d_internal.hpp -- structures shared by the host library and the kernels.
+//
+// Field layout in HBM: the reference's nodes(nx,ny,ne) as four planes
+// (mass, momentum_x, momentum_y, total energy), each of ny rows of `pitch`
+// elements (pitch >= nx, padded to a multiple of 32 elements so every row
+// starts on a 256-byte boundary).  Element (i,j,k) of the reference (1-based)
+// sits at (k-1)*plane + (j-1)*pitch + (i-1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dim2d {
+
+constexpr int kBcReflection = 0;
+constexpr int kBcPeriodic = 1;
+constexpr int kBcWall = 2;
+constexpr int kBcSize = 2;
+// the reference's debug_real (NF90_FILL_DOUBLE), written into the ghost cells
+// of a time derivative as td_operators_class.f:150-154 does
+constexpr double kDebugReal = 9.9692099683868690e+36;
+
+// Per-simulation constants, passed by value (single simulation) or read from
+// a device array (batch).  dx, dy are x_map(2)-x_map(1), y_map(2)-y_map(1) as
+// in td_operators_class.f:156-157.
+struct DevParams {
+  int nx, ny, pitch, bc_w;
+  int bc_e, bc_s, bc_n, gravity_ac;
+  long long plane;     // pitch * ny
+  long long offset;    // element offset of this simulation inside batch buffers
+  double dx, dy;
+  double idx, idy;         // 1/dx, 1/dy
+  double half_idx, half_idy; // 0.5/dx, 0.5/dy  (central gradients)
+  double q_idx, q_idy;     // 0.25/dx, 0.25/dy (transverse face gradients)
+  double h2x, h2y;         // 0.5/dx^2, 0.5/dy^2
+  double re, we, pr, vr, cv;
+  double inv_re, inv_we, inv_pr, inv_repr;
+  double c15cv, c3cv, c38cv;  // 1.5/cv, 3/cv, 3/(8cv)
+  double two_vr;           // 2 + viscous_r
+  double gravity;
+  double wall_cos_angle;   // cos(contact angle)
+  double wall_q;           // constant wall heat flux
+  double sigma_pref;       // 2 sqrt(2)/(3 sqrt(We)) (surface tension prefactor)
+};
+
+// Step control block in device memory: dt and time of the running step and
+// the reductions (for the CFL time step and the diagnostics) of the state at
+// the end of each step, double-buffered by step parity.
+struct StepCtl {
+  double dt, t;
+  double cfl, h;                 // CFL number and min(dx,dy)
+  double re, we;
+  double sum[2][4];              // sum rho, rho u, rho v, E over interior cells
+  unsigned long long amax[2];    // max(|u|+c) as bits of a non-negative double
+  unsigned long long rmin[2];    // min rho as bits (non-negative)
+  unsigned long long rmax[2];    // max rho as bits
+  unsigned int nonfinite[2];
+  unsigned int pad[2];
+};
+
+// one tile of the batch work list
+struct WorkItem {
+  int sim, bx, by, pad;
+};
+
+// kernel launch helpers (dim2d_kernels.hip)
+struct StageArgs {
+  const double* U;    // u^n (FP64)
+  const double* S64;  // stage input state (FP64 mode)
+  const float* S32;   // stage increment (mixed mode), state = U + S32
+  double* O64;        // FP64 output (FP64 stages, mixed stage 3, time_dev)
+  float* O32;         // FP32 output (mixed stages 1, 2)
+  StepCtl* ctl;
+  const double* dts;  // per-simulation dt (batch)
+  double dt_fixed;
+  int cfl_mode;       // 1: stage 1 takes dt from the reductions
+  int parity;         // step parity (reduction slot)
+};
+
+hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArgs& a, int block_x,
+                        int rows, hipStream_t s);
+hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
+                              int mixed, int stage, const StageArgs& a, int block_x, int rows,
+                              int workgroups, hipStream_t s);
+// wall ghost cells of the state `st` (= U + S32 in mixed mode) written into
+// O64 or O32; tmp is u^n for the root bracket guess.
+hipError_t launch_wall(const DevParams& p, int mixed_in, const double* U, const double* S64,
+                       const float* S32, double* O64, float* O32, const double* tmp,
+                       hipStream_t s);
+hipError_t launch_wall_batch(const DevParams* h_params, const DevParams* d_params, int nsim,
+                             int mixed_in, const double* U, const double* S64, const float* S32,
+                             double* O64, float* O32, const double* tmp, hipStream_t s);
+// store reflection/periodic ghost cells of an FP64 field
+hipError_t launch_fill_ghosts(const DevParams& p, double* F, hipStream_t s);
+// reductions of an FP64 field into ctl slot `slot` (slot cleared first)
+hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int slot,
+                         hipStream_t s);
+// set ghost cells of a time derivative to debug_real
+hipError_t launch_fill_frame(const DevParams& p, double* F, double value, hipStream_t s);
+
+int max_resident_workgroups(int block_x);
+
+}  // namespace dim2d

@@ -1,0 +1,976 @@
+// dim2d_kernels.hip -- gfx950 kernels of the DIM RK3-TVD hot path.
+//
+// One fused kernel per RK stage.  A workgroup owns a strip of BW columns (one
+// lane per column: BW-4 output columns plus a 2-column halo on each side) and
+// `rows` output rows; it marches up the strip one row per iteration.  Per row:
+//   * load the conservative variables of row r+2 (coalesced FP64 rows; ghost
+//     cells of reflection/periodic edges are resolved by index maps here, so no
+//     ghost-fill launch exists for them);
+//   * compute that row's primitives once per cell (1/rho, u, v, P, T, 1/(3-rho)
+//     and the transport products the fluxes need) and the capillary
+//     temperature of row r+1;
+//   * x-face flux at (i-1/2, r) and y-face flux at (i, r+1/2) from the lane's
+//     own column (registers) and its neighbours (a 4-row LDS ring);
+//   * flux-difference divergence with the right x-face from the next lane and
+//     the lower y-face from the previous iteration, then the RK stage update.
+// One workgroup barrier per row.  Stage 3 also reduces max(|u|+c), min/max rho
+// and the conserved sums for the next CFL time step and the diagnostics.
+//
+// Reference semantics followed here (jdesmarais/dim2D_finite_volume):
+//   fluxes  physical_models/dim2d/dim2d_fluxes_module.f:103-1649
+//   stencils sd_operators/mattsson_operators/sd_operators_class.f:185-1039
+//   divergence td_operators/finitevolume/td_operators_class.f:181-194
+//   RK stages td_integrator/rungekutta3rdtvd/rk3tvd_steps_module.f:52-490
+//   wall  boundary_conditions/wall_xy/wall_xy_equilibrium_module.f, ridders_method_module.f
+#include "dim2d_internal.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <type_traits>
+
+namespace dim2d {
+
+// ------------------------------------------------------------------------
+// virtual ghost cells: index maps of reflection and periodic edges
+// ------------------------------------------------------------------------
+// reflection_x: ghost i mirrors about the boundary face (bc_operators_reflection_x_class.f:206-234),
+// periodic: i +/- (nx - 2*bc_size) (periodic_xy/bc_operators_class.f:127-128,156-175);
+// wall ghosts are stored cells.
+__device__ __forceinline__ int map_col(const DevParams& p, int c, bool& flip) {
+  flip = false;
+  if (c < kBcSize) {
+    if (p.bc_w == kBcReflection) { flip = true; return 2 * kBcSize - 1 - c; }
+    if (p.bc_w == kBcPeriodic) return c + p.nx - 2 * kBcSize;
+  } else if (c >= p.nx - kBcSize) {
+    if (p.bc_e == kBcReflection) { flip = true; return 2 * p.nx - 2 * kBcSize - 1 - c; }
+    if (p.bc_e == kBcPeriodic) return c - (p.nx - 2 * kBcSize);
+  }
+  return c;
+}
+__device__ __forceinline__ int map_row(const DevParams& p, int r, bool& flip) {
+  flip = false;
+  if (r < kBcSize) {
+    if (p.bc_s == kBcReflection) { flip = true; return 2 * kBcSize - 1 - r; }
+    if (p.bc_s == kBcPeriodic) return r + p.ny - 2 * kBcSize;
+  } else if (r >= p.ny - kBcSize) {
+    if (p.bc_n == kBcReflection) { flip = true; return 2 * p.ny - 2 * kBcSize - 1 - r; }
+    if (p.bc_n == kBcPeriodic) return r - (p.ny - 2 * kBcSize);
+  }
+  return r;
+}
+
+// state value of cell (cm, rm) (already mapped) with the reflection signs
+// (momentum normal to a reflecting edge changes sign, reflection_xy_module.f:50-110)
+// SRC 0: FP64 field S64; 1: U alone; 2: U + S32
+template <int SRC>
+__device__ __forceinline__ void load_state(const DevParams& p, const double* __restrict__ U,
+                                           const double* __restrict__ S64,
+                                           const float* __restrict__ S32, long long idx,
+                                           bool fx, bool fy, double v[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (SRC == 0)
+      v[k] = S64[k * p.plane + idx];
+    else if (SRC == 1)
+      v[k] = U[k * p.plane + idx];
+    else
+      v[k] = U[k * p.plane + idx] + (double)S32[k * p.plane + idx];
+  }
+  if (fx) v[1] = -v[1];
+  if (fy) v[2] = -v[2];
+}
+
+// ------------------------------------------------------------------------
+// per-cell quantities (dim2d_prim_module.f)
+// ------------------------------------------------------------------------
+template <typename RD, typename R>
+struct Cell {
+  RD rho;
+  double irho;
+  R qx, qy, u, v, w, Tcl, Tcap;
+  R Pxx, Pyy, qxv, Hu, Hv, uw, vw;  // qx*u+P, qy*v+P, qx*v, (E+P)u, (E+P)v, u w, v w
+};
+
+// velocities, classical pressure (:425-453), classical temperature (:515-544),
+// capillarity pressure 1/(3-rho) (:1030-1062) and the transport products
+// (:830-1150), computed in FP64 from the FP64 state.
+template <typename RD, typename R>
+__device__ __forceinline__ void make_cell(const DevParams& p, const double v[4], Cell<RD, R>& c) {
+  const double rho = v[0];
+  const double ir = 1.0 / rho;
+  const double u = v[1] * ir;
+  const double vv = v[2] * ir;
+  const double rho2 = rho * rho;
+  const double Tn = v[3] - 0.5 * rho * (u * u + vv * vv) + 3.0 * rho2;
+  const double w = 1.0 / (3.0 - rho);
+  const double P = p.c3cv * w * Tn - 3.0 * rho2;
+  const double H = v[3] + P;
+  c.rho = (RD)rho;
+  c.irho = ir;
+  c.qx = (R)v[1];
+  c.qy = (R)v[2];
+  c.u = (R)u;
+  c.v = (R)vv;
+  c.w = (R)w;
+  c.Tcl = (R)(Tn * ir);
+  c.Tcap = (R)0;
+  c.Pxx = (R)(v[1] * u + P);
+  c.Pyy = (R)(v[2] * vv + P);
+  c.qxv = (R)(v[1] * vv);
+  c.Hu = (R)(H * u);
+  c.Hv = (R)(H * vv);
+  c.uw = (R)(u * w);
+  c.vw = (R)(vv * w);
+}
+
+// LDS ring of the four rows r-1..r+2 (slot = row & 3) and the x-face fluxes
+template <typename RD, typename R, int BW>
+struct Ring {
+  RD rho[4][BW];
+  R qx[4][BW], u[4][BW], v[4][BW], w[4][BW], Tcl[4][BW], Tcap[4][BW];
+  R Pxx[4][BW], qxv[4][BW], Hu[4][BW], uw[4][BW];
+  R fx[2][4][BW];
+};
+
+template <typename RD, typename R, int BW>
+__device__ __forceinline__ void put_slot(Ring<RD, R, BW>& sm, int s, int lane,
+                                         const Cell<RD, R>& c) {
+  sm.rho[s][lane] = c.rho;
+  sm.qx[s][lane] = c.qx;
+  sm.u[s][lane] = c.u;
+  sm.v[s][lane] = c.v;
+  sm.w[s][lane] = c.w;
+  sm.Tcl[s][lane] = c.Tcl;
+  sm.Pxx[s][lane] = c.Pxx;
+  sm.qxv[s][lane] = c.qxv;
+  sm.Hu[s][lane] = c.Hu;
+  sm.uw[s][lane] = c.uw;
+}
+
+// capillarity_temperature_eff (:582-613) with gradient_x/y_interior
+template <typename RD, typename R>
+__device__ __forceinline__ R cap_temperature(const DevParams& p, double irho, RD rho_w, RD rho_e,
+                                             RD rho_s, RD rho_n) {
+  const RD gx = (rho_e - rho_w) * (RD)p.half_idx;
+  const RD gy = (rho_n - rho_s) * (RD)p.half_idy;
+  return (R)(0.5 * irho * (double)(gx * gx + gy * gy));
+}
+
+// pressure(T, rho) of the wall (dim2d_prim_module.f:476-488)
+__device__ __forceinline__ double wall_pressure(double T, double md) {
+  return 8.0 * md * T / (3.0 - md) - 3.0 * md * md;
+}
+
+// x-face flux at (i-1/2, j): L = (i-1, j), R = (i, j)  (dim2d_fluxes_module.f,
+// Mattsson f/dfdx/dfdy/d2fdx2/d2fdy2/dfdx_nl stencils).
+template <typename RD, typename R, int BW>
+__device__ __forceinline__ void x_face(const DevParams& p, const Ring<RD, R, BW>& sm, int s0,
+                                       int sm1, int sp1, int lane, const Cell<RD, R>& Rc,
+                                       RD rhoR_dn, R uR_dn, R vR_dn, RD rhoR_up, R uR_up,
+                                       R vR_up, bool wall_w, bool wall_e, R F[4]) {
+  const int lm1 = lane > 0 ? lane - 1 : 0;
+  const int lm2 = lane > 1 ? lane - 2 : 0;
+  const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
+  const RD rhoL = sm.rho[s0][lm1];
+  const RD rhoLL = sm.rho[s0][lm2];
+  const RD rhoRR = sm.rho[s0][lp1];
+  const RD rhoL_dn = sm.rho[sm1][lm1], rhoL_up = sm.rho[sp1][lm1];
+  const R uL = sm.u[s0][lm1], vL = sm.v[s0][lm1];
+  const R uL_dn = sm.u[sm1][lm1], vL_dn = sm.v[sm1][lm1];
+  const R uL_up = sm.u[sp1][lm1], vL_up = sm.v[sp1][lm1];
+  const R qxL = sm.qx[s0][lm1], wL = sm.w[s0][lm1], TclL = sm.Tcl[s0][lm1];
+  const R TcapL = sm.Tcap[s0][lm1], PxxL = sm.Pxx[s0][lm1], qxvL = sm.qxv[s0][lm1];
+  const R HuL = sm.Hu[s0][lm1], uwL = sm.uw[s0][lm1];
+
+  const R half = (R)0.5;
+  const R fr = (R)(half * (rhoL + Rc.rho));
+  const R rx = (R)((Rc.rho - rhoL) * (RD)p.idx);
+  const R ry = (R)(((rhoL_up + rhoR_up) - (rhoL_dn + rhoR_dn)) * (RD)p.q_idy);
+  const R rxx = (R)(((rhoLL + rhoRR) - (rhoL + Rc.rho)) * (RD)p.h2x);
+  const R ryy = (R)(((rhoL_dn + rhoR_dn) - (RD)2 * (rhoL + Rc.rho) + (rhoL_up + rhoR_up)) *
+                    (RD)p.h2y);
+  const R ux = half * (uL + Rc.u);
+  const R uy = half * (vL + Rc.v);
+  const R dudx = (Rc.u - uL) * (R)p.idx;
+  const R dvdx = (Rc.v - vL) * (R)p.idx;
+  const R dudy = ((uL_up + uR_up) - (uL_dn + uR_dn)) * (R)p.q_idy;
+  const R dvdy = ((vL_up + vR_up) - (vL_dn + vR_dn)) * (R)p.q_idy;
+  const R grad2 = rx * rx + ry * ry;
+  const R lap = rxx + ryy;
+  const R rx2d = ry * ry - rx * rx;
+
+  if (wall_w || wall_e) {
+    // compute_wall_flux_x (wall_xy_equilibrium_module.f:249-341)
+    const double TeffL = (double)TclL - p.inv_we * (double)TcapL;
+    const double TeffR = (double)Rc.Tcl - p.inv_we * (double)Rc.Tcap;
+    const double Tl = p.c38cv * (wall_w ? TeffR : TeffL);
+    const double Th = Tl + p.dx * p.c38cv * p.re * p.pr * p.wall_q;
+    const double Pw = wall_pressure(0.5 * (Tl + Th), (double)fr);
+    F[0] = (R)0;
+    F[1] = (R)(Pw - p.inv_we * (double)(fr * lap + half * rx2d));
+    F[2] = (R)(-p.inv_re * (double)dvdx - p.inv_we * (double)(-rx * ry));
+    F[3] = (R)(wall_w ? p.wall_q : -p.wall_q);
+    return;
+  }
+  const R visc = (R)p.two_vr * dudx + (R)p.vr * dvdy;
+  const R shear = dudy + dvdx;
+  F[0] = half * (qxL + Rc.qx);
+  F[1] = half * (PxxL + Rc.Pxx) - (R)p.inv_re * visc -
+         (R)p.inv_we * ((R)p.c15cv * half * (wL + Rc.w) * grad2 + fr * lap + half * rx2d);
+  F[2] = half * (qxvL + Rc.qxv) - (R)p.inv_re * shear + (R)p.inv_we * (rx * ry);
+  const R visc_e = ux * visc + uy * shear + (R)p.inv_pr * (Rc.Tcl - TclL) * (R)p.idx;
+  const R cap_e = (R)p.c15cv * half * (uwL + Rc.uw) * grad2 + F[0] * lap + half * ux * rx2d -
+                  rx * ry * uy - fr * rx * (dudx + dvdy) -
+                  (R)p.inv_repr * (Rc.Tcap - TcapL) * (R)p.idx;
+  F[3] = half * (HuL + Rc.Hu) - (R)p.inv_re * visc_e - (R)p.inv_we * cap_e;
+}
+
+// y-face flux at (i, j-1/2): D = (i, j-1), U = (i, j)  (g/dgdx/dgdy/d2gdx2/d2gdy2/dgdy_nl)
+template <typename RD, typename R, int BW>
+__device__ __forceinline__ void y_face(const DevParams& p, const Ring<RD, R, BW>& sm, int sD,
+                                       int sU, int lane, const Cell<RD, R>& D,
+                                       const Cell<RD, R>& Uc, RD rho_dd, RD rho_uu,
+                                       bool wall_s, bool wall_n, R G[4]) {
+  const int lm1 = lane > 0 ? lane - 1 : 0;
+  const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
+  const RD rWD = sm.rho[sD][lm1], rED = sm.rho[sD][lp1];
+  const RD rWU = sm.rho[sU][lm1], rEU = sm.rho[sU][lp1];
+  const R uWD = sm.u[sD][lm1], uED = sm.u[sD][lp1], uWU = sm.u[sU][lm1], uEU = sm.u[sU][lp1];
+  const R vWD = sm.v[sD][lm1], vED = sm.v[sD][lp1], vWU = sm.v[sU][lm1], vEU = sm.v[sU][lp1];
+
+  const R half = (R)0.5;
+  const R gr = (R)(half * (D.rho + Uc.rho));
+  const R ry = (R)((Uc.rho - D.rho) * (RD)p.idy);
+  const R rx = (R)(((rED + rEU) - (rWD + rWU)) * (RD)p.q_idx);
+  const R ryy = (R)(((rho_dd + rho_uu) - (D.rho + Uc.rho)) * (RD)p.h2y);
+  const R rxx = (R)(((rWD + rWU) - (RD)2 * (D.rho + Uc.rho) + (rED + rEU)) * (RD)p.h2x);
+  const R ux = half * (D.u + Uc.u);
+  const R uy = half * (D.v + Uc.v);
+  const R dudy = (Uc.u - D.u) * (R)p.idy;
+  const R dvdy = (Uc.v - D.v) * (R)p.idy;
+  const R dudx = ((uED + uEU) - (uWD + uWU)) * (R)p.q_idx;
+  const R dvdx = ((vED + vEU) - (vWD + vWU)) * (R)p.q_idx;
+  const R grad2 = rx * rx + ry * ry;
+  const R lap = rxx + ryy;
+  const R ry2d = rx * rx - ry * ry;
+
+  if (wall_s || wall_n) {
+    // compute_wall_flux_y (wall_xy_equilibrium_module.f:342-444)
+    const double TeffD = (double)D.Tcl - p.inv_we * (double)D.Tcap;
+    const double TeffU = (double)Uc.Tcl - p.inv_we * (double)Uc.Tcap;
+    const double Tl = p.c38cv * (wall_s ? TeffU : TeffD);
+    const double Th = Tl + p.dy * p.re * p.pr * p.c38cv * p.wall_q;
+    const double Pw = wall_pressure(0.5 * (Tl + Th), (double)gr);
+    G[0] = (R)0;
+    G[1] = (R)(-p.inv_re * (double)dudy - p.inv_we * (double)(-rx * ry));
+    G[2] = (R)(Pw - p.inv_we * (double)(gr * lap + half * ry2d));
+    G[3] = (R)(wall_s ? p.wall_q : -p.wall_q);
+    return;
+  }
+  const R shear = dudy + dvdx;
+  const R visc = (R)p.vr * dudx + (R)p.two_vr * dvdy;
+  G[0] = half * (D.qy + Uc.qy);
+  G[1] = half * (D.qxv + Uc.qxv) - (R)p.inv_re * shear + (R)p.inv_we * (rx * ry);
+  G[2] = half * (D.Pyy + Uc.Pyy) - (R)p.inv_re * visc -
+         (R)p.inv_we * ((R)p.c15cv * half * (D.w + Uc.w) * grad2 + gr * lap + half * ry2d);
+  const R visc_e = ux * shear + uy * visc + (R)p.inv_pr * (Uc.Tcl - D.Tcl) * (R)p.idy;
+  const R cap_e = (R)p.c15cv * half * (D.vw + Uc.vw) * grad2 + G[0] * lap + half * uy * ry2d -
+                  rx * ry * ux - gr * ry * (dudx + dvdy) -
+                  (R)p.inv_repr * (Uc.Tcap - D.Tcap) * (R)p.idy;
+  G[3] = half * (D.Hv + Uc.Hv) - (R)p.inv_re * visc_e - (R)p.inv_we * cap_e;
+}
+
+// speed_of_sound (dim2d_prim_module.f:1179-1215), algebraically reduced to
+// c^2 = 3 (e + (P + 3 rho^2)/cv) / (rho (3 - rho)),  e = P + rho^2 (2 rho - 3);
+// c = 0 where c^2 <= 0 (the spinodal part of an interface has no real sound speed)
+__device__ __forceinline__ double sound_speed(const DevParams& p, const double v[4],
+                                              double* speed) {
+  const double rho = v[0];
+  const double ir = 1.0 / rho;
+  const double u = v[1] * ir, vv = v[2] * ir;
+  const double rho2 = rho * rho;
+  const double Tn = v[3] - 0.5 * rho * (u * u + vv * vv) + 3.0 * rho2;
+  const double w = 1.0 / (3.0 - rho);
+  const double P = p.c3cv * w * Tn - 3.0 * rho2;
+  const double e = P + rho2 * (2.0 * rho - 3.0);
+  const double c2 = 3.0 * (e + (P + 3.0 * rho2) / p.cv) * w * ir;
+  *speed = sqrt(u * u + vv * vv);
+  return c2 > 0.0 ? sqrt(c2) : 0.0;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ unsigned wave_sum_u(unsigned x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+struct RedAcc {
+  double s[4];
+  double amax, rmin, rmax;
+  unsigned nonfinite;
+  __device__ void init() {
+    s[0] = s[1] = s[2] = s[3] = 0.0;
+    amax = 0.0;
+    rmin = INFINITY;
+    rmax = 0.0;
+    nonfinite = 0;
+  }
+  __device__ void add(const DevParams& p, const double v[4]) {
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      s[k] += v[k];
+      bad |= !isfinite(v[k]);
+    }
+    double sp;
+    const double c = sound_speed(p, v, &sp);
+    const double a = sp + c;
+    bad |= !isfinite(a);
+    nonfinite += bad ? 1u : 0u;
+    if (!bad) {
+      amax = fmax(amax, a);
+      rmin = fmin(rmin, v[0]);
+      rmax = fmax(rmax, v[0]);
+    }
+  }
+  // one wave's contribution into slot `slot` of ctl
+  __device__ void flush(StepCtl* ctl, int slot) {
+    double t[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] = wave_sum(s[k]);
+    const double am = wave_max(amax);
+    const double rn = wave_min(rmin);
+    const double rx = wave_max(rmax);
+    const unsigned nf = wave_sum_u(nonfinite);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) atomicAdd(&ctl->sum[slot][k], t[k]);
+      atomicMax(&ctl->amax[slot], (unsigned long long)__double_as_longlong(am));
+      if (rn >= 0.0 && rn < INFINITY)
+        atomicMin(&ctl->rmin[slot], (unsigned long long)__double_as_longlong(rn));
+      atomicMax(&ctl->rmax[slot], (unsigned long long)__double_as_longlong(fmax(rx, 0.0)));
+      if (nf) atomicAdd(&ctl->nonfinite[slot], nf);
+    }
+  }
+};
+
+__device__ __forceinline__ void clear_slot(StepCtl* ctl, int slot) {
+  for (int k = 0; k < 4; ++k) ctl->sum[slot][k] = 0.0;
+  ctl->amax[slot] = 0ull;
+  ctl->rmin[slot] = (unsigned long long)__double_as_longlong(INFINITY);
+  ctl->rmax[slot] = 0ull;
+  ctl->nonfinite[slot] = 0u;
+}
+
+// time step rule of dim2d_cfl_dt (include/dim2d_hip.h)
+__device__ __forceinline__ double cfl_dt_from(const StepCtl* ctl, int slot) {
+  const double amax = __longlong_as_double((long long)ctl->amax[slot]);
+  const double rmin = __longlong_as_double((long long)ctl->rmin[slot]);
+  const double rmax = __longlong_as_double((long long)ctl->rmax[slot]);
+  const double h = ctl->h;
+  double dt = h / amax;
+  dt = fmin(dt, ctl->re * rmin * h * h);
+  dt = fmin(dt, sqrt(ctl->we / rmax) * h * h);
+  return ctl->cfl * dt;
+}
+
+// ------------------------------------------------------------------------
+// the fused stage tile
+// ------------------------------------------------------------------------
+// STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages.
+// FP64 mode (MIXED=0): stage input S64; out = b*U + (1-b)*(S64 + dt L) with
+// b = 0 (stage 1, S64 = U), 3/4, 1/3 (rk3tvd_steps_module.f:225,444).
+// Mixed mode: state = U + S32 and out increments
+//   stage 1: S32' = dt L;  stage 2: S32' = 1/4 (S32 + dt L);
+//   stage 3: O64 = U + (1 - 1/3) (S32 + dt L)   (FP64 accumulation).
+template <int BW, typename RD, typename R, int MIXED, int STAGE, bool REDUCE>
+__device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& a, double dt,
+                                           Ring<RD, R, BW>& sm, int bx, int by, int rows,
+                                           RedAcc& red) {
+  // mixed stages 2 and 3 read u^n + increment; mixed stage 0/1 read u^n alone
+  constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
+  const int lane = threadIdx.x;
+  const int c = bx * (BW - 2 * kBcSize) + lane;  // column of this lane (0-based)
+  const int r0 = kBcSize + by * rows;
+  const int r1 = min(r0 + rows, p.ny - kBcSize);
+  const bool out_col = lane >= kBcSize && lane < BW - kBcSize && c <= p.nx - kBcSize - 1;
+  const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
+
+  bool flx;
+  const int cm = map_col(p, min(max(c, 0), p.nx - 1), flx);
+  const bool wall_w = (p.bc_w == kBcWall) && c == kBcSize;
+  const bool wall_e = (p.bc_e == kBcWall) && c == p.nx - kBcSize;
+
+  auto load_row = [&](int r, Cell<RD, R>& cell) {
+    bool fly;
+    const int rm = map_row(p, min(max(r, 0), p.ny - 1), fly);
+    double v[4];
+    load_state<SRC>(p, a.U, a.S64, a.S32, (long long)rm * p.pitch + cm, flx, fly, v);
+    make_cell<RD, R>(p, v, cell);
+  };
+
+  Cell<RD, R> cm2, cm1, c0, c1, c2;
+  load_row(r0 - 2, cm2);
+  load_row(r0 - 1, cm1);
+  load_row(r0, c0);
+  load_row(r0 + 1, c1);
+  put_slot(sm, (r0 - 1) & 3, lane, cm1);
+  put_slot(sm, r0 & 3, lane, c0);
+  put_slot(sm, (r0 + 1) & 3, lane, c1);
+  __syncthreads();
+  {
+    const int lm1 = lane > 0 ? lane - 1 : 0;
+    const int sa = (r0 - 1) & 3, sb = r0 & 3;
+    cm1.Tcap = cap_temperature<RD, R>(p, cm1.irho, sm.rho[sa][lm1], sm.rho[sa][lp1], cm2.rho, c0.rho);
+    c0.Tcap = cap_temperature<RD, R>(p, c0.irho, sm.rho[sb][lm1], sm.rho[sb][lp1], cm1.rho, c1.rho);
+    sm.Tcap[sb][lane] = c0.Tcap;
+  }
+  R fy_prev[4];
+  y_face<RD, R, BW>(p, sm, (r0 - 1) & 3, r0 & 3, lane, cm1, c0, cm2.rho, c1.rho,
+                    p.bc_s == kBcWall && r0 - 1 == kBcSize - 1, false, fy_prev);
+  __syncthreads();
+
+  const R idx = (R)p.idx, idy = (R)p.idy;
+  for (int r = r0; r < r1; ++r) {
+    const int s0 = r & 3, sm1 = (r - 1) & 3, sp1 = (r + 1) & 3, sp2 = (r + 2) & 3;
+    load_row(r + 2, c2);
+    {
+      const int lm1 = lane > 0 ? lane - 1 : 0;
+      c1.Tcap = cap_temperature<RD, R>(p, c1.irho, sm.rho[sp1][lm1], sm.rho[sp1][lp1], c0.rho, c2.rho);
+    }
+    put_slot(sm, sp2, lane, c2);
+    sm.Tcap[sp1][lane] = c1.Tcap;
+
+    R F[4], G[4];
+    x_face<RD, R, BW>(p, sm, s0, sm1, sp1, lane, c0, cm1.rho, cm1.u, cm1.v, c1.rho, c1.u, c1.v,
+                      wall_w, wall_e, F);
+    y_face<RD, R, BW>(p, sm, s0, sp1, lane, c0, c1, cm1.rho, c2.rho, false,
+                      p.bc_n == kBcWall && r == p.ny - kBcSize - 1, G);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sm.fx[r & 1][k][lane] = F[k];
+    __syncthreads();
+    R L[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L[k] = (F[k] - sm.fx[r & 1][k][lp1]) * idx + (fy_prev[k] - G[k]) * idy;
+    if (p.gravity_ac) {
+      // compute_body_forces (pmodel_eq_class.f:1840-1918)
+      L[2] -= (R)((double)c0.rho * p.gravity);
+      L[3] -= (R)((double)c0.qy * p.gravity);
+    }
+
+    if (out_col) {
+      const long long o = (long long)r * p.pitch + c;
+      double vout[4];
+      if (STAGE == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a.O64[k * p.plane + o] = (double)L[k];
+      } else if (!MIXED) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double s = a.S64[k * p.plane + o];
+          const double inc = s + dt * (double)L[k];
+          double val;
+          if (STAGE == 1) val = inc;
+          else if (STAGE == 2) val = 0.75 * a.U[k * p.plane + o] + (1.0 - 0.75) * inc;
+          else val = (1.0 / 3.0) * a.U[k * p.plane + o] + (1.0 - 1.0 / 3.0) * inc;
+          a.O64[k * p.plane + o] = val;
+          vout[k] = val;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double dl = dt * (double)L[k];
+          if (STAGE == 1) {
+            a.O32[k * p.plane + o] = (float)dl;
+          } else if (STAGE == 2) {
+            a.O32[k * p.plane + o] = (float)(0.25 * ((double)a.S32[k * p.plane + o] + dl));
+          } else {
+            const double val = a.U[k * p.plane + o] +
+                               (1.0 - 1.0 / 3.0) * ((double)a.S32[k * p.plane + o] + dl);
+            a.O64[k * p.plane + o] = val;
+            vout[k] = val;
+          }
+        }
+      }
+      if (REDUCE && STAGE == 3) red.add(p, vout);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fy_prev[k] = G[k];
+    cm1 = c0;
+    c0 = c1;
+    c1 = c2;
+  }
+}
+
+template <int BW, int MIXED, int STAGE>
+__global__ void __launch_bounds__(BW) stage_kernel(const DevParams p, const StageArgs a, int rows) {
+  using RD = double;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  __shared__ Ring<RD, R, BW> sm;
+  double dt = a.dt_fixed;
+  if (STAGE == 1) {
+    if (a.cfl_mode) dt = cfl_dt_from(a.ctl, a.parity);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      a.ctl->dt = dt;
+      clear_slot(a.ctl, a.parity ^ 1);
+    }
+  } else if (STAGE != 0) {
+    dt = a.ctl->dt;
+  }
+  RedAcc red;
+  red.init();
+  stage_tile<BW, RD, R, MIXED, STAGE, STAGE == 3>(p, a, dt, sm, blockIdx.x, blockIdx.y, rows, red);
+  if (STAGE == 3) {
+    red.flush(a.ctl, a.parity ^ 1);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctl->t += dt;
+  }
+}
+
+// persistent batch kernel: one grid walks the tile list of all simulations;
+// each simulation has a fixed dt (a.S64/U/O pointers are batch bases, offset
+// by DevParams::offset); no reductions.
+template <int BW, int MIXED, int STAGE>
+__global__ void __launch_bounds__(BW) stage_batch_kernel(const DevParams* __restrict__ P,
+                                                         const WorkItem* __restrict__ items,
+                                                         int nitems, const StageArgs a,
+                                                         const double* __restrict__ dts, int rows) {
+  using RD = double;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  __shared__ Ring<RD, R, BW> sm;
+  RedAcc red;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const DevParams p = P[w.sim];
+    StageArgs b = a;
+    b.U = a.U ? a.U + p.offset : nullptr;
+    b.S64 = a.S64 ? a.S64 + p.offset : nullptr;
+    b.S32 = a.S32 ? a.S32 + p.offset : nullptr;
+    b.O64 = a.O64 ? a.O64 + p.offset : nullptr;
+    b.O32 = a.O32 ? a.O32 + p.offset : nullptr;
+    stage_tile<BW, RD, R, MIXED, STAGE, false>(p, b, dts[w.sim], sm, w.bx, w.by, rows, red);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------
+// wall ghost cells (bc_operators_wall_xy_class.f:232-330 and
+// wall_xy_equilibrium_module.f:948-1437, 1747-2290, ridders_method_module.f:112-216)
+// ------------------------------------------------------------------------
+struct WallRoot {
+  double far, first;   // md_x = [far, first] along the wall normal
+  double ys0, ys1;     // md_y: transverse neighbours of the first interior cell
+  double ux, uy, Ed;
+  double dn, dt;       // normal and transverse spacings
+};
+
+__device__ double wall_root_f(const DevParams& p, const WallRoot& w, double x) {
+  // wall_x_equilibrium_root_fct(md = first, md_x = [far, x]) (:2151-2236)
+  const double md = w.first;
+  const double dmddx_half = (-md + x) / w.dn;
+  const double md_av = 0.5 * (md + x);
+  const double gn = (x - w.far) / (2.0 * w.dn);
+  const double gt = (w.ys1 - w.ys0) / (2.0 * w.dt);
+  const double g2 = gn * gn + gt * gt;
+  const double T1 = p.c38cv * (w.Ed / md - 0.5 * (w.ux * w.ux + w.uy * w.uy) -
+                               0.5 / p.we * g2 / md + 3.0 * md);
+  const double Tav = T1 + 0.5 * w.dn * p.re * p.pr * p.c38cv * p.wall_q;
+  // dwallInternalEnergy_dmd (:2238-2293)
+  const double sq = sqrt(1.0 - Tav);
+  const double ml = 1.0 + 2.06 * sq;
+  const double mv = 1.0 - 1.91 * sq;
+  const double sigma = p.sigma_pref * (9.691686257306483 * sq * sq * sq);
+  const double d = ml - mv;
+  const double dw = 6.0 * (md_av - ml) * (md_av - mv) / (d * d * d) * sigma * p.wall_cos_angle;
+  return p.inv_we * dmddx_half + dw;
+}
+
+__device__ __forceinline__ bool same_sign(double a, double b) {
+  return copysign(1.0, a) * copysign(1.0, b) > 0.0;
+}
+__device__ __forceinline__ bool validated(double v, double c) { return fabs(v - c) < 1e-10; }
+
+__device__ double wall_ghost_density(const DevParams& p, const WallRoot& w, double T_guess,
+                                     double md_guess) {
+  // get_wall_x_root_brackets (:1846-1941)
+  double xl = 1.0 - 1.91 * sqrt(1.0 - T_guess);
+  double xh = 1.0 + 2.06 * sqrt(1.0 - T_guess);
+  double fl = wall_root_f(p, w, xl);
+  double fh = wall_root_f(p, w, xh);
+  if (same_sign(fl, fh)) {
+    const double dl = md_guess / 100.0, dr = (3.0 - md_guess) / 100.0;
+    const double fmd = wall_root_f(p, w, md_guess);
+    bool found = false;
+    for (int j = 1; j <= 99 && !found; ++j) {
+      const double a = md_guess - j * dl, b = md_guess + j * dr;
+      const double fa = wall_root_f(p, w, a), fb = wall_root_f(p, w, b);
+      if (!same_sign(fa, fmd)) {
+        xl = a; fl = fa; xh = md_guess; fh = fmd; found = true;
+      } else if (!same_sign(fb, fmd)) {
+        xl = md_guess; fl = fmd; xh = b; fh = fb; found = true;
+      }
+    }
+    if (!found) return NAN;
+  }
+  // get_root_ridder_method with xacc = 1e-12 (default-kind literal)
+  const double xacc = (double)1e-12f;
+  if ((fl > 0.0 && fh < 0.0) || (fl < 0.0 && fh > 0.0)) {
+    double x = (double)(-1.11e30f);
+    for (int j = 1; j <= 60; ++j) {
+      const double xm = 0.5 * (xl + xh);
+      const double fm = wall_root_f(p, w, xm);
+      const double s = sqrt(fm * fm - fl * fh);
+      if (validated(s, 0.0)) return x;
+      const double xnew = xm + (xm - xl) * (copysign(1.0, fl - fh) * fm / s);
+      if (fabs(xnew - x) <= xacc) return x;
+      x = xnew;
+      const double fnew = wall_root_f(p, w, x);
+      if (validated(fnew, 0.0)) return x;
+      if (!validated(copysign(fm, fnew), fm)) {
+        xl = xm; fl = fm; xh = x; fh = fnew;
+      } else if (!validated(copysign(fl, fnew), fl)) {
+        xh = x; fh = fnew;
+      } else if (!validated(copysign(fh, fnew), fh)) {
+        xl = x; fl = fnew;
+      } else {
+        return NAN;
+      }
+      if (fabs(xh - xl) <= xacc) return x;
+    }
+    return NAN;
+  }
+  if (validated(fl, 0.0)) return xl;
+  if (validated(fh, 0.0)) return xh;
+  return NAN;
+}
+
+// state of cell (c, r) (mapped) from an FP64 field or U + increment
+__device__ __forceinline__ void wall_load(const DevParams& p, const double* U, const double* S64,
+                                          const float* S32, int mixed, int c, int r, double v[4]) {
+  bool fx, fy;
+  const int cm = map_col(p, c, fx), rm = map_row(p, r, fy);
+  const long long idx = (long long)rm * p.pitch + cm;
+  for (int k = 0; k < 4; ++k)
+    v[k] = mixed ? U[k * p.plane + idx] + (double)S32[k * p.plane + idx] : S64[k * p.plane + idx];
+  if (fx) v[1] = -v[1];
+  if (fy) v[2] = -v[2];
+}
+__device__ __forceinline__ double rho_at(const DevParams& p, const double* F, int c, int r) {
+  bool fx, fy;
+  const int cm = map_col(p, c, fx), rm = map_row(p, r, fy);
+  return F[(long long)rm * p.pitch + cm];
+}
+
+// temperature_eff (:652-691) at (c, r) of the FP64 field F (for the bracket guess)
+__device__ double temperature_eff_at(const DevParams& p, const double* F, int c, int r) {
+  double v[4];
+  wall_load(p, nullptr, F, nullptr, 0, c, r, v);
+  const double gx = 0.5 / p.dx * (-rho_at(p, F, c - 1, r) + rho_at(p, F, c + 1, r));
+  const double gy = 0.5 / p.dy * (-rho_at(p, F, c, r - 1) + rho_at(p, F, c, r + 1));
+  const double rho = v[0];
+  return 1.0 / rho *
+         (v[3] - 0.5 * rho * ((v[1] / rho) * (v[1] / rho) + (v[2] / rho) * (v[2] / rho)) -
+          0.5 / p.we * (gx * gx + gy * gy) + 3.0 * rho * rho);
+}
+
+// one thread per wall cell; edge e: 0 S, 1 N, 2 W, 3 E
+__device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const double* U,
+                          const double* S64, const float* S32, double* O64, float* O32,
+                          const double* tmp) {
+  int c, r, c1, r1, c2, r2, co, ro;  // ghost, first interior, far interior, outer ghost
+  if (e <= 1) {
+    c = c1 = c2 = co = kBcSize + t;
+    if (e == 0) { r = kBcSize - 1; r1 = r + 1; r2 = r + 2; ro = r - 1; }
+    else { r = p.ny - kBcSize; r1 = r - 1; r2 = r - 2; ro = r + 1; }
+  } else {
+    r = r1 = r2 = ro = kBcSize + t;
+    if (e == 2) { c = kBcSize - 1; c1 = c + 1; c2 = c + 2; co = c - 1; }
+    else { c = p.nx - kBcSize; c1 = c - 1; c2 = c - 2; co = c + 1; }
+  }
+  double v1[4], v2[4];
+  wall_load(p, U, S64, S32, mixed_in, c1, r1, v1);
+  wall_load(p, U, S64, S32, mixed_in, c2, r2, v2);
+  WallRoot w;
+  w.far = v2[0];
+  w.first = v1[0];
+  w.Ed = v1[3];
+  double vy0[4], vy1[4];
+  if (e <= 1) {
+    wall_load(p, U, S64, S32, mixed_in, c1 - 1, r1, vy0);
+    wall_load(p, U, S64, S32, mixed_in, c1 + 1, r1, vy1);
+    w.ux = (e == 0 ? -v1[2] : v1[2]) / v1[0];  // normal velocity (sign of the S/N cell)
+    w.uy = v1[1] / v1[0];
+    w.dn = p.dy;
+    w.dt = p.dx;
+  } else {
+    wall_load(p, U, S64, S32, mixed_in, c1, r1 - 1, vy0);
+    wall_load(p, U, S64, S32, mixed_in, c1, r1 + 1, vy1);
+    w.ux = (e == 2 ? -v1[1] : v1[1]) / v1[0];
+    w.uy = v1[2] / v1[0];
+    w.dn = p.dx;
+    w.dt = p.dy;
+  }
+  w.ys0 = vy0[0];
+  w.ys1 = vy1[0];
+  // compute_wall_xy_guess from u^n (bc_operators_wall_xy_class.f:898-930)
+  const double T_guess = p.c38cv * temperature_eff_at(p, tmp, c1, r1);
+  const double md_guess = tmp[(long long)r * p.pitch + c];
+  const double rg = wall_ghost_density(p, w, T_guess, md_guess);
+  // sign_normal = +1, sign_trans = -1 (wall_xy_equilibrium_module.f:136-140)
+  double g[4], go[4];
+  g[0] = rg;
+  go[0] = v2[0] + 3.0 * (-v1[0] + rg);  // wall_dmddx_high_order
+  if (e <= 1) {
+    g[1] = -1.0 * v1[1] / v1[0] * rg;
+    g[2] = 1.0 * v1[2] / v1[0] * rg;
+  } else {
+    g[1] = 1.0 * v1[1] / v1[0] * rg;
+    g[2] = -1.0 * v1[2] / v1[0] * rg;
+  }
+  // the reference leaves the ghost energy and the outer momenta unset; they are
+  // never read by an interior stencil.  Keep finite values there.
+  g[3] = v1[3];
+  go[1] = g[1];
+  go[2] = g[2];
+  go[3] = v1[3];
+  const long long ig = (long long)r * p.pitch + c, io = (long long)ro * p.pitch + co;
+  for (int k = 0; k < 4; ++k) {
+    if (O32) {
+      O32[k * p.plane + ig] = (float)(g[k] - U[k * p.plane + ig]);
+      O32[k * p.plane + io] = (float)(go[k] - U[k * p.plane + io]);
+    } else {
+      O64[k * p.plane + ig] = g[k];
+      O64[k * p.plane + io] = go[k];
+    }
+  }
+}
+
+__global__ void wall_kernel(const DevParams p, int mixed_in, const double* U, const double* S64,
+                            const float* S32, double* O64, float* O32, const double* tmp) {
+  const int nS = (p.bc_s == kBcWall) ? p.nx - 2 * kBcSize : 0;
+  const int nN = (p.bc_n == kBcWall) ? p.nx - 2 * kBcSize : 0;
+  const int nW = (p.bc_w == kBcWall) ? p.ny - 2 * kBcSize : 0;
+  const int nE = (p.bc_e == kBcWall) ? p.ny - 2 * kBcSize : 0;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int e;
+  if (t < nS) e = 0;
+  else if ((t -= nS) < nN) e = 1;
+  else if ((t -= nN) < nW) e = 2;
+  else if ((t -= nW) < nE) e = 3;
+  else return;
+  wall_cell(p, e, t, mixed_in, U, S64, S32, O64, O32, tmp);
+}
+
+__global__ void wall_batch_kernel(const DevParams* P, int nsim, int mixed_in, const double* U,
+                                  const double* S64, const float* S32, double* O64, float* O32,
+                                  const double* tmp) {
+  const int sim = blockIdx.y;
+  if (sim >= nsim) return;
+  const DevParams p = P[sim];
+  const long long o = p.offset;
+  const int nS = (p.bc_s == kBcWall) ? p.nx - 2 * kBcSize : 0;
+  const int nN = (p.bc_n == kBcWall) ? p.nx - 2 * kBcSize : 0;
+  const int nW = (p.bc_w == kBcWall) ? p.ny - 2 * kBcSize : 0;
+  const int nE = (p.bc_e == kBcWall) ? p.ny - 2 * kBcSize : 0;
+  for (int t0 = blockIdx.x * blockDim.x + threadIdx.x; t0 < nS + nN + nW + nE;
+       t0 += gridDim.x * blockDim.x) {
+    int t = t0, e;
+    if (t < nS) e = 0;
+    else if ((t -= nS) < nN) e = 1;
+    else if ((t -= nN) < nW) e = 2;
+    else { t -= nW; e = 3; }
+    wall_cell(p, e, t, mixed_in, U ? U + o : nullptr, S64 ? S64 + o : nullptr,
+              S32 ? S32 + o : nullptr, O64 ? O64 + o : nullptr, O32 ? O32 + o : nullptr,
+              tmp + o);
+  }
+}
+
+// ------------------------------------------------------------------------
+// ghost materialisation, reductions
+// ------------------------------------------------------------------------
+__global__ void fill_ghosts_kernel(const DevParams p, double* F) {
+  const int nrow = 2 * kBcSize * p.nx;
+  const int ncol = 2 * kBcSize * (p.ny - 2 * kBcSize);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int c, r;
+  if (t < nrow) {
+    const int q = t / p.nx;
+    c = t % p.nx;
+    r = q < kBcSize ? q : p.ny - 2 * kBcSize + q;
+  } else if (t < nrow + ncol) {
+    const int u = t - nrow;
+    const int q = u / (p.ny - 2 * kBcSize);
+    r = kBcSize + u % (p.ny - 2 * kBcSize);
+    c = q < kBcSize ? q : p.nx - 2 * kBcSize + q;
+  } else {
+    return;
+  }
+  bool fx, fy;
+  const int cm = map_col(p, c, fx), rm = map_row(p, r, fy);
+  if (cm == c && rm == r) return;  // stored (wall) ghost
+  const long long src = (long long)rm * p.pitch + cm, dst = (long long)r * p.pitch + c;
+  for (int k = 0; k < 4; ++k) {
+    double v = F[k * p.plane + src];
+    if ((k == 1 && fx) || (k == 2 && fy)) v = -v;
+    F[k * p.plane + dst] = v;
+  }
+}
+
+__global__ void clear_slot_kernel(StepCtl* ctl, int slot) { clear_slot(ctl, slot); }
+
+__global__ void __launch_bounds__(256) reduce_kernel(const DevParams p, const double* F,
+                                                     StepCtl* ctl, int slot) {
+  RedAcc red;
+  red.init();
+  const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
+  const long long n = (long long)ni * nj;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = kBcSize + (int)(t % ni), j = kBcSize + (int)(t / ni);
+    const long long o = (long long)j * p.pitch + i;
+    double v[4];
+    for (int k = 0; k < 4; ++k) v[k] = F[k * p.plane + o];
+    red.add(p, v);
+  }
+  red.flush(ctl, slot);
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+template <int BW, int MIXED, int STAGE>
+static hipError_t launch_one(const DevParams& p, const StageArgs& a, int rows, hipStream_t s) {
+  const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
+  dim3 grid((ni + BW - 2 * kBcSize - 1) / (BW - 2 * kBcSize), (nj + rows - 1) / rows);
+  hipLaunchKernelGGL((stage_kernel<BW, MIXED, STAGE>), grid, dim3(BW), 0, s, p, a, rows);
+  return hipGetLastError();
+}
+
+template <int BW>
+static hipError_t launch_bw(const DevParams& p, int mixed, int stage, const StageArgs& a, int rows,
+                            hipStream_t s) {
+  if (!mixed) {
+    switch (stage) {
+      case 0: return launch_one<BW, 0, 0>(p, a, rows, s);
+      case 1: return launch_one<BW, 0, 1>(p, a, rows, s);
+      case 2: return launch_one<BW, 0, 2>(p, a, rows, s);
+      default: return launch_one<BW, 0, 3>(p, a, rows, s);
+    }
+  }
+  switch (stage) {
+    case 0: return launch_one<BW, 1, 0>(p, a, rows, s);
+    case 1: return launch_one<BW, 1, 1>(p, a, rows, s);
+    case 2: return launch_one<BW, 1, 2>(p, a, rows, s);
+    default: return launch_one<BW, 1, 3>(p, a, rows, s);
+  }
+}
+
+hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArgs& a, int block_x,
+                        int rows, hipStream_t s) {
+  if (block_x == 128) return launch_bw<128>(p, mixed, stage, a, rows, s);
+  return launch_bw<64>(p, mixed, stage, a, rows, s);
+}
+
+template <int BW>
+static hipError_t launch_batch_bw(const DevParams* P, const WorkItem* items, int n, int mixed,
+                                  int stage, const StageArgs& a, int rows, int wg, hipStream_t s) {
+  const double* dts = a.dts;
+#define DIM2D_BL(M, S)                                                                     \
+  hipLaunchKernelGGL((stage_batch_kernel<BW, M, S>), dim3(wg), dim3(BW), 0, s, P, items, n, a, \
+                     dts, rows)
+  if (!mixed) {
+    if (stage == 1) DIM2D_BL(0, 1);
+    else if (stage == 2) DIM2D_BL(0, 2);
+    else DIM2D_BL(0, 3);
+  } else {
+    if (stage == 1) DIM2D_BL(1, 1);
+    else if (stage == 2) DIM2D_BL(1, 2);
+    else DIM2D_BL(1, 3);
+  }
+#undef DIM2D_BL
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
+                              int mixed, int stage, const StageArgs& a, int block_x, int rows,
+                              int workgroups, hipStream_t s) {
+  if (block_x == 128)
+    return launch_batch_bw<128>(d_params, d_items, nitems, mixed, stage, a, rows, workgroups, s);
+  return launch_batch_bw<64>(d_params, d_items, nitems, mixed, stage, a, rows, workgroups, s);
+}
+
+hipError_t launch_wall(const DevParams& p, int mixed_in, const double* U, const double* S64,
+                       const float* S32, double* O64, float* O32, const double* tmp,
+                       hipStream_t s) {
+  const int n = ((p.bc_s == kBcWall) + (p.bc_n == kBcWall)) * (p.nx - 2 * kBcSize) +
+                ((p.bc_w == kBcWall) + (p.bc_e == kBcWall)) * (p.ny - 2 * kBcSize);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(wall_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, mixed_in, U, S64, S32,
+                     O64, O32, tmp);
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_batch(const DevParams* h_params, const DevParams* d_params, int nsim,
+                             int mixed_in, const double* U, const double* S64, const float* S32,
+                             double* O64, float* O32, const double* tmp, hipStream_t s) {
+  int nmax = 0;
+  for (int i = 0; i < nsim; ++i) {
+    const DevParams& p = h_params[i];
+    const int n = ((p.bc_s == kBcWall) + (p.bc_n == kBcWall)) * (p.nx - 2 * kBcSize) +
+                  ((p.bc_w == kBcWall) + (p.bc_e == kBcWall)) * (p.ny - 2 * kBcSize);
+    nmax = n > nmax ? n : nmax;
+  }
+  if (nmax == 0) return hipSuccess;
+  hipLaunchKernelGGL(wall_batch_kernel, dim3((nmax + 63) / 64, nsim), dim3(64), 0, s, d_params,
+                     nsim, mixed_in, U, S64, S32, O64, O32, tmp);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_ghosts(const DevParams& p, double* F, hipStream_t s) {
+  const int n = 2 * kBcSize * p.nx + 2 * kBcSize * (p.ny - 2 * kBcSize);
+  hipLaunchKernelGGL(fill_ghosts_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int slot,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(clear_slot_kernel, dim3(1), dim3(1), 0, s, ctl, slot);
+  const long long n = (long long)(p.nx - 2 * kBcSize) * (p.ny - 2 * kBcSize);
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, F, ctl, slot);
+  return hipGetLastError();
+}
+
+int max_resident_workgroups(int block_x) {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  int per = 0;
+  hipError_t e;
+  if (block_x == 128)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, stage_batch_kernel<128, 0, 2>, 128, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, stage_batch_kernel<64, 0, 2>, 64, 0);
+  if (e != hipSuccess || per <= 0) per = 4;
+  return ncu * per;
+}
+
+}  // namespace dim2d

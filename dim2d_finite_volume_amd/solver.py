@@ -1,0 +1,213 @@
+"""Host-side mirror of the reference's field / td_operators / td_integrator /
+bc_operators interface, running on the HIP library through its C ABI.
+
+Reference interface mirrored here (jdesmarais/dim2D_finite_volume):
+  field%integrate(dt)                       field/field_class.f:61-72
+  td_integrator%integrate(field, dt)        td_integrator/rungekutta3rdtvd/td_integrator_class.f:103-161
+  td_operators%compute_time_dev(...)        td_operators/finitevolume/td_operators_class.f:119-235
+  bc_operators_gen%apply_bc_on_nodes(...)   boundary_conditions/bc_operators_gen_class.f:238-381
+  field%ini coordinates and borders         field/field_abstract_class.f:244-315
+
+Arrays are the reference's nodes(nx, ny, ne) in Fortran order, held in numpy
+as C arrays of shape (4, ny, nx) -- the same bytes.  Errors raise
+capi.Dim2dError (the reference prints and stops).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field as dc_field
+
+import numpy as np
+
+from . import capi
+
+BC_SIZE = 2
+
+
+@dataclass
+class ModelParams:
+    """Compile-time parameters of the reference (parameters_input.f,
+    dim2d_parameters.f) as run-time values.  Defaults are the reference's
+    test physics (dim2d_parameters.f:182-187) on [-0.5, 0.5]^2."""
+    nx: int
+    ny: int
+    bc: tuple = (capi.BC_REFLECTION,) * 4  # (W, E, S, N)
+    precision: int = capi.PREC_FP64
+    x_min: float = -0.5
+    x_max: float = 0.5
+    y_min: float = -0.5
+    y_max: float = 0.5
+    re: float = 5.0
+    we: float = 10.0
+    pr: float = 20.0
+    viscous_r: float = -1.5
+    cv_r: float = 2.5
+    gravity: float = 0.0
+    wall_contact_angle: float = 90.0
+    wall_heat_flux: float = 0.0
+
+    def to_c(self) -> capi.Params:
+        return capi.Params(
+            self.nx, self.ny, BC_SIZE, self.bc[0], self.bc[1], self.bc[2], self.bc[3],
+            self.precision, int(self.gravity != 0.0), self.x_min, self.x_max, self.y_min,
+            self.y_max, self.re, self.we, self.pr, self.viscous_r, self.cv_r, self.gravity,
+            self.wall_contact_angle, self.wall_heat_flux)
+
+    def coordinates(self):
+        """x_map, y_map (field_abstract_class.f:286-315)."""
+        dx = (self.x_max - self.x_min) / (self.nx - 1 - 2 * BC_SIZE)
+        dy = (self.y_max - self.y_min) / (self.ny - 1 - 2 * BC_SIZE)
+        x_map = np.array([self.x_min + (i - 1 - BC_SIZE) * dx for i in range(1, self.nx + 1)])
+        y_map = np.array([self.y_min + (j - 1 - BC_SIZE) * dy for j in range(1, self.ny + 1)])
+        return x_map, y_map
+
+    def spacing(self):
+        """dx, dy as td_operators_class.f:156-157 takes them."""
+        x_map, y_map = self.coordinates()
+        return x_map[1] - x_map[0], y_map[1] - y_map[0]
+
+
+def _dptr(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Field:
+    """One simulation resident in HBM (field_abstract + its operators)."""
+
+    def __init__(self, params: ModelParams, nodes: np.ndarray | None = None, time: float = 0.0):
+        self.params = params
+        self.x_map, self.y_map = params.coordinates()
+        self.time = time
+        self._lib = capi.load()
+        self._ctx = ctypes.c_void_p()
+        cp = params.to_c()
+        capi.check(self._lib.dim2d_ctx_create(ctypes.byref(cp), ctypes.byref(self._ctx)),
+                   "dim2d_ctx_create")
+        if nodes is not None:
+            self.upload(nodes)
+
+    # -- state transfer -------------------------------------------------
+    def _check_shape(self, a: np.ndarray):
+        shape = (4, self.params.ny, self.params.nx)
+        if a.shape != shape:
+            raise ValueError(f"nodes must have shape {shape} (nodes(nx,ny,ne) in Fortran order)")
+
+    def upload(self, nodes: np.ndarray) -> None:
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        self._check_shape(nodes)
+        capi.check(self._lib.dim2d_upload(self._ctx, _dptr(nodes)), "dim2d_upload")
+
+    def download(self) -> np.ndarray:
+        out = np.empty((4, self.params.ny, self.params.nx))
+        capi.check(self._lib.dim2d_download(self._ctx, _dptr(out)), "dim2d_download")
+        return out
+
+    @property
+    def nodes(self) -> np.ndarray:
+        return self.download()
+
+    # -- operators ----------------------------------------------------------
+    def compute_time_dev(self) -> np.ndarray:
+        """td_operators%compute_time_dev of the current state."""
+        out = np.empty((4, self.params.ny, self.params.nx))
+        capi.check(self._lib.dim2d_time_dev(self._ctx, self.time, _dptr(out)), "dim2d_time_dev")
+        return out
+
+    def apply_bc_on_nodes(self) -> None:
+        capi.check(self._lib.dim2d_apply_bc(self._ctx, self.time), "dim2d_apply_bc")
+
+    def integrate(self, dt: float) -> None:
+        """field%integrate: one RK3-TVD step, then time += dt (field_class.f:61-72)."""
+        capi.check(self._lib.dim2d_rk3_step(self._ctx, self.time, dt), "dim2d_rk3_step")
+        self.time += dt
+
+    def run(self, nsteps: int, dt: float = 0.0, cfl: float = 0.0, sync: bool = True) -> None:
+        """nsteps steps with a fixed dt, or with the CFL rule when cfl > 0."""
+        capi.check(self._lib.dim2d_integrate(self._ctx, self.time, dt, cfl, nsteps),
+                   "dim2d_integrate")
+        if sync:
+            self.synchronize()
+            self.time = self.device_time()[0]
+
+    def synchronize(self) -> None:
+        capi.check(self._lib.dim2d_synchronize(self._ctx), "dim2d_synchronize")
+
+    def device_time(self):
+        out = np.zeros(2)
+        capi.check(self._lib.dim2d_time(self._ctx, _dptr(out)), "dim2d_time")
+        return out
+
+    def cfl_dt(self, cfl: float):
+        out = np.zeros(4)
+        capi.check(self._lib.dim2d_cfl_dt(self._ctx, cfl, _dptr(out)), "dim2d_cfl_dt")
+        return out
+
+    def diagnostics(self):
+        out = np.zeros(7)
+        capi.check(self._lib.dim2d_diagnostics(self._ctx, _dptr(out)), "dim2d_diagnostics")
+        return out
+
+    def set_tiles(self, block_x: int = 0, rows: int = 0) -> None:
+        capi.check(self._lib.dim2d_set_tiles(self._ctx, block_x, rows), "dim2d_set_tiles")
+
+    def close(self) -> None:
+        if self._ctx:
+            self._lib.dim2d_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent simulations advanced together (persistent stage
+    kernels over one work list, replayed from a captured HIP graph)."""
+
+    def __init__(self, params: list[ModelParams]):
+        self.params = list(params)
+        self._lib = capi.load()
+        arr = (capi.Params * len(params))(*[p.to_c() for p in params])
+        self._b = ctypes.c_void_p()
+        capi.check(self._lib.dim2d_batch_create(arr, len(params), ctypes.byref(self._b)),
+                   "dim2d_batch_create")
+
+    def upload(self, sim: int, nodes: np.ndarray) -> None:
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        capi.check(self._lib.dim2d_batch_upload(self._b, sim, _dptr(nodes)), "dim2d_batch_upload")
+
+    def download(self, sim: int) -> np.ndarray:
+        p = self.params[sim]
+        out = np.empty((4, p.ny, p.nx))
+        capi.check(self._lib.dim2d_batch_download(self._b, sim, _dptr(out)),
+                   "dim2d_batch_download")
+        return out
+
+    def integrate(self, dts, nsteps: int, sync: bool = True) -> None:
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        capi.check(self._lib.dim2d_batch_integrate(self._b, _dptr(d), nsteps),
+                   "dim2d_batch_integrate")
+        if sync:
+            self.synchronize()
+
+    def synchronize(self) -> None:
+        capi.check(self._lib.dim2d_batch_synchronize(self._b), "dim2d_batch_synchronize")
+
+    def set_tiles(self, block_x: int = 0, rows: int = 0, workgroups: int = 0) -> None:
+        capi.check(self._lib.dim2d_batch_set_tiles(self._b, block_x, rows, workgroups),
+                   "dim2d_batch_set_tiles")
+
+    def close(self) -> None:
+        if self._b:
+            self._lib.dim2d_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

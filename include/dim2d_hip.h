@@ -1,0 +1,162 @@
+/*
+ * dim2d_hip.h -- C ABI of the MI355X (gfx950) DIM time-integration library.
+ *
+ * The reference solver (Fortran, jdesmarais/dim2D_finite_volume) binds these
+ * entry points through ISO_C_BINDING in place of its td_operators /
+ * td_integrator / bc_operators modules for the serial hot path.  Every entry
+ * point returns 0 on success and a negative DIM2D_E* code otherwise; the
+ * message of the last error on the calling thread is available from
+ * dim2d_last_error().  The reference prints and stops on errors
+ * (errors_module.f:69,106); its Fortran shim keeps that convention by
+ * stopping on a non-zero status.
+ *
+ * Arrays exchanged with the host have the reference's layout nodes(nx,ny,ne):
+ * Fortran column-major, i (x) fastest, then j (y), then the variable
+ * k = 1 mass density, 2 momentum_x, 3 momentum_y, 4 total energy; nx and ny
+ * include the bc_size = 2 ghost layers on each side.
+ *
+ * One context owns one simulation's device buffers and one HIP stream; calls on
+ * one context must not overlap (one host thread per context).
+ */
+#ifndef DIM2D_HIP_H
+#define DIM2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DIM2D_OK 0
+#define DIM2D_EINVAL (-1)   /* invalid argument or unsupported configuration */
+#define DIM2D_EDEVICE (-2)  /* HIP runtime error */
+#define DIM2D_ENOMEM (-3)   /* device allocation failed */
+#define DIM2D_ESTATE (-4)   /* call out of order (e.g. step before upload) */
+#define DIM2D_ENONFINITE (-5) /* non-finite values found by a diagnostic */
+
+/* boundary kinds per edge (bc_operators_gen_class.f dispatch choices) */
+#define DIM2D_BC_REFLECTION 0 /* reflection_x / reflection_y */
+#define DIM2D_BC_PERIODIC 1   /* periodic_xy */
+#define DIM2D_BC_WALL 2       /* wall_xy (equilibrium wall, constant heat flux) */
+
+/* arithmetic of the stage kernels */
+#define DIM2D_PREC_FP64 0  /* everything in FP64 */
+#define DIM2D_PREC_MIXED 1 /* FP64 u^n and RK accumulation, FP32 stage increments and stencils */
+
+typedef struct dim2d_params {
+  int32_t nx, ny;          /* grid points including ghosts (parameters_input.f nx, ny) */
+  int32_t bc_size;         /* must be 2 (Mattsson operators, sd_operators_class.f get_bc_size) */
+  int32_t bc_w, bc_e, bc_s, bc_n; /* DIM2D_BC_* per edge */
+  int32_t precision;       /* DIM2D_PREC_* */
+  int32_t gravity_ac;      /* body force switch (parameters_input.f gravity_ac) */
+  double x_min, x_max, y_min, y_max; /* coordinates of the first/last interior points */
+  double re, we, pr, viscous_r, cv_r; /* dim2d_parameters.f */
+  double gravity;          /* gravity_amp */
+  double wall_contact_angle; /* degrees (wall_micro_contact_angle) */
+  double wall_heat_flux;   /* constant wall heat flux, 0 = no_heat_source */
+} dim2d_params;
+
+typedef struct dim2d_ctx dim2d_ctx;
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* dim2d_last_error(void);
+
+/* Library version string. */
+const char* dim2d_version(void);
+
+/* Create a context for one simulation on the current HIP device.
+ * Replaces the set-up half of field_abstract%ini (field_abstract_class.f:148-198):
+ * ini_coordinates (:286-315) and ini_for_timeInt (:244-270). */
+int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out);
+
+/* Release a context and its device buffers. */
+int dim2d_ctx_destroy(dim2d_ctx* ctx);
+
+/* Copy the host field nodes(nx,ny,4) into the device state u^n (blocking).
+ * Replaces the host ownership of field_abstract%nodes (field_abstract_class.f:87). */
+int dim2d_upload(dim2d_ctx* ctx, const double* nodes);
+
+/* Copy the device state back to nodes(nx,ny,4), ghosts filled as after
+ * bc_operators_gen%apply_bc_on_nodes (blocking). */
+int dim2d_download(dim2d_ctx* ctx, double* nodes);
+
+/* Same two calls on device memory owned by the caller (stream-ordered on the
+ * context's stream, no host synchronisation). */
+int dim2d_upload_device(dim2d_ctx* ctx, const double* d_nodes);
+int dim2d_download_device(dim2d_ctx* ctx, double* d_nodes);
+
+/* time_dev(nx,ny,4) of the current state, interior cells only; ghost cells are
+ * set to the reference's debug_real (NF90_FILL_DOUBLE).  Replaces
+ * td_operators%compute_time_dev (td_operators/finitevolume/td_operators_class.f:119-235)
+ * including bc_operators_gen%apply_bc_on_fluxes for wall edges (:171-177). */
+int dim2d_time_dev(dim2d_ctx* ctx, double t, double* time_dev);
+
+/* Fill the ghost cells of the device state (bc_operators_gen%apply_bc_on_nodes,
+ * bc_operators_gen_class.f:238-381).  Reflection and periodic ghosts are also
+ * resolved on the fly inside the stage kernels. */
+int dim2d_apply_bc(dim2d_ctx* ctx, double t);
+
+/* One RK3-TVD step with time step dt: three fused stages, each computing the
+ * time derivative, the stage update and the wall ghost cells.  Replaces
+ * td_integrator%integrate (td_integrator/rungekutta3rdtvd/td_integrator_class.f:103-161)
+ * with compute_{1st,2nd,3rd}_step (rk3tvd_steps_module.f:52-100,205-269,424-490). */
+int dim2d_rk3_step(dim2d_ctx* ctx, double t, double dt);
+
+/* nsteps RK3-TVD steps without host synchronisation.
+ * cfl > 0: dt of each step from the CFL rule below, evaluated on the device
+ *          (fused into the third stage of the previous step); dt_fixed ignored.
+ * cfl <= 0: every step uses dt_fixed.
+ * Replaces the time loop of sim_dim2d.f:81-92 for the steps between outputs. */
+int dim2d_integrate(dim2d_ctx* ctx, double t0, double dt_fixed, double cfl, int32_t nsteps);
+
+/* Time step rule (no counterpart in the reference, whose dt is a compile-time
+ * constant, parameters_input.f:63):
+ *   dt = cfl * min( h / max(|u|+c), Re*min(rho)*h^2, sqrt(We/max(rho))*h^2 ),
+ * h = min(dx,dy), c the speed of sound (dim2d_prim_module.f:1179-1215) in the
+ * reduced form c^2 = 3 (e + (P + 3 rho^2)/cv) / (rho (3 - rho)), e = P + rho^2 (2 rho - 3),
+ * with c = 0 where c^2 <= 0 (spinodal part of an interface), over interior cells.  out[0]=dt, out[1]=max(|u|+c), out[2]=min rho, out[3]=max rho. */
+int dim2d_cfl_dt(dim2d_ctx* ctx, double cfl, double out[4]);
+
+/* Interior sums and extrema of the current state:
+ * out[0..3] = sum of rho, rho*u, rho*v, E; out[4] = min rho; out[5] = max rho;
+ * out[6] = number of non-finite values.  (No counterpart in the reference.) */
+int dim2d_diagnostics(dim2d_ctx* ctx, double out[7]);
+
+/* Simulated time and the dt of the last step after dim2d_integrate
+ * (blocking): out[0] = t, out[1] = dt. */
+int dim2d_time(dim2d_ctx* ctx, double out[2]);
+
+/* Block until the context's stream is idle. */
+int dim2d_synchronize(dim2d_ctx* ctx);
+
+/* Kernel tile selection: lanes per workgroup along x (64 or 128) and rows per
+ * workgroup along y; 0 keeps the current value. */
+int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
+
+/* ------------------------------------------------------------------ */
+/* Batches of independent simulations (uneven grid sizes).              */
+/* ------------------------------------------------------------------ */
+typedef struct dim2d_batch dim2d_batch;
+
+/* Create a batch of n simulations; every simulation has its own params
+ * (grid size, physics, boundaries; all must share the precision mode).  The
+ * stage kernels walk one work list over all simulations in one persistent
+ * launch per RK stage, and dim2d_batch_integrate replays those launches from a
+ * captured HIP graph. */
+int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out);
+int dim2d_batch_destroy(dim2d_batch* b);
+int dim2d_batch_upload(dim2d_batch* b, int32_t sim, const double* nodes);
+int dim2d_batch_download(dim2d_batch* b, int32_t sim, double* nodes);
+/* nsteps RK3 steps of every simulation, each with its own fixed dt (dts[n]). */
+int dim2d_batch_integrate(dim2d_batch* b, const double* dts, int32_t nsteps);
+/* Tile selection for the batch (same meaning as dim2d_set_tiles) and the
+ * number of persistent workgroups (0: one per work item, capped by residency). */
+int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows_per_block,
+                          int32_t workgroups);
+int dim2d_batch_synchronize(dim2d_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

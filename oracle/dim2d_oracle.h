@@ -1,0 +1,126 @@
+/*
+ * dim2d_oracle.h -- CPU restatement of the reference DIM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker for the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product library never links or calls it.
+ *
+ * It restates, function by function and in the same evaluation order, the
+ * reference's serial Fortran:
+ *   physical_models/dim2d/dim2d_prim_module.f       (primitives, EOS)
+ *   physical_models/dim2d/dim2d_fluxes_module.f     (26 face fluxes)
+ *   sd_operators/mattsson_operators/sd_operators_class.f, sd_operators_fd_module.f
+ *   physical_models/dim2d/pmodel_eq_class.f:1073-1214 (flux loops), :1840-1918 (body forces)
+ *   td_operators/finitevolume/td_operators_class.f:119-235 (compute_time_dev)
+ *   td_integrator/rungekutta3rdtvd/{td_integrator_class.f:103-161, rk3tvd_steps_module.f}
+ *   boundary_conditions/bc_operators_gen_class.f:95-102,238-381 (sections, order)
+ *   boundary_conditions/reflection_xy/, periodic_xy/bc_operators_class.f:156-210
+ *   boundary_conditions/wall_xy/{bc_operators_wall_xy_class.f, wall_xy_equilibrium_module.f,
+ *                                ridders_method_module.f}
+ *   physical_models/dim2d/dim2d_state_eq_module.f
+ *
+ * Arrays use the reference's Fortran layout nodes(nx,ny,ne): i fastest, then j,
+ * then the variable k (1: mass, 2: momentum_x, 3: momentum_y, 4: total energy).
+ * Indices in this file are 1-based like the reference.
+ */
+#ifndef DIM2D_ORACLE_H
+#define DIM2D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary kinds per edge */
+enum { ORC_BC_REFLECTION = 0, ORC_BC_PERIODIC = 1, ORC_BC_WALL = 2 };
+
+typedef struct orc_model {
+  int nx, ny;            /* total grid points including bc_size=2 ghosts */
+  double re, we, pr, viscous_r, cv_r;
+  double gravity;        /* gravity_amp; body force active when gravity_ac != 0 */
+  int gravity_ac;
+  int bc_w, bc_e, bc_s, bc_n;
+  /* wall parameters (wall_xy_equilibrium_module.f, parameters_input.f:524-611) */
+  double wall_contact_angle;   /* micro contact angle in degrees */
+  double wall_heat_flux;       /* constant heat flux (0: no_heat_source) */
+} orc_model;
+
+/* ---- primitives (point functions) ---- */
+double orc_mass_density(const orc_model*, const double*, int, int);
+double orc_velocity_x(const orc_model*, const double*, int, int);
+double orc_velocity_y(const orc_model*, const double*, int, int);
+double orc_classical_pressure(const orc_model*, const double*, int, int);
+double orc_classical_temperature_eff(const orc_model*, const double*, int, int);
+double orc_speed_of_sound_local(const orc_model*, const double nodes4[4]);
+
+/* ---- Mattsson stencil values of the mass density (pins) ----
+ * which: 0 f, 1 dfdx, 2 dfdy, 3 d2fdx2, 4 d2fdy2, 5 d2fdxdy, 6 gradient_x,
+ *        7 g, 8 dgdx, 9 dgdy, 10 d2gdx2, 11 d2gdy2, 12 d2gdxdy, 13 gradient_y,
+ *        14 dfdx_nl, 15 dgdy_nl (of gradient_x + gradient_y of the density)  */
+double orc_mattsson_density_stencil(const orc_model*, const double* nodes, int which,
+                                    int i, int j, double dx, double dy);
+
+/* ---- the 26 DIM flux functions, in the order of the reference's flux test ---- */
+double orc_flux_term(const orc_model*, const double* nodes, int which, int i, int j,
+                     double dx, double dy);
+
+/* ---- finite-volume time derivative (compute_time_dev) ----
+ * time_dev(nx,ny,4); ghosts are set to the reference's debug_real. */
+void orc_compute_time_dev(const orc_model*, double t, const double* nodes,
+                          const double* x_map, const double* y_map, double* time_dev);
+
+/* ---- ghost cells (apply_bc_on_nodes) ---- */
+void orc_apply_bc_on_nodes(const orc_model*, double t, const double* x_map,
+                           const double* y_map, const double* nodes_tmp, double* nodes);
+
+/* ---- one RK3-TVD step (td_integrator%integrate); work: 2*nx*ny*4 doubles ---- */
+void orc_rk3_step(const orc_model*, double t, double dt, const double* x_map,
+                  const double* y_map, double* nodes, double* work);
+
+/* nsteps RK3 steps with a given dt sequence (dts[nsteps]); t advances by dt.
+ * Returns 0, or the 1-based step at which a non-finite interior value appeared. */
+int orc_run(const orc_model*, double t0, int nsteps, const double* dts,
+            const double* x_map, const double* y_map, double* nodes);
+
+/* coordinates (field_abstract_class.f:286-315) */
+void orc_coordinates(int nx, int ny, double x_min, double x_max, double y_min,
+                     double y_max, double* x_map, double* y_map);
+
+/* time step rule used by the build (not in the reference): dt = cfl*min(
+ * dx/max(|u|+c), Re*min(rho)*dx^2, sqrt(We/max(rho))*dx^2 ), dx = min(dx,dy),
+ * over interior cells; c = sqrt(max(c^2, 0)) with c^2 the reduced form of
+ * speed_of_sound (0 inside the spinodal region).  out[0]=dt, out[1]=max(|u|+c), out[2]=min rho, out[3]=max rho */
+void orc_cfl_dt(const orc_model*, const double* nodes, double dx, double dy,
+                double cfl, double out[4]);
+
+/* conserved sums over interior cells: out[0]=sum rho, [1]=sum qx, [2]=sum qy, [3]=sum E */
+void orc_sums(const orc_model*, const double* nodes, double out[4]);
+
+/* wall ghost density root solve, exposed for pins:
+ * returns the Ridders root of the wall equilibrium function
+ * (wall_xy_equilibrium_module.f:1747-1821) for one ghost cell. */
+double orc_wall_md_ghost(const orc_model*, double T_guess, double md_guess,
+                         const double md_x[2], const double md_y[2], double velocity_x,
+                         double velocity_y, double Ed, double delta_x, double delta_y);
+double orc_wall_root_f(const orc_model*, double md, const double md_x[2],
+                       const double md_y[2], double velocity_x, double velocity_y,
+                       double Ed, double delta_x, double delta_y);
+double orc_mass_density_liquid(double T);
+double orc_mass_density_vapor(double T);
+double orc_surface_tension(const orc_model*, double T);
+double orc_dwall_internal_energy_dmd(const orc_model*, double md, double temperature,
+                                     double micro_angle);
+double orc_wall_temperature(const orc_model*, double md, double md_grad_squared, double ux,
+                            double uy, double Ed);
+/* generic bracket search and Ridders root for pins with the reference's test functions */
+int orc_root_brackets(double (*f)(void*, double), void* ctx, double T_guess, double md_guess,
+                      double out[2]);
+double orc_ridders(double (*f)(void*, double), void* ctx, double x1, double x2, double xacc,
+                   int* status);
+/* wall face flux (compute_wall_flux_x / _y) at face (i,j); dir_y selects y */
+void orc_wall_flux(const orc_model*, const double* nodes, int dir_y, int i, int j, double dx,
+                   double dy, int side_right, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
