@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the DIM RK3-TVD hot path on one MI355X.
+
+Workload (the reference's headline configuration that fits one GPU): two vapour
+bubbles of radius 0.08 in liquid at T0 = 0.95 with mean flow u = 0.1 on a
+4096 x 4096 interior (plus 2 ghost layers), periodic in x, walls in y, FP64,
+RK3-TVD with dt from the device-side CFL reduction (CFL 0.1).  Test physics
+Re=5, We=10, Pr=20, viscous_r=-1.5, cv_r=2.5.
+
+One step = one full RK3-TVD step (three fused stage kernels + wall ghost
+kernels) over the whole field, inputs resident in HBM.  value = interior cells
+x steps / wall time (grid-point updates/s).  The same run in the mixed
+FP32/FP64 mode is reported beside it with its error against the FP64 run, and
+the CPU oracle is timed on a bounded sample on the host.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt updates/s + %HBM peak; FP64 vs mixed FP32 at ≤1e-6 rel err"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# compulsory HBM bytes per interior cell and launch (FP64): stage 1 reads u^n and
+# writes u1 (64 B); stages 2 and 3 read the stage state and u^n and write (96 B).
+# Mixed mode: u1/u2 increments in FP32 -> 48, 64, 80 B.
+ALG_BYTES = {"f64": (64, 96, 96), "mixed": (48, 64, 80)}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_errors(a, b):
+    """max and L2 relative error of density and total energy (interior)."""
+    import numpy as np
+    out = {}
+    for k, name in ((0, "rho"), (3, "E")):
+        x, y = a[k, 2:-2, 2:-2], b[k, 2:-2, 2:-2]
+        out[f"{name}_max_rel"] = float(np.abs(x - y).max() / np.abs(y).max())
+        out[f"{name}_l2_rel"] = float(np.sqrt(((x - y) ** 2).sum() / (y ** 2).sum()))
+    return out
+
+
+def timed_run(field, steps, warmup, cfl, dt=0.0):
+    """warmup untimed steps, then `steps` timed steps with HIP-event kernel timing."""
+    import torch
+    field.run(warmup, dt=dt, cfl=cfl)
+    field.profile(True)
+    torch.cuda.synchronize()
+    field.synchronize()
+    t0 = time.perf_counter()
+    field.run(steps, dt=dt, cfl=cfl, sync=False)
+    field.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kt = field.kernel_times()
+    field.profile(False)
+    return el, kt
+
+
+def cpu_baseline(seconds_hint=15.0):
+    """The CPU oracle (one core) on a bounded sample of the workload: the same
+    two-bubble physics and boundaries at 1024^2, a few RK3 steps."""
+    import numpy as np
+
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    from oracle import oracle as orc
+    n = 1024
+    p, nodes = ic.two_bubbles(n=n)
+    m = orc.make_model(p.nx, p.ny, bc=p.bc)
+    xm, ym = p.coordinates()
+    orc.apply_bc(m, nodes, xm, ym)
+    dt = ic.cfl_dt(p, nodes, 0.1)
+    # one step to size the sample, then enough steps for ~seconds_hint
+    t0 = time.perf_counter()
+    orc.run(m, nodes, xm, ym, np.array([dt]))
+    one = time.perf_counter() - t0
+    steps = max(1, min(8, int(seconds_hint / max(one, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    bad = orc.run(m, nodes, xm, ym, np.full(steps, dt))
+    el = time.perf_counter() - t0
+    total_steps = steps + 1
+    return {"value": n * n * total_steps / (el + one), "unit": "grid-pt updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"CPU oracle (C restatement of the reference's serial Fortran, gcc -O2, "
+                      f"one thread) on the same two-bubble case at {n}x{n}, periodic x / wall y, "
+                      f"FP64, {total_steps} RK3 steps, {el + one:.1f} s"
+                      + (" (diverged)" if bad else "")}
+
+
+def read_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096, help="interior points per axis")
+    ap.add_argument("--block-x", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain device copy of 2 x 1 GiB (for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+              f"--gpus {args.gpus}); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+
+    torch.cuda.init()
+    dev = torch.cuda.get_device_name(0)
+    if args.calibrate_copy:
+        # a copy kernel of known size: reads 1 GiB and writes 1 GiB per launch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    n = args.n
+    cells = n * n
+    log(f"[bench] device {dev}; building the {n}^2 two-bubble state")
+    p, nodes = ic.two_bubbles(n=n)
+
+    # ---- FP64, CFL time step (the headline) ----
+    f = d.Field(p, nodes)
+    if args.block_x or args.rows:
+        f.set_tiles(args.block_x, args.rows)
+    f.apply_bc_on_nodes()
+    diag0 = f.diagnostics()
+    log(f"[bench] FP64: {args.warmup} warmup + {args.steps} timed steps")
+    el, kt = timed_run(f, args.steps, args.warmup, cfl=0.1)
+    diag1 = f.diagnostics()
+    t_end, dt_last = f.device_time()
+    value = cells * args.steps / el
+    ms_step = 1e3 * el / args.steps
+    log(f"[bench] FP64 {value:.4e} updates/s, {ms_step:.4f} ms/step, kernels {kt}")
+    # dominant stage kernel
+    totals = {k: kt[k][0] * kt[k][1] for k in ("stage1", "stage2", "stage3")}
+    dom = max(totals, key=totals.get)
+    si = ("stage1", "stage2", "stage3").index(dom)
+    alg = ALG_BYTES["f64"][si] * cells
+    achieved = alg / (kt[dom][0] * 1e-3)
+    roof = {"bound": "hbm", "kernel": f"stage_kernel<64,FP64,{si + 1}>", "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": read_traffic(f"stage{si + 1}_fp64_two_bubbles_4096_traffic.json"),
+            "algorithmic_bytes_per_launch": alg, "mean_launch_ms": kt[dom][0]}
+    f.close()
+    del f
+
+    # ---- mixed FP32/FP64 ----
+    mixed = None
+    if not args.no_mixed:
+        pm, _ = ic.two_bubbles(n=n, precision=d.PREC_MIXED)
+        fm = d.Field(pm, nodes)
+        if args.block_x or args.rows:
+            fm.set_tiles(args.block_x, args.rows)
+        fm.apply_bc_on_nodes()
+        log(f"[bench] mixed: {args.warmup} warmup + {args.steps} timed steps")
+        elm, ktm = timed_run(fm, args.steps, args.warmup, cfl=0.1)
+        vm = cells * args.steps / elm
+        totm = {k: ktm[k][0] * ktm[k][1] for k in ("stage1", "stage2", "stage3")}
+        fm.close()
+        del fm
+        # error against FP64 with a common fixed dt, at step 100 and at `steps`
+        dt0 = ic.cfl_dt(p, nodes, 0.1)
+        errs = {}
+        for nst in sorted({min(100, args.steps), args.steps}):
+            a = d.Field(p, nodes)
+            a.apply_bc_on_nodes()
+            a.run(nst, dt=dt0)
+            ra = a.download()
+            da = a.diagnostics()
+            a.close()
+            b = d.Field(pm, nodes)
+            b.apply_bc_on_nodes()
+            b.run(nst, dt=dt0)
+            rb = b.download()
+            db = b.diagnostics()
+            b.close()
+            e = rel_errors(rb, ra)
+            e["mass_drift_mixed"] = float((db[0] - da[0]) / da[0])
+            errs[f"step_{nst}"] = e
+            log(f"[bench] mixed vs FP64 after {nst} steps: {e}")
+        mixed = {"value": vm, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * elm / args.steps,
+                 "speedup_vs_fp64": vm / value,
+                 "hbm_frac_algorithmic": vm * sum(ALG_BYTES["mixed"]) / HBM_PEAK,
+                 "kernel_ms": {k: ktm[k][0] for k in ktm},
+                 "dominant_kernel_frac": None,
+                 "error_vs_fp64": errs}
+        domm = max(totm, key=totm.get)
+        sim = ("stage1", "stage2", "stage3").index(domm)
+        mixed["dominant_kernel_frac"] = (ALG_BYTES["mixed"][sim] * cells /
+                                         (ktm[domm][0] * 1e-3)) / HBM_PEAK
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": METRIC, "value": value, "unit": "grid-pt updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"two vapour bubbles r=0.08 in liquid, T0=0.95, u=0.1, "
+                               f"{n}x{n} interior + 2 ghost layers, periodic x / wall y, "
+                               f"RK3-TVD, CFL 0.1 device-side dt, FP64",
+                   "grid": [n + 4, n + 4], "physics": "Re=5 We=10 Pr=20 viscous_r=-1.5 cv_r=2.5",
+                   "device": dev},
+        "hbm_frac_algorithmic": value * sum(ALG_BYTES["f64"]) / HBM_PEAK,
+        "kernel_ms": {k: kt[k][0] for k in kt},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "mixed": mixed,
+        "checks": {"mass_drift": float((diag1[0] - diag0[0]) / diag0[0]),
+                   "nonfinite": int(diag1[6]), "t_end": t_end, "dt_last": dt_last},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

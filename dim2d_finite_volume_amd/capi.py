@@ -1,7 +1,8 @@
 """ctypes binding of the C ABI declared in include/dim2d_hip.h.
 
 The library is the in-tree build `dim2d_finite_volume_amd/lib/libdim2d_hip.so`
-(built by `make -C dim2d_finite_volume_amd` or `__graft_entry__.build()`).
+(built by `make -C dim2d_finite_volume_amd` or `__graft_entry__.build()`); the
+environment variable DIM2D_HIP_LIBRARY selects another build of the same ABI.
 There is no fallback: if the library is missing, `load()` raises.
 """
 from __future__ import annotations
@@ -10,7 +11,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libdim2d_hip.so")
+LIB_PATH = os.environ.get("DIM2D_HIP_LIBRARY") or os.path.join(HERE, "lib", "libdim2d_hip.so")
 
 OK = 0
 EINVAL, EDEVICE, ENOMEM, ESTATE, ENONFINITE = -1, -2, -3, -4, -5
@@ -23,6 +24,7 @@ EXPORTS = (
     "dim2d_upload", "dim2d_download", "dim2d_upload_device", "dim2d_download_device",
     "dim2d_time_dev", "dim2d_apply_bc", "dim2d_rk3_step", "dim2d_integrate", "dim2d_cfl_dt",
     "dim2d_diagnostics", "dim2d_time", "dim2d_synchronize", "dim2d_set_tiles",
+    "dim2d_profile_enable", "dim2d_profile_read",
     "dim2d_batch_create", "dim2d_batch_destroy", "dim2d_batch_upload", "dim2d_batch_download",
     "dim2d_batch_integrate", "dim2d_batch_set_tiles", "dim2d_batch_synchronize",
 )
@@ -83,6 +85,8 @@ def load() -> ctypes.CDLL:
         "dim2d_time": (ctypes.c_int, [vp, dp]),
         "dim2d_synchronize": (ctypes.c_int, [vp]),
         "dim2d_set_tiles": (ctypes.c_int, [vp, i32, i32]),
+        "dim2d_profile_enable": (ctypes.c_int, [vp, i32]),
+        "dim2d_profile_read": (ctypes.c_int, [vp, dp]),
         "dim2d_batch_create": (ctypes.c_int, [ctypes.POINTER(Params), i32, ctypes.POINTER(vp)]),
         "dim2d_batch_destroy": (ctypes.c_int, [vp]),
         "dim2d_batch_upload": (ctypes.c_int, [vp, i32, dp]),

@@ -110,6 +110,12 @@ DevParams make_dev_params(const dim2d_params& q) {
   p.c3cv = 3.0 / q.cv_r;
   p.c38cv = 3.0 / (8.0 * q.cv_r);
   p.two_vr = 2.0 + q.viscous_r;
+  p.two_inv_re = 2.0 / q.re;
+  p.two_inv_we = 2.0 / q.we;
+  p.k_heat_x = 2.0 / q.pr * p.idx;
+  p.k_heat_y = 2.0 / q.pr * p.idy;
+  p.k_cap_x = 2.0 * p.inv_repr * p.idx;
+  p.k_cap_y = 2.0 * p.inv_repr * p.idy;
   p.gravity = q.gravity;
   p.wall_cos_angle = cos(q.wall_contact_angle * acos(-1.0) / 180.0);
   p.wall_q = q.wall_heat_flux;
@@ -141,7 +147,51 @@ struct dim2d_ctx {
   int rows = 64;
   bool uploaded = false;
   long long steps = 0;
+  // kernel timing: event pairs around launches, tagged by kernel kind
+  bool profiling = false;
+  struct Timed {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> pool;
 };
+
+namespace {
+
+hipEvent_t take_event(dim2d_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// record an event pair around `launch` when profiling (kind 0..3)
+template <typename F>
+hipError_t timed_launch(dim2d_ctx* c, int kind, F&& launch) {
+  if (!c->profiling) return launch();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, c->stream);
+  if (e == hipSuccess) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+  if (e == hipSuccess) c->timed.push_back({a, b, kind});
+  return e;
+}
+
+void release_timed(dim2d_ctx* c) {
+  for (auto& t : c->timed) {
+    c->pool.push_back(t.a);
+    c->pool.push_back(t.b);
+  }
+  c->timed.clear();
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -204,6 +254,8 @@ int dim2d_ctx_destroy(dim2d_ctx* c) {
   (void)hipFree(c->D2);
   (void)hipFree(c->T);
   (void)hipFree(c->ctl);
+  release_timed(c);
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DIM2D_OK;
@@ -216,6 +268,32 @@ int dim2d_set_tiles(dim2d_ctx* c, int32_t block_x, int32_t rows) {
   if (rows < 0 || rows > 4096) return fail(DIM2D_EINVAL, "rows_per_block out of range: %d", rows);
   if (block_x) c->block_x = block_x;
   if (rows) c->rows = rows;
+  return DIM2D_OK;
+}
+
+int dim2d_profile_enable(dim2d_ctx* c, int32_t enable) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  release_timed(c);
+  c->profiling = enable != 0;
+  return DIM2D_OK;
+}
+
+int dim2d_profile_read(dim2d_ctx* c, double out[8]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double sum[4] = {0, 0, 0, 0};
+  double cnt[4] = {0, 0, 0, 0};
+  for (auto& t : c->timed) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+    sum[t.kind] += ms;
+    cnt[t.kind] += 1;
+  }
+  for (int k = 0; k < 4; ++k) {
+    out[2 * k] = cnt[k] > 0 ? sum[k] / cnt[k] : 0.0;
+    out[2 * k + 1] = cnt[k];
+  }
   return DIM2D_OK;
 }
 
@@ -323,30 +401,30 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     a.U = c->U;
     a.S64 = c->U;
     a.O64 = c->A;
-    HIP_TRY(launch_stage(p, 0, 1, a, c->block_x, c->rows, c->stream));
-    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->A, nullptr, c->A, nullptr, c->U, c->stream));
+    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage(p, 0, 1, a, c->block_x, c->rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->A, nullptr, c->A, nullptr, c->U, c->stream); }));
     a.S64 = c->A;
     a.O64 = c->B;
-    HIP_TRY(launch_stage(p, 0, 2, a, c->block_x, c->rows, c->stream));
-    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->B, nullptr, c->B, nullptr, c->U, c->stream));
+    HIP_TRY(timed_launch(c, 1, [&] { return launch_stage(p, 0, 2, a, c->block_x, c->rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->B, nullptr, c->B, nullptr, c->U, c->stream); }));
     a.S64 = c->B;
     a.O64 = c->U2;
-    HIP_TRY(launch_stage(p, 0, 3, a, c->block_x, c->rows, c->stream));
-    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream));
+    HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 0, 3, a, c->block_x, c->rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
   } else {
     a.U = c->U;
     a.O32 = c->D1;
-    HIP_TRY(launch_stage(p, 1, 1, a, c->block_x, c->rows, c->stream));
-    if (wall) HIP_TRY(launch_wall(p, 1, c->U, nullptr, c->D1, nullptr, c->D1, c->U, c->stream));
+    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage(p, 1, 1, a, c->block_x, c->rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D1, nullptr, c->D1, c->U, c->stream); }));
     a.S32 = c->D1;
     a.O32 = c->D2;
-    HIP_TRY(launch_stage(p, 1, 2, a, c->block_x, c->rows, c->stream));
-    if (wall) HIP_TRY(launch_wall(p, 1, c->U, nullptr, c->D2, nullptr, c->D2, c->U, c->stream));
+    HIP_TRY(timed_launch(c, 1, [&] { return launch_stage(p, 1, 2, a, c->block_x, c->rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D2, nullptr, c->D2, c->U, c->stream); }));
     a.S32 = c->D2;
     a.O32 = nullptr;
     a.O64 = c->U2;
-    HIP_TRY(launch_stage(p, 1, 3, a, c->block_x, c->rows, c->stream));
-    if (wall) HIP_TRY(launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream));
+    HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 1, 3, a, c->block_x, c->rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
   }
   std::swap(c->U, c->U2);
   c->steps++;

@@ -37,6 +37,10 @@ struct DevParams {
   double inv_re, inv_we, inv_pr, inv_repr;
   double c15cv, c3cv, c38cv;  // 1.5/cv, 3/cv, 3/(8cv)
   double two_vr;           // 2 + viscous_r
+  double two_inv_re, two_inv_we;   // 2/Re, 2/We
+  double k_heat_x, k_heat_y;       // 2/(Pr dx), 2/(Pr dy)
+  double k_cap_x, k_cap_y;         // 2/(Re Pr dx), 2/(Re Pr dy)
+
   double gravity;
   double wall_cos_angle;   // cos(contact angle)
   double wall_q;           // constant wall heat flux

@@ -61,21 +61,24 @@ __device__ __forceinline__ int map_row(const DevParams& p, int r, bool& flip) {
 }
 
 // state value of cell (cm, rm) (already mapped) with the reflection signs
-// (momentum normal to a reflecting edge changes sign, reflection_xy_module.f:50-110)
+// (momentum normal to a reflecting edge changes sign, reflection_xy_module.f:50-110).
+// The row offset is uniform across the wave, the column a 32-bit lane offset.
 // SRC 0: FP64 field S64; 1: U alone; 2: U + S32
 template <int SRC>
 __device__ __forceinline__ void load_state(const DevParams& p, const double* __restrict__ U,
                                            const double* __restrict__ S64,
-                                           const float* __restrict__ S32, long long idx,
+                                           const float* __restrict__ S32, int rm, unsigned cm,
                                            bool fx, bool fy, double v[4]) {
+  const long long rofs = (long long)rm * p.pitch;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
+    const long long base = rofs + k * p.plane;
     if (SRC == 0)
-      v[k] = S64[k * p.plane + idx];
+      v[k] = (S64 + base)[cm];
     else if (SRC == 1)
-      v[k] = U[k * p.plane + idx];
+      v[k] = (U + base)[cm];
     else
-      v[k] = U[k * p.plane + idx] + (double)S32[k * p.plane + idx];
+      v[k] = (U + base)[cm] + (double)(S32 + base)[cm];
   }
   if (fx) v[1] = -v[1];
   if (fy) v[2] = -v[2];
@@ -84,68 +87,94 @@ __device__ __forceinline__ void load_state(const DevParams& p, const double* __r
 // ------------------------------------------------------------------------
 // per-cell quantities (dim2d_prim_module.f)
 // ------------------------------------------------------------------------
+// 1/x by the hardware reciprocal and two Newton steps (error ~1 ulp)
+__device__ __forceinline__ double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Own-column values a lane keeps in registers (what the y-faces need)
 template <typename RD, typename R>
 struct Cell {
   RD rho;
   double irho;
-  R qx, qy, u, v, w, Tcl, Tcap;
-  R Pxx, Pyy, qxv, Hu, Hv, uw, vw;  // qx*u+P, qy*v+P, qx*v, (E+P)u, (E+P)v, u w, v w
+  R u, v, w, Tcl, Tcap;
+  R qy, qxv, Pyy, Hv, vw;  // qy, qx*v, qy*v+P, (E+P)v, v w
+};
+// x-face values of a cell, exchanged through LDS
+template <typename R>
+struct XVals {
+  R qx, Pxx, Hu, uw;  // qx, qx*u+P, (E+P)u, u w
 };
 
 // velocities, classical pressure (:425-453), classical temperature (:515-544),
 // capillarity pressure 1/(3-rho) (:1030-1062) and the transport products
 // (:830-1150), computed in FP64 from the FP64 state.
 template <typename RD, typename R>
-__device__ __forceinline__ void make_cell(const DevParams& p, const double v[4], Cell<RD, R>& c) {
+__device__ __forceinline__ void make_cell(const DevParams& p, const double v[4], Cell<RD, R>& c,
+                                          XVals<R>& x) {
   const double rho = v[0];
-  const double ir = 1.0 / rho;
+  const double ir = fast_rcp(rho);
   const double u = v[1] * ir;
   const double vv = v[2] * ir;
   const double rho2 = rho * rho;
   const double Tn = v[3] - 0.5 * rho * (u * u + vv * vv) + 3.0 * rho2;
-  const double w = 1.0 / (3.0 - rho);
+  const double w = fast_rcp(3.0 - rho);
   const double P = p.c3cv * w * Tn - 3.0 * rho2;
   const double H = v[3] + P;
   c.rho = (RD)rho;
   c.irho = ir;
-  c.qx = (R)v[1];
-  c.qy = (R)v[2];
   c.u = (R)u;
   c.v = (R)vv;
   c.w = (R)w;
   c.Tcl = (R)(Tn * ir);
   c.Tcap = (R)0;
-  c.Pxx = (R)(v[1] * u + P);
-  c.Pyy = (R)(v[2] * vv + P);
+  c.qy = (R)v[2];
   c.qxv = (R)(v[1] * vv);
-  c.Hu = (R)(H * u);
+  c.Pyy = (R)(v[2] * vv + P);
   c.Hv = (R)(H * vv);
-  c.uw = (R)(u * w);
   c.vw = (R)(vv * w);
+  x.qx = (R)v[1];
+  x.Pxx = (R)(v[1] * u + P);
+  x.Hu = (R)(H * u);
+  x.uw = (R)(u * w);
 }
 
-// LDS ring of the four rows r-1..r+2 (slot = row & 3) and the x-face fluxes
+template <typename R>
+struct Pair {
+  R a, b;
+};
+
+// LDS: density and velocities of rows r-1, r, r+1 (slot = row % 3), the
+// x-face values of row r in pairs read by one instruction each, and the
+// x-face fluxes.
 template <typename RD, typename R, int BW>
 struct Ring {
-  RD rho[4][BW];
-  R qx[4][BW], u[4][BW], v[4][BW], w[4][BW], Tcl[4][BW], Tcap[4][BW];
-  R Pxx[4][BW], qxv[4][BW], Hu[4][BW], uw[4][BW];
-  R fx[2][4][BW];
+  RD rho[3][BW];
+  Pair<R> uv[3][BW];
+  Pair<R> wT[BW];  // w, Tcl
+  Pair<R> qP[BW];  // qx, qx*u+P
+  Pair<R> Hw[BW];  // (E+P)u, u w
+  Pair<R> Cq[BW];  // Tcap, qx*v
+  Pair<R> f01[BW], f23[BW];
 };
 
 template <typename RD, typename R, int BW>
-__device__ __forceinline__ void put_slot(Ring<RD, R, BW>& sm, int s, int lane,
-                                         const Cell<RD, R>& c) {
-  sm.rho[s][lane] = c.rho;
-  sm.qx[s][lane] = c.qx;
-  sm.u[s][lane] = c.u;
-  sm.v[s][lane] = c.v;
-  sm.w[s][lane] = c.w;
-  sm.Tcl[s][lane] = c.Tcl;
-  sm.Pxx[s][lane] = c.Pxx;
-  sm.qxv[s][lane] = c.qxv;
-  sm.Hu[s][lane] = c.Hu;
-  sm.uw[s][lane] = c.uw;
+__device__ __forceinline__ void put_rho_uv(Ring<RD, R, BW>& sm, int s3, int lane,
+                                           const Cell<RD, R>& c) {
+  sm.rho[s3][lane] = c.rho;
+  sm.uv[s3][lane] = Pair<R>{c.u, c.v};
+}
+template <typename RD, typename R, int BW>
+__device__ __forceinline__ void put_xvals(Ring<RD, R, BW>& sm, int lane, const Cell<RD, R>& c,
+                                          const XVals<R>& x) {
+  sm.wT[lane] = Pair<R>{c.w, c.Tcl};
+  sm.qP[lane] = Pair<R>{x.qx, x.Pxx};
+  sm.Hw[lane] = Pair<R>{x.Hu, x.uw};
+  sm.Cq[lane] = Pair<R>{c.Tcap, c.qxv};
 }
 
 // capillarity_temperature_eff (:582-613) with gradient_x/y_interior
@@ -163,12 +192,14 @@ __device__ __forceinline__ double wall_pressure(double T, double md) {
 }
 
 // x-face flux at (i-1/2, j): L = (i-1, j), R = (i, j)  (dim2d_fluxes_module.f,
-// Mattsson f/dfdx/dfdy/d2fdx2/d2fdy2/dfdx_nl stencils).
+// Mattsson f/dfdx/dfdy/d2fdx2/d2fdy2/dfdx_nl stencils), returned doubled
+// (F2 = 2 F) so the face averages are plain sums; the divergence applies the
+// 1/2.  s0/sm1/sp1: row slots of j, j-1, j+1; the x-values are those of row j.
 template <typename RD, typename R, int BW>
 __device__ __forceinline__ void x_face(const DevParams& p, const Ring<RD, R, BW>& sm, int s0,
                                        int sm1, int sp1, int lane, const Cell<RD, R>& Rc,
                                        RD rhoR_dn, R uR_dn, R vR_dn, RD rhoR_up, R uR_up,
-                                       R vR_up, bool wall_w, bool wall_e, R F[4]) {
+                                       R vR_up, bool wall_w, bool wall_e, R F2[4]) {
   const int lm1 = lane > 0 ? lane - 1 : 0;
   const int lm2 = lane > 1 ? lane - 2 : 0;
   const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
@@ -176,84 +207,83 @@ __device__ __forceinline__ void x_face(const DevParams& p, const Ring<RD, R, BW>
   const RD rhoLL = sm.rho[s0][lm2];
   const RD rhoRR = sm.rho[s0][lp1];
   const RD rhoL_dn = sm.rho[sm1][lm1], rhoL_up = sm.rho[sp1][lm1];
-  const R uL = sm.u[s0][lm1], vL = sm.v[s0][lm1];
-  const R uL_dn = sm.u[sm1][lm1], vL_dn = sm.v[sm1][lm1];
-  const R uL_up = sm.u[sp1][lm1], vL_up = sm.v[sp1][lm1];
-  const R qxL = sm.qx[s0][lm1], wL = sm.w[s0][lm1], TclL = sm.Tcl[s0][lm1];
-  const R TcapL = sm.Tcap[s0][lm1], PxxL = sm.Pxx[s0][lm1], qxvL = sm.qxv[s0][lm1];
-  const R HuL = sm.Hu[s0][lm1], uwL = sm.uw[s0][lm1];
+  const Pair<R> uvL = sm.uv[s0][lm1], uvL_dn = sm.uv[sm1][lm1], uvL_up = sm.uv[sp1][lm1];
+  const Pair<R> wTL = sm.wT[lm1];
+  const Pair<R> qPL = sm.qP[lm1], qPR = sm.qP[lane];
+  const Pair<R> HwL = sm.Hw[lm1], HwR = sm.Hw[lane];
+  const Pair<R> CqL = sm.Cq[lm1];
 
-  const R half = (R)0.5;
-  const R fr = (R)(half * (rhoL + Rc.rho));
+  const RD srho = rhoL + Rc.rho;
   const R rx = (R)((Rc.rho - rhoL) * (RD)p.idx);
   const R ry = (R)(((rhoL_up + rhoR_up) - (rhoL_dn + rhoR_dn)) * (RD)p.q_idy);
-  const R rxx = (R)(((rhoLL + rhoRR) - (rhoL + Rc.rho)) * (RD)p.h2x);
-  const R ryy = (R)(((rhoL_dn + rhoR_dn) - (RD)2 * (rhoL + Rc.rho) + (rhoL_up + rhoR_up)) *
-                    (RD)p.h2y);
-  const R ux = half * (uL + Rc.u);
-  const R uy = half * (vL + Rc.v);
-  const R dudx = (Rc.u - uL) * (R)p.idx;
-  const R dvdx = (Rc.v - vL) * (R)p.idx;
-  const R dudy = ((uL_up + uR_up) - (uL_dn + uR_dn)) * (R)p.q_idy;
-  const R dvdy = ((vL_up + vR_up) - (vL_dn + vR_dn)) * (R)p.q_idy;
+  const R rxx = (R)(((rhoLL + rhoRR) - srho) * (RD)p.h2x);
+  const R ryy = (R)(((rhoL_dn + rhoR_dn) + (rhoL_up + rhoR_up) - (RD)2 * srho) * (RD)p.h2y);
+  const R su = uvL.a + Rc.u;
+  const R sv = uvL.b + Rc.v;
+  const R dudx = (Rc.u - uvL.a) * (R)p.idx;
+  const R dvdx = (Rc.v - uvL.b) * (R)p.idx;
+  const R dudy = ((uvL_up.a + uR_up) - (uvL_dn.a + uR_dn)) * (R)p.q_idy;
+  const R dvdy = ((uvL_up.b + vR_up) - (uvL_dn.b + vR_dn)) * (R)p.q_idy;
+  const R rxy = rx * ry;
   const R grad2 = rx * rx + ry * ry;
   const R lap = rxx + ryy;
   const R rx2d = ry * ry - rx * rx;
+  const R srhoR = (R)srho;
 
   if (wall_w || wall_e) {
     // compute_wall_flux_x (wall_xy_equilibrium_module.f:249-341)
-    const double TeffL = (double)TclL - p.inv_we * (double)TcapL;
+    const double TeffL = (double)wTL.b - p.inv_we * (double)CqL.a;
     const double TeffR = (double)Rc.Tcl - p.inv_we * (double)Rc.Tcap;
     const double Tl = p.c38cv * (wall_w ? TeffR : TeffL);
     const double Th = Tl + p.dx * p.c38cv * p.re * p.pr * p.wall_q;
-    const double Pw = wall_pressure(0.5 * (Tl + Th), (double)fr);
-    F[0] = (R)0;
-    F[1] = (R)(Pw - p.inv_we * (double)(fr * lap + half * rx2d));
-    F[2] = (R)(-p.inv_re * (double)dvdx - p.inv_we * (double)(-rx * ry));
-    F[3] = (R)(wall_w ? p.wall_q : -p.wall_q);
+    const double Pw = wall_pressure(0.5 * (Tl + Th), 0.5 * (double)srho);
+    F2[0] = (R)0;
+    F2[1] = (R)(2.0 * Pw - p.inv_we * (double)(srhoR * lap + rx2d));
+    F2[2] = (R)(2.0 * (-p.inv_re * (double)dvdx + p.inv_we * (double)rxy));
+    F2[3] = (R)(2.0 * (wall_w ? p.wall_q : -p.wall_q));
     return;
   }
   const R visc = (R)p.two_vr * dudx + (R)p.vr * dvdy;
   const R shear = dudy + dvdx;
-  F[0] = half * (qxL + Rc.qx);
-  F[1] = half * (PxxL + Rc.Pxx) - (R)p.inv_re * visc -
-         (R)p.inv_we * ((R)p.c15cv * half * (wL + Rc.w) * grad2 + fr * lap + half * rx2d);
-  F[2] = half * (qxvL + Rc.qxv) - (R)p.inv_re * shear + (R)p.inv_we * (rx * ry);
-  const R visc_e = ux * visc + uy * shear + (R)p.inv_pr * (Rc.Tcl - TclL) * (R)p.idx;
-  const R cap_e = (R)p.c15cv * half * (uwL + Rc.uw) * grad2 + F[0] * lap + half * ux * rx2d -
-                  rx * ry * uy - fr * rx * (dudx + dvdy) -
-                  (R)p.inv_repr * (Rc.Tcap - TcapL) * (R)p.idx;
-  F[3] = half * (HuL + Rc.Hu) - (R)p.inv_re * visc_e - (R)p.inv_we * cap_e;
+  F2[0] = qPL.a + qPR.a;
+  F2[1] = (qPL.b + qPR.b) - (R)p.two_inv_re * visc -
+          (R)p.inv_we * ((R)p.c15cv * (wTL.a + Rc.w) * grad2 + srhoR * lap + rx2d);
+  F2[2] = (CqL.b + Rc.qxv) - (R)p.two_inv_re * shear + (R)p.two_inv_we * rxy;
+  const R visc_e2 = su * visc + sv * shear + (R)p.k_heat_x * (Rc.Tcl - wTL.b);
+  const R cap_e2 = (R)p.c15cv * (HwL.b + HwR.b) * grad2 + F2[0] * lap + (R)0.5 * su * rx2d -
+                   rxy * sv - srhoR * rx * (dudx + dvdy) - (R)p.k_cap_x * (Rc.Tcap - CqL.a);
+  F2[3] = (HwL.a + HwR.a) - (R)p.inv_re * visc_e2 - (R)p.inv_we * cap_e2;
 }
 
-// y-face flux at (i, j-1/2): D = (i, j-1), U = (i, j)  (g/dgdx/dgdy/d2gdx2/d2gdy2/dgdy_nl)
+// y-face flux at (i, j-1/2): D = (i, j-1), U = (i, j)  (g/dgdx/dgdy/d2gdx2/d2gdy2/dgdy_nl),
+// returned doubled like x_face.
 template <typename RD, typename R, int BW>
 __device__ __forceinline__ void y_face(const DevParams& p, const Ring<RD, R, BW>& sm, int sD,
                                        int sU, int lane, const Cell<RD, R>& D,
                                        const Cell<RD, R>& Uc, RD rho_dd, RD rho_uu,
-                                       bool wall_s, bool wall_n, R G[4]) {
+                                       bool wall_s, bool wall_n, R G2[4]) {
   const int lm1 = lane > 0 ? lane - 1 : 0;
   const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
   const RD rWD = sm.rho[sD][lm1], rED = sm.rho[sD][lp1];
   const RD rWU = sm.rho[sU][lm1], rEU = sm.rho[sU][lp1];
-  const R uWD = sm.u[sD][lm1], uED = sm.u[sD][lp1], uWU = sm.u[sU][lm1], uEU = sm.u[sU][lp1];
-  const R vWD = sm.v[sD][lm1], vED = sm.v[sD][lp1], vWU = sm.v[sU][lm1], vEU = sm.v[sU][lp1];
+  const Pair<R> WD = sm.uv[sD][lm1], ED = sm.uv[sD][lp1], WU = sm.uv[sU][lm1], EU = sm.uv[sU][lp1];
 
-  const R half = (R)0.5;
-  const R gr = (R)(half * (D.rho + Uc.rho));
+  const RD srho = D.rho + Uc.rho;
   const R ry = (R)((Uc.rho - D.rho) * (RD)p.idy);
   const R rx = (R)(((rED + rEU) - (rWD + rWU)) * (RD)p.q_idx);
-  const R ryy = (R)(((rho_dd + rho_uu) - (D.rho + Uc.rho)) * (RD)p.h2y);
-  const R rxx = (R)(((rWD + rWU) - (RD)2 * (D.rho + Uc.rho) + (rED + rEU)) * (RD)p.h2x);
-  const R ux = half * (D.u + Uc.u);
-  const R uy = half * (D.v + Uc.v);
+  const R ryy = (R)(((rho_dd + rho_uu) - srho) * (RD)p.h2y);
+  const R rxx = (R)(((rWD + rWU) + (rED + rEU) - (RD)2 * srho) * (RD)p.h2x);
+  const R su = D.u + Uc.u;
+  const R sv = D.v + Uc.v;
   const R dudy = (Uc.u - D.u) * (R)p.idy;
   const R dvdy = (Uc.v - D.v) * (R)p.idy;
-  const R dudx = ((uED + uEU) - (uWD + uWU)) * (R)p.q_idx;
-  const R dvdx = ((vED + vEU) - (vWD + vWU)) * (R)p.q_idx;
+  const R dudx = ((ED.a + EU.a) - (WD.a + WU.a)) * (R)p.q_idx;
+  const R dvdx = ((ED.b + EU.b) - (WD.b + WU.b)) * (R)p.q_idx;
+  const R rxy = rx * ry;
   const R grad2 = rx * rx + ry * ry;
   const R lap = rxx + ryy;
   const R ry2d = rx * rx - ry * ry;
+  const R srhoR = (R)srho;
 
   if (wall_s || wall_n) {
     // compute_wall_flux_y (wall_xy_equilibrium_module.f:342-444)
@@ -261,24 +291,23 @@ __device__ __forceinline__ void y_face(const DevParams& p, const Ring<RD, R, BW>
     const double TeffU = (double)Uc.Tcl - p.inv_we * (double)Uc.Tcap;
     const double Tl = p.c38cv * (wall_s ? TeffU : TeffD);
     const double Th = Tl + p.dy * p.re * p.pr * p.c38cv * p.wall_q;
-    const double Pw = wall_pressure(0.5 * (Tl + Th), (double)gr);
-    G[0] = (R)0;
-    G[1] = (R)(-p.inv_re * (double)dudy - p.inv_we * (double)(-rx * ry));
-    G[2] = (R)(Pw - p.inv_we * (double)(gr * lap + half * ry2d));
-    G[3] = (R)(wall_s ? p.wall_q : -p.wall_q);
+    const double Pw = wall_pressure(0.5 * (Tl + Th), 0.5 * (double)srho);
+    G2[0] = (R)0;
+    G2[1] = (R)(2.0 * (-p.inv_re * (double)dudy + p.inv_we * (double)rxy));
+    G2[2] = (R)(2.0 * Pw - p.inv_we * (double)(srhoR * lap + ry2d));
+    G2[3] = (R)(2.0 * (wall_s ? p.wall_q : -p.wall_q));
     return;
   }
   const R shear = dudy + dvdx;
   const R visc = (R)p.vr * dudx + (R)p.two_vr * dvdy;
-  G[0] = half * (D.qy + Uc.qy);
-  G[1] = half * (D.qxv + Uc.qxv) - (R)p.inv_re * shear + (R)p.inv_we * (rx * ry);
-  G[2] = half * (D.Pyy + Uc.Pyy) - (R)p.inv_re * visc -
-         (R)p.inv_we * ((R)p.c15cv * half * (D.w + Uc.w) * grad2 + gr * lap + half * ry2d);
-  const R visc_e = ux * shear + uy * visc + (R)p.inv_pr * (Uc.Tcl - D.Tcl) * (R)p.idy;
-  const R cap_e = (R)p.c15cv * half * (D.vw + Uc.vw) * grad2 + G[0] * lap + half * uy * ry2d -
-                  rx * ry * ux - gr * ry * (dudx + dvdy) -
-                  (R)p.inv_repr * (Uc.Tcap - D.Tcap) * (R)p.idy;
-  G[3] = half * (D.Hv + Uc.Hv) - (R)p.inv_re * visc_e - (R)p.inv_we * cap_e;
+  G2[0] = D.qy + Uc.qy;
+  G2[1] = (D.qxv + Uc.qxv) - (R)p.two_inv_re * shear + (R)p.two_inv_we * rxy;
+  G2[2] = (D.Pyy + Uc.Pyy) - (R)p.two_inv_re * visc -
+          (R)p.inv_we * ((R)p.c15cv * (D.w + Uc.w) * grad2 + srhoR * lap + ry2d);
+  const R visc_e2 = su * shear + sv * visc + (R)p.k_heat_y * (Uc.Tcl - D.Tcl);
+  const R cap_e2 = (R)p.c15cv * (D.vw + Uc.vw) * grad2 + G2[0] * lap + (R)0.5 * sv * ry2d -
+                   rxy * su - srhoR * ry * (dudx + dvdy) - (R)p.k_cap_y * (Uc.Tcap - D.Tcap);
+  G2[3] = (D.Hv + Uc.Hv) - (R)p.inv_re * visc_e2 - (R)p.inv_we * cap_e2;
 }
 
 // speed_of_sound (dim2d_prim_module.f:1179-1215), algebraically reduced to
@@ -287,14 +316,14 @@ __device__ __forceinline__ void y_face(const DevParams& p, const Ring<RD, R, BW>
 __device__ __forceinline__ double sound_speed(const DevParams& p, const double v[4],
                                               double* speed) {
   const double rho = v[0];
-  const double ir = 1.0 / rho;
+  const double ir = fast_rcp(rho);
   const double u = v[1] * ir, vv = v[2] * ir;
   const double rho2 = rho * rho;
   const double Tn = v[3] - 0.5 * rho * (u * u + vv * vv) + 3.0 * rho2;
-  const double w = 1.0 / (3.0 - rho);
+  const double w = fast_rcp(3.0 - rho);
   const double P = p.c3cv * w * Tn - 3.0 * rho2;
   const double e = P + rho2 * (2.0 * rho - 3.0);
-  const double c2 = 3.0 * (e + (P + 3.0 * rho2) / p.cv) * w * ir;
+  const double c2 = 3.0 * (e + (P + 3.0 * rho2) * (p.c3cv / 3.0)) * w * ir;
   *speed = sqrt(u * u + vv * vv);
   return c2 > 0.0 ? sqrt(c2) : 0.0;
 }
@@ -320,12 +349,16 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned x) {
   return x;
 }
 
+// reductions of a state: the CFL quantities (max |u|+c, min/max rho, count of
+// non-finite cells) and, when SUMS, the conserved sums
+template <bool SUMS>
 struct RedAcc {
   double s[4];
   double amax, rmin, rmax;
   unsigned nonfinite;
   __device__ void init() {
-    s[0] = s[1] = s[2] = s[3] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] = 0.0;
     amax = 0.0;
     rmin = INFINITY;
     rmax = 0.0;
@@ -335,7 +368,7 @@ struct RedAcc {
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      s[k] += v[k];
+      if (SUMS) s[k] += v[k];
       bad |= !isfinite(v[k]);
     }
     double sp;
@@ -352,15 +385,19 @@ struct RedAcc {
   // one wave's contribution into slot `slot` of ctl
   __device__ void flush(StepCtl* ctl, int slot) {
     double t[4];
+    if (SUMS) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t[k] = wave_sum(s[k]);
+      for (int k = 0; k < 4; ++k) t[k] = wave_sum(s[k]);
+    }
     const double am = wave_max(amax);
     const double rn = wave_min(rmin);
     const double rx = wave_max(rmax);
     const unsigned nf = wave_sum_u(nonfinite);
     if ((threadIdx.x & 63) == 0) {
+      if (SUMS) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) atomicAdd(&ctl->sum[slot][k], t[k]);
+        for (int k = 0; k < 4; ++k) atomicAdd(&ctl->sum[slot][k], t[k]);
+      }
       atomicMax(&ctl->amax[slot], (unsigned long long)__double_as_longlong(am));
       if (rn >= 0.0 && rn < INFINITY)
         atomicMin(&ctl->rmin[slot], (unsigned long long)__double_as_longlong(rn));
@@ -399,10 +436,15 @@ __device__ __forceinline__ double cfl_dt_from(const StepCtl* ctl, int slot) {
 // Mixed mode: state = U + S32 and out increments
 //   stage 1: S32' = dt L;  stage 2: S32' = 1/4 (S32 + dt L);
 //   stage 3: O64 = U + (1 - 1/3) (S32 + dt L)   (FP64 accumulation).
+// Pipeline per row r: the raw state of rows r+1 and r+2 was loaded one and two
+// iterations earlier; this iteration issues the loads of row r+3 and of the RK
+// inputs at (i, r), builds the cell of row r+1, exchanges it through LDS
+// (barrier 1), computes the x-face of row r and the y-face r+1/2, exchanges
+// the x-face fluxes (barrier 2) and writes the updated cell (i, r).
 template <int BW, typename RD, typename R, int MIXED, int STAGE, bool REDUCE>
 __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& a, double dt,
                                            Ring<RD, R, BW>& sm, int bx, int by, int rows,
-                                           RedAcc& red) {
+                                           RedAcc<false>& red) {
   // mixed stages 2 and 3 read u^n + increment; mixed stage 0/1 read u^n alone
   constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
   const int lane = threadIdx.x;
@@ -410,86 +452,107 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   const int r0 = kBcSize + by * rows;
   const int r1 = min(r0 + rows, p.ny - kBcSize);
   const bool out_col = lane >= kBcSize && lane < BW - kBcSize && c <= p.nx - kBcSize - 1;
+  const int lm1 = lane > 0 ? lane - 1 : 0;
   const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
+  const int ccl = min(max(c, 0), p.nx - 1);
 
   bool flx;
-  const int cm = map_col(p, min(max(c, 0), p.nx - 1), flx);
+  const int cm = map_col(p, ccl, flx);
   const bool wall_w = (p.bc_w == kBcWall) && c == kBcSize;
   const bool wall_e = (p.bc_e == kBcWall) && c == p.nx - kBcSize;
 
-  auto load_row = [&](int r, Cell<RD, R>& cell) {
+  auto load_raw = [&](int r, double v[4]) {
     bool fly;
     const int rm = map_row(p, min(max(r, 0), p.ny - 1), fly);
-    double v[4];
-    load_state<SRC>(p, a.U, a.S64, a.S32, (long long)rm * p.pitch + cm, flx, fly, v);
-    make_cell<RD, R>(p, v, cell);
+    load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, flx, fly, v);
   };
 
-  Cell<RD, R> cm2, cm1, c0, c1, c2;
-  load_row(r0 - 2, cm2);
-  load_row(r0 - 1, cm1);
-  load_row(r0, c0);
-  load_row(r0 + 1, c1);
-  put_slot(sm, (r0 - 1) & 3, lane, cm1);
-  put_slot(sm, r0 & 3, lane, c0);
-  put_slot(sm, (r0 + 1) & 3, lane, c1);
-  __syncthreads();
+  // The loop starts two rows early: iterations r0-2 and r0-1 only fill the
+  // pipeline (their outputs are not written), so every output row runs the
+  // same instruction stream whatever the tiling.
+  double raw1[4], raw2[4], raw3[4];
+  Cell<RD, R> cm1, c0, c1;
+  XVals<R> x1;
   {
-    const int lm1 = lane > 0 ? lane - 1 : 0;
-    const int sa = (r0 - 1) & 3, sb = r0 & 3;
-    cm1.Tcap = cap_temperature<RD, R>(p, cm1.irho, sm.rho[sa][lm1], sm.rho[sa][lp1], cm2.rho, c0.rho);
-    c0.Tcap = cap_temperature<RD, R>(p, c0.irho, sm.rho[sb][lm1], sm.rho[sb][lp1], cm1.rho, c1.rho);
-    sm.Tcap[sb][lane] = c0.Tcap;
+    double t[4];
+    load_raw(r0 - 2, t);
+    make_cell<RD, R>(p, t, c0, x1);
+    c0.Tcap = (R)0;
+    cm1 = c0;
+    load_raw(r0 - 1, raw1);
+    load_raw(r0, raw2);
   }
-  R fy_prev[4];
-  y_face<RD, R, BW>(p, sm, (r0 - 1) & 3, r0 & 3, lane, cm1, c0, cm2.rho, c1.rho,
-                    p.bc_s == kBcWall && r0 - 1 == kBcSize - 1, false, fy_prev);
-  __syncthreads();
+  put_rho_uv(sm, (r0 - 2 + 3) % 3, lane, c0);
+  put_rho_uv(sm, (r0 - 3 + 3) % 3, lane, c0);
+  put_xvals(sm, lane, c0, x1);
+  R fy_prev[4] = {(R)0, (R)0, (R)0, (R)0};
 
-  const R idx = (R)p.idx, idy = (R)p.idy;
-  for (int r = r0; r < r1; ++r) {
-    const int s0 = r & 3, sm1 = (r - 1) & 3, sp1 = (r + 1) & 3, sp2 = (r + 2) & 3;
-    load_row(r + 2, c2);
-    {
-      const int lm1 = lane > 0 ? lane - 1 : 0;
-      c1.Tcap = cap_temperature<RD, R>(p, c1.irho, sm.rho[sp1][lm1], sm.rho[sp1][lp1], c0.rho, c2.rho);
+  // fluxes are doubled (face sums), hence the 1/2
+  const R hidx = (R)p.half_idx, hidy = (R)p.half_idy;
+  int s0 = (r0 - 2) % 3;  // row slot of r
+  for (int r = r0 - 2; r < r1; ++r) {
+    const int sp1 = s0 == 2 ? 0 : s0 + 1;
+    const int sm1 = s0 == 0 ? 2 : s0 - 1;
+    // prefetch: row r+3 and the RK inputs of (i, r)
+    load_raw(r + 3, raw3);
+    const long long ro = (long long)r * p.pitch;  // uniform row offset
+    const unsigned uc = (unsigned)ccl;
+    double rkU[4], rkS[4];
+    if (STAGE >= 1 && r >= r0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const long long b = ro + k * p.plane;
+        if (!MIXED) {
+          rkS[k] = (a.S64 + b)[uc];
+          rkU[k] = (STAGE >= 2) ? (a.U + b)[uc] : 0.0;
+        } else {
+          rkS[k] = (STAGE >= 2) ? (double)(a.S32 + b)[uc] : 0.0;
+          rkU[k] = (STAGE == 3) ? (a.U + b)[uc] : 0.0;
+        }
+      }
     }
-    put_slot(sm, sp2, lane, c2);
-    sm.Tcap[sp1][lane] = c1.Tcap;
+    // cell of row r+1
+    make_cell<RD, R>(p, raw1, c1, x1);
+    put_rho_uv(sm, sp1, lane, c1);
+    __syncthreads();
+    c1.Tcap = cap_temperature<RD, R>(p, c1.irho, sm.rho[sp1][lm1], sm.rho[sp1][lp1], c0.rho, (RD)raw2[0]);
 
     R F[4], G[4];
-    x_face<RD, R, BW>(p, sm, s0, sm1, sp1, lane, c0, cm1.rho, cm1.u, cm1.v, c1.rho, c1.u, c1.v,
-                      wall_w, wall_e, F);
-    y_face<RD, R, BW>(p, sm, s0, sp1, lane, c0, c1, cm1.rho, c2.rho, false,
+    x_face<RD, R, BW>(p, sm, s0, sm1, sp1, lane, c0, cm1.rho, cm1.u, cm1.v, c1.rho, c1.u,
+                      c1.v, wall_w, wall_e, F);
+    y_face<RD, R, BW>(p, sm, s0, sp1, lane, c0, c1, cm1.rho, (RD)raw2[0],
+                      p.bc_s == kBcWall && r == kBcSize - 1,
                       p.bc_n == kBcWall && r == p.ny - kBcSize - 1, G);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sm.fx[r & 1][k][lane] = F[k];
+    sm.f01[lane] = Pair<R>{F[0], F[1]};
+    sm.f23[lane] = Pair<R>{F[2], F[3]};
     __syncthreads();
+    const Pair<R> fa = sm.f01[lp1], fb = sm.f23[lp1];
+    const R FR[4] = {fa.a, fa.b, fb.a, fb.b};
+    // the x-values of row r are no longer read: store those of row r+1
+    put_xvals(sm, lane, c1, x1);
     R L[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) L[k] = (F[k] - sm.fx[r & 1][k][lp1]) * idx + (fy_prev[k] - G[k]) * idy;
+    for (int k = 0; k < 4; ++k) L[k] = (F[k] - FR[k]) * hidx + (fy_prev[k] - G[k]) * hidy;
     if (p.gravity_ac) {
       // compute_body_forces (pmodel_eq_class.f:1840-1918)
       L[2] -= (R)((double)c0.rho * p.gravity);
       L[3] -= (R)((double)c0.qy * p.gravity);
     }
 
-    if (out_col) {
-      const long long o = (long long)r * p.pitch + c;
+    if (out_col && r >= r0) {
       double vout[4];
       if (STAGE == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) a.O64[k * p.plane + o] = (double)L[k];
+        for (int k = 0; k < 4; ++k) (a.O64 + ro + k * p.plane)[uc] = (double)L[k];
       } else if (!MIXED) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const double s = a.S64[k * p.plane + o];
-          const double inc = s + dt * (double)L[k];
+          const double inc = rkS[k] + dt * (double)L[k];
           double val;
           if (STAGE == 1) val = inc;
-          else if (STAGE == 2) val = 0.75 * a.U[k * p.plane + o] + (1.0 - 0.75) * inc;
-          else val = (1.0 / 3.0) * a.U[k * p.plane + o] + (1.0 - 1.0 / 3.0) * inc;
-          a.O64[k * p.plane + o] = val;
+          else if (STAGE == 2) val = 0.75 * rkU[k] + (1.0 - 0.75) * inc;
+          else val = (1.0 / 3.0) * rkU[k] + (1.0 - 1.0 / 3.0) * inc;
+          (a.O64 + ro + k * p.plane)[uc] = val;
           vout[k] = val;
         }
       } else {
@@ -497,13 +560,12 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
         for (int k = 0; k < 4; ++k) {
           const double dl = dt * (double)L[k];
           if (STAGE == 1) {
-            a.O32[k * p.plane + o] = (float)dl;
+            (a.O32 + ro + k * p.plane)[uc] = (float)dl;
           } else if (STAGE == 2) {
-            a.O32[k * p.plane + o] = (float)(0.25 * ((double)a.S32[k * p.plane + o] + dl));
+            (a.O32 + ro + k * p.plane)[uc] = (float)(0.25 * (rkS[k] + dl));
           } else {
-            const double val = a.U[k * p.plane + o] +
-                               (1.0 - 1.0 / 3.0) * ((double)a.S32[k * p.plane + o] + dl);
-            a.O64[k * p.plane + o] = val;
+            const double val = rkU[k] + (1.0 - 1.0 / 3.0) * (rkS[k] + dl);
+            (a.O64 + ro + k * p.plane)[uc] = val;
             vout[k] = val;
           }
         }
@@ -511,15 +573,32 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
       if (REDUCE && STAGE == 3) red.add(p, vout);
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) fy_prev[k] = G[k];
+    for (int k = 0; k < 4; ++k) {
+      fy_prev[k] = G[k];
+      raw1[k] = raw2[k];
+      raw2[k] = raw3[k];
+    }
     cm1 = c0;
     c0 = c1;
-    c1 = c2;
+    s0 = sp1;
   }
 }
 
+// minimum waves per SIMD the register allocation must allow: FP64 stages 3
+// (168 VGPRs), mixed stages 4 (128 VGPRs); stage 3 (with the CFL reduction)
+// one fewer
+#ifndef DIM2D_WAVES_FP64
+#define DIM2D_WAVES_FP64 3
+#endif
+#ifndef DIM2D_WAVES_MIXED
+#define DIM2D_WAVES_MIXED 4
+#endif
 template <int BW, int MIXED, int STAGE>
-__global__ void __launch_bounds__(BW) stage_kernel(const DevParams p, const StageArgs a, int rows) {
+__global__ void __launch_bounds__(BW)
+__attribute__((amdgpu_waves_per_eu((MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64) -
+                                       (STAGE == 3 ? 1 : 0),
+                                   8)))
+stage_kernel(const DevParams p, const StageArgs a, int rows) {
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
   __shared__ Ring<RD, R, BW> sm;
@@ -533,7 +612,7 @@ __global__ void __launch_bounds__(BW) stage_kernel(const DevParams p, const Stag
   } else if (STAGE != 0) {
     dt = a.ctl->dt;
   }
-  RedAcc red;
+  RedAcc<false> red;
   red.init();
   stage_tile<BW, RD, R, MIXED, STAGE, STAGE == 3>(p, a, dt, sm, blockIdx.x, blockIdx.y, rows, red);
   if (STAGE == 3) {
@@ -546,14 +625,16 @@ __global__ void __launch_bounds__(BW) stage_kernel(const DevParams p, const Stag
 // each simulation has a fixed dt (a.S64/U/O pointers are batch bases, offset
 // by DevParams::offset); no reductions.
 template <int BW, int MIXED, int STAGE>
-__global__ void __launch_bounds__(BW) stage_batch_kernel(const DevParams* __restrict__ P,
+__global__ void __launch_bounds__(BW)
+__attribute__((amdgpu_waves_per_eu(MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64, 8)))
+stage_batch_kernel(const DevParams* __restrict__ P,
                                                          const WorkItem* __restrict__ items,
                                                          int nitems, const StageArgs a,
                                                          const double* __restrict__ dts, int rows) {
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
   __shared__ Ring<RD, R, BW> sm;
-  RedAcc red;
+  RedAcc<false> red;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem w = items[it];
     const DevParams p = P[w.sim];
@@ -835,7 +916,7 @@ __global__ void clear_slot_kernel(StepCtl* ctl, int slot) { clear_slot(ctl, slot
 
 __global__ void __launch_bounds__(256) reduce_kernel(const DevParams p, const double* F,
                                                      StepCtl* ctl, int slot) {
-  RedAcc red;
+  RedAcc<true> red;
   red.init();
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
   const long long n = (long long)ni * nj;

@@ -152,6 +152,17 @@ class Field:
     def set_tiles(self, block_x: int = 0, rows: int = 0) -> None:
         capi.check(self._lib.dim2d_set_tiles(self._ctx, block_x, rows), "dim2d_set_tiles")
 
+    def profile(self, enable: bool) -> None:
+        """Record HIP events around every kernel launch of the following steps."""
+        capi.check(self._lib.dim2d_profile_enable(self._ctx, int(enable)), "dim2d_profile_enable")
+
+    def kernel_times(self) -> dict:
+        """Mean duration (ms) and count per kernel since profile(True)."""
+        out = np.zeros(8)
+        capi.check(self._lib.dim2d_profile_read(self._ctx, _dptr(out)), "dim2d_profile_read")
+        names = ("stage1", "stage2", "stage3", "wall")
+        return {n: (float(out[2 * i]), int(out[2 * i + 1])) for i, n in enumerate(names)}
+
     def close(self) -> None:
         if self._ctx:
             self._lib.dim2d_ctx_destroy(self._ctx)

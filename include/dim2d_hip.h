@@ -134,6 +134,17 @@ int dim2d_synchronize(dim2d_ctx* ctx);
  * workgroup along y; 0 keeps the current value. */
 int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
 
+/* Kernel timing with HIP events recorded on the context's stream around every
+ * kernel launch of the following steps (enable = 1), or off (enable = 0, which
+ * also discards what was recorded).  No counterpart in the reference, which
+ * only reports CPU_TIME totals (sim_dim2d.f:46,64,106). */
+int dim2d_profile_enable(dim2d_ctx* ctx, int32_t enable);
+
+/* Mean duration (ms) and launch count per kernel since enabling (blocking):
+ * out[0],out[1] stage 1; out[2],out[3] stage 2; out[4],out[5] stage 3;
+ * out[6],out[7] wall ghost kernel. */
+int dim2d_profile_read(dim2d_ctx* ctx, double out[8]);
+
 /* ------------------------------------------------------------------ */
 /* Batches of independent simulations (uneven grid sizes).              */
 /* ------------------------------------------------------------------ */

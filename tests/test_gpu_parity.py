@@ -130,7 +130,9 @@ def test_drop64_200_steps_fp64(orc):
 
 def test_drop64_mixed_precision(orc):
     """Mixed FP32/FP64 path against the FP64 oracle after 200 steps: density
-    error <= 1e-6 relative; mass conserved to rounding."""
+    error <= 1e-6 relative; mass conserved to 1e-10 relative (the stage
+    increments are stored in FP32, so the telescoping of the face fluxes holds
+    to FP32 rounding of dt*L per cell)."""
     from dim2d_finite_volume_amd import initial_conditions as ic
     d = _lib()
     p, nodes = ic.drop64(precision=1)
@@ -147,7 +149,7 @@ def test_drop64_mixed_precision(orc):
     err = max_rel_err(got, ref)
     print("mixed drop64 200 steps rel err", err)
     assert err[0] < 1e-6 and err[3] < 1e-6, err
-    assert abs(f.diagnostics()[0] - mass0) / mass0 < 1e-12
+    assert abs(f.diagnostics()[0] - mass0) / mass0 < 1e-10
 
 
 def test_cfl_integrate(orc):
