@@ -263,8 +263,8 @@ int dim2d_ctx_destroy(dim2d_ctx* c) {
 
 int dim2d_set_tiles(dim2d_ctx* c, int32_t block_x, int32_t rows) {
   if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
-  if (block_x != 0 && block_x != 64 && block_x != 128)
-    return fail(DIM2D_EINVAL, "block_x must be 64 or 128, got %d", block_x);
+  if (block_x != 0 && block_x != 64)
+    return fail(DIM2D_EINVAL, "block_x must be 64 (one wave per column strip), got %d", block_x);
   if (rows < 0 || rows > 4096) return fail(DIM2D_EINVAL, "rows_per_block out of range: %d", rows);
   if (block_x) c->block_x = block_x;
   if (rows) c->rows = rows;
@@ -713,8 +713,8 @@ int dim2d_batch_destroy(dim2d_batch* b) {
 
 int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows, int32_t workgroups) {
   if (!b) return fail(DIM2D_EINVAL, "batch is NULL");
-  if (block_x != 0 && block_x != 64 && block_x != 128)
-    return fail(DIM2D_EINVAL, "block_x must be 64 or 128, got %d", block_x);
+  if (block_x != 0 && block_x != 64)
+    return fail(DIM2D_EINVAL, "block_x must be 64 (one wave per column strip), got %d", block_x);
   if (rows < 0 || rows > 4096 || workgroups < 0)
     return fail(DIM2D_EINVAL, "invalid tile arguments");
   if (block_x) b->block_x = block_x;

@@ -143,38 +143,45 @@ __device__ __forceinline__ void make_cell(const DevParams& p, const double v[4],
   x.uw = (R)(u * w);
 }
 
-template <typename R>
-struct Pair {
-  R a, b;
-};
-
-// LDS: density and velocities of rows r-1, r, r+1 (slot = row % 3), the
-// x-face values of row r in pairs read by one instruction each, and the
-// x-face fluxes.
-template <typename RD, typename R, int BW>
-struct Ring {
-  RD rho[3][BW];
-  Pair<R> uv[3][BW];
-  Pair<R> wT[BW];  // w, Tcl
-  Pair<R> qP[BW];  // qx, qx*u+P
-  Pair<R> Hw[BW];  // (E+P)u, u w
-  Pair<R> Cq[BW];  // Tcap, qx*v
-  Pair<R> f01[BW], f23[BW];
-};
-
-template <typename RD, typename R, int BW>
-__device__ __forceinline__ void put_rho_uv(Ring<RD, R, BW>& sm, int s3, int lane,
-                                           const Cell<RD, R>& c) {
-  sm.rho[s3][lane] = c.rho;
-  sm.uv[s3][lane] = Pair<R>{c.u, c.v};
+// ------------------------------------------------------------------------
+// cross-lane exchange: one wave is one strip of 64 columns, so the values of
+// the neighbouring columns live in the neighbouring lanes' registers and move
+// with DPP wavefront shifts (VALU register moves, no LDS, no barrier).
+// shr: lane i receives lane i-1; shl: lane i receives lane i+1; a lane
+// without a source keeps its own value (edge lanes never produce outputs).
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int dpp_shr_i(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false);
 }
-template <typename RD, typename R, int BW>
-__device__ __forceinline__ void put_xvals(Ring<RD, R, BW>& sm, int lane, const Cell<RD, R>& c,
-                                          const XVals<R>& x) {
-  sm.wT[lane] = Pair<R>{c.w, c.Tcl};
-  sm.qP[lane] = Pair<R>{x.qx, x.Pxx};
-  sm.Hw[lane] = Pair<R>{x.Hu, x.uw};
-  sm.Cq[lane] = Pair<R>{c.Tcap, c.qxv};
+__device__ __forceinline__ int dpp_shl_i(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float shr(float x) { return __int_as_float(dpp_shr_i(__float_as_int(x))); }
+__device__ __forceinline__ float shl(float x) { return __int_as_float(dpp_shl_i(__float_as_int(x))); }
+__device__ __forceinline__ double shr(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_shr_i((int)b), hi = dpp_shr_i((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double shl(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_shl_i((int)b), hi = dpp_shl_i((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// density and velocities of a neighbouring cell
+template <typename RD, typename R>
+struct Nb {
+  RD rho;
+  R u, v;
+};
+template <typename RD, typename R>
+__device__ __forceinline__ Nb<RD, R> nb_shr(const Cell<RD, R>& c) {
+  return Nb<RD, R>{shr(c.rho), shr(c.u), shr(c.v)};
+}
+template <typename RD, typename R>
+__device__ __forceinline__ Nb<RD, R> nb_shl(const Cell<RD, R>& c) {
+  return Nb<RD, R>{shl(c.rho), shl(c.u), shl(c.v)};
 }
 
 // capillarity_temperature_eff (:582-613) with gradient_x/y_interior
@@ -194,36 +201,26 @@ __device__ __forceinline__ double wall_pressure(double T, double md) {
 // x-face flux at (i-1/2, j): L = (i-1, j), R = (i, j)  (dim2d_fluxes_module.f,
 // Mattsson f/dfdx/dfdy/d2fdx2/d2fdy2/dfdx_nl stencils), returned doubled
 // (F2 = 2 F) so the face averages are plain sums; the divergence applies the
-// 1/2.  s0/sm1/sp1: row slots of j, j-1, j+1; the x-values are those of row j.
-template <typename RD, typename R, int BW>
-__device__ __forceinline__ void x_face(const DevParams& p, const Ring<RD, R, BW>& sm, int s0,
-                                       int sm1, int sp1, int lane, const Cell<RD, R>& Rc,
-                                       RD rhoR_dn, R uR_dn, R vR_dn, RD rhoR_up, R uR_up,
-                                       R vR_up, bool wall_w, bool wall_e, R F2[4]) {
-  const int lm1 = lane > 0 ? lane - 1 : 0;
-  const int lm2 = lane > 1 ? lane - 2 : 0;
-  const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
-  const RD rhoL = sm.rho[s0][lm1];
-  const RD rhoLL = sm.rho[s0][lm2];
-  const RD rhoRR = sm.rho[s0][lp1];
-  const RD rhoL_dn = sm.rho[sm1][lm1], rhoL_up = sm.rho[sp1][lm1];
-  const Pair<R> uvL = sm.uv[s0][lm1], uvL_dn = sm.uv[sm1][lm1], uvL_up = sm.uv[sp1][lm1];
-  const Pair<R> wTL = sm.wT[lm1];
-  const Pair<R> qPL = sm.qP[lm1], qPR = sm.qP[lane];
-  const Pair<R> HwL = sm.Hw[lm1], HwR = sm.Hw[lane];
-  const Pair<R> CqL = sm.Cq[lm1];
-
-  const RD srho = rhoL + Rc.rho;
-  const R rx = (R)((Rc.rho - rhoL) * (RD)p.idx);
-  const R ry = (R)(((rhoL_up + rhoR_up) - (rhoL_dn + rhoR_dn)) * (RD)p.q_idy);
+// 1/2.  L: left cell (its x-values in Lx); LL/RR: densities of (i-2, j) and
+// (i+1, j); Ldn/Lup, Rdn/Rup: rows j-1 and j+1 of the two columns.
+template <typename RD, typename R>
+__device__ __forceinline__ void x_face(const DevParams& p, const Nb<RD, R>& L, const Cell<RD, R>& Lc,
+                                       const XVals<R>& Lx, RD rhoLL, RD rhoRR,
+                                       const Nb<RD, R>& Ldn, const Nb<RD, R>& Lup,
+                                       const Cell<RD, R>& Rc, const XVals<R>& Rx,
+                                       const Nb<RD, R>& Rdn, const Nb<RD, R>& Rup, bool wall_w,
+                                       bool wall_e, R F2[4]) {
+  const RD srho = L.rho + Rc.rho;
+  const R rx = (R)((Rc.rho - L.rho) * (RD)p.idx);
+  const R ry = (R)(((Lup.rho + Rup.rho) - (Ldn.rho + Rdn.rho)) * (RD)p.q_idy);
   const R rxx = (R)(((rhoLL + rhoRR) - srho) * (RD)p.h2x);
-  const R ryy = (R)(((rhoL_dn + rhoR_dn) + (rhoL_up + rhoR_up) - (RD)2 * srho) * (RD)p.h2y);
-  const R su = uvL.a + Rc.u;
-  const R sv = uvL.b + Rc.v;
-  const R dudx = (Rc.u - uvL.a) * (R)p.idx;
-  const R dvdx = (Rc.v - uvL.b) * (R)p.idx;
-  const R dudy = ((uvL_up.a + uR_up) - (uvL_dn.a + uR_dn)) * (R)p.q_idy;
-  const R dvdy = ((uvL_up.b + vR_up) - (uvL_dn.b + vR_dn)) * (R)p.q_idy;
+  const R ryy = (R)(((Ldn.rho + Rdn.rho) + (Lup.rho + Rup.rho) - (RD)2 * srho) * (RD)p.h2y);
+  const R su = L.u + Rc.u;
+  const R sv = L.v + Rc.v;
+  const R dudx = (Rc.u - L.u) * (R)p.idx;
+  const R dvdx = (Rc.v - L.v) * (R)p.idx;
+  const R dudy = ((Lup.u + Rup.u) - (Ldn.u + Rdn.u)) * (R)p.q_idy;
+  const R dvdy = ((Lup.v + Rup.v) - (Ldn.v + Rdn.v)) * (R)p.q_idy;
   const R rxy = rx * ry;
   const R grad2 = rx * rx + ry * ry;
   const R lap = rxx + ryy;
@@ -232,7 +229,7 @@ __device__ __forceinline__ void x_face(const DevParams& p, const Ring<RD, R, BW>
 
   if (wall_w || wall_e) {
     // compute_wall_flux_x (wall_xy_equilibrium_module.f:249-341)
-    const double TeffL = (double)wTL.b - p.inv_we * (double)CqL.a;
+    const double TeffL = (double)Lc.Tcl - p.inv_we * (double)Lc.Tcap;
     const double TeffR = (double)Rc.Tcl - p.inv_we * (double)Rc.Tcap;
     const double Tl = p.c38cv * (wall_w ? TeffR : TeffL);
     const double Th = Tl + p.dx * p.c38cv * p.re * p.pr * p.wall_q;
@@ -245,40 +242,35 @@ __device__ __forceinline__ void x_face(const DevParams& p, const Ring<RD, R, BW>
   }
   const R visc = (R)p.two_vr * dudx + (R)p.vr * dvdy;
   const R shear = dudy + dvdx;
-  F2[0] = qPL.a + qPR.a;
-  F2[1] = (qPL.b + qPR.b) - (R)p.two_inv_re * visc -
-          (R)p.inv_we * ((R)p.c15cv * (wTL.a + Rc.w) * grad2 + srhoR * lap + rx2d);
-  F2[2] = (CqL.b + Rc.qxv) - (R)p.two_inv_re * shear + (R)p.two_inv_we * rxy;
-  const R visc_e2 = su * visc + sv * shear + (R)p.k_heat_x * (Rc.Tcl - wTL.b);
-  const R cap_e2 = (R)p.c15cv * (HwL.b + HwR.b) * grad2 + F2[0] * lap + (R)0.5 * su * rx2d -
-                   rxy * sv - srhoR * rx * (dudx + dvdy) - (R)p.k_cap_x * (Rc.Tcap - CqL.a);
-  F2[3] = (HwL.a + HwR.a) - (R)p.inv_re * visc_e2 - (R)p.inv_we * cap_e2;
+  F2[0] = Lx.qx + Rx.qx;
+  F2[1] = (Lx.Pxx + Rx.Pxx) - (R)p.two_inv_re * visc -
+          (R)p.inv_we * ((R)p.c15cv * (Lc.w + Rc.w) * grad2 + srhoR * lap + rx2d);
+  F2[2] = (Lc.qxv + Rc.qxv) - (R)p.two_inv_re * shear + (R)p.two_inv_we * rxy;
+  const R visc_e2 = su * visc + sv * shear + (R)p.k_heat_x * (Rc.Tcl - Lc.Tcl);
+  const R cap_e2 = (R)p.c15cv * (Lx.uw + Rx.uw) * grad2 + F2[0] * lap + (R)0.5 * su * rx2d -
+                   rxy * sv - srhoR * rx * (dudx + dvdy) - (R)p.k_cap_x * (Rc.Tcap - Lc.Tcap);
+  F2[3] = (Lx.Hu + Rx.Hu) - (R)p.inv_re * visc_e2 - (R)p.inv_we * cap_e2;
 }
 
 // y-face flux at (i, j-1/2): D = (i, j-1), U = (i, j)  (g/dgdx/dgdy/d2gdx2/d2gdy2/dgdy_nl),
-// returned doubled like x_face.
-template <typename RD, typename R, int BW>
-__device__ __forceinline__ void y_face(const DevParams& p, const Ring<RD, R, BW>& sm, int sD,
-                                       int sU, int lane, const Cell<RD, R>& D,
-                                       const Cell<RD, R>& Uc, RD rho_dd, RD rho_uu,
-                                       bool wall_s, bool wall_n, R G2[4]) {
-  const int lm1 = lane > 0 ? lane - 1 : 0;
-  const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
-  const RD rWD = sm.rho[sD][lm1], rED = sm.rho[sD][lp1];
-  const RD rWU = sm.rho[sU][lm1], rEU = sm.rho[sU][lp1];
-  const Pair<R> WD = sm.uv[sD][lm1], ED = sm.uv[sD][lp1], WU = sm.uv[sU][lm1], EU = sm.uv[sU][lp1];
-
+// returned doubled like x_face.  WD/ED, WU/EU: columns i-1 and i+1 of both rows.
+template <typename RD, typename R>
+__device__ __forceinline__ void y_face(const DevParams& p, const Cell<RD, R>& D,
+                                       const Cell<RD, R>& Uc, const Nb<RD, R>& WD,
+                                       const Nb<RD, R>& ED, const Nb<RD, R>& WU,
+                                       const Nb<RD, R>& EU, RD rho_dd, RD rho_uu, bool wall_s,
+                                       bool wall_n, R G2[4]) {
   const RD srho = D.rho + Uc.rho;
   const R ry = (R)((Uc.rho - D.rho) * (RD)p.idy);
-  const R rx = (R)(((rED + rEU) - (rWD + rWU)) * (RD)p.q_idx);
+  const R rx = (R)(((ED.rho + EU.rho) - (WD.rho + WU.rho)) * (RD)p.q_idx);
   const R ryy = (R)(((rho_dd + rho_uu) - srho) * (RD)p.h2y);
-  const R rxx = (R)(((rWD + rWU) + (rED + rEU) - (RD)2 * srho) * (RD)p.h2x);
+  const R rxx = (R)(((WD.rho + WU.rho) + (ED.rho + EU.rho) - (RD)2 * srho) * (RD)p.h2x);
   const R su = D.u + Uc.u;
   const R sv = D.v + Uc.v;
   const R dudy = (Uc.u - D.u) * (R)p.idy;
   const R dvdy = (Uc.v - D.v) * (R)p.idy;
-  const R dudx = ((ED.a + EU.a) - (WD.a + WU.a)) * (R)p.q_idx;
-  const R dvdx = ((ED.b + EU.b) - (WD.b + WU.b)) * (R)p.q_idx;
+  const R dudx = ((ED.u + EU.u) - (WD.u + WU.u)) * (R)p.q_idx;
+  const R dvdx = ((ED.v + EU.v) - (WD.v + WU.v)) * (R)p.q_idx;
   const R rxy = rx * ry;
   const R grad2 = rx * rx + ry * ry;
   const R lap = rxx + ryy;
@@ -430,30 +422,34 @@ __device__ __forceinline__ double cfl_dt_from(const StepCtl* ctl, int slot) {
 // ------------------------------------------------------------------------
 // the fused stage tile
 // ------------------------------------------------------------------------
+// One wave (64 lanes) owns a strip of 64 columns (60 output columns plus a
+// 2-column halo on each side) and `rows` output rows, and marches up it.
 // STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages.
 // FP64 mode (MIXED=0): stage input S64; out = b*U + (1-b)*(S64 + dt L) with
 // b = 0 (stage 1, S64 = U), 3/4, 1/3 (rk3tvd_steps_module.f:225,444).
 // Mixed mode: state = U + S32 and out increments
 //   stage 1: S32' = dt L;  stage 2: S32' = 1/4 (S32 + dt L);
 //   stage 3: O64 = U + (1 - 1/3) (S32 + dt L)   (FP64 accumulation).
-// Pipeline per row r: the raw state of rows r+1 and r+2 was loaded one and two
+// Per row r: the raw state of rows r+1 and r+2 was loaded one and two
 // iterations earlier; this iteration issues the loads of row r+3 and of the RK
-// inputs at (i, r), builds the cell of row r+1, exchanges it through LDS
-// (barrier 1), computes the x-face of row r and the y-face r+1/2, exchanges
-// the x-face fluxes (barrier 2) and writes the updated cell (i, r).
-template <int BW, typename RD, typename R, int MIXED, int STAGE, bool REDUCE>
+// inputs at (i, r), builds the cell of row r+1, fetches the neighbouring
+// columns' values by DPP, computes the x-face of row r and the y-face r+1/2,
+// and writes the updated cell (i, r).  The loop starts two rows early to fill
+// the pipeline (no outputs there), so every output row runs the same
+// instruction stream whatever the tiling.
+constexpr int kLanes = 64;
+constexpr int kOutCols = kLanes - 2 * kBcSize;
+
+template <typename RD, typename R, int MIXED, int STAGE, bool REDUCE>
 __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& a, double dt,
-                                           Ring<RD, R, BW>& sm, int bx, int by, int rows,
-                                           RedAcc<false>& red) {
+                                           int bx, int by, int rows, RedAcc<false>& red) {
   // mixed stages 2 and 3 read u^n + increment; mixed stage 0/1 read u^n alone
   constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
-  const int lane = threadIdx.x;
-  const int c = bx * (BW - 2 * kBcSize) + lane;  // column of this lane (0-based)
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int c = bx * kOutCols + lane;  // column of this lane (0-based)
   const int r0 = kBcSize + by * rows;
   const int r1 = min(r0 + rows, p.ny - kBcSize);
-  const bool out_col = lane >= kBcSize && lane < BW - kBcSize && c <= p.nx - kBcSize - 1;
-  const int lm1 = lane > 0 ? lane - 1 : 0;
-  const int lp1 = lane < BW - 1 ? lane + 1 : BW - 1;
+  const bool out_col = lane >= kBcSize && lane < kLanes - kBcSize && c <= p.nx - kBcSize - 1;
   const int ccl = min(max(c, 0), p.nx - 1);
 
   bool flx;
@@ -467,32 +463,27 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
     load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, flx, fly, v);
   };
 
-  // The loop starts two rows early: iterations r0-2 and r0-1 only fill the
-  // pipeline (their outputs are not written), so every output row runs the
-  // same instruction stream whatever the tiling.
   double raw1[4], raw2[4], raw3[4];
   Cell<RD, R> cm1, c0, c1;
-  XVals<R> x1;
+  XVals<R> x0, x1;
+  Nb<RD, R> cm1W, c0W, c0E;  // west/east neighbours of rows r-1 and r
   {
     double t[4];
     load_raw(r0 - 2, t);
-    make_cell<RD, R>(p, t, c0, x1);
+    make_cell<RD, R>(p, t, c0, x0);
     c0.Tcap = (R)0;
     cm1 = c0;
+    c0W = nb_shr(c0);
+    c0E = nb_shl(c0);
+    cm1W = c0W;
     load_raw(r0 - 1, raw1);
     load_raw(r0, raw2);
   }
-  put_rho_uv(sm, (r0 - 2 + 3) % 3, lane, c0);
-  put_rho_uv(sm, (r0 - 3 + 3) % 3, lane, c0);
-  put_xvals(sm, lane, c0, x1);
   R fy_prev[4] = {(R)0, (R)0, (R)0, (R)0};
-
   // fluxes are doubled (face sums), hence the 1/2
   const R hidx = (R)p.half_idx, hidy = (R)p.half_idy;
-  int s0 = (r0 - 2) % 3;  // row slot of r
+
   for (int r = r0 - 2; r < r1; ++r) {
-    const int sp1 = s0 == 2 ? 0 : s0 + 1;
-    const int sm1 = s0 == 0 ? 2 : s0 - 1;
     // prefetch: row r+3 and the RK inputs of (i, r)
     load_raw(r + 3, raw3);
     const long long ro = (long long)r * p.pitch;  // uniform row offset
@@ -511,28 +502,34 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
         }
       }
     }
-    // cell of row r+1
+    // cell of row r+1 and its neighbours
     make_cell<RD, R>(p, raw1, c1, x1);
-    put_rho_uv(sm, sp1, lane, c1);
-    __syncthreads();
-    c1.Tcap = cap_temperature<RD, R>(p, c1.irho, sm.rho[sp1][lm1], sm.rho[sp1][lp1], c0.rho, (RD)raw2[0]);
+    const Nb<RD, R> c1W = nb_shr(c1), c1E = nb_shl(c1);
+    c1.Tcap = cap_temperature<RD, R>(p, c1.irho, c1W.rho, c1E.rho, c0.rho, (RD)raw2[0]);
+
+    // the west cell of row r for the x-face
+    Cell<RD, R> Lc;
+    XVals<R> Lx;
+    Lc.w = shr(c0.w);
+    Lc.Tcl = shr(c0.Tcl);
+    Lc.Tcap = shr(c0.Tcap);
+    Lc.qxv = shr(c0.qxv);
+    Lx.qx = shr(x0.qx);
+    Lx.Pxx = shr(x0.Pxx);
+    Lx.Hu = shr(x0.Hu);
+    Lx.uw = shr(x0.uw);
+    const RD rhoLL = shr(c0W.rho);
 
     R F[4], G[4];
-    x_face<RD, R, BW>(p, sm, s0, sm1, sp1, lane, c0, cm1.rho, cm1.u, cm1.v, c1.rho, c1.u,
-                      c1.v, wall_w, wall_e, F);
-    y_face<RD, R, BW>(p, sm, s0, sp1, lane, c0, c1, cm1.rho, (RD)raw2[0],
-                      p.bc_s == kBcWall && r == kBcSize - 1,
-                      p.bc_n == kBcWall && r == p.ny - kBcSize - 1, G);
-    sm.f01[lane] = Pair<R>{F[0], F[1]};
-    sm.f23[lane] = Pair<R>{F[2], F[3]};
-    __syncthreads();
-    const Pair<R> fa = sm.f01[lp1], fb = sm.f23[lp1];
-    const R FR[4] = {fa.a, fa.b, fb.a, fb.b};
-    // the x-values of row r are no longer read: store those of row r+1
-    put_xvals(sm, lane, c1, x1);
+    x_face<RD, R>(p, c0W, Lc, Lx, rhoLL, c0E.rho, cm1W, c1W, c0, x0,
+                  Nb<RD, R>{cm1.rho, cm1.u, cm1.v}, Nb<RD, R>{c1.rho, c1.u, c1.v}, wall_w,
+                  wall_e, F);
+    y_face<RD, R>(p, c0, c1, c0W, c0E, c1W, c1E, cm1.rho, (RD)raw2[0],
+                  p.bc_s == kBcWall && r == kBcSize - 1,
+                  p.bc_n == kBcWall && r == p.ny - kBcSize - 1, G);
     R L[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) L[k] = (F[k] - FR[k]) * hidx + (fy_prev[k] - G[k]) * hidy;
+    for (int k = 0; k < 4; ++k) L[k] = (F[k] - shl(F[k])) * hidx + (fy_prev[k] - G[k]) * hidy;
     if (p.gravity_ac) {
       // compute_body_forces (pmodel_eq_class.f:1840-1918)
       L[2] -= (R)((double)c0.rho * p.gravity);
@@ -580,28 +577,30 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
     }
     cm1 = c0;
     c0 = c1;
-    s0 = sp1;
+    x0 = x1;
+    cm1W = c0W;
+    c0W = c1W;
+    c0E = c1E;
   }
 }
 
-// minimum waves per SIMD the register allocation must allow: FP64 stages 3
-// (168 VGPRs), mixed stages 4 (128 VGPRs); stage 3 (with the CFL reduction)
-// one fewer
+// minimum waves per SIMD the register allocation must allow (FP64 stages 2,
+// i.e. up to 256 VGPRs; mixed stages 3, up to 168 VGPRs; stage 3 with the CFL
+// reduction one fewer): tighter bounds make the allocator spill to scratch.
 #ifndef DIM2D_WAVES_FP64
-#define DIM2D_WAVES_FP64 3
+#define DIM2D_WAVES_FP64 2
 #endif
 #ifndef DIM2D_WAVES_MIXED
-#define DIM2D_WAVES_MIXED 4
+#define DIM2D_WAVES_MIXED 3
 #endif
-template <int BW, int MIXED, int STAGE>
-__global__ void __launch_bounds__(BW)
+template <int MIXED, int STAGE>
+__global__ void __launch_bounds__(kLanes)
 __attribute__((amdgpu_waves_per_eu((MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64) -
                                        (STAGE == 3 ? 1 : 0),
                                    8)))
 stage_kernel(const DevParams p, const StageArgs a, int rows) {
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
-  __shared__ Ring<RD, R, BW> sm;
   double dt = a.dt_fixed;
   if (STAGE == 1) {
     if (a.cfl_mode) dt = cfl_dt_from(a.ctl, a.parity);
@@ -614,7 +613,7 @@ stage_kernel(const DevParams p, const StageArgs a, int rows) {
   }
   RedAcc<false> red;
   red.init();
-  stage_tile<BW, RD, R, MIXED, STAGE, STAGE == 3>(p, a, dt, sm, blockIdx.x, blockIdx.y, rows, red);
+  stage_tile<RD, R, MIXED, STAGE, STAGE == 3>(p, a, dt, blockIdx.x, blockIdx.y, rows, red);
   if (STAGE == 3) {
     red.flush(a.ctl, a.parity ^ 1);
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctl->t += dt;
@@ -624,16 +623,13 @@ stage_kernel(const DevParams p, const StageArgs a, int rows) {
 // persistent batch kernel: one grid walks the tile list of all simulations;
 // each simulation has a fixed dt (a.S64/U/O pointers are batch bases, offset
 // by DevParams::offset); no reductions.
-template <int BW, int MIXED, int STAGE>
-__global__ void __launch_bounds__(BW)
+template <int MIXED, int STAGE>
+__global__ void __launch_bounds__(kLanes)
 __attribute__((amdgpu_waves_per_eu(MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64, 8)))
-stage_batch_kernel(const DevParams* __restrict__ P,
-                                                         const WorkItem* __restrict__ items,
-                                                         int nitems, const StageArgs a,
-                                                         const double* __restrict__ dts, int rows) {
+stage_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restrict__ items, int nitems,
+                   const StageArgs a, const double* __restrict__ dts, int rows) {
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
-  __shared__ Ring<RD, R, BW> sm;
   RedAcc<false> red;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem w = items[it];
@@ -644,8 +640,7 @@ stage_batch_kernel(const DevParams* __restrict__ P,
     b.S32 = a.S32 ? a.S32 + p.offset : nullptr;
     b.O64 = a.O64 ? a.O64 + p.offset : nullptr;
     b.O32 = a.O32 ? a.O32 + p.offset : nullptr;
-    stage_tile<BW, RD, R, MIXED, STAGE, false>(p, b, dts[w.sim], sm, w.bx, w.by, rows, red);
-    __syncthreads();
+    stage_tile<RD, R, MIXED, STAGE, false>(p, b, dts[w.sim], w.bx, w.by, rows, red);
   }
 }
 
@@ -934,46 +929,41 @@ __global__ void __launch_bounds__(256) reduce_kernel(const DevParams p, const do
 // ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
-template <int BW, int MIXED, int STAGE>
+template <int MIXED, int STAGE>
 static hipError_t launch_one(const DevParams& p, const StageArgs& a, int rows, hipStream_t s) {
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
-  dim3 grid((ni + BW - 2 * kBcSize - 1) / (BW - 2 * kBcSize), (nj + rows - 1) / rows);
-  hipLaunchKernelGGL((stage_kernel<BW, MIXED, STAGE>), grid, dim3(BW), 0, s, p, a, rows);
+  dim3 grid((ni + kOutCols - 1) / kOutCols, (nj + rows - 1) / rows);
+  hipLaunchKernelGGL((stage_kernel<MIXED, STAGE>), grid, dim3(kLanes), 0, s, p, a, rows);
   return hipGetLastError();
-}
-
-template <int BW>
-static hipError_t launch_bw(const DevParams& p, int mixed, int stage, const StageArgs& a, int rows,
-                            hipStream_t s) {
-  if (!mixed) {
-    switch (stage) {
-      case 0: return launch_one<BW, 0, 0>(p, a, rows, s);
-      case 1: return launch_one<BW, 0, 1>(p, a, rows, s);
-      case 2: return launch_one<BW, 0, 2>(p, a, rows, s);
-      default: return launch_one<BW, 0, 3>(p, a, rows, s);
-    }
-  }
-  switch (stage) {
-    case 0: return launch_one<BW, 1, 0>(p, a, rows, s);
-    case 1: return launch_one<BW, 1, 1>(p, a, rows, s);
-    case 2: return launch_one<BW, 1, 2>(p, a, rows, s);
-    default: return launch_one<BW, 1, 3>(p, a, rows, s);
-  }
 }
 
 hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArgs& a, int block_x,
                         int rows, hipStream_t s) {
-  if (block_x == 128) return launch_bw<128>(p, mixed, stage, a, rows, s);
-  return launch_bw<64>(p, mixed, stage, a, rows, s);
+  (void)block_x;
+  if (!mixed) {
+    switch (stage) {
+      case 0: return launch_one<0, 0>(p, a, rows, s);
+      case 1: return launch_one<0, 1>(p, a, rows, s);
+      case 2: return launch_one<0, 2>(p, a, rows, s);
+      default: return launch_one<0, 3>(p, a, rows, s);
+    }
+  }
+  switch (stage) {
+    case 0: return launch_one<1, 0>(p, a, rows, s);
+    case 1: return launch_one<1, 1>(p, a, rows, s);
+    case 2: return launch_one<1, 2>(p, a, rows, s);
+    default: return launch_one<1, 3>(p, a, rows, s);
+  }
 }
 
-template <int BW>
-static hipError_t launch_batch_bw(const DevParams* P, const WorkItem* items, int n, int mixed,
-                                  int stage, const StageArgs& a, int rows, int wg, hipStream_t s) {
-  const double* dts = a.dts;
+hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
+                              int mixed, int stage, const StageArgs& a, int block_x, int rows,
+                              int workgroups, hipStream_t s) {
+  (void)block_x;
+  const int wg = workgroups;
 #define DIM2D_BL(M, S)                                                                     \
-  hipLaunchKernelGGL((stage_batch_kernel<BW, M, S>), dim3(wg), dim3(BW), 0, s, P, items, n, a, \
-                     dts, rows)
+  hipLaunchKernelGGL((stage_batch_kernel<M, S>), dim3(wg), dim3(kLanes), 0, s, d_params,  \
+                     d_items, nitems, a, a.dts, rows)
   if (!mixed) {
     if (stage == 1) DIM2D_BL(0, 1);
     else if (stage == 2) DIM2D_BL(0, 2);
@@ -985,14 +975,6 @@ static hipError_t launch_batch_bw(const DevParams* P, const WorkItem* items, int
   }
 #undef DIM2D_BL
   return hipGetLastError();
-}
-
-hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
-                              int mixed, int stage, const StageArgs& a, int block_x, int rows,
-                              int workgroups, hipStream_t s) {
-  if (block_x == 128)
-    return launch_batch_bw<128>(d_params, d_items, nitems, mixed, stage, a, rows, workgroups, s);
-  return launch_batch_bw<64>(d_params, d_items, nitems, mixed, stage, a, rows, workgroups, s);
 }
 
 hipError_t launch_wall(const DevParams& p, int mixed_in, const double* U, const double* S64,
@@ -1044,12 +1026,10 @@ int max_resident_workgroups(int block_x) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   }
+  (void)block_x;
   int per = 0;
-  hipError_t e;
-  if (block_x == 128)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, stage_batch_kernel<128, 0, 2>, 128, 0);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, stage_batch_kernel<64, 0, 2>, 64, 0);
+  hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, stage_batch_kernel<0, 2>, kLanes, 0);
   if (e != hipSuccess || per <= 0) per = 4;
   return ncu * per;
 }

@@ -30,7 +30,7 @@ def test_fortran_driver_matches_oracle(orc, tmp_path, bc, nsync):
     nodes.tofile(fin)
     nt = 15
     r = subprocess.run([DRIVER, str(fin), str(fout), str(p.nx), str(p.ny), *map(str, bc), str(nt),
-                        repr(dt), "0", "0", str(nsync)], capture_output=True, text=True,
+                        f"{float(dt):.17g}", "0", "0", str(nsync)], capture_output=True, text=True,
                        timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
