@@ -77,7 +77,7 @@ DevParams make_dev_params(const dim2d_params& q) {
   p.nx = q.nx;
   p.ny = q.ny;
   p.pitch = (q.nx + 31) / 32 * 32;
-  p.plane = (long long)p.pitch * q.ny;
+  p.ncell = (long long)p.pitch * q.ny;
   p.bc_w = q.bc_w;
   p.bc_e = q.bc_e;
   p.bc_s = q.bc_s;
@@ -141,10 +141,10 @@ struct dim2d_ctx {
   double* B = nullptr;   // FP64 stage 2 output
   float* D1 = nullptr;   // mixed stage increments
   float* D2 = nullptr;
-  double* T = nullptr;   // time derivative scratch
+  double* T = nullptr;   // planar staging of uploads, downloads and time derivatives
   StepCtl* ctl = nullptr;
   int block_x = 64;
-  int rows = 64;
+  int rows = 16;
   bool uploaded = false;
   long long steps = 0;
   // kernel timing: event pairs around launches, tagged by kernel kind
@@ -209,7 +209,7 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   c->prm = *params;
   c->dp = make_dev_params(*params);
   c->mixed = params->precision == DIM2D_PREC_MIXED;
-  const size_t n = (size_t)c->dp.plane * 4;
+  const size_t n = (size_t)c->dp.ncell * 4;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->U, n * sizeof(double));
@@ -218,6 +218,7 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (e == hipSuccess && !c->mixed) e = hipMalloc(&c->B, n * sizeof(double));
   if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D1, n * sizeof(float));
   if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D2, n * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&c->T, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->ctl, sizeof(StepCtl));
   if (e == hipSuccess) e = hipMemsetAsync(c->U, 0, n * sizeof(double), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->U2, 0, n * sizeof(double), c->stream);
@@ -297,23 +298,29 @@ int dim2d_profile_read(dim2d_ctx* c, double out[8]) {
   return DIM2D_OK;
 }
 
+// host or device planar nodes(nx,ny,4) -> interleaved cells of U
 static int copy_in(dim2d_ctx* c, const double* src, hipMemcpyKind kind) {
   const DevParams& p = c->dp;
-  for (int k = 0; k < 4; ++k)
-    HIP_TRY(hipMemcpy2DAsync(c->U + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
-                             src + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
-                             (size_t)p.nx * sizeof(double), p.ny, kind, c->stream));
+  const double* planar = src;
+  if (kind == hipMemcpyHostToDevice) {
+    HIP_TRY(hipMemcpyAsync(c->T, src, (size_t)p.nx * p.ny * 4 * sizeof(double), kind, c->stream));
+    planar = c->T;
+  }
+  HIP_TRY(launch_pack(p, planar, c->U, c->stream));
   c->uploaded = true;
   return DIM2D_OK;
 }
 
+// U with its ghost cells stored -> host or device planar nodes(nx,ny,4)
 static int copy_out(dim2d_ctx* c, double* dst, hipMemcpyKind kind) {
   const DevParams& p = c->dp;
   HIP_TRY(launch_fill_ghosts(p, c->U, c->stream));
-  for (int k = 0; k < 4; ++k)
-    HIP_TRY(hipMemcpy2DAsync(dst + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
-                             c->U + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
-                             (size_t)p.nx * sizeof(double), p.ny, kind, c->stream));
+  if (kind == hipMemcpyDeviceToHost) {
+    HIP_TRY(launch_unpack(p, c->U, c->T, 0, 0.0, c->stream));
+    HIP_TRY(hipMemcpyAsync(dst, c->T, (size_t)p.nx * p.ny * 4 * sizeof(double), kind, c->stream));
+  } else {
+    HIP_TRY(launch_unpack(p, c->U, dst, 0, 0.0, c->stream));
+  }
   return DIM2D_OK;
 }
 
@@ -352,7 +359,7 @@ int dim2d_apply_bc(dim2d_ctx* c, double t) {
   HIP_TRY(launch_fill_ghosts(c->dp, c->U, c->stream));
   if (has_wall(c->dp)) {
     // bracket guess from the same state, as at initialisation (field_abstract_class.f:447-463)
-    const size_t n = (size_t)c->dp.plane * 4;
+    const size_t n = (size_t)c->dp.ncell * 4;
     HIP_TRY(hipMemcpyAsync(c->U2, c->U, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(launch_wall(c->dp, 0, nullptr, c->U, nullptr, c->U, nullptr, c->U2, c->stream));
   }
@@ -365,25 +372,18 @@ int dim2d_time_dev(dim2d_ctx* c, double t, double* time_dev) {
   if (!c || !time_dev) return fail(DIM2D_EINVAL, "NULL argument");
   if (!c->uploaded) return fail(DIM2D_ESTATE, "time_dev before upload");
   const DevParams& p = c->dp;
-  if (!c->T) HIP_TRY(hipMalloc(&c->T, (size_t)p.plane * 4 * sizeof(double)));
+  // U2 is free between steps: the derivative goes there, then to planar order
+  // with the ghost cells set to debug_real (td_operators_class.f:150-154)
   StageArgs a;
   memset(&a, 0, sizeof a);
   a.U = c->U;
   a.S64 = c->U;
-  a.O64 = c->T;
+  a.O64 = c->U2;
   HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
-  for (int k = 0; k < 4; ++k)
-    HIP_TRY(hipMemcpy2DAsync(time_dev + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
-                             c->T + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
-                             (size_t)p.nx * sizeof(double), p.ny, hipMemcpyDeviceToHost,
-                             c->stream));
+  HIP_TRY(launch_unpack(p, c->U2, c->T, 1, kDebugReal, c->stream));
+  HIP_TRY(hipMemcpyAsync(time_dev, c->T, (size_t)p.nx * p.ny * 4 * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  // ghost cells hold debug_real (td_operators_class.f:150-154)
-  for (int k = 0; k < 4; ++k)
-    for (int j = 0; j < p.ny; ++j)
-      for (int i = 0; i < p.nx; ++i)
-        if (i < kBcSize || i >= p.nx - kBcSize || j < kBcSize || j >= p.ny - kBcSize)
-          time_dev[((size_t)k * p.ny + j) * p.nx + i] = kDebugReal;
   return DIM2D_OK;
 }
 
@@ -539,13 +539,15 @@ struct dim2d_batch {
   std::vector<dim2d_params> prm;
   std::vector<DevParams> hp;
   DevParams* dp = nullptr;
-  size_t total = 0;  // elements per variable-set buffer (sum of 4*plane)
-  double *U = nullptr, *U2 = nullptr, *A = nullptr, *B = nullptr, *tmp = nullptr;
+  size_t total = 0;  // elements per variable-set buffer (sum of 4*ncell)
+  double *U = nullptr, *U2 = nullptr, *A = nullptr, *B = nullptr;
+  double* stage = nullptr;  // planar staging of one simulation (uploads, downloads)
   float *D1 = nullptr, *D2 = nullptr;
   double* d_dts = nullptr;
   std::vector<double> h_dts;
   WorkItem* d_items = nullptr;
   int nitems = 0;
+  size_t stage_n = 0;
   int block_x = 64, rows = 32, wg = 0;
   int wg_eff = 1;  // persistent workgroups actually launched
   bool any_wall = false;
@@ -653,8 +655,9 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
   for (int i = 0; i < n; ++i) {
     DevParams p = make_dev_params(params[i]);
     p.offset = (long long)off;
-    off += (size_t)p.plane * 4;
+    off += (size_t)p.ncell * 4;
     off = (off + 63) / 64 * 64;
+    b->stage_n = std::max(b->stage_n, (size_t)p.nx * p.ny * 4);
     b->any_wall |= has_wall(p);
     b->hp.push_back(p);
   }
@@ -672,6 +675,7 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
   if (e == hipSuccess && b->mixed) e = hipMalloc(&b->D1, off * sizeof(float));
   if (e == hipSuccess && b->mixed) e = hipMalloc(&b->D2, off * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&b->d_dts, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&b->stage, b->stage_n * sizeof(double));
   if (e == hipSuccess) e = hipMemset(b->U, 0, off * sizeof(double));
   if (e == hipSuccess) e = hipMemset(b->U2, 0, off * sizeof(double));
   if (e == hipSuccess && b->A) e = hipMemset(b->A, 0, off * sizeof(double));
@@ -705,6 +709,7 @@ int dim2d_batch_destroy(dim2d_batch* b) {
   (void)hipFree(b->D1);
   (void)hipFree(b->D2);
   (void)hipFree(b->d_dts);
+  (void)hipFree(b->stage);
   (void)hipFree(b->d_items);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -726,23 +731,21 @@ int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows, int32_t
 int dim2d_batch_upload(dim2d_batch* b, int32_t sim, const double* nodes) {
   if (!b || !nodes || sim < 0 || sim >= b->n) return fail(DIM2D_EINVAL, "invalid argument");
   const DevParams& p = b->hp[sim];
-  for (int k = 0; k < 4; ++k)
-    HIP_TRY(hipMemcpy2DAsync(b->U + p.offset + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
-                             nodes + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
-                             (size_t)p.nx * sizeof(double), p.ny, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->stage, nodes, (size_t)p.nx * p.ny * 4 * sizeof(double),
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(launch_pack(p, b->stage, b->U + p.offset, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return DIM2D_OK;
 }
 
 int dim2d_batch_download(dim2d_batch* b, int32_t sim, double* nodes) {
   if (!b || !nodes || sim < 0 || sim >= b->n) return fail(DIM2D_EINVAL, "invalid argument");
-  DevParams p = b->hp[sim];
+  const DevParams& p = b->hp[sim];
   double* base = b->U + p.offset;
   HIP_TRY(launch_fill_ghosts(p, base, b->stream));
-  for (int k = 0; k < 4; ++k)
-    HIP_TRY(hipMemcpy2DAsync(nodes + (size_t)k * p.nx * p.ny, (size_t)p.nx * sizeof(double),
-                             base + (size_t)k * p.plane, (size_t)p.pitch * sizeof(double),
-                             (size_t)p.nx * sizeof(double), p.ny, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(launch_unpack(p, base, b->stage, 0, 0.0, b->stream));
+  HIP_TRY(hipMemcpyAsync(nodes, b->stage, (size_t)p.nx * p.ny * 4 * sizeof(double),
+                         hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return DIM2D_OK;
 }

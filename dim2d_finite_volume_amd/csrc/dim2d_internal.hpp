@@ -1,10 +1,12 @@
 // dim2d_internal.hpp -- structures shared by the host library and the kernels.
 //
-// Field layout in HBM: the reference's nodes(nx,ny,ne) as four planes
-// (mass, momentum_x, momentum_y, total energy), each of ny rows of `pitch`
-// elements (pitch >= nx, padded to a multiple of 32 elements so every row
-// starts on a 256-byte boundary).  Element (i,j,k) of the reference (1-based)
-// sits at (k-1)*plane + (j-1)*pitch + (i-1).
+// Field layout in HBM: cells interleaved, each cell the four conservative
+// variables (mass, momentum_x, momentum_y, total energy) in 32 contiguous bytes
+// (16 for the FP32 increments of the mixed mode), ny rows of `pitch` cells
+// (pitch >= nx, a multiple of 32 cells).  Element (i,j,k) of the reference's
+// nodes(nx,ny,ne) (1-based) sits at ((j-1)*pitch + (i-1))*4 + (k-1).  A wave
+// reading one row of 64 cells reads 2 KiB contiguous bytes; the upload and
+// download calls convert from and to the reference's planar order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,7 +28,7 @@ constexpr double kDebugReal = 9.9692099683868690e+36;
 struct DevParams {
   int nx, ny, pitch, bc_w;
   int bc_e, bc_s, bc_n, gravity_ac;
-  long long plane;     // pitch * ny
+  long long ncell;     // pitch * ny cells (4 elements each)
   long long offset;    // element offset of this simulation inside batch buffers
   double dx, dy;
   double idx, idy;         // 1/dx, 1/dy
@@ -99,8 +101,12 @@ hipError_t launch_fill_ghosts(const DevParams& p, double* F, hipStream_t s);
 // reductions of an FP64 field into ctl slot `slot` (slot cleared first)
 hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int slot,
                          hipStream_t s);
-// set ghost cells of a time derivative to debug_real
-hipError_t launch_fill_frame(const DevParams& p, double* F, double value, hipStream_t s);
+// planar nodes(nx,ny,4) (the reference's order, contiguous) -> interleaved cells
+hipError_t launch_pack(const DevParams& p, const double* planar, double* F, hipStream_t s);
+// interleaved cells -> planar nodes(nx,ny,4); with fill_ghosts the ghost cells
+// receive `ghost_value` instead of the field's values
+hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, int fill_ghosts,
+                         double ghost_value, hipStream_t s);
 
 int max_resident_workgroups(int block_x);
 

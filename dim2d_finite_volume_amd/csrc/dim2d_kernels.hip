@@ -1,20 +1,23 @@
 // dim2d_kernels.hip -- gfx950 kernels of the DIM RK3-TVD hot path.
 //
-// One fused kernel per RK stage.  A workgroup owns a strip of BW columns (one
-// lane per column: BW-4 output columns plus a 2-column halo on each side) and
-// `rows` output rows; it marches up the strip one row per iteration.  Per row:
-//   * load the conservative variables of row r+2 (coalesced FP64 rows; ghost
-//     cells of reflection/periodic edges are resolved by index maps here, so no
-//     ghost-fill launch exists for them);
-//   * compute that row's primitives once per cell (1/rho, u, v, P, T, 1/(3-rho)
-//     and the transport products the fluxes need) and the capillary
-//     temperature of row r+1;
-//   * x-face flux at (i-1/2, r) and y-face flux at (i, r+1/2) from the lane's
-//     own column (registers) and its neighbours (a 4-row LDS ring);
+// One fused kernel per RK stage.  A workgroup is one wave of 64 lanes that owns
+// a strip of 64 columns (60 output columns plus a 2-column halo on each side,
+// one lane per column) and `rows` output rows, and marches up the strip one row
+// per iteration.  Per row:
+//   * issue the loads of the cells of row r+4 (two 16-byte loads per lane from
+//     the interleaved cell layout: 2 KiB contiguous per wave; ghost cells of
+//     reflection/periodic edges are resolved by index maps, so no ghost-fill
+//     launch exists for them) and of the RK inputs of row r+1;
+//   * compute the primitives of row r+1 once per cell (1/rho, u, v, P, T,
+//     1/(3-rho) and the transport products the fluxes need) and the capillary
+//     temperature of that row;
+//   * x-face flux at (i-1/2, r) and y-face flux at (i, r+1/2); the values of
+//     the neighbouring columns move between lanes by DPP wavefront shifts (no
+//     LDS, no barrier);
 //   * flux-difference divergence with the right x-face from the next lane and
 //     the lower y-face from the previous iteration, then the RK stage update.
-// One workgroup barrier per row.  Stage 3 also reduces max(|u|+c), min/max rho
-// and the conserved sums for the next CFL time step and the diagnostics.
+// Stage 3 also reduces max(|u|+c) and min/max rho of the new state for the
+// next CFL time step.
 //
 // Reference semantics followed here (jdesmarais/dim2D_finite_volume):
 //   fluxes  physical_models/dim2d/dim2d_fluxes_module.f:103-1649
@@ -60,25 +63,54 @@ __device__ __forceinline__ int map_row(const DevParams& p, int r, bool& flip) {
   return r;
 }
 
+// ------------------------------------------------------------------------
+// cell access: a cell is 4 consecutive FP64 (two 16-byte loads) or FP32 (one
+// 16-byte load) values; the row offset is uniform across the wave, the column
+// a 32-bit lane offset.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void ld_cell(const double* __restrict__ row, unsigned col, double v[4]) {
+  const double2* q = reinterpret_cast<const double2*>(row) + 2 * col;
+  const double2 a = q[0], b = q[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void ld_cell(const float* __restrict__ row, unsigned col, double v[4]) {
+  const float4 a = reinterpret_cast<const float4*>(row)[col];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+__device__ __forceinline__ void st_cell(double* __restrict__ row, unsigned col, const double v[4]) {
+  double2* q = reinterpret_cast<double2*>(row) + 2 * col;
+  q[0] = make_double2(v[0], v[1]);
+  q[1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void st_cell(float* __restrict__ row, unsigned col, const float v[4]) {
+  reinterpret_cast<float4*>(row)[col] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
 // state value of cell (cm, rm) (already mapped) with the reflection signs
 // (momentum normal to a reflecting edge changes sign, reflection_xy_module.f:50-110).
-// The row offset is uniform across the wave, the column a 32-bit lane offset.
 // SRC 0: FP64 field S64; 1: U alone; 2: U + S32
 template <int SRC>
 __device__ __forceinline__ void load_state(const DevParams& p, const double* __restrict__ U,
                                            const double* __restrict__ S64,
                                            const float* __restrict__ S32, int rm, unsigned cm,
                                            bool fx, bool fy, double v[4]) {
-  const long long rofs = (long long)rm * p.pitch;
+  const long long row = (long long)rm * p.pitch * 4;
+  if (SRC == 0) {
+    ld_cell(S64 + row, cm, v);
+  } else if (SRC == 1) {
+    ld_cell(U + row, cm, v);
+  } else {
+    double d[4];
+    ld_cell(U + row, cm, v);
+    ld_cell(S32 + row, cm, d);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const long long base = rofs + k * p.plane;
-    if (SRC == 0)
-      v[k] = (S64 + base)[cm];
-    else if (SRC == 1)
-      v[k] = (U + base)[cm];
-    else
-      v[k] = (U + base)[cm] + (double)(S32 + base)[cm];
+    for (int k = 0; k < 4; ++k) v[k] += d[k];
   }
   if (fx) v[1] = -v[1];
   if (fy) v[2] = -v[2];
@@ -148,25 +180,23 @@ __device__ __forceinline__ void make_cell(const DevParams& p, const double v[4],
 // the neighbouring columns live in the neighbouring lanes' registers and move
 // with DPP wavefront shifts (VALU register moves, no LDS, no barrier).
 // shr: lane i receives lane i-1; shl: lane i receives lane i+1; a lane
-// without a source keeps its own value (edge lanes never produce outputs).
+// without a source (lane 0 for shr, 63 for shl) receives 0.  Only lanes 0, 1,
+// 62 and 63 ever depend on those zeros, and they are halo lanes whose results
+// are never stored nor reduced.
 // ------------------------------------------------------------------------
 __device__ __forceinline__ int dpp_shr_i(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true);  // wave_shr:1, bound_ctrl
 }
 __device__ __forceinline__ int dpp_shl_i(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true);  // wave_shl:1, bound_ctrl
 }
 __device__ __forceinline__ float shr(float x) { return __int_as_float(dpp_shr_i(__float_as_int(x))); }
 __device__ __forceinline__ float shl(float x) { return __int_as_float(dpp_shl_i(__float_as_int(x))); }
 __device__ __forceinline__ double shr(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = dpp_shr_i((int)b), hi = dpp_shr_i((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  return __hiloint2double(dpp_shr_i(__double2hiint(x)), dpp_shr_i(__double2loint(x)));
 }
 __device__ __forceinline__ double shl(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = dpp_shl_i((int)b), hi = dpp_shl_i((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  return __hiloint2double(dpp_shl_i(__double2hiint(x)), dpp_shl_i(__double2loint(x)));
 }
 
 // density and velocities of a neighbouring cell
@@ -341,6 +371,11 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned x) {
   return x;
 }
 
+// relaxed device-scope read of a reduction word
+__device__ __forceinline__ unsigned long long peek(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // reductions of a state: the CFL quantities (max |u|+c, min/max rho, count of
 // non-finite cells) and, when SUMS, the conserved sums
 template <bool SUMS>
@@ -390,10 +425,17 @@ struct RedAcc {
 #pragma unroll
         for (int k = 0; k < 4; ++k) atomicAdd(&ctl->sum[slot][k], t[k]);
       }
-      atomicMax(&ctl->amax[slot], (unsigned long long)__double_as_longlong(am));
-      if (rn >= 0.0 && rn < INFINITY)
-        atomicMin(&ctl->rmin[slot], (unsigned long long)__double_as_longlong(rn));
-      atomicMax(&ctl->rmax[slot], (unsigned long long)__double_as_longlong(fmax(rx, 0.0)));
+      // The stored max (min) only grows (shrinks) while the kernel runs, so a
+      // wave whose value cannot change it skips the device-scope atomic: most
+      // waves do, which keeps thousands of waves off one cache line.
+      const unsigned long long ab = (unsigned long long)__double_as_longlong(am);
+      if (ab > peek(&ctl->amax[slot])) atomicMax(&ctl->amax[slot], ab);
+      if (rn >= 0.0 && rn < INFINITY) {
+        const unsigned long long nb = (unsigned long long)__double_as_longlong(rn);
+        if (nb < peek(&ctl->rmin[slot])) atomicMin(&ctl->rmin[slot], nb);
+      }
+      const unsigned long long xb = (unsigned long long)__double_as_longlong(fmax(rx, 0.0));
+      if (xb > peek(&ctl->rmax[slot])) atomicMax(&ctl->rmax[slot], xb);
       if (nf) atomicAdd(&ctl->nonfinite[slot], nf);
     }
   }
@@ -430,27 +472,42 @@ __device__ __forceinline__ double cfl_dt_from(const StepCtl* ctl, int slot) {
 // Mixed mode: state = U + S32 and out increments
 //   stage 1: S32' = dt L;  stage 2: S32' = 1/4 (S32 + dt L);
 //   stage 3: O64 = U + (1 - 1/3) (S32 + dt L)   (FP64 accumulation).
-// Per row r: the raw state of rows r+1 and r+2 was loaded one and two
-// iterations earlier; this iteration issues the loads of row r+3 and of the RK
-// inputs at (i, r), builds the cell of row r+1, fetches the neighbouring
-// columns' values by DPP, computes the x-face of row r and the y-face r+1/2,
-// and writes the updated cell (i, r).  The loop starts two rows early to fill
-// the pipeline (no outputs there), so every output row runs the same
-// instruction stream whatever the tiling.
+// Row pipeline: the loads of a row are issued four rows before it is needed
+// (its density two rows before), and the RK inputs of (i, r) one row before.
+// Per output row r the wave builds the cell of row r+1, fetches the
+// neighbouring columns' values by DPP, computes the x-face of row r and the
+// y-face r+1/2, and writes the updated cell (i, r).  The loop-carried values
+// live in rings of kRing slots indexed by compile-time constants; the loop body
+// is unrolled kRing times so that the ring rotation is register renaming (no
+// moves, and no wait for a load before its value is used).  The loop starts two
+// rows early to fill the pipeline (no outputs there), so every output row runs
+// the same instruction stream whatever the tiling.
+// FP64 stages take S64(i, r) from the raw row already in registers (an output
+// cell is interior, where no ghost map applies).
 constexpr int kLanes = 64;
 constexpr int kOutCols = kLanes - 2 * kBcSize;
+constexpr int kRing = 5;
+
+template <int K>
+struct Slot {
+  static constexpr int k = K;
+};
 
 template <typename RD, typename R, int MIXED, int STAGE, bool REDUCE>
 __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& a, double dt,
                                            int bx, int by, int rows, RedAcc<false>& red) {
   // mixed stages 2 and 3 read u^n + increment; mixed stage 0/1 read u^n alone
   constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
+  // RK inputs loaded apart from the stencil rows
+  constexpr bool LOAD_U = (!MIXED && STAGE >= 2) || (MIXED && STAGE == 3);
+  constexpr bool LOAD_S = MIXED && STAGE >= 2;
   const int lane = threadIdx.x & (kLanes - 1);
   const int c = bx * kOutCols + lane;  // column of this lane (0-based)
   const int r0 = kBcSize + by * rows;
   const int r1 = min(r0 + rows, p.ny - kBcSize);
   const bool out_col = lane >= kBcSize && lane < kLanes - kBcSize && c <= p.nx - kBcSize - 1;
   const int ccl = min(max(c, 0), p.nx - 1);
+  const unsigned uc = (unsigned)ccl;
 
   bool flx;
   const int cm = map_col(p, ccl, flx);
@@ -458,54 +515,70 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   const bool wall_e = (p.bc_e == kBcWall) && c == p.nx - kBcSize;
 
   auto load_raw = [&](int r, double v[4]) {
-    bool fly;
-    const int rm = map_row(p, min(max(r, 0), p.ny - 1), fly);
+    bool fly = false;
+    int rm = r;
+    if (r < kBcSize || r >= p.ny - kBcSize) rm = map_row(p, min(max(r, 0), p.ny - 1), fly);
     load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, flx, fly, v);
   };
+  // RK inputs of (i, r); r stays within [r0 - 1, r1] (inside the array)
+  auto load_rk = [&](int r, double u[4], double sv[4]) {
+    const long long row = (long long)r * p.pitch * 4;
+    if (LOAD_U) ld_cell(a.U + row, uc, u);
+    if (LOAD_S) ld_cell(a.S32 + row, uc, sv);
+  };
 
-  double raw1[4], raw2[4], raw3[4];
-  Cell<RD, R> cm1, c0, c1;
-  XVals<R> x0, x1;
-  Nb<RD, R> cm1W, c0W, c0E;  // west/east neighbours of rows r-1 and r
-  {
-    double t[4];
-    load_raw(r0 - 2, t);
-    make_cell<RD, R>(p, t, c0, x0);
-    c0.Tcap = (R)0;
-    cm1 = c0;
-    c0W = nb_shr(c0);
-    c0E = nb_shl(c0);
-    cm1W = c0W;
-    load_raw(r0 - 1, raw1);
-    load_raw(r0, raw2);
+  // rings: slot (q - (r0 - 2)) % kRing holds row q
+  double raw[kRing][4];
+  Cell<RD, R> cell[kRing];
+  XVals<R> xv[kRing];
+  Nb<RD, R> wst[kRing], est[kRing];
+  R fy[kRing][4];
+  double rkU[kRing][4], rkS[kRing][4];
+
+  load_raw(r0 - 2, raw[0]);
+  load_raw(r0 - 1, raw[1]);
+  load_raw(r0, raw[2]);
+  load_raw(r0 + 1, raw[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    fy[kRing - 1][k] = (R)0;
+    rkU[0][k] = 0.0;
+    rkS[0][k] = 0.0;
   }
-  R fy_prev[4] = {(R)0, (R)0, (R)0, (R)0};
+  make_cell<RD, R>(p, raw[0], cell[0], xv[0]);
+  cell[0].Tcap = (R)0;
+  cell[kRing - 1] = cell[0];
+  wst[0] = nb_shr(cell[0]);
+  est[0] = nb_shl(cell[0]);
+  wst[kRing - 1] = wst[0];
+
   // fluxes are doubled (face sums), hence the 1/2
   const R hidx = (R)p.half_idx, hidy = (R)p.half_idy;
 
-  for (int r = r0 - 2; r < r1; ++r) {
-    // prefetch: row r+3 and the RK inputs of (i, r)
-    load_raw(r + 3, raw3);
-    const long long ro = (long long)r * p.pitch;  // uniform row offset
-    const unsigned uc = (unsigned)ccl;
-    double rkU[4], rkS[4];
-    if (STAGE >= 1 && r >= r0) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const long long b = ro + k * p.plane;
-        if (!MIXED) {
-          rkS[k] = (a.S64 + b)[uc];
-          rkU[k] = (STAGE >= 2) ? (a.U + b)[uc] : 0.0;
-        } else {
-          rkS[k] = (STAGE >= 2) ? (double)(a.S32 + b)[uc] : 0.0;
-          rkU[k] = (STAGE == 3) ? (a.U + b)[uc] : 0.0;
-        }
-      }
-    }
+  auto row = [&](auto slot, int r) __attribute__((always_inline)) {
+    constexpr int K = decltype(slot)::k;
+    constexpr int KM = (K + kRing - 1) % kRing;  // row r-1
+    constexpr int K1 = (K + 1) % kRing;          // row r+1
+    constexpr int K2 = (K + 2) % kRing;          // row r+2
+    constexpr int K4 = (K + 4) % kRing;          // row r+4
+    // prefetch: stencil row r+4 and the RK inputs of (i, r+1)
+    load_raw(r + 4, raw[K4]);
+    if (LOAD_U || LOAD_S) load_rk(r + 1, rkU[K1], rkS[K1]);
+
+    Cell<RD, R>& cm1 = cell[KM];
+    Cell<RD, R>& c0 = cell[K];
+    Cell<RD, R>& c1 = cell[K1];
+    const XVals<R>& x0 = xv[K];
     // cell of row r+1 and its neighbours
-    make_cell<RD, R>(p, raw1, c1, x1);
-    const Nb<RD, R> c1W = nb_shr(c1), c1E = nb_shl(c1);
-    c1.Tcap = cap_temperature<RD, R>(p, c1.irho, c1W.rho, c1E.rho, c0.rho, (RD)raw2[0]);
+    make_cell<RD, R>(p, raw[K1], c1, xv[K1]);
+    wst[K1] = nb_shr(c1);
+    est[K1] = nb_shl(c1);
+    const Nb<RD, R>& c1W = wst[K1];
+    const Nb<RD, R>& c1E = est[K1];
+    const Nb<RD, R>& c0W = wst[K];
+    const Nb<RD, R>& c0E = est[K];
+    const Nb<RD, R>& cm1W = wst[KM];
+    c1.Tcap = cap_temperature<RD, R>(p, c1.irho, c1W.rho, c1E.rho, c0.rho, (RD)raw[K2][0]);
 
     // the west cell of row r for the x-face
     Cell<RD, R> Lc;
@@ -520,11 +593,13 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
     Lx.uw = shr(x0.uw);
     const RD rhoLL = shr(c0W.rho);
 
-    R F[4], G[4];
+    R F[4];
+    R* G = fy[K];
+    const R* fy_prev = fy[KM];
     x_face<RD, R>(p, c0W, Lc, Lx, rhoLL, c0E.rho, cm1W, c1W, c0, x0,
                   Nb<RD, R>{cm1.rho, cm1.u, cm1.v}, Nb<RD, R>{c1.rho, c1.u, c1.v}, wall_w,
                   wall_e, F);
-    y_face<RD, R>(p, c0, c1, c0W, c0E, c1W, c1E, cm1.rho, (RD)raw2[0],
+    y_face<RD, R>(p, c0, c1, c0W, c0E, c1W, c1E, cm1.rho, (RD)raw[K2][0],
                   p.bc_s == kBcWall && r == kBcSize - 1,
                   p.bc_n == kBcWall && r == p.ny - kBcSize - 1, G);
     R L[4];
@@ -536,62 +611,64 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
       L[3] -= (R)((double)c0.qy * p.gravity);
     }
 
+    const long long row = (long long)r * p.pitch * 4;  // uniform row offset
     if (out_col && r >= r0) {
+      const double* S = raw[K];  // S64(i, r) in FP64 mode
       double vout[4];
       if (STAGE == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) (a.O64 + ro + k * p.plane)[uc] = (double)L[k];
+        for (int k = 0; k < 4; ++k) vout[k] = (double)L[k];
+        st_cell(a.O64 + row, uc, vout);
       } else if (!MIXED) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const double inc = rkS[k] + dt * (double)L[k];
-          double val;
-          if (STAGE == 1) val = inc;
-          else if (STAGE == 2) val = 0.75 * rkU[k] + (1.0 - 0.75) * inc;
-          else val = (1.0 / 3.0) * rkU[k] + (1.0 - 1.0 / 3.0) * inc;
-          (a.O64 + ro + k * p.plane)[uc] = val;
-          vout[k] = val;
+          const double inc = S[k] + dt * (double)L[k];
+          if (STAGE == 1) vout[k] = inc;
+          else if (STAGE == 2) vout[k] = 0.75 * rkU[K][k] + (1.0 - 0.75) * inc;
+          else vout[k] = (1.0 / 3.0) * rkU[K][k] + (1.0 - 1.0 / 3.0) * inc;
         }
-      } else {
+        st_cell(a.O64 + row, uc, vout);
+      } else if (STAGE == 1 || STAGE == 2) {
+        float o[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const double dl = dt * (double)L[k];
-          if (STAGE == 1) {
-            (a.O32 + ro + k * p.plane)[uc] = (float)dl;
-          } else if (STAGE == 2) {
-            (a.O32 + ro + k * p.plane)[uc] = (float)(0.25 * (rkS[k] + dl));
-          } else {
-            const double val = rkU[k] + (1.0 - 1.0 / 3.0) * (rkS[k] + dl);
-            (a.O64 + ro + k * p.plane)[uc] = val;
-            vout[k] = val;
-          }
+          o[k] = STAGE == 1 ? (float)dl : (float)(0.25 * (rkS[K][k] + dl));
         }
+        st_cell(a.O32 + row, uc, o);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          vout[k] = rkU[K][k] + (1.0 - 1.0 / 3.0) * (rkS[K][k] + dt * (double)L[k]);
+        st_cell(a.O64 + row, uc, vout);
       }
       if (REDUCE && STAGE == 3) red.add(p, vout);
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      fy_prev[k] = G[k];
-      raw1[k] = raw2[k];
-      raw2[k] = raw3[k];
-    }
-    cm1 = c0;
-    c0 = c1;
-    x0 = x1;
-    cm1W = c0W;
-    c0W = c1W;
-    c0E = c1E;
+  };
+
+  int r = r0 - 2;
+  for (;;) {
+    row(Slot<0>{}, r);
+    if (++r >= r1) break;
+    row(Slot<1>{}, r);
+    if (++r >= r1) break;
+    row(Slot<2>{}, r);
+    if (++r >= r1) break;
+    row(Slot<3>{}, r);
+    if (++r >= r1) break;
+    row(Slot<4>{}, r);
+    if (++r >= r1) break;
   }
 }
 
-// minimum waves per SIMD the register allocation must allow (FP64 stages 2,
-// i.e. up to 256 VGPRs; mixed stages 3, up to 168 VGPRs; stage 3 with the CFL
-// reduction one fewer): tighter bounds make the allocator spill to scratch.
+// minimum waves per SIMD the register allocation must allow: 2, i.e. up to 256
+// VGPRs (stage 3 with the CFL reduction one fewer).  Tighter bounds make the
+// allocator spill to scratch; the 5-row pipeline needs about 200-230 VGPRs.
 #ifndef DIM2D_WAVES_FP64
 #define DIM2D_WAVES_FP64 2
 #endif
 #ifndef DIM2D_WAVES_MIXED
-#define DIM2D_WAVES_MIXED 3
+#define DIM2D_WAVES_MIXED 2
 #endif
 template <int MIXED, int STAGE>
 __global__ void __launch_bounds__(kLanes)
@@ -740,16 +817,16 @@ __device__ __forceinline__ void wall_load(const DevParams& p, const double* U, c
                                           const float* S32, int mixed, int c, int r, double v[4]) {
   bool fx, fy;
   const int cm = map_col(p, c, fx), rm = map_row(p, r, fy);
-  const long long idx = (long long)rm * p.pitch + cm;
+  const long long idx = ((long long)rm * p.pitch + cm) * 4;
   for (int k = 0; k < 4; ++k)
-    v[k] = mixed ? U[k * p.plane + idx] + (double)S32[k * p.plane + idx] : S64[k * p.plane + idx];
+    v[k] = mixed ? U[idx + k] + (double)S32[idx + k] : S64[idx + k];
   if (fx) v[1] = -v[1];
   if (fy) v[2] = -v[2];
 }
 __device__ __forceinline__ double rho_at(const DevParams& p, const double* F, int c, int r) {
   bool fx, fy;
   const int cm = map_col(p, c, fx), rm = map_row(p, r, fy);
-  return F[(long long)rm * p.pitch + cm];
+  return F[((long long)rm * p.pitch + cm) * 4];
 }
 
 // temperature_eff (:652-691) at (c, r) of the FP64 field F (for the bracket guess)
@@ -805,7 +882,7 @@ __device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const 
   w.ys1 = vy1[0];
   // compute_wall_xy_guess from u^n (bc_operators_wall_xy_class.f:898-930)
   const double T_guess = p.c38cv * temperature_eff_at(p, tmp, c1, r1);
-  const double md_guess = tmp[(long long)r * p.pitch + c];
+  const double md_guess = tmp[((long long)r * p.pitch + c) * 4];
   const double rg = wall_ghost_density(p, w, T_guess, md_guess);
   // sign_normal = +1, sign_trans = -1 (wall_xy_equilibrium_module.f:136-140)
   double g[4], go[4];
@@ -824,14 +901,14 @@ __device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const 
   go[1] = g[1];
   go[2] = g[2];
   go[3] = v1[3];
-  const long long ig = (long long)r * p.pitch + c, io = (long long)ro * p.pitch + co;
+  const long long ig = ((long long)r * p.pitch + c) * 4, io = ((long long)ro * p.pitch + co) * 4;
   for (int k = 0; k < 4; ++k) {
     if (O32) {
-      O32[k * p.plane + ig] = (float)(g[k] - U[k * p.plane + ig]);
-      O32[k * p.plane + io] = (float)(go[k] - U[k * p.plane + io]);
+      O32[ig + k] = (float)(g[k] - U[ig + k]);
+      O32[io + k] = (float)(go[k] - U[io + k]);
     } else {
-      O64[k * p.plane + ig] = g[k];
-      O64[k * p.plane + io] = go[k];
+      O64[ig + k] = g[k];
+      O64[io + k] = go[k];
     }
   }
 }
@@ -899,11 +976,11 @@ __global__ void fill_ghosts_kernel(const DevParams p, double* F) {
   bool fx, fy;
   const int cm = map_col(p, c, fx), rm = map_row(p, r, fy);
   if (cm == c && rm == r) return;  // stored (wall) ghost
-  const long long src = (long long)rm * p.pitch + cm, dst = (long long)r * p.pitch + c;
+  const long long src = ((long long)rm * p.pitch + cm) * 4, dst = ((long long)r * p.pitch + c) * 4;
   for (int k = 0; k < 4; ++k) {
-    double v = F[k * p.plane + src];
+    double v = F[src + k];
     if ((k == 1 && fx) || (k == 2 && fy)) v = -v;
-    F[k * p.plane + dst] = v;
+    F[dst + k] = v;
   }
 }
 
@@ -918,12 +995,41 @@ __global__ void __launch_bounds__(256) reduce_kernel(const DevParams p, const do
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     const int i = kBcSize + (int)(t % ni), j = kBcSize + (int)(t / ni);
-    const long long o = (long long)j * p.pitch + i;
     double v[4];
-    for (int k = 0; k < 4; ++k) v[k] = F[k * p.plane + o];
+    ld_cell(F + (long long)j * p.pitch * 4, (unsigned)i, v);
     red.add(p, v);
   }
   red.flush(ctl, slot);
+}
+
+// planar nodes(nx,ny,4) <-> interleaved cells, one thread per cell
+__global__ void pack_kernel(const DevParams p, const double* __restrict__ planar,
+                            double* __restrict__ F) {
+  const long long n = (long long)p.nx * p.ny;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % p.nx), j = (int)(t / p.nx);
+  double v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = planar[k * n + t];
+  st_cell(F + (long long)j * p.pitch * 4, (unsigned)i, v);
+}
+
+__global__ void unpack_kernel(const DevParams p, const double* __restrict__ F,
+                              double* __restrict__ planar, int fill_ghosts, double ghost_value) {
+  const long long n = (long long)p.nx * p.ny;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % p.nx), j = (int)(t / p.nx);
+  double v[4];
+  ld_cell(F + (long long)j * p.pitch * 4, (unsigned)i, v);
+  if (fill_ghosts &&
+      (i < kBcSize || i >= p.nx - kBcSize || j < kBcSize || j >= p.ny - kBcSize)) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ghost_value;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) planar[k * n + t] = v[k];
 }
 
 // ------------------------------------------------------------------------
@@ -1007,6 +1113,20 @@ hipError_t launch_wall_batch(const DevParams* h_params, const DevParams* d_param
 hipError_t launch_fill_ghosts(const DevParams& p, double* F, hipStream_t s) {
   const int n = 2 * kBcSize * p.nx + 2 * kBcSize * (p.ny - 2 * kBcSize);
   hipLaunchKernelGGL(fill_ghosts_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(const DevParams& p, const double* planar, double* F, hipStream_t s) {
+  const long long n = (long long)p.nx * p.ny;
+  hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, planar, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, int fill_ghosts,
+                         double ghost_value, hipStream_t s) {
+  const long long n = (long long)p.nx * p.ny;
+  hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, F,
+                     planar, fill_ghosts, ghost_value);
   return hipGetLastError();
 }
 
