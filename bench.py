@@ -118,7 +118,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a plain device copy of 2 x 1 GiB (for HBM counter scaling)")
+                    help="also run an elementwise kernel reading and writing 1 GiB "
+                         "(for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -134,11 +135,13 @@ def main():
     torch.cuda.init()
     dev = torch.cuda.get_device_name(0)
     if args.calibrate_copy:
-        # a copy kernel of known size: reads 1 GiB and writes 1 GiB per launch
+        # a kernel of known traffic for HBM counter scaling: an elementwise
+        # product that reads 1 GiB and writes 1 GiB per launch with 16-byte
+        # vector accesses, like the stage kernels
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     n = args.n
@@ -165,9 +168,10 @@ def main():
     si = ("stage1", "stage2", "stage3").index(dom)
     alg = ALG_BYTES["f64"][si] * cells
     achieved = alg / (kt[dom][0] * 1e-3)
-    roof = {"bound": "hbm", "kernel": f"stage_kernel<64,FP64,{si + 1}>", "achieved": achieved / 1e9,
+    roof = {"bound": "hbm", "kernel": f"stage_kernel<0, {si + 1}> (FP64 stage {si + 1})",
+            "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-            "traffic": read_traffic(f"stage{si + 1}_fp64_two_bubbles_4096_traffic.json"),
+            "traffic": (read_traffic(f"stage{si + 1}_fp64_two_bubbles_{n}_traffic.json")),
             "algorithmic_bytes_per_launch": alg, "mean_launch_ms": kt[dom][0]}
     f.close()
     del f

@@ -94,23 +94,26 @@ __device__ __forceinline__ void st_cell(float* __restrict__ row, unsigned col, c
 
 // state value of cell (cm, rm) (already mapped) with the reflection signs
 // (momentum normal to a reflecting edge changes sign, reflection_xy_module.f:50-110).
-// SRC 0: FP64 field S64; 1: U alone; 2: U + S32
+// SRC 0: FP64 field S64; 1: U alone; 2: U + S32, the increment kept in `inc`
 template <int SRC>
 __device__ __forceinline__ void load_state(const DevParams& p, const double* __restrict__ U,
                                            const double* __restrict__ S64,
                                            const float* __restrict__ S32, int rm, unsigned cm,
-                                           bool fx, bool fy, double v[4]) {
+                                           bool fx, bool fy, double v[4], float inc[4]) {
   const long long row = (long long)rm * p.pitch * 4;
   if (SRC == 0) {
     ld_cell(S64 + row, cm, v);
   } else if (SRC == 1) {
     ld_cell(U + row, cm, v);
   } else {
-    double d[4];
+    const float4 d = reinterpret_cast<const float4*>(S32 + row)[cm];
     ld_cell(U + row, cm, v);
-    ld_cell(S32 + row, cm, d);
+    inc[0] = d.x;
+    inc[1] = d.y;
+    inc[2] = d.z;
+    inc[3] = d.w;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] += d[k];
+    for (int k = 0; k < 4; ++k) v[k] += (double)inc[k];
   }
   if (fx) v[1] = -v[1];
   if (fy) v[2] = -v[2];
@@ -498,9 +501,9 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
                                            int bx, int by, int rows, RedAcc<false>& red) {
   // mixed stages 2 and 3 read u^n + increment; mixed stage 0/1 read u^n alone
   constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
-  // RK inputs loaded apart from the stencil rows
-  constexpr bool LOAD_U = (!MIXED && STAGE >= 2) || (MIXED && STAGE == 3);
-  constexpr bool LOAD_S = MIXED && STAGE >= 2;
+  // u^n is loaded apart from the stencil rows in FP64 stages 2 and 3; every
+  // other RK input is the stencil row itself (S64, or U + S32 with S32 kept)
+  constexpr bool LOAD_U = !MIXED && STAGE >= 2;
   const int lane = threadIdx.x & (kLanes - 1);
   const int c = bx * kOutCols + lane;  // column of this lane (0-based)
   const int r0 = kBcSize + by * rows;
@@ -514,18 +517,14 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   const bool wall_w = (p.bc_w == kBcWall) && c == kBcSize;
   const bool wall_e = (p.bc_e == kBcWall) && c == p.nx - kBcSize;
 
-  auto load_raw = [&](int r, double v[4]) {
+  auto load_raw = [&](int r, double v[4], float inc[4]) {
     bool fly = false;
     int rm = r;
     if (r < kBcSize || r >= p.ny - kBcSize) rm = map_row(p, min(max(r, 0), p.ny - 1), fly);
-    load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, flx, fly, v);
+    load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, flx, fly, v, inc);
   };
-  // RK inputs of (i, r); r stays within [r0 - 1, r1] (inside the array)
-  auto load_rk = [&](int r, double u[4], double sv[4]) {
-    const long long row = (long long)r * p.pitch * 4;
-    if (LOAD_U) ld_cell(a.U + row, uc, u);
-    if (LOAD_S) ld_cell(a.S32 + row, uc, sv);
-  };
+  // u^n at (i, r); r stays within [r0 - 1, r1 - 1] (inside the array)
+  auto load_u = [&](int r, double u[4]) { ld_cell(a.U + (long long)r * p.pitch * 4, uc, u); };
 
   // rings: slot (q - (r0 - 2)) % kRing holds row q
   double raw[kRing][4];
@@ -533,17 +532,17 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   XVals<R> xv[kRing];
   Nb<RD, R> wst[kRing], est[kRing];
   R fy[kRing][4];
-  double rkU[kRing][4], rkS[kRing][4];
+  float inc[kRing][4];  // S32 of the raw rows (mixed stages 2 and 3)
+  double rkU[kRing][4];
 
-  load_raw(r0 - 2, raw[0]);
-  load_raw(r0 - 1, raw[1]);
-  load_raw(r0, raw[2]);
-  load_raw(r0 + 1, raw[3]);
+  load_raw(r0 - 2, raw[0], inc[0]);
+  load_raw(r0 - 1, raw[1], inc[1]);
+  load_raw(r0, raw[2], inc[2]);
+  load_raw(r0 + 1, raw[3], inc[3]);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     fy[kRing - 1][k] = (R)0;
     rkU[0][k] = 0.0;
-    rkS[0][k] = 0.0;
   }
   make_cell<RD, R>(p, raw[0], cell[0], xv[0]);
   cell[0].Tcap = (R)0;
@@ -561,9 +560,11 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
     constexpr int K1 = (K + 1) % kRing;          // row r+1
     constexpr int K2 = (K + 2) % kRing;          // row r+2
     constexpr int K4 = (K + 4) % kRing;          // row r+4
-    // prefetch: stencil row r+4 and the RK inputs of (i, r+1)
-    load_raw(r + 4, raw[K4]);
-    if (LOAD_U || LOAD_S) load_rk(r + 1, rkU[K1], rkS[K1]);
+    // prefetch: stencil row r+4 and u^n at (i, r+1).  Rows past the last one
+    // the tile reads (r1 + 1, resp. r1 - 1) are clamped to it: the repeated
+    // load hits in cache and its value is never used.
+    load_raw(min(r + 4, r1 + 1), raw[K4], inc[K4]);
+    if (LOAD_U) load_u(min(r + 1, r1 - 1), rkU[K1]);
 
     Cell<RD, R>& cm1 = cell[KM];
     Cell<RD, R>& c0 = cell[K];
@@ -613,7 +614,7 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
 
     const long long row = (long long)r * p.pitch * 4;  // uniform row offset
     if (out_col && r >= r0) {
-      const double* S = raw[K];  // S64(i, r) in FP64 mode
+      const double* S = raw[K];  // the stencil row at (i, r): S64, or u^n + S32
       double vout[4];
       if (STAGE == 0) {
 #pragma unroll
@@ -633,13 +634,15 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const double dl = dt * (double)L[k];
-          o[k] = STAGE == 1 ? (float)dl : (float)(0.25 * (rkS[K][k] + dl));
+          o[k] = STAGE == 1 ? (float)dl : (float)(0.25 * ((double)inc[K][k] + dl));
         }
         st_cell(a.O32 + row, uc, o);
       } else {
+        // u^n + 2/3 (S32 + dt L) = (u^n + S32) + (2/3 dt L - 1/3 S32), from the
+        // stencil row (u^n + S32 rounded once) and its kept increment
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          vout[k] = rkU[K][k] + (1.0 - 1.0 / 3.0) * (rkS[K][k] + dt * (double)L[k]);
+          vout[k] = S[k] + ((2.0 / 3.0) * (dt * (double)L[k]) - (1.0 / 3.0) * (double)inc[K][k]);
         st_cell(a.O64 + row, uc, vout);
       }
       if (REDUCE && STAGE == 3) red.add(p, vout);

@@ -1,0 +1,127 @@
+"""Summaries of rocprofv3 CSV output for the stage kernels.
+
+Two reports, each written as a short Markdown file (and, for traffic, one JSON
+file per FP64 stage kernel that bench.py reads for the roofline's `traffic`):
+
+  stats:    per-kernel call count and durations from `--kernel-trace --stats`
+            (<prefix>_kernel_stats.csv)
+  traffic:  HBM bytes per dispatch from two `--pmc` runs (FETCH_SIZE, then
+            WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
+            reads for a calibration kernel of known size in the same run
+
+    python -m dim2d_finite_volume_amd.rocprof_summary stats STATS.csv OUT.md --command "..."
+    python -m dim2d_finite_volume_amd.rocprof_summary traffic FETCH.csv WRITE.csv OUTDIR \
+        --calib SUBSTRING:READ_BYTES:WRITE_BYTES --workload two_bubbles_4096 --command "..."
+"""
+import argparse
+import csv
+import json
+import os
+import re
+from collections import defaultdict
+
+STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
+
+
+def short_name(name):
+    m = STAGE_RE.search(name)
+    if m:
+        return f"stage_kernel<{m.group(1)}, {m.group(2)}> ({'mixed' if m.group(1) == '1' else 'FP64'} stage {m.group(2)})"
+    name = name.replace("void ", "")
+    return name.split("(")[0][:90]
+
+
+def stats(args):
+    rows = []
+    with open(args.csv) as f:
+        for r in csv.DictReader(f):
+            rows.append(r)
+    lines = ["# Kernel durations (rocprofv3 --kernel-trace --stats)", "",
+             f"Command: `{args.command}`", "",
+             "| kernel | calls | mean us | min us | max us | % of GPU time |",
+             "|---|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        if args.all or "dim2d::" in r["Name"]:
+            lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | "
+                         f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                         f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def per_dispatch(path, counter):
+    """mean counter value (KiB) per dispatch, per kernel name"""
+    acc = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def traffic(args):
+    fetch = per_dispatch(args.fetch, "FETCH_SIZE")
+    write = per_dispatch(args.write, "WRITE_SIZE")
+    sub, rb, wb = args.calib.split(":")
+    rb, wb = float(rb), float(wb)
+
+    def calib(d, nbytes):
+        cands = [(k, v) for k, v in d.items() if sub in k]
+        if not cands or nbytes <= 0:
+            return None, None
+        k, (kib, _) = max(cands, key=lambda kv: kv[1][0])
+        return nbytes / (kib * 1024.0), kib
+
+    fr, fk = calib(fetch, rb)
+    fw, wk = calib(write, wb)
+    lines = ["# HBM traffic per dispatch (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE)", "",
+             f"Command: `{args.command}`", "",
+             f"Calibration kernel: name contains `{sub}`, reads {rb:.0f} B and writes {wb:.0f} B "
+             f"per dispatch; the counters read {fk} KiB and {wk} KiB, so FETCH_SIZE is scaled by "
+             f"{fr:.4f} and WRITE_SIZE by {fw:.4f}.", "",
+             "| kernel | dispatches | FETCH KiB | WRITE KiB | scaled HBM bytes / dispatch |",
+             "|---|---:|---:|---:|---:|"]
+    os.makedirs(args.outdir, exist_ok=True)
+    for name in sorted(set(fetch) | set(write)):
+        if "dim2d::" not in name:
+            continue
+        f_kib, n = fetch.get(name, (0.0, 0))
+        w_kib, _ = write.get(name, (0.0, 0))
+        total = f_kib * 1024.0 * (fr or 1.0) + w_kib * 1024.0 * (fw or 1.0)
+        lines.append(f"| {short_name(name)} | {n} | {f_kib:.1f} | {w_kib:.1f} | {total:.4e} |")
+        m = STAGE_RE.search(name)
+        if m and m.group(1) == "0":
+            js = {"kernel": short_name(name), "workload": args.workload,
+                  "fetch_kib_per_dispatch": f_kib, "write_kib_per_dispatch": w_kib,
+                  "fetch_scale": fr, "write_scale": fw, "hbm_bytes_per_launch": total,
+                  "command": args.command}
+            with open(os.path.join(args.outdir, f"stage{m.group(2)}_fp64_{args.workload}_traffic.json"),
+                      "w") as f:
+                json.dump(js, f, indent=1)
+    with open(os.path.join(args.outdir, f"stage_kernels_{args.workload}_traffic.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sp = ap.add_subparsers(dest="mode", required=True)
+    a = sp.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("out")
+    a.add_argument("--command", required=True)
+    a.add_argument("--all", action="store_true", help="list every kernel, not only dim2d's")
+    b = sp.add_parser("traffic")
+    b.add_argument("fetch")
+    b.add_argument("write")
+    b.add_argument("outdir")
+    b.add_argument("--calib", required=True)
+    b.add_argument("--workload", required=True)
+    b.add_argument("--command", required=True)
+    args = ap.parse_args()
+    stats(args) if args.mode == "stats" else traffic(args)
+
+
+if __name__ == "__main__":
+    main()
