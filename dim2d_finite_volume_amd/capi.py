@@ -17,6 +17,7 @@ OK = 0
 EINVAL, EDEVICE, ENOMEM, ESTATE, ENONFINITE = -1, -2, -3, -4, -5
 BC_REFLECTION, BC_PERIODIC, BC_WALL = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+INT_RK3TVD, INT_LSRK3 = 0, 1
 
 # every symbol declared in include/dim2d_hip.h
 EXPORTS = (
@@ -40,7 +41,7 @@ class Params(ctypes.Structure):
         ("y_max", ctypes.c_double), ("re", ctypes.c_double), ("we", ctypes.c_double),
         ("pr", ctypes.c_double), ("viscous_r", ctypes.c_double), ("cv_r", ctypes.c_double),
         ("gravity", ctypes.c_double), ("wall_contact_angle", ctypes.c_double),
-        ("wall_heat_flux", ctypes.c_double),
+        ("wall_heat_flux", ctypes.c_double), ("integrator", ctypes.c_int32),
     ]
 
 

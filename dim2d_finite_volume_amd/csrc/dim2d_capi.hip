@@ -63,6 +63,10 @@ int validate(const dim2d_params* p) {
                 "library_config.py");
   if (p->precision != DIM2D_PREC_FP64 && p->precision != DIM2D_PREC_MIXED)
     return fail(DIM2D_EINVAL, "unknown precision mode %d", p->precision);
+  if (p->integrator != DIM2D_INT_RK3TVD && p->integrator != DIM2D_INT_LSRK3)
+    return fail(DIM2D_EINVAL, "unknown integrator %d", p->integrator);
+  if (p->integrator == DIM2D_INT_LSRK3 && p->precision != DIM2D_PREC_FP64)
+    return fail(DIM2D_EINVAL, "the low-storage RK3 runs in FP64 only");
   if (!(p->x_max > p->x_min) || !(p->y_max > p->y_min))
     return fail(DIM2D_EINVAL, "empty coordinate range");
   if (!(p->re > 0) || !(p->we > 0) || !(p->pr > 0) || !(p->cv_r > 0))
@@ -134,11 +138,13 @@ struct dim2d_ctx {
   DevParams dp;
   int device = 0;
   int mixed = 0;
+  int lsrk = 0;          // low-storage RK3 instead of RK3-TVD
   hipStream_t stream = nullptr;
   double* U = nullptr;   // u^n
   double* U2 = nullptr;  // u^{n+1}
   double* A = nullptr;   // FP64 stage 1 output
   double* B = nullptr;   // FP64 stage 2 output
+  double* Q = nullptr;   // low-storage RK register q
   float* D1 = nullptr;   // mixed stage increments
   float* D2 = nullptr;
   double* T = nullptr;   // planar staging of uploads, downloads and time derivatives
@@ -209,13 +215,15 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   c->prm = *params;
   c->dp = make_dev_params(*params);
   c->mixed = params->precision == DIM2D_PREC_MIXED;
+  c->lsrk = params->integrator == DIM2D_INT_LSRK3;
   const size_t n = (size_t)c->dp.ncell * 4;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->U, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->U2, n * sizeof(double));
-  if (e == hipSuccess && !c->mixed) e = hipMalloc(&c->A, n * sizeof(double));
-  if (e == hipSuccess && !c->mixed) e = hipMalloc(&c->B, n * sizeof(double));
+  if (e == hipSuccess && !c->mixed && !c->lsrk) e = hipMalloc(&c->A, n * sizeof(double));
+  if (e == hipSuccess && !c->mixed && !c->lsrk) e = hipMalloc(&c->B, n * sizeof(double));
+  if (e == hipSuccess && c->lsrk) e = hipMalloc(&c->Q, n * sizeof(double));
   if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D1, n * sizeof(float));
   if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D2, n * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&c->T, n * sizeof(double));
@@ -224,6 +232,7 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (e == hipSuccess) e = hipMemsetAsync(c->U2, 0, n * sizeof(double), c->stream);
   if (e == hipSuccess && c->A) e = hipMemsetAsync(c->A, 0, n * sizeof(double), c->stream);
   if (e == hipSuccess && c->B) e = hipMemsetAsync(c->B, 0, n * sizeof(double), c->stream);
+  if (e == hipSuccess && c->Q) e = hipMemsetAsync(c->Q, 0, n * sizeof(double), c->stream);
   if (e == hipSuccess && c->D1) e = hipMemsetAsync(c->D1, 0, n * sizeof(float), c->stream);
   if (e == hipSuccess && c->D2) e = hipMemsetAsync(c->D2, 0, n * sizeof(float), c->stream);
   if (e == hipSuccess) {
@@ -251,6 +260,7 @@ int dim2d_ctx_destroy(dim2d_ctx* c) {
   (void)hipFree(c->U2);
   (void)hipFree(c->A);
   (void)hipFree(c->B);
+  (void)hipFree(c->Q);
   (void)hipFree(c->D1);
   (void)hipFree(c->D2);
   (void)hipFree(c->T);
@@ -397,7 +407,21 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
   a.cfl_mode = cfl_mode;
   a.parity = (int)(c->steps & 1);
   const bool wall = has_wall(p);
-  if (!c->mixed) {
+  if (c->lsrk) {
+    // low-storage RK3: the state ping-pongs U -> U2 -> U -> U2, q stays in Q;
+    // the wall root guess is each stage's input state
+    double* in = c->U;
+    double* out = c->U2;
+    a.Q = c->Q;
+    for (int s = 0; s < 3; ++s) {
+      a.S64 = in;
+      a.O64 = out;
+      HIP_TRY(timed_launch(c, s, [&] { return launch_stage(p, 0, 4 + s, a, c->block_x, c->rows, c->stream); }));
+      if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, out, nullptr, out, nullptr, in, c->stream); }));
+      std::swap(in, out);
+    }
+    // three swaps: the new state is in U2, as for RK3-TVD
+  } else if (!c->mixed) {
     a.U = c->U;
     a.S64 = c->U;
     a.O64 = c->A;
@@ -645,6 +669,8 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
     if (st) return fail(st, "simulation %d: %s", i, g_err.c_str());
     if (params[i].precision != params[0].precision)
       return fail(DIM2D_EINVAL, "all simulations of a batch must share the precision mode");
+    if (params[i].integrator != DIM2D_INT_RK3TVD)
+      return fail(DIM2D_EINVAL, "batches run the RK3-TVD integrator only");
   }
   dim2d_batch* b = new (std::nothrow) dim2d_batch();
   if (!b) return fail(DIM2D_ENOMEM, "host allocation failed");

@@ -76,6 +76,7 @@ struct StageArgs {
   const float* S32;   // stage increment (mixed mode), state = U + S32
   double* O64;        // FP64 output (FP64 stages, mixed stage 3, time_dev)
   float* O32;         // FP32 output (mixed stages 1, 2)
+  double* Q;          // low-storage RK register q (read and written in place)
   StepCtl* ctl;
   const double* dts;  // per-simulation dt (batch)
   double dt_fixed;

@@ -469,12 +469,17 @@ __device__ __forceinline__ double cfl_dt_from(const StepCtl* ctl, int slot) {
 // ------------------------------------------------------------------------
 // One wave (64 lanes) owns a strip of 64 columns (60 output columns plus a
 // 2-column halo on each side) and `rows` output rows, and marches up it.
-// STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages.
+// STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages; 4, 5, 6: the
+// stages of the low-storage RK3 (FP64 only).
 // FP64 mode (MIXED=0): stage input S64; out = b*U + (1-b)*(S64 + dt L) with
 // b = 0 (stage 1, S64 = U), 3/4, 1/3 (rk3tvd_steps_module.f:225,444).
 // Mixed mode: state = U + S32 and out increments
 //   stage 1: S32' = dt L;  stage 2: S32' = 1/4 (S32 + dt L);
-//   stage 3: O64 = U + (1 - 1/3) (S32 + dt L)   (FP64 accumulation).
+//   stage 3: O64 = U + (1 - 1/3) (S32 + dt L)   (FP64 accumulation, formed
+//   as (U + S32) + (2/3 dt L - 1/3 S32) from the loaded stencil row).
+// Low-storage RK3 (Williamson's 2N-storage scheme, FP64): stage s reads the
+// stencil input S64 and q, writes q' = A_s q + dt L to Q (stages 1, 2) and
+// S64 + B_s q' to O64; A = (0, -5/9, -153/128), B = (1/3, 15/16, 8/15).
 // Row pipeline: the loads of a row are issued four rows before it is needed
 // (its density two rows before), and the RK inputs of (i, r) one row before.
 // Per output row r the wave builds the cell of row r+1, fetches the
@@ -501,9 +506,11 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
                                            int bx, int by, int rows, RedAcc<false>& red) {
   // mixed stages 2 and 3 read u^n + increment; mixed stage 0/1 read u^n alone
   constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
-  // u^n is loaded apart from the stencil rows in FP64 stages 2 and 3; every
-  // other RK input is the stencil row itself (S64, or U + S32 with S32 kept)
-  constexpr bool LOAD_U = !MIXED && STAGE >= 2;
+  // u^n (RK3-TVD FP64 stages 2, 3) or q (low-storage stages 2, 3) is loaded
+  // apart from the stencil rows; every other RK input is the stencil row
+  // itself (S64, or U + S32 with S32 kept)
+  constexpr bool LOAD_U = !MIXED && (STAGE == 2 || STAGE == 3 || STAGE == 5 || STAGE == 6);
+  const double* __restrict__ rk_src = STAGE >= 5 ? a.Q : a.U;
   const int lane = threadIdx.x & (kLanes - 1);
   const int c = bx * kOutCols + lane;  // column of this lane (0-based)
   const int r0 = kBcSize + by * rows;
@@ -524,7 +531,7 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
     load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, flx, fly, v, inc);
   };
   // u^n at (i, r); r stays within [r0 - 1, r1 - 1] (inside the array)
-  auto load_u = [&](int r, double u[4]) { ld_cell(a.U + (long long)r * p.pitch * 4, uc, u); };
+  auto load_u = [&](int r, double u[4]) { ld_cell(rk_src + (long long)r * p.pitch * 4, uc, u); };
 
   // rings: slot (q - (r0 - 2)) % kRing holds row q
   double raw[kRing][4];
@@ -620,6 +627,19 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
 #pragma unroll
         for (int k = 0; k < 4; ++k) vout[k] = (double)L[k];
         st_cell(a.O64 + row, uc, vout);
+      } else if (STAGE >= 4) {
+        // low-storage RK3 stage s: q = A_s q + dt L, u = u + B_s q
+        constexpr int s = STAGE - 4;
+        constexpr double LA = s == 0 ? 0.0 : (s == 1 ? -5.0 / 9.0 : -153.0 / 128.0);
+        constexpr double LB = s == 0 ? 1.0 / 3.0 : (s == 1 ? 15.0 / 16.0 : 8.0 / 15.0);
+        double q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          q[k] = (s == 0 ? 0.0 : LA * rkU[K][k]) + dt * (double)L[k];
+          vout[k] = S[k] + LB * q[k];
+        }
+        st_cell(a.O64 + row, uc, vout);
+        if (s < 2) st_cell(a.Q + row, uc, q);
       } else if (!MIXED) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -645,7 +665,7 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
           vout[k] = S[k] + ((2.0 / 3.0) * (dt * (double)L[k]) - (1.0 / 3.0) * (double)inc[K][k]);
         st_cell(a.O64 + row, uc, vout);
       }
-      if (REDUCE && STAGE == 3) red.add(p, vout);
+      if (REDUCE) red.add(p, vout);
     }
   };
 
@@ -676,13 +696,15 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
 template <int MIXED, int STAGE>
 __global__ void __launch_bounds__(kLanes)
 __attribute__((amdgpu_waves_per_eu((MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64) -
-                                       (STAGE == 3 ? 1 : 0),
+                                       (STAGE == 3 || STAGE == 6 ? 1 : 0),
                                    8)))
 stage_kernel(const DevParams p, const StageArgs a, int rows) {
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
+  constexpr bool FIRST = STAGE == 1 || STAGE == 4;  // takes dt, clears the next slot
+  constexpr bool LAST = STAGE == 3 || STAGE == 6;    // reduces the new state, t += dt
   double dt = a.dt_fixed;
-  if (STAGE == 1) {
+  if (FIRST) {
     if (a.cfl_mode) dt = cfl_dt_from(a.ctl, a.parity);
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
       a.ctl->dt = dt;
@@ -693,8 +715,8 @@ stage_kernel(const DevParams p, const StageArgs a, int rows) {
   }
   RedAcc<false> red;
   red.init();
-  stage_tile<RD, R, MIXED, STAGE, STAGE == 3>(p, a, dt, blockIdx.x, blockIdx.y, rows, red);
-  if (STAGE == 3) {
+  stage_tile<RD, R, MIXED, STAGE, LAST>(p, a, dt, blockIdx.x, blockIdx.y, rows, red);
+  if (LAST) {
     red.flush(a.ctl, a.parity ^ 1);
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctl->t += dt;
   }
@@ -1054,7 +1076,10 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
       case 0: return launch_one<0, 0>(p, a, rows, s);
       case 1: return launch_one<0, 1>(p, a, rows, s);
       case 2: return launch_one<0, 2>(p, a, rows, s);
-      default: return launch_one<0, 3>(p, a, rows, s);
+      case 3: return launch_one<0, 3>(p, a, rows, s);
+      case 4: return launch_one<0, 4>(p, a, rows, s);
+      case 5: return launch_one<0, 5>(p, a, rows, s);
+      default: return launch_one<0, 6>(p, a, rows, s);
     }
   }
   switch (stage) {

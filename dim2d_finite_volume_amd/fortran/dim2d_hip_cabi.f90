@@ -14,6 +14,8 @@ module dim2d_hip_cabi
   integer(c_int), parameter :: DIM2D_BC_WALL = 2
   integer(c_int), parameter :: DIM2D_PREC_FP64 = 0
   integer(c_int), parameter :: DIM2D_PREC_MIXED = 1
+  integer(c_int), parameter :: DIM2D_INT_RK3TVD = 0
+  integer(c_int), parameter :: DIM2D_INT_LSRK3 = 1
 
   !> dim2d_params
   type, bind(C) :: dim2d_params
@@ -23,6 +25,7 @@ module dim2d_hip_cabi
     real(c_double) :: x_min, x_max, y_min, y_max
     real(c_double) :: re, we, pr, viscous_r, cv_r
     real(c_double) :: gravity, wall_contact_angle, wall_heat_flux
+    integer(c_int32_t) :: integrator = 0
   end type dim2d_params
 
   interface

@@ -40,6 +40,12 @@ extern "C" {
 #define DIM2D_BC_PERIODIC 1   /* periodic_xy */
 #define DIM2D_BC_WALL 2       /* wall_xy (equilibrium wall, constant heat flux) */
 
+/* time integrators */
+#define DIM2D_INT_RK3TVD 0 /* the reference's RK3-TVD (rk3tvd_steps_module.f) */
+#define DIM2D_INT_LSRK3 1  /* low-storage RK3, Williamson (1980) 2N-storage; no counterpart
+                              in the reference (its only integrator is RK3-TVD); FP64 only,
+                              single contexts only; three state-sized buffers instead of four */
+
 /* arithmetic of the stage kernels */
 #define DIM2D_PREC_FP64 0  /* everything in FP64 */
 #define DIM2D_PREC_MIXED 1 /* FP64 u^n and RK accumulation, FP32 stage increments and stencils */
@@ -55,6 +61,7 @@ typedef struct dim2d_params {
   double gravity;          /* gravity_amp */
   double wall_contact_angle; /* degrees (wall_micro_contact_angle) */
   double wall_heat_flux;   /* constant wall heat flux, 0 = no_heat_source */
+  int32_t integrator;      /* DIM2D_INT_* (ti_choice of the reference's makefile_ti.mk) */
 } dim2d_params;
 
 typedef struct dim2d_ctx dim2d_ctx;
@@ -97,13 +104,15 @@ int dim2d_time_dev(dim2d_ctx* ctx, double t, double* time_dev);
  * resolved on the fly inside the stage kernels. */
 int dim2d_apply_bc(dim2d_ctx* ctx, double t);
 
-/* One RK3-TVD step with time step dt: three fused stages, each computing the
- * time derivative, the stage update and the wall ghost cells.  Replaces
- * td_integrator%integrate (td_integrator/rungekutta3rdtvd/td_integrator_class.f:103-161)
- * with compute_{1st,2nd,3rd}_step (rk3tvd_steps_module.f:52-100,205-269,424-490). */
+/* One step of the context's integrator with time step dt: three fused
+ * stages, each computing the time derivative, the stage update and the wall
+ * ghost cells.  Replaces td_integrator%integrate
+ * (td_integrator/rungekutta3rdtvd/td_integrator_class.f:103-161) with
+ * compute_{1st,2nd,3rd}_step (rk3tvd_steps_module.f:52-100,205-269,424-490)
+ * for DIM2D_INT_RK3TVD. */
 int dim2d_rk3_step(dim2d_ctx* ctx, double t, double dt);
 
-/* nsteps RK3-TVD steps without host synchronisation.
+/* nsteps steps of the context's integrator without host synchronisation.
  * cfl > 0: dt of each step from the CFL rule below, evaluated on the device
  *          (fused into the third stage of the previous step); dt_fixed ignored.
  * cfl <= 0: every step uses dt_fixed.

@@ -1064,14 +1064,54 @@ void orc_rk3_step(const orc_model* m, double t, double dt, const double* x_map,
   orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
 }
 
+/* Low-storage RK3, Williamson (1980) 2N-storage scheme (coefficients from
+ * that paper's third-order case).  Not in the reference; the stage recurrence
+ * is shared by the field step and the scalar amplification factor below. */
+static const double LS_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+static const double LS_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+
+static inline void lsrk_point(int k, double dtL, double* q, double* u) {
+  *q = LS_A[k] * *q + dtL;
+  *u = *u + LS_B[k] * *q;
+}
+
+double orc_lsrk3_amplification(double z) {
+  double u = 1.0, q = 0.0;
+  for (int k = 0; k < 3; ++k) lsrk_point(k, z * u, &q, &u);
+  return u;
+}
+
+void orc_lsrk3_step(const orc_model* m, double t, double dt, const double* x_map,
+                    const double* y_map, double* nodes, double* work) {
+  const int nx = m->nx, ny = m->ny;
+  const size_t plane = (size_t)nx * ny;
+  double* q = work;
+  double* time_dev = work + plane * 4;
+  double* guess = work + plane * 8;
+  const int B = BC_SIZE;
+  for (size_t i = 0; i < plane * 4; ++i) q[i] = 0.0;
+  for (int s = 0; s < 3; ++s) {
+    memcpy(guess, nodes, plane * 4 * sizeof(double));
+    orc_compute_time_dev(m, t, nodes, x_map, y_map, time_dev);
+    for (int k = 1; k <= 4; ++k)
+      for (int j = B + 1; j <= ny - B; ++j)
+        for (int i = B + 1; i <= nx - B; ++i)
+          lsrk_point(s, dt * ND(time_dev, i, j, k), &ND(q, i, j, k), &ND(nodes, i, j, k));
+    orc_apply_bc_on_nodes(m, t, x_map, y_map, guess, nodes);
+  }
+}
+
 int orc_run(const orc_model* m, double t0, int nsteps, const double* dts, const double* x_map,
             const double* y_map, double* nodes) {
   const size_t plane = (size_t)m->nx * m->ny;
-  double* work = (double*)malloc(plane * 8 * sizeof(double));
+  double* work = (double*)malloc(plane * 12 * sizeof(double));
   double t = t0;
   int bad = 0;
   for (int s = 0; s < nsteps && !bad; ++s) {
-    orc_rk3_step(m, t, dts[s], x_map, y_map, nodes, work);
+    if (m->integrator == ORC_INT_LSRK3)
+      orc_lsrk3_step(m, t, dts[s], x_map, y_map, nodes, work);
+    else
+      orc_rk3_step(m, t, dts[s], x_map, y_map, nodes, work);
     t += dts[s];
     for (int k = 1; k <= 4 && !bad; ++k)
       for (int j = 1 + BC_SIZE; j <= m->ny - BC_SIZE && !bad; ++j)

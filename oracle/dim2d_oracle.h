@@ -42,7 +42,12 @@ typedef struct orc_model {
   /* wall parameters (wall_xy_equilibrium_module.f, parameters_input.f:524-611) */
   double wall_contact_angle;   /* micro contact angle in degrees */
   double wall_heat_flux;       /* constant heat flux (0: no_heat_source) */
+  int integrator;              /* ORC_INT_RK3TVD or ORC_INT_LSRK3 */
 } orc_model;
+
+/* time integrators: the reference's RK3-TVD, and a low-storage RK3 that the
+ * reference does not have (see orc_lsrk3_step) */
+enum { ORC_INT_RK3TVD = 0, ORC_INT_LSRK3 = 1 };
 
 /* ---- primitives (point functions) ---- */
 double orc_mass_density(const orc_model*, const double*, int, int);
@@ -76,7 +81,20 @@ void orc_apply_bc_on_nodes(const orc_model*, double t, const double* x_map,
 void orc_rk3_step(const orc_model*, double t, double dt, const double* x_map,
                   const double* y_map, double* nodes, double* work);
 
-/* nsteps RK3 steps with a given dt sequence (dts[nsteps]); t advances by dt.
+/* one step of the 2N-storage low-storage RK3 of Williamson (1980, J. Comput.
+ * Phys. 35:48), no counterpart in the reference: for stages k = 1..3,
+ *   q = A_k q + dt L(u),  u = u + B_k q   on the interior cells,
+ *   A = (0, -5/9, -153/128), B = (1/3, 15/16, 8/15),
+ * then apply_bc_on_nodes with the stage's input state as the wall root guess.
+ * work: 3*nx*ny*4 doubles. */
+void orc_lsrk3_step(const orc_model*, double t, double dt, const double* x_map,
+                    const double* y_map, double* nodes, double* work);
+/* the same stage recurrence on u' = lambda u from u = 1: the amplification
+ * factor R(z), z = lambda dt (for the pins: 1 + z + z^2/2 + z^3/6) */
+double orc_lsrk3_amplification(double z);
+
+/* nsteps steps of the model's integrator with a given dt sequence
+ * (dts[nsteps]); t advances by dt.
  * Returns 0, or the 1-based step at which a non-finite interior value appeared. */
 int orc_run(const orc_model*, double t0, int nsteps, const double* dts,
             const double* x_map, const double* y_map, double* nodes);

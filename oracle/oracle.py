@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "libdim2d_oracle.so")
 
 BC_REFLECTION, BC_PERIODIC, BC_WALL = 0, 1, 2
+INT_RK3TVD, INT_LSRK3 = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -30,6 +31,7 @@ class Model(ctypes.Structure):
         ("bc_w", ctypes.c_int), ("bc_e", ctypes.c_int), ("bc_s", ctypes.c_int),
         ("bc_n", ctypes.c_int),
         ("wall_contact_angle", ctypes.c_double), ("wall_heat_flux", ctypes.c_double),
+        ("integrator", ctypes.c_int),
     ]
 
 
@@ -56,6 +58,9 @@ def lib():
         L.orc_compute_time_dev.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_apply_bc_on_nodes.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_rk3_step.argtypes = [M, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_lsrk3_step.argtypes = L.orc_rk3_step.argtypes
+        L.orc_lsrk3_amplification.restype = ctypes.c_double
+        L.orc_lsrk3_amplification.argtypes = [ctypes.c_double]
         L.orc_run.restype = ctypes.c_int
         L.orc_run.argtypes = [M, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_coordinates.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp, _dp]
@@ -97,11 +102,12 @@ def _ptr(a: np.ndarray):
 
 
 def make_model(nx, ny, re=5.0, we=10.0, pr=20.0, viscous_r=-1.5, cv_r=2.5, gravity=0.0,
-               bc=(BC_REFLECTION,) * 4, wall_contact_angle=90.0, wall_heat_flux=0.0) -> Model:
+               bc=(BC_REFLECTION,) * 4, wall_contact_angle=90.0, wall_heat_flux=0.0,
+               integrator=INT_RK3TVD) -> Model:
     """bc = (W, E, S, N).  Default physics is the reference's test set
     (dim2d_parameters.f:182-187)."""
     return Model(nx, ny, re, we, pr, viscous_r, cv_r, gravity, int(gravity != 0.0),
-                 bc[0], bc[1], bc[2], bc[3], wall_contact_angle, wall_heat_flux)
+                 bc[0], bc[1], bc[2], bc[3], wall_contact_angle, wall_heat_flux, integrator)
 
 
 def flux_term(m: Model, nodes: np.ndarray, which: int, i: int, j: int, dx: float, dy: float):
@@ -134,10 +140,15 @@ def apply_bc(m: Model, nodes, x_map, y_map, nodes_tmp=None, t=0.0):
 
 
 def run(m: Model, nodes, x_map, y_map, dts, t0=0.0):
-    """Advance `nodes` in place by len(dts) RK3 steps; returns the failing step or 0."""
+    """Advance `nodes` in place by len(dts) steps of the model's integrator;
+    returns the failing step or 0."""
     dts = np.ascontiguousarray(dts, dtype=np.float64)
     return lib().orc_run(ctypes.byref(m), t0, len(dts), _ptr(dts), _ptr(x_map), _ptr(y_map),
                          _ptr(nodes))
+
+
+def lsrk3_amplification(z):
+    return lib().orc_lsrk3_amplification(float(z))
 
 
 def cfl_dt(m: Model, nodes, dx, dy, cfl):

@@ -47,6 +47,7 @@ def test_version_and_error_text():
     ("bc_w", 7, b"unknown boundary"),
     ("precision", 5, b"precision"),
     ("re", -1.0, b"positive"),
+    ("integrator", 7, b"unknown integrator"),
 ])
 def test_invalid_params_rejected(field, value, msg):
     import dim2d_finite_volume_amd as d
@@ -70,6 +71,21 @@ def test_unsupported_boundary_combinations():
     p = d.ModelParams(nx=40, ny=40, bc=(2, 0, 2, 0)).to_c()
     assert L.dim2d_ctx_create(ctypes.byref(p), ctypes.byref(ctx)) == d.capi.EINVAL
     assert b"wall_xy" in L.dim2d_last_error()
+
+
+def test_low_storage_rk_limits():
+    """The low-storage RK3 runs in FP64 and in single contexts only."""
+    import dim2d_finite_volume_amd as d
+    L = d.capi.load()
+    ctx = ctypes.c_void_p()
+    p = d.ModelParams(nx=40, ny=40, precision=d.PREC_MIXED, integrator=d.INT_LSRK3).to_c()
+    assert L.dim2d_ctx_create(ctypes.byref(p), ctypes.byref(ctx)) == d.capi.EINVAL
+    assert b"FP64 only" in L.dim2d_last_error()
+    ps = (d.capi.Params * 2)(d.ModelParams(nx=40, ny=40).to_c(),
+                             d.ModelParams(nx=40, ny=40, integrator=d.INT_LSRK3).to_c())
+    b = ctypes.c_void_p()
+    assert L.dim2d_batch_create(ps, 2, ctypes.byref(b)) == d.capi.EINVAL
+    assert b"RK3-TVD" in L.dim2d_last_error()
 
 
 def test_null_arguments():

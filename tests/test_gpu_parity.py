@@ -46,12 +46,12 @@ def perturbed_state(params, seed=3):
 def oracle_model(orc, p):
     return orc.make_model(p.nx, p.ny, re=p.re, we=p.we, pr=p.pr, viscous_r=p.viscous_r,
                           cv_r=p.cv_r, bc=p.bc, wall_contact_angle=p.wall_contact_angle,
-                          wall_heat_flux=p.wall_heat_flux)
+                          wall_heat_flux=p.wall_heat_flux, integrator=p.integrator)
 
 
-def setup_case(orc, bc, nx=74, ny=41, precision=0):
+def setup_case(orc, bc, nx=74, ny=41, precision=0, integrator=0):
     d = _lib()
-    p = d.ModelParams(nx=nx, ny=ny, bc=bc, precision=precision)
+    p = d.ModelParams(nx=nx, ny=ny, bc=bc, precision=precision, integrator=integrator)
     m = oracle_model(orc, p)
     x_map, y_map = p.coordinates()
     nodes = perturbed_state(p)
@@ -169,6 +169,41 @@ def test_cfl_integrate(orc):
     assert abs(last_dt - dts[-1]) < 1e-12 * dts[-1]
     err = max_rel_err(got, ref)
     print("cfl run rel err", err)
+    assert max(err) < 1e-10, err
+
+
+@pytest.mark.parametrize("case", ["reflection", "periodic_x_wall_y", "wall_s_reflection"])
+def test_lsrk3_steps_fp64(orc, case):
+    """Low-storage RK3 (2N-storage, Williamson 1980): 20 fixed-dt steps against
+    the oracle's restatement of the same scheme, FP64 parity 1e-10."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], integrator=1)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(20, dt)) == 0
+    f = d.Field(p, nodes.copy())
+    f.run(20, dt=dt)
+    got = f.download()
+    err = max_rel_err(got, ref)
+    print(case, "lsrk3 20-step rel err", err)
+    assert max(err) < 1e-10, err
+
+
+def test_lsrk3_cfl_integrate(orc):
+    """Low-storage RK3 with the device CFL time step."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["periodic"], integrator=1)
+    f = d.Field(p, nodes.copy())
+    ref = nodes.copy()
+    dts = []
+    for _ in range(10):
+        dt = orc.cfl_dt(m, ref, *p.spacing(), 0.1)[0]
+        dts.append(dt)
+        assert orc.run(m, ref, xm, ym, np.array([dt])) == 0
+    f.run(10, cfl=0.1)
+    got = f.download()
+    t, _ = f.device_time()
+    assert abs(t - sum(dts)) < 1e-12 * sum(dts)
+    err = max_rel_err(got, ref)
+    print("lsrk3 cfl run rel err", err)
     assert max(err) < 1e-10, err
 
 

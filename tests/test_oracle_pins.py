@@ -242,3 +242,50 @@ def test_cfl_rule(orc):
     red = ic.cfl_sound_speed(p, it)
     ok = np.isfinite(ref)
     assert np.allclose(red[ok], ref[ok], rtol=1e-12)
+
+
+# ---- low-storage RK3 (no counterpart in the reference) ----
+
+def test_lsrk3_amplification_is_third_order_taylor():
+    """On u' = lambda u any 3-stage third-order RK has R(z) = 1 + z + z^2/2 + z^3/6;
+    a wrong A or B coefficient of the 2N-storage recurrence changes it."""
+    from oracle import oracle as orc_mod
+    for z in (-2.4, -1.3, -0.7, -0.1, 0.05, 0.4):
+        exact = 1 + z + z * z / 2 + z ** 3 / 6
+        assert math.isclose(orc_mod.lsrk3_amplification(z), exact, rel_tol=1e-14, abs_tol=1e-15), z
+
+
+def test_lsrk3_exact_for_gravity_quadratic(orc):
+    """The same polynomial-in-t solution as for RK3-TVD is integrated exactly."""
+    g = 2.0
+    m, xm, ym = _grid(orc, 10, bc=(1, 1, 1, 1), gravity=g)
+    m.integrator = orc.INT_LSRK3
+    rho, E0 = 1.1, 5.0
+    nodes = np.empty((4, m.ny, m.nx))
+    nodes[0], nodes[1], nodes[2], nodes[3] = rho, 0.0, 0.0, E0
+    dt, n = 0.01, 7
+    assert orc.run(m, nodes, xm, ym, np.full(n, dt)) == 0
+    t = n * dt
+    assert np.allclose(nodes[2, 2:-2, 2:-2], -rho * g * t, rtol=1e-13)
+    assert np.allclose(nodes[3, 2:-2, 2:-2], E0 + 0.5 * rho * g * g * t * t, rtol=1e-13)
+
+
+def test_lsrk3_converges_to_rk3tvd_at_third_order(orc):
+    """Both integrators are third order: their difference at a fixed time
+    shrinks by ~2^3 when dt halves, and mass is conserved by both."""
+    p, m, xm, ym, nodes0 = _drop(orc, 20, (0, 0, 0, 0), flow=(0.1, 0.0), noise=1e-4)
+    dt0 = orc.cfl_dt(m, nodes0, *p.spacing(), 0.2)[0]
+    diffs = []
+    for level in range(3):
+        n = 4 * 2 ** level
+        dt = dt0 / 2 ** level
+        a, b = nodes0.copy(), nodes0.copy()
+        m.integrator = orc.INT_RK3TVD
+        assert orc.run(m, a, xm, ym, np.full(n, dt)) == 0
+        m.integrator = orc.INT_LSRK3
+        s0 = orc.sums(m, b)
+        assert orc.run(m, b, xm, ym, np.full(n, dt)) == 0
+        assert abs(orc.sums(m, b)[0] - s0[0]) < 1e-13 * s0[0]
+        diffs.append(np.abs(a[:, 2:-2, 2:-2] - b[:, 2:-2, 2:-2]).max())
+    r1, r2 = diffs[0] / diffs[1], diffs[1] / diffs[2]
+    assert 6.0 < r1 < 10.5 and 6.0 < r2 < 10.5, (diffs, r1, r2)
