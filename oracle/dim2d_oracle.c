@@ -239,14 +239,39 @@ static double s_d2gdxdy(const orc_model* m, const double* n, int i, int j, prim_
          0.5 / (dx * dy);
 }
 
+/* An sd_operators object: the face stencils a flux evaluation uses.  The
+ * interior (Mattsson) set and the one-sided sets of the open boundaries
+ * (sd_operators_{x,y}_oneside_{L0,R0}_class.f) differ in a few members; the
+ * L1/R1 classes override only the other face family, so for the fluxes they
+ * are the interior set. */
+typedef double (*st0_fn)(const orc_model*, const double*, int, int, prim_fn);
+typedef double (*st1_fn)(const orc_model*, const double*, int, int, prim_fn, double);
+typedef double (*stnl_fn)(const orc_model*, const double*, int, int, sec_fn, double, double);
+typedef struct sd_ops {
+  st0_fn f;
+  st1_fn dfdx;
+  stnl_fn dfdx_nl;
+  st1_fn dfdy, d2fdx2, d2fdy2;
+  st0_fn g;
+  st1_fn dgdx, dgdy;
+  stnl_fn dgdy_nl;
+  st1_fn d2gdx2, d2gdy2;
+} sd_ops;
+static const sd_ops MATTSSON = {s_f,    s_dfdx, s_dfdx_nl, s_dfdy,    s_d2fdx2, s_d2fdy2,
+                                s_g,    s_dgdx, s_dgdy,    s_dgdy_nl, s_d2gdx2, s_d2gdy2};
+
 /* gradient_av of the reference stencil test (test_mattsson_operators.f:1219-1238) */
 static double gradient_av(const orc_model* m, const double* n, int i, int j, double dx, double dy,
                           grad_fn gx, grad_fn gy) {
   return gx(m, n, i, j, orc_mass_density, dx) + gy(m, n, i, j, orc_mass_density, dy);
 }
 
+double orc_oneside_density_stencil(const orc_model* m, const double* n, int which, int i, int j,
+                                   double dx, double dy);
+
 double orc_mattsson_density_stencil(const orc_model* m, const double* n, int which, int i, int j,
                                     double dx, double dy) {
+  if (which >= 16) return orc_oneside_density_stencil(m, n, which, i, j, dx, dy);
   prim_fn p = orc_mass_density;
   switch (which) {
     case 0: return s_f(m, n, i, j, p);
@@ -273,197 +298,197 @@ double orc_mattsson_density_stencil(const orc_model* m, const double* n, int whi
 /* fluxes: dim2d_fluxes_module.f                                       */
 /* ------------------------------------------------------------------ */
 
-static double flux_x_mass_density(const orc_model* m, const double* n, int i, int j) {
-  return s_f(m, n, i, j, momentum_x);
+static double flux_x_mass_density(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->f(m, n, i, j, momentum_x);
 }
-static double flux_y_mass_density(const orc_model* m, const double* n, int i, int j) {
-  return s_g(m, n, i, j, momentum_y);
+static double flux_y_mass_density(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->g(m, n, i, j, momentum_y);
 }
 
 /* momentum_x */
-static double flux_x_inviscid_momentum_x(const orc_model* m, const double* n, int i, int j) {
-  return s_f(m, n, i, j, qx_transport_x) + s_f(m, n, i, j, orc_classical_pressure);
+static double flux_x_inviscid_momentum_x(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->f(m, n, i, j, qx_transport_x) + S->f(m, n, i, j, orc_classical_pressure);
 }
 static double flux_x_viscid_momentum_x(const orc_model* m, const double* n, int i, int j, double dx,
-                                       double dy) {
-  return (2.0 + m->viscous_r) * s_dfdx(m, n, i, j, orc_velocity_x, dx) +
-         m->viscous_r * s_dfdy(m, n, i, j, orc_velocity_y, dy);
+                                       double dy, const sd_ops* S) {
+  return (2.0 + m->viscous_r) * S->dfdx(m, n, i, j, orc_velocity_x, dx) +
+         m->viscous_r * S->dfdy(m, n, i, j, orc_velocity_y, dy);
 }
 static double flux_x_capillarity_momentum_x(const orc_model* m, const double* n, int i, int j,
-                                            double dx, double dy) {
+                                            double dx, double dy, const sd_ops* S) {
   prim_fn r = orc_mass_density;
-  return 1.5 / m->cv_r * s_f(m, n, i, j, capillarity_pressure) *
-             (sq(s_dfdx(m, n, i, j, r, dx)) + sq(s_dfdy(m, n, i, j, r, dy))) +
-         s_f(m, n, i, j, r) * (s_d2fdx2(m, n, i, j, r, dx) + s_d2fdy2(m, n, i, j, r, dy)) +
-         0.5 * (-sq(s_dfdx(m, n, i, j, r, dx)) + sq(s_dfdy(m, n, i, j, r, dy)));
+  return 1.5 / m->cv_r * S->f(m, n, i, j, capillarity_pressure) *
+             (sq(S->dfdx(m, n, i, j, r, dx)) + sq(S->dfdy(m, n, i, j, r, dy))) +
+         S->f(m, n, i, j, r) * (S->d2fdx2(m, n, i, j, r, dx) + S->d2fdy2(m, n, i, j, r, dy)) +
+         0.5 * (-sq(S->dfdx(m, n, i, j, r, dx)) + sq(S->dfdy(m, n, i, j, r, dy)));
 }
-static double flux_y_inviscid_momentum_x(const orc_model* m, const double* n, int i, int j) {
-  return s_g(m, n, i, j, qx_transport_y);
+static double flux_y_inviscid_momentum_x(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->g(m, n, i, j, qx_transport_y);
 }
 static double flux_y_viscid_momentum_x(const orc_model* m, const double* n, int i, int j, double dx,
-                                       double dy) {
-  return s_dgdy(m, n, i, j, orc_velocity_x, dy) + s_dgdx(m, n, i, j, orc_velocity_y, dx);
+                                       double dy, const sd_ops* S) {
+  return S->dgdy(m, n, i, j, orc_velocity_x, dy) + S->dgdx(m, n, i, j, orc_velocity_y, dx);
 }
 static double flux_y_capillarity_momentum_x(const orc_model* m, const double* n, int i, int j,
-                                            double dx, double dy) {
-  return -s_dgdx(m, n, i, j, orc_mass_density, dx) * s_dgdy(m, n, i, j, orc_mass_density, dy);
+                                            double dx, double dy, const sd_ops* S) {
+  return -S->dgdx(m, n, i, j, orc_mass_density, dx) * S->dgdy(m, n, i, j, orc_mass_density, dy);
 }
 
 /* momentum_y */
-static double flux_x_inviscid_momentum_y(const orc_model* m, const double* n, int i, int j) {
-  return s_f(m, n, i, j, qy_transport_x);
+static double flux_x_inviscid_momentum_y(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->f(m, n, i, j, qy_transport_x);
 }
 static double flux_x_viscid_momentum_y(const orc_model* m, const double* n, int i, int j, double dx,
-                                       double dy) {
-  return s_dfdy(m, n, i, j, orc_velocity_x, dy) + s_dfdx(m, n, i, j, orc_velocity_y, dx);
+                                       double dy, const sd_ops* S) {
+  return S->dfdy(m, n, i, j, orc_velocity_x, dy) + S->dfdx(m, n, i, j, orc_velocity_y, dx);
 }
 static double flux_x_capillarity_momentum_y(const orc_model* m, const double* n, int i, int j,
-                                            double dx, double dy) {
-  return -s_dfdx(m, n, i, j, orc_mass_density, dx) * s_dfdy(m, n, i, j, orc_mass_density, dy);
+                                            double dx, double dy, const sd_ops* S) {
+  return -S->dfdx(m, n, i, j, orc_mass_density, dx) * S->dfdy(m, n, i, j, orc_mass_density, dy);
 }
-static double flux_y_inviscid_momentum_y(const orc_model* m, const double* n, int i, int j) {
-  return s_g(m, n, i, j, qy_transport_y) + s_g(m, n, i, j, orc_classical_pressure);
+static double flux_y_inviscid_momentum_y(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->g(m, n, i, j, qy_transport_y) + S->g(m, n, i, j, orc_classical_pressure);
 }
 static double flux_y_viscid_momentum_y(const orc_model* m, const double* n, int i, int j, double dx,
-                                       double dy) {
-  return m->viscous_r * s_dgdx(m, n, i, j, orc_velocity_x, dx) +
-         (2.0 + m->viscous_r) * s_dgdy(m, n, i, j, orc_velocity_y, dy);
+                                       double dy, const sd_ops* S) {
+  return m->viscous_r * S->dgdx(m, n, i, j, orc_velocity_x, dx) +
+         (2.0 + m->viscous_r) * S->dgdy(m, n, i, j, orc_velocity_y, dy);
 }
 static double flux_y_capillarity_momentum_y(const orc_model* m, const double* n, int i, int j,
-                                            double dx, double dy) {
+                                            double dx, double dy, const sd_ops* S) {
   prim_fn r = orc_mass_density;
-  return 1.5 / m->cv_r * s_g(m, n, i, j, capillarity_pressure) *
-             (sq(s_dgdx(m, n, i, j, r, dx)) + sq(s_dgdy(m, n, i, j, r, dy))) +
-         s_g(m, n, i, j, r) * (s_d2gdx2(m, n, i, j, r, dx) + s_d2gdy2(m, n, i, j, r, dy)) +
-         0.5 * (sq(s_dgdx(m, n, i, j, r, dx)) - sq(s_dgdy(m, n, i, j, r, dy)));
+  return 1.5 / m->cv_r * S->g(m, n, i, j, capillarity_pressure) *
+             (sq(S->dgdx(m, n, i, j, r, dx)) + sq(S->dgdy(m, n, i, j, r, dy))) +
+         S->g(m, n, i, j, r) * (S->d2gdx2(m, n, i, j, r, dx) + S->d2gdy2(m, n, i, j, r, dy)) +
+         0.5 * (sq(S->dgdx(m, n, i, j, r, dx)) - sq(S->dgdy(m, n, i, j, r, dy)));
 }
 
 /* total_energy */
-static double flux_x_inviscid_total_energy(const orc_model* m, const double* n, int i, int j) {
-  return s_f(m, n, i, j, energy_transport_x) + s_f(m, n, i, j, classical_pressure_xwork);
+static double flux_x_inviscid_total_energy(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->f(m, n, i, j, energy_transport_x) + S->f(m, n, i, j, classical_pressure_xwork);
 }
 static double flux_x_viscid_total_energy(const orc_model* m, const double* n, int i, int j,
-                                         double dx, double dy) {
-  const double ux = s_f(m, n, i, j, orc_velocity_x);
-  const double uy = s_f(m, n, i, j, orc_velocity_y);
-  return ux * ((2.0 + m->viscous_r) * s_dfdx(m, n, i, j, orc_velocity_x, dx) +
-               m->viscous_r * s_dfdy(m, n, i, j, orc_velocity_y, dy)) +
-         uy * (s_dfdy(m, n, i, j, orc_velocity_x, dy) + s_dfdx(m, n, i, j, orc_velocity_y, dx)) +
-         1.0 / m->pr * s_dfdx(m, n, i, j, orc_classical_temperature_eff, dx);
+                                         double dx, double dy, const sd_ops* S) {
+  const double ux = S->f(m, n, i, j, orc_velocity_x);
+  const double uy = S->f(m, n, i, j, orc_velocity_y);
+  return ux * ((2.0 + m->viscous_r) * S->dfdx(m, n, i, j, orc_velocity_x, dx) +
+               m->viscous_r * S->dfdy(m, n, i, j, orc_velocity_y, dy)) +
+         uy * (S->dfdy(m, n, i, j, orc_velocity_x, dy) + S->dfdx(m, n, i, j, orc_velocity_y, dx)) +
+         1.0 / m->pr * S->dfdx(m, n, i, j, orc_classical_temperature_eff, dx);
 }
 static double flux_x_capillarity_total_energy(const orc_model* m, const double* n, int i, int j,
-                                              double dx, double dy) {
+                                              double dx, double dy, const sd_ops* S) {
   prim_fn r = orc_mass_density;
-  const double ux = s_f(m, n, i, j, orc_velocity_x);
-  const double uy = s_f(m, n, i, j, orc_velocity_y);
-  const double duxdx = s_dfdx(m, n, i, j, orc_velocity_x, dx);
-  const double duydy = s_dfdy(m, n, i, j, orc_velocity_y, dy);
-  const double drhodx = s_dfdx(m, n, i, j, r, dx);
-  const double drhody = s_dfdy(m, n, i, j, r, dy);
-  return 1.5 / m->cv_r * s_f(m, n, i, j, capillarity_pressure_xwork) *
+  const double ux = S->f(m, n, i, j, orc_velocity_x);
+  const double uy = S->f(m, n, i, j, orc_velocity_y);
+  const double duxdx = S->dfdx(m, n, i, j, orc_velocity_x, dx);
+  const double duydy = S->dfdy(m, n, i, j, orc_velocity_y, dy);
+  const double drhodx = S->dfdx(m, n, i, j, r, dx);
+  const double drhody = S->dfdy(m, n, i, j, r, dy);
+  return 1.5 / m->cv_r * S->f(m, n, i, j, capillarity_pressure_xwork) *
              (sq(drhodx) + sq(drhody)) +
-         s_f(m, n, i, j, momentum_x) * (s_d2fdx2(m, n, i, j, r, dx) + s_d2fdy2(m, n, i, j, r, dy)) +
+         S->f(m, n, i, j, momentum_x) * (S->d2fdx2(m, n, i, j, r, dx) + S->d2fdy2(m, n, i, j, r, dy)) +
          0.5 * ux * (-sq(drhodx) + sq(drhody)) - drhodx * drhody * uy -
-         s_f(m, n, i, j, r) * drhodx * (duxdx + duydy) -
-         1.0 / (m->re * m->pr) * s_dfdx_nl(m, n, i, j, capillarity_temperature_eff, dx, dy);
+         S->f(m, n, i, j, r) * drhodx * (duxdx + duydy) -
+         1.0 / (m->re * m->pr) * S->dfdx_nl(m, n, i, j, capillarity_temperature_eff, dx, dy);
 }
-static double flux_y_inviscid_total_energy(const orc_model* m, const double* n, int i, int j) {
-  return s_g(m, n, i, j, energy_transport_y) + s_g(m, n, i, j, classical_pressure_ywork);
+static double flux_y_inviscid_total_energy(const orc_model* m, const double* n, int i, int j, const sd_ops* S) {
+  return S->g(m, n, i, j, energy_transport_y) + S->g(m, n, i, j, classical_pressure_ywork);
 }
 static double flux_y_viscid_total_energy(const orc_model* m, const double* n, int i, int j,
-                                         double dx, double dy) {
-  const double ux = s_g(m, n, i, j, orc_velocity_x);
-  const double uy = s_g(m, n, i, j, orc_velocity_y);
-  const double duxdx = s_dgdx(m, n, i, j, orc_velocity_x, dx);
-  const double duydy = s_dgdy(m, n, i, j, orc_velocity_y, dy);
-  return ux * (s_dgdy(m, n, i, j, orc_velocity_x, dy) + s_dgdx(m, n, i, j, orc_velocity_y, dx)) +
+                                         double dx, double dy, const sd_ops* S) {
+  const double ux = S->g(m, n, i, j, orc_velocity_x);
+  const double uy = S->g(m, n, i, j, orc_velocity_y);
+  const double duxdx = S->dgdx(m, n, i, j, orc_velocity_x, dx);
+  const double duydy = S->dgdy(m, n, i, j, orc_velocity_y, dy);
+  return ux * (S->dgdy(m, n, i, j, orc_velocity_x, dy) + S->dgdx(m, n, i, j, orc_velocity_y, dx)) +
          uy * (m->viscous_r * duxdx + (2.0 + m->viscous_r) * duydy) +
-         1.0 / m->pr * s_dgdy(m, n, i, j, orc_classical_temperature_eff, dy);
+         1.0 / m->pr * S->dgdy(m, n, i, j, orc_classical_temperature_eff, dy);
 }
 static double flux_y_capillarity_total_energy(const orc_model* m, const double* n, int i, int j,
-                                              double dx, double dy) {
+                                              double dx, double dy, const sd_ops* S) {
   prim_fn r = orc_mass_density;
-  const double ux = s_g(m, n, i, j, orc_velocity_x);
-  const double uy = s_g(m, n, i, j, orc_velocity_y);
-  const double duxdx = s_dgdx(m, n, i, j, orc_velocity_x, dx);
-  const double duydy = s_dgdy(m, n, i, j, orc_velocity_y, dy);
-  const double drhodx = s_dgdx(m, n, i, j, r, dx);
-  const double drhody = s_dgdy(m, n, i, j, r, dy);
-  return 1.5 / m->cv_r * s_g(m, n, i, j, capillarity_pressure_ywork) *
-             (sq(s_dgdx(m, n, i, j, r, dx)) + sq(s_dgdy(m, n, i, j, r, dy))) +
-         s_g(m, n, i, j, momentum_y) * (s_d2gdx2(m, n, i, j, r, dx) + s_d2gdy2(m, n, i, j, r, dy)) +
+  const double ux = S->g(m, n, i, j, orc_velocity_x);
+  const double uy = S->g(m, n, i, j, orc_velocity_y);
+  const double duxdx = S->dgdx(m, n, i, j, orc_velocity_x, dx);
+  const double duydy = S->dgdy(m, n, i, j, orc_velocity_y, dy);
+  const double drhodx = S->dgdx(m, n, i, j, r, dx);
+  const double drhody = S->dgdy(m, n, i, j, r, dy);
+  return 1.5 / m->cv_r * S->g(m, n, i, j, capillarity_pressure_ywork) *
+             (sq(S->dgdx(m, n, i, j, r, dx)) + sq(S->dgdy(m, n, i, j, r, dy))) +
+         S->g(m, n, i, j, momentum_y) * (S->d2gdx2(m, n, i, j, r, dx) + S->d2gdy2(m, n, i, j, r, dy)) +
          0.5 * uy * (sq(drhodx) - sq(drhody)) - drhodx * drhody * ux -
-         s_g(m, n, i, j, r) * drhody * (duxdx + duydy) -
-         1.0 / (m->re * m->pr) * s_dgdy_nl(m, n, i, j, capillarity_temperature_eff, dx, dy);
+         S->g(m, n, i, j, r) * drhody * (duxdx + duydy) -
+         1.0 / (m->re * m->pr) * S->dgdy_nl(m, n, i, j, capillarity_temperature_eff, dx, dy);
 }
 
 /* total fluxes: inviscid - 1/Re viscid - 1/We capillarity */
 static double flux_x_momentum_x(const orc_model* m, const double* n, int i, int j, double dx,
-                                double dy) {
-  return flux_x_inviscid_momentum_x(m, n, i, j) -
-         1.0 / m->re * flux_x_viscid_momentum_x(m, n, i, j, dx, dy) -
-         1.0 / m->we * flux_x_capillarity_momentum_x(m, n, i, j, dx, dy);
+                                double dy, const sd_ops* S) {
+  return flux_x_inviscid_momentum_x(m, n, i, j, S) -
+         1.0 / m->re * flux_x_viscid_momentum_x(m, n, i, j, dx, dy, S) -
+         1.0 / m->we * flux_x_capillarity_momentum_x(m, n, i, j, dx, dy, S);
 }
 static double flux_y_momentum_x(const orc_model* m, const double* n, int i, int j, double dx,
-                                double dy) {
-  return flux_y_inviscid_momentum_x(m, n, i, j) -
-         1.0 / m->re * flux_y_viscid_momentum_x(m, n, i, j, dx, dy) -
-         1.0 / m->we * flux_y_capillarity_momentum_x(m, n, i, j, dx, dy);
+                                double dy, const sd_ops* S) {
+  return flux_y_inviscid_momentum_x(m, n, i, j, S) -
+         1.0 / m->re * flux_y_viscid_momentum_x(m, n, i, j, dx, dy, S) -
+         1.0 / m->we * flux_y_capillarity_momentum_x(m, n, i, j, dx, dy, S);
 }
 static double flux_x_momentum_y(const orc_model* m, const double* n, int i, int j, double dx,
-                                double dy) {
-  return flux_x_inviscid_momentum_y(m, n, i, j) -
-         1.0 / m->re * flux_x_viscid_momentum_y(m, n, i, j, dx, dy) -
-         1.0 / m->we * flux_x_capillarity_momentum_y(m, n, i, j, dx, dy);
+                                double dy, const sd_ops* S) {
+  return flux_x_inviscid_momentum_y(m, n, i, j, S) -
+         1.0 / m->re * flux_x_viscid_momentum_y(m, n, i, j, dx, dy, S) -
+         1.0 / m->we * flux_x_capillarity_momentum_y(m, n, i, j, dx, dy, S);
 }
 static double flux_y_momentum_y(const orc_model* m, const double* n, int i, int j, double dx,
-                                double dy) {
-  return flux_y_inviscid_momentum_y(m, n, i, j) -
-         1.0 / m->re * flux_y_viscid_momentum_y(m, n, i, j, dx, dy) -
-         1.0 / m->we * flux_y_capillarity_momentum_y(m, n, i, j, dx, dy);
+                                double dy, const sd_ops* S) {
+  return flux_y_inviscid_momentum_y(m, n, i, j, S) -
+         1.0 / m->re * flux_y_viscid_momentum_y(m, n, i, j, dx, dy, S) -
+         1.0 / m->we * flux_y_capillarity_momentum_y(m, n, i, j, dx, dy, S);
 }
 static double flux_x_total_energy(const orc_model* m, const double* n, int i, int j, double dx,
-                                  double dy) {
-  return flux_x_inviscid_total_energy(m, n, i, j) -
-         1.0 / m->re * flux_x_viscid_total_energy(m, n, i, j, dx, dy) -
-         1.0 / m->we * flux_x_capillarity_total_energy(m, n, i, j, dx, dy);
+                                  double dy, const sd_ops* S) {
+  return flux_x_inviscid_total_energy(m, n, i, j, S) -
+         1.0 / m->re * flux_x_viscid_total_energy(m, n, i, j, dx, dy, S) -
+         1.0 / m->we * flux_x_capillarity_total_energy(m, n, i, j, dx, dy, S);
 }
 static double flux_y_total_energy(const orc_model* m, const double* n, int i, int j, double dx,
-                                  double dy) {
-  return flux_y_inviscid_total_energy(m, n, i, j) -
-         1.0 / m->re * flux_y_viscid_total_energy(m, n, i, j, dx, dy) -
-         1.0 / m->we * flux_y_capillarity_total_energy(m, n, i, j, dx, dy);
+                                  double dy, const sd_ops* S) {
+  return flux_y_inviscid_total_energy(m, n, i, j, S) -
+         1.0 / m->re * flux_y_viscid_total_energy(m, n, i, j, dx, dy, S) -
+         1.0 / m->we * flux_y_capillarity_total_energy(m, n, i, j, dx, dy, S);
 }
 
 double orc_flux_term(const orc_model* m, const double* n, int which, int i, int j, double dx,
                      double dy) {
   switch (which) {
-    case 0: return flux_x_mass_density(m, n, i, j);
-    case 1: return flux_y_mass_density(m, n, i, j);
-    case 2: return flux_x_inviscid_momentum_x(m, n, i, j);
-    case 3: return flux_x_viscid_momentum_x(m, n, i, j, dx, dy);
-    case 4: return flux_x_capillarity_momentum_x(m, n, i, j, dx, dy);
-    case 5: return flux_x_momentum_x(m, n, i, j, dx, dy);
-    case 6: return flux_y_inviscid_momentum_x(m, n, i, j);
-    case 7: return flux_y_viscid_momentum_x(m, n, i, j, dx, dy);
-    case 8: return flux_y_capillarity_momentum_x(m, n, i, j, dx, dy);
-    case 9: return flux_y_momentum_x(m, n, i, j, dx, dy);
-    case 10: return flux_x_inviscid_momentum_y(m, n, i, j);
-    case 11: return flux_x_viscid_momentum_y(m, n, i, j, dx, dy);
-    case 12: return flux_x_capillarity_momentum_y(m, n, i, j, dx, dy);
-    case 13: return flux_x_momentum_y(m, n, i, j, dx, dy);
-    case 14: return flux_y_inviscid_momentum_y(m, n, i, j);
-    case 15: return flux_y_viscid_momentum_y(m, n, i, j, dx, dy);
-    case 16: return flux_y_capillarity_momentum_y(m, n, i, j, dx, dy);
-    case 17: return flux_y_momentum_y(m, n, i, j, dx, dy);
-    case 18: return flux_x_inviscid_total_energy(m, n, i, j);
-    case 19: return flux_x_viscid_total_energy(m, n, i, j, dx, dy);
-    case 20: return flux_x_capillarity_total_energy(m, n, i, j, dx, dy);
-    case 21: return flux_x_total_energy(m, n, i, j, dx, dy);
-    case 22: return flux_y_inviscid_total_energy(m, n, i, j);
-    case 23: return flux_y_viscid_total_energy(m, n, i, j, dx, dy);
-    case 24: return flux_y_capillarity_total_energy(m, n, i, j, dx, dy);
-    case 25: return flux_y_total_energy(m, n, i, j, dx, dy);
+    case 0: return flux_x_mass_density(m, n, i, j, &MATTSSON);
+    case 1: return flux_y_mass_density(m, n, i, j, &MATTSSON);
+    case 2: return flux_x_inviscid_momentum_x(m, n, i, j, &MATTSSON);
+    case 3: return flux_x_viscid_momentum_x(m, n, i, j, dx, dy, &MATTSSON);
+    case 4: return flux_x_capillarity_momentum_x(m, n, i, j, dx, dy, &MATTSSON);
+    case 5: return flux_x_momentum_x(m, n, i, j, dx, dy, &MATTSSON);
+    case 6: return flux_y_inviscid_momentum_x(m, n, i, j, &MATTSSON);
+    case 7: return flux_y_viscid_momentum_x(m, n, i, j, dx, dy, &MATTSSON);
+    case 8: return flux_y_capillarity_momentum_x(m, n, i, j, dx, dy, &MATTSSON);
+    case 9: return flux_y_momentum_x(m, n, i, j, dx, dy, &MATTSSON);
+    case 10: return flux_x_inviscid_momentum_y(m, n, i, j, &MATTSSON);
+    case 11: return flux_x_viscid_momentum_y(m, n, i, j, dx, dy, &MATTSSON);
+    case 12: return flux_x_capillarity_momentum_y(m, n, i, j, dx, dy, &MATTSSON);
+    case 13: return flux_x_momentum_y(m, n, i, j, dx, dy, &MATTSSON);
+    case 14: return flux_y_inviscid_momentum_y(m, n, i, j, &MATTSSON);
+    case 15: return flux_y_viscid_momentum_y(m, n, i, j, dx, dy, &MATTSSON);
+    case 16: return flux_y_capillarity_momentum_y(m, n, i, j, dx, dy, &MATTSSON);
+    case 17: return flux_y_momentum_y(m, n, i, j, dx, dy, &MATTSSON);
+    case 18: return flux_x_inviscid_total_energy(m, n, i, j, &MATTSSON);
+    case 19: return flux_x_viscid_total_energy(m, n, i, j, dx, dy, &MATTSSON);
+    case 20: return flux_x_capillarity_total_energy(m, n, i, j, dx, dy, &MATTSSON);
+    case 21: return flux_x_total_energy(m, n, i, j, dx, dy, &MATTSSON);
+    case 22: return flux_y_inviscid_total_energy(m, n, i, j, &MATTSSON);
+    case 23: return flux_y_viscid_total_energy(m, n, i, j, dx, dy, &MATTSSON);
+    case 24: return flux_y_capillarity_total_energy(m, n, i, j, dx, dy, &MATTSSON);
+    case 25: return flux_y_total_energy(m, n, i, j, dx, dy, &MATTSSON);
     default: return NAN;
   }
 }
@@ -799,6 +824,246 @@ void orc_wall_flux(const orc_model* m, const double* n, int dir_y, int i, int j,
 /* compute_time_dev: td_operators_class.f:119-235                      */
 /* ------------------------------------------------------------------ */
 
+/* ------------------------------------------------------------------ */
+/* open boundaries (hedstrom_xy): one-sided stencils and the           */
+/* characteristic time derivative                                       */
+/* ------------------------------------------------------------------ */
+
+/* one-sided gradients, mattsson_operators/sd_operators_fd_module.f:182-450 */
+static double gradient_x_x_oneside_L0(const orc_model* m, const double* n, int i, int j,
+                                      prim_fn p, double dn) {
+  return 1.0 / dn * (-p(m, n, i, j) + p(m, n, i + 1, j));
+}
+static double gradient_x_x_oneside_R0(const orc_model* m, const double* n, int i, int j,
+                                      prim_fn p, double dn) {
+  return 1.0 / dn * (-p(m, n, i - 1, j) + p(m, n, i, j));
+}
+static double gradient_y_y_oneside_L0(const orc_model* m, const double* n, int i, int j,
+                                      prim_fn p, double dn) {
+  return 1.0 / dn * (-p(m, n, i, j) + p(m, n, i, j + 1));
+}
+static double gradient_y_y_oneside_R0(const orc_model* m, const double* n, int i, int j,
+                                      prim_fn p, double dn) {
+  return 1.0 / dn * (-p(m, n, i, j - 1) + p(m, n, i, j));
+}
+
+/* x-face stencils of the outer S ghost row, sd_operators_y_oneside_L0_class.f */
+static double dfdy_y_L0(const orc_model* m, const double* n, int i, int j, prim_fn p, double dy) {
+  return 1.0 / dy *
+         (-0.5 * p(m, n, i - 1, j) - 0.5 * p(m, n, i, j) + 0.5 * p(m, n, i - 1, j + 1) +
+          0.5 * p(m, n, i, j + 1));
+}
+static double d2fdy2_y_L0(const orc_model* m, const double* n, int i, int j, prim_fn p,
+                          double dy) {
+  return 1.0 / (dy * dy) *
+         (0.5 * p(m, n, i - 1, j) + 0.5 * p(m, n, i, j) -
+          2.0 * (+0.5 * p(m, n, i - 1, j + 1) + 0.5 * p(m, n, i, j + 1)) +
+          0.5 * p(m, n, i - 1, j + 2) + 0.5 * p(m, n, i, j + 2));
+}
+static double dfdx_y_L0_nl(const orc_model* m, const double* n, int i, int j, sec_fn p,
+                           double dx, double dy) {
+  return 1.0 / dx *
+         (-p(m, n, i - 1, j, dx, dy, gradient_x_interior, gradient_y_y_oneside_L0) +
+          p(m, n, i, j, dx, dy, gradient_x_interior, gradient_y_y_oneside_L0));
+}
+/* x-face stencils of the outer N ghost row, sd_operators_y_oneside_R0_class.f */
+static double dfdy_y_R0(const orc_model* m, const double* n, int i, int j, prim_fn p, double dy) {
+  return 1.0 / dy *
+         (-0.5 * p(m, n, i - 1, j - 1) - 0.5 * p(m, n, i, j - 1) + 0.5 * p(m, n, i - 1, j) +
+          0.5 * p(m, n, i, j));
+}
+static double d2fdy2_y_R0(const orc_model* m, const double* n, int i, int j, prim_fn p,
+                          double dy) {
+  return 1.0 / (dy * dy) *
+         (0.5 * p(m, n, i - 1, j - 2) + 0.5 * p(m, n, i, j - 2) -
+          2.0 * (+0.5 * p(m, n, i - 1, j - 1) + 0.5 * p(m, n, i, j - 1)) +
+          0.5 * p(m, n, i - 1, j) + 0.5 * p(m, n, i, j));
+}
+static double dfdx_y_R0_nl(const orc_model* m, const double* n, int i, int j, sec_fn p,
+                           double dx, double dy) {
+  return 1.0 / dx *
+         (-p(m, n, i - 1, j, dx, dy, gradient_x_interior, gradient_y_y_oneside_R0) +
+          p(m, n, i, j, dx, dy, gradient_x_interior, gradient_y_y_oneside_R0));
+}
+/* y-face stencils of the outer W ghost column, sd_operators_x_oneside_L0_class.f */
+static double dgdx_x_L0(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx) {
+  return 1.0 / dx *
+         (-0.5 * p(m, n, i, j - 1) - 0.5 * p(m, n, i, j) + 0.5 * p(m, n, i + 1, j - 1) +
+          0.5 * p(m, n, i + 1, j));
+}
+static double d2gdx2_x_L0(const orc_model* m, const double* n, int i, int j, prim_fn p,
+                          double dx) {
+  return 1.0 / (dx * dx) *
+         (0.5 * p(m, n, i, j - 1) + 0.5 * p(m, n, i, j) -
+          2.0 * (+0.5 * p(m, n, i + 1, j - 1) + 0.5 * p(m, n, i + 1, j)) +
+          0.5 * p(m, n, i + 2, j - 1) + 0.5 * p(m, n, i + 2, j));
+}
+static double dgdy_x_L0_nl(const orc_model* m, const double* n, int i, int j, sec_fn p,
+                           double dx, double dy) {
+  return 1.0 / dy *
+         (-p(m, n, i, j - 1, dx, dy, gradient_x_x_oneside_L0, gradient_y_interior) +
+          p(m, n, i, j, dx, dy, gradient_x_x_oneside_L0, gradient_y_interior));
+}
+/* y-face stencils of the outer E ghost column, sd_operators_x_oneside_R0_class.f */
+static double dgdx_x_R0(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx) {
+  return 1.0 / dx *
+         (-0.5 * p(m, n, i - 1, j - 1) - 0.5 * p(m, n, i - 1, j) + 0.5 * p(m, n, i, j - 1) +
+          0.5 * p(m, n, i, j));
+}
+static double d2gdx2_x_R0(const orc_model* m, const double* n, int i, int j, prim_fn p,
+                          double dx) {
+  return 1.0 / (dx * dx) *
+         (0.5 * p(m, n, i - 2, j - 1) + 0.5 * p(m, n, i - 2, j) -
+          2.0 * (+0.5 * p(m, n, i - 1, j - 1) + 0.5 * p(m, n, i - 1, j)) +
+          0.5 * p(m, n, i, j - 1) + 0.5 * p(m, n, i, j));
+}
+static double dgdy_x_R0_nl(const orc_model* m, const double* n, int i, int j, sec_fn p,
+                           double dx, double dy) {
+  return 1.0 / dy *
+         (-p(m, n, i, j - 1, dx, dy, gradient_x_x_oneside_R0, gradient_y_interior) +
+          p(m, n, i, j, dx, dy, gradient_x_x_oneside_R0, gradient_y_interior));
+}
+/* members a face family does not use keep the interior stencils */
+static const sd_ops Y_L0 = {s_f,    s_dfdx, dfdx_y_L0_nl, dfdy_y_L0, s_d2fdx2, d2fdy2_y_L0,
+                            s_g,    s_dgdx, s_dgdy,       s_dgdy_nl, s_d2gdx2, s_d2gdy2};
+static const sd_ops Y_R0 = {s_f,    s_dfdx, dfdx_y_R0_nl, dfdy_y_R0, s_d2fdx2, d2fdy2_y_R0,
+                            s_g,    s_dgdx, s_dgdy,       s_dgdy_nl, s_d2gdx2, s_d2gdy2};
+static const sd_ops X_L0 = {s_f,    s_dfdx,    s_dfdx_nl, s_dfdy,       s_d2fdx2,    s_d2fdy2,
+                            s_g,    dgdx_x_L0, s_dgdy,    dgdy_x_L0_nl, d2gdx2_x_L0, s_d2gdy2};
+static const sd_ops X_R0 = {s_f,    s_dfdx,    s_dfdx_nl, s_dfdy,       s_d2fdx2,    s_d2fdy2,
+                            s_g,    dgdx_x_R0, s_dgdy,    dgdy_x_R0_nl, d2gdx2_x_R0, s_d2gdy2};
+
+/* the one-sided stencils of the density, for the pins: 16 dfdy, 17 d2fdy2,
+ * 18 dfdx_nl of sd_operators_y_oneside_L0; 19-21 the same of _y_oneside_R0;
+ * 22 dgdx, 23 d2gdx2, 24 dgdy_nl of sd_operators_x_oneside_L0; 25-27 of
+ * _x_oneside_R0 (nl: of gradient_x + gradient_y of the density) */
+double orc_oneside_density_stencil(const orc_model* m, const double* n, int which, int i, int j,
+                                   double dx, double dy) {
+  prim_fn p = orc_mass_density;
+  switch (which) {
+    case 16: return dfdy_y_L0(m, n, i, j, p, dy);
+    case 17: return d2fdy2_y_L0(m, n, i, j, p, dy);
+    case 18: return dfdx_y_L0_nl(m, n, i, j, gradient_av, dx, dy);
+    case 19: return dfdy_y_R0(m, n, i, j, p, dy);
+    case 20: return d2fdy2_y_R0(m, n, i, j, p, dy);
+    case 21: return dfdx_y_R0_nl(m, n, i, j, gradient_av, dx, dy);
+    case 22: return dgdx_x_L0(m, n, i, j, p, dx);
+    case 23: return d2gdx2_x_L0(m, n, i, j, p, dx);
+    case 24: return dgdy_x_L0_nl(m, n, i, j, gradient_av, dx, dy);
+    case 25: return dgdx_x_R0(m, n, i, j, p, dx);
+    case 26: return d2gdx2_x_R0(m, n, i, j, p, dx);
+    case 27: return dgdy_x_R0_nl(m, n, i, j, gradient_av, dx, dy);
+    default: return NAN;
+  }
+}
+
+/* compute_flux_x_oneside / compute_flux_y_oneside with obc_edge_flux_capillarity
+ * (pmodel_eq_class.f:1389-1595): the full fluxes with the given operators */
+static void flux_x_ops(const orc_model* m, const double* n, int i, int j, double dx, double dy,
+                       const sd_ops* S, double f[4]) {
+  f[0] = flux_x_mass_density(m, n, i, j, S);
+  f[1] = flux_x_momentum_x(m, n, i, j, dx, dy, S);
+  f[2] = flux_x_momentum_y(m, n, i, j, dx, dy, S);
+  f[3] = flux_x_total_energy(m, n, i, j, dx, dy, S);
+}
+static void flux_y_ops(const orc_model* m, const double* n, int i, int j, double dx, double dy,
+                       const sd_ops* S, double f[4]) {
+  f[0] = flux_y_mass_density(m, n, i, j, S);
+  f[1] = flux_y_momentum_x(m, n, i, j, dx, dy, S);
+  f[2] = flux_y_momentum_y(m, n, i, j, dx, dy, S);
+  f[3] = flux_y_total_energy(m, n, i, j, dx, dy, S);
+}
+
+/* get_prim_obc_eigenqties with obc_eigenqties_bc (pmodel_eq_class.f:2142-2195):
+ * [rho, ux, uy, P, c] of the boundary point itself */
+static void obc_eigenqties_bc(const orc_model* m, const double v[4], double pe[5]) {
+  pe[0] = v[0];
+  pe[1] = v[1] / v[0];
+  pe[2] = v[2] / v[0];
+  pe[3] = classical_pressure_local(m, v);
+  pe[4] = orc_speed_of_sound_local(m, v);
+}
+
+/* compute_timedev_with_openbc (hedstrom_xy_module.f:445-558) with the
+ * eigen-quantities of ns_vdw2d_prim_module.f:249-891 (Fortran matrices
+ * A(row,col) stored as A[row-1][col-1]) */
+void orc_openbc_timedev(const orc_model* m, const double* n, int i, int j, int ydir, int right,
+                        int grad_type, double dn, const double pe_in[5], double out[4]) {
+  double v[4] = {ND(n, i, j, 1), ND(n, i, j, 2), ND(n, i, j, 3), ND(n, i, j, 4)};
+  double pe[5];
+  if (pe_in)
+    for (int q = 0; q < 5; ++q) pe[q] = pe_in[q];
+  else
+    obc_eigenqties_bc(m, v, pe);
+  const double md = pe[0], ux = pe[1], uy = pe[2], P = pe[3], c = pe[4];
+  double lam[4], L[4][4], R[4][4], J[4][4];
+  const double q_c = 0.5 * md * c;
+  const double inv_c = 1.0 / (c * c);
+  const double inv_q_c = 1.0 / (md * c);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) L[a][b] = R[a][b] = J[a][b] = 0.0;
+  if (!ydir) {
+    /* compute_x_eigenvalues/left/righteigenvector_ns_vdw2d :504-722 */
+    lam[0] = ux; lam[1] = ux; lam[2] = ux - c; lam[3] = ux + c;
+    L[2][0] = 1.0;
+    L[0][1] = 1.0; L[3][1] = -1.0 / (c * c);
+    L[1][2] = -q_c; L[3][2] = 0.5;
+    L[1][3] = q_c; L[3][3] = 0.5;
+    R[1][0] = 1.0; R[2][0] = inv_c; R[3][0] = inv_c;
+    R[2][1] = -inv_q_c; R[3][1] = inv_q_c;
+    R[0][2] = 1.0;
+    R[2][3] = 1.0; R[3][3] = 1.0;
+  } else {
+    /* compute_y_eigenvalues/left/righteigenvector_ns_vdw2d :539-891 */
+    lam[0] = uy; lam[1] = uy; lam[2] = uy - c; lam[3] = uy + c;
+    L[1][0] = 1.0;
+    L[0][1] = 1.0; L[3][1] = -1.0 / (c * c);
+    L[2][2] = -q_c; L[3][2] = 0.5;
+    L[2][3] = q_c; L[3][3] = 0.5;
+    R[1][0] = 1.0; R[2][0] = inv_c; R[3][0] = inv_c;
+    R[0][1] = 1.0;
+    R[2][2] = -inv_q_c; R[3][2] = inv_q_c;
+    R[2][3] = 1.0; R[3][3] = 1.0;
+  }
+  /* compute_jacobian_cons_to_prim_ns_vdw2d :249-320 */
+  const double dEdrho = 0.5 * (ux * ux + uy * uy) - 6.0 * md -
+                        1.0 / 3.0 * m->cv_r * (P + 9.0 * (md - 2.0) * md);
+  J[0][0] = 1.0;
+  J[0][1] = ux; J[1][1] = md;
+  J[0][2] = uy; J[2][2] = md;
+  J[0][3] = dEdrho; J[1][3] = md * ux; J[2][3] = md * uy; J[3][3] = m->cv_r * (1.0 - md / 3.0);
+  /* compute_gradient_prim (pmodel_eq_class.f:3113-3153) */
+  grad_fn gr = grad_type == 0 ? (ydir ? gradient_y_y_oneside_L0 : gradient_x_x_oneside_L0)
+                              : (ydir ? gradient_y_y_oneside_R0 : gradient_x_x_oneside_R0);
+  const double g[4] = {gr(m, n, i, j, orc_mass_density, dn), gr(m, n, i, j, orc_velocity_x, dn),
+                       gr(m, n, i, j, orc_velocity_y, dn),
+                       gr(m, n, i, j, orc_classical_pressure, dn)};
+  /* incoming_left: lambda >= 0, incoming_right: lambda <= 0
+   * (openbc_operators_module.f:97-136): no amplitude for incoming waves */
+  double amp[4];
+  for (int k = 0; k < 4; ++k) {
+    const int incoming = right ? (lam[k] <= 0.0) : (lam[k] >= 0.0);
+    if (incoming) {
+      amp[k] = 0.0;
+    } else {
+      double dot = 0.0;
+      for (int r = 0; r < 4; ++r) dot += L[r][k] * g[r];
+      amp[k] = -lam[k] * dot;
+    }
+  }
+  double tp[4];
+  for (int col = 0; col < 4; ++col) {
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k) acc += amp[k] * R[k][col];
+    tp[col] = acc;
+  }
+  for (int col = 0; col < 4; ++col) {
+    double acc = 0.0;
+    for (int l = 0; l < 4; ++l) acc += tp[l] * J[l][col];
+    out[col] = acc;
+  }
+}
+
 /* compute_body_forces, pmodel_eq_class.f:1840-1918 (conservative form) */
 static double body_force(const orc_model* m, const double* v, int k) {
   if (!m->gravity_ac) return 0.0;
@@ -824,18 +1089,18 @@ void orc_compute_time_dev(const orc_model* m, double t, const double* nodes, con
   /* compute_flux_x, pmodel_eq_class.f:1073-1139 */
   for (int j = 1 + BC_SIZE; j <= ny - BC_SIZE; ++j)
     for (int i = 1 + BC_SIZE; i <= nx + 1 - BC_SIZE; ++i) {
-      FX(i, j, 1) = flux_x_mass_density(m, nodes, i, j);
-      FX(i, j, 2) = flux_x_momentum_x(m, nodes, i, j, dx, dy);
-      FX(i, j, 3) = flux_x_momentum_y(m, nodes, i, j, dx, dy);
-      FX(i, j, 4) = flux_x_total_energy(m, nodes, i, j, dx, dy);
+      FX(i, j, 1) = flux_x_mass_density(m, nodes, i, j, &MATTSSON);
+      FX(i, j, 2) = flux_x_momentum_x(m, nodes, i, j, dx, dy, &MATTSSON);
+      FX(i, j, 3) = flux_x_momentum_y(m, nodes, i, j, dx, dy, &MATTSSON);
+      FX(i, j, 4) = flux_x_total_energy(m, nodes, i, j, dx, dy, &MATTSSON);
     }
   /* compute_flux_y, :1140-1214 */
   for (int j = 1 + BC_SIZE; j <= ny + 1 - BC_SIZE; ++j)
     for (int i = 1 + BC_SIZE; i <= nx - BC_SIZE; ++i) {
-      FY(i, j, 1) = flux_y_mass_density(m, nodes, i, j);
-      FY(i, j, 2) = flux_y_momentum_x(m, nodes, i, j, dx, dy);
-      FY(i, j, 3) = flux_y_momentum_y(m, nodes, i, j, dx, dy);
-      FY(i, j, 4) = flux_y_total_energy(m, nodes, i, j, dx, dy);
+      FY(i, j, 1) = flux_y_mass_density(m, nodes, i, j, &MATTSSON);
+      FY(i, j, 2) = flux_y_momentum_x(m, nodes, i, j, dx, dy, &MATTSSON);
+      FY(i, j, 3) = flux_y_momentum_y(m, nodes, i, j, dx, dy, &MATTSSON);
+      FY(i, j, 4) = flux_y_total_energy(m, nodes, i, j, dx, dy, &MATTSSON);
     }
 
   /* apply_bc_on_fluxes for wall edges (bc_operators_wall_xy_class.f:702-854),
@@ -870,6 +1135,67 @@ void orc_compute_time_dev(const orc_model* m, double t, const double* nodes, con
         ND(time_dev, i, j, k) = (FX(i, j, k) / dx - FX(i + 1, j, k) / dx) +
                                 (FY(i, j, k) / dy - FY(i, j + 1, k) / dy) + body_force(m, v, k);
       }
+
+  /* apply_bc_on_timedev of hedstrom_xy on every edge
+   * (bc_operators_hedstrom_xy_class.f:182-496): sections of
+   * bc_operators_gen_class.f:95-102, edges with one-sided fluxes
+   * (bc_operators_openbc_normal_class.f:563-787) and the characteristic term
+   * (hedstrom_xy_module.f:175-400), corners with both characteristic terms */
+  if (m->bc_n == ORC_BC_OPEN) {
+    const int B = BC_SIZE;
+    double f[4], o[4];
+    /* N and S edges: x-fluxes of the ghost rows, i in [B+1, nx-B+1] */
+    for (int e = 0; e < 2; ++e) {
+      const int j0 = e == 0 ? ny - B + 1 : 1;
+      for (int jj = 0; jj < 2; ++jj) {
+        const int j = j0 + jj;
+        const sd_ops* S = e == 0 ? (jj == 0 ? &MATTSSON : &Y_R0) : (jj == 0 ? &Y_L0 : &MATTSSON);
+        for (int i = B + 1; i <= nx - B + 1; ++i) {
+          flux_x_ops(m, nodes, i, j, dx, dy, S, f);
+          for (int k = 1; k <= 4; ++k) FX(i, j, k) = f[k - 1];
+        }
+        for (int i = B + 1; i <= nx - B; ++i) {
+          double v[4] = {ND(nodes, i, j, 1), ND(nodes, i, j, 2), ND(nodes, i, j, 3), ND(nodes, i, j, 4)};
+          orc_openbc_timedev(m, nodes, i, j, 1, e == 0, e == 0, dy, NULL, o);
+          for (int k = 1; k <= 4; ++k)
+            ND(time_dev, i, j, k) = 1.0 / dx * (FX(i, j, k) - FX(i + 1, j, k)) + o[k - 1] +
+                                    body_force(m, v, k);
+        }
+      }
+    }
+    /* E and W edges: y-fluxes of the ghost columns, j in [B+1, ny-B+1] */
+    for (int e = 0; e < 2; ++e) {
+      const int i0 = e == 0 ? nx - B + 1 : 1;
+      for (int ii = 0; ii < 2; ++ii) {
+        const int i = i0 + ii;
+        const sd_ops* S = e == 0 ? (ii == 0 ? &MATTSSON : &X_R0) : (ii == 0 ? &X_L0 : &MATTSSON);
+        for (int j = B + 1; j <= ny - B + 1; ++j) {
+          flux_y_ops(m, nodes, i, j, dx, dy, S, f);
+          for (int k = 1; k <= 4; ++k) FY(i, j, k) = f[k - 1];
+        }
+        for (int j = B + 1; j <= ny - B; ++j) {
+          double v[4] = {ND(nodes, i, j, 1), ND(nodes, i, j, 2), ND(nodes, i, j, 3), ND(nodes, i, j, 4)};
+          orc_openbc_timedev(m, nodes, i, j, 0, e == 0, e == 0, dx, NULL, o);
+          for (int k = 1; k <= 4; ++k)
+            ND(time_dev, i, j, k) = o[k - 1] + 1.0 / dy * (FY(i, j, k) - FY(i, j + 1, k)) +
+                                    body_force(m, v, k);
+        }
+      }
+    }
+    /* corners: x- and y-characteristic terms */
+    for (int cy = 0; cy < 2; ++cy)
+      for (int cx = 0; cx < 2; ++cx)
+        for (int jj = 0; jj < 2; ++jj)
+          for (int ii = 0; ii < 2; ++ii) {
+            const int i = (cx ? nx - B + 1 : 1) + ii, j = (cy ? ny - B + 1 : 1) + jj;
+            double v[4] = {ND(nodes, i, j, 1), ND(nodes, i, j, 2), ND(nodes, i, j, 3), ND(nodes, i, j, 4)};
+            double ox[4], oy[4];
+            orc_openbc_timedev(m, nodes, i, j, 0, cx, cx, dx, NULL, ox);
+            orc_openbc_timedev(m, nodes, i, j, 1, cy, cy, dy, NULL, oy);
+            for (int k = 1; k <= 4; ++k)
+              ND(time_dev, i, j, k) = ox[k - 1] + oy[k - 1] + body_force(m, v, k);
+          }
+  }
 #undef FX
 #undef FY
   free(flux_x);
@@ -983,6 +1309,7 @@ void orc_apply_bc_on_nodes(const orc_model* m, double t, const double* x_map, co
   const double dx = x_map[1] - x_map[0];
   const double dy = y_map[1] - y_map[0];
   const int B = BC_SIZE;
+  if (m->bc_n == ORC_BC_OPEN) return; /* hedstrom_xy: the ghost cells are integrated */
   const int allrefl = m->bc_w == ORC_BC_REFLECTION && m->bc_e == ORC_BC_REFLECTION &&
                       m->bc_s == ORC_BC_REFLECTION && m->bc_n == ORC_BC_REFLECTION;
   if (allrefl) {
@@ -1045,11 +1372,17 @@ void orc_rk3_step(const orc_model* m, double t, double dt, const double* x_map,
   }
   orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
 
+  /* x/y_borders (field_abstract_class.f:244-270): widened to the ghost cells
+   * on edges whose time derivative is computed (open boundaries) */
+  const int op = m->bc_n == ORC_BC_OPEN;
+  const int i0 = op ? 1 : B + 1, i1 = op ? nx : nx - B;
+  const int j0 = op ? 1 : B + 1, j1 = op ? ny : ny - B;
+
   /* stage 2 (compute_2nd_step :205-269) over x/y_borders */
   orc_compute_time_dev(m, t, nodes, x_map, y_map, time_dev);
   for (int k = 1; k <= 4; ++k)
-    for (int j = B + 1; j <= ny - B; ++j)
-      for (int i = B + 1; i <= nx - B; ++i)
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i)
         ND(nodes, i, j, k) =
             b2 * ND(nodes_tmp, i, j, k) + (1.0 - b2) * (ND(nodes, i, j, k) + dt * ND(time_dev, i, j, k));
   orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
@@ -1057,8 +1390,8 @@ void orc_rk3_step(const orc_model* m, double t, double dt, const double* x_map,
   /* stage 3 (compute_3rd_step :424-490) */
   orc_compute_time_dev(m, t, nodes, x_map, y_map, time_dev);
   for (int k = 1; k <= 4; ++k)
-    for (int j = B + 1; j <= ny - B; ++j)
-      for (int i = B + 1; i <= nx - B; ++i)
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i)
         ND(nodes, i, j, k) =
             b3 * ND(nodes_tmp, i, j, k) + (1.0 - b3) * (ND(nodes, i, j, k) + dt * ND(time_dev, i, j, k));
   orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
@@ -1089,13 +1422,16 @@ void orc_lsrk3_step(const orc_model* m, double t, double dt, const double* x_map
   double* time_dev = work + plane * 4;
   double* guess = work + plane * 8;
   const int B = BC_SIZE;
+  const int op = m->bc_n == ORC_BC_OPEN; /* borders as in orc_rk3_step */
+  const int i0 = op ? 1 : B + 1, i1 = op ? nx : nx - B;
+  const int j0 = op ? 1 : B + 1, j1 = op ? ny : ny - B;
   for (size_t i = 0; i < plane * 4; ++i) q[i] = 0.0;
   for (int s = 0; s < 3; ++s) {
     memcpy(guess, nodes, plane * 4 * sizeof(double));
     orc_compute_time_dev(m, t, nodes, x_map, y_map, time_dev);
     for (int k = 1; k <= 4; ++k)
-      for (int j = B + 1; j <= ny - B; ++j)
-        for (int i = B + 1; i <= nx - B; ++i)
+      for (int j = j0; j <= j1; ++j)
+        for (int i = i0; i <= i1; ++i)
           lsrk_point(s, dt * ND(time_dev, i, j, k), &ND(q, i, j, k), &ND(nodes, i, j, k));
     orc_apply_bc_on_nodes(m, t, x_map, y_map, guess, nodes);
   }

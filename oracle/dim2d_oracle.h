@@ -31,7 +31,8 @@ extern "C" {
 #endif
 
 /* boundary kinds per edge */
-enum { ORC_BC_REFLECTION = 0, ORC_BC_PERIODIC = 1, ORC_BC_WALL = 2 };
+enum { ORC_BC_REFLECTION = 0, ORC_BC_PERIODIC = 1, ORC_BC_WALL = 2, ORC_BC_OPEN = 3 };
+/* ORC_BC_OPEN: hedstrom_xy, only on all four edges at once */
 
 typedef struct orc_model {
   int nx, ny;            /* total grid points including bc_size=2 ghosts */
@@ -60,7 +61,11 @@ double orc_speed_of_sound_local(const orc_model*, const double nodes4[4]);
 /* ---- Mattsson stencil values of the mass density (pins) ----
  * which: 0 f, 1 dfdx, 2 dfdy, 3 d2fdx2, 4 d2fdy2, 5 d2fdxdy, 6 gradient_x,
  *        7 g, 8 dgdx, 9 dgdy, 10 d2gdx2, 11 d2gdy2, 12 d2gdxdy, 13 gradient_y,
- *        14 dfdx_nl, 15 dgdy_nl (of gradient_x + gradient_y of the density)  */
+ *        14 dfdx_nl, 15 dgdy_nl (of gradient_x + gradient_y of the density),
+ *        16-27 the one-sided stencils of the open boundaries
+ *        (sd_operators_{y,x}_oneside_{L0,R0}_class.f): 16 dfdy, 17 d2fdy2,
+ *        18 dfdx_nl of y_L0; 19-21 of y_R0; 22 dgdx, 23 d2gdx2, 24 dgdy_nl of
+ *        x_L0; 25-27 of x_R0  */
 double orc_mattsson_density_stencil(const orc_model*, const double* nodes, int which,
                                     int i, int j, double dx, double dy);
 
@@ -98,6 +103,15 @@ double orc_lsrk3_amplification(double z);
  * Returns 0, or the 1-based step at which a non-finite interior value appeared. */
 int orc_run(const orc_model*, double t0, int nsteps, const double* dts,
             const double* x_map, const double* y_map, double* nodes);
+
+/* characteristic time derivative of an open boundary point
+ * (hedstrom_xy_module.f:445-558): direction ydir (0: x, 1: y), side right
+ * (0: left, 1: right), one-sided gradient grad_type (0: L0, 1: R0) with
+ * spacing dn; eigen-quantities [rho, ux, uy, P, c] from the point itself
+ * (obc_eigenqties_bc) when pe is NULL, else the given ones
+ * (obc_eigenqties_lin). */
+void orc_openbc_timedev(const orc_model*, const double* nodes, int i, int j, int ydir, int right,
+                        int grad_type, double dn, const double pe[5], double out[4]);
 
 /* coordinates (field_abstract_class.f:286-315) */
 void orc_coordinates(int nx, int ny, double x_min, double x_max, double y_min,

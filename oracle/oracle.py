@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "libdim2d_oracle.so")
 
-BC_REFLECTION, BC_PERIODIC, BC_WALL = 0, 1, 2
+BC_REFLECTION, BC_PERIODIC, BC_WALL, BC_OPEN = 0, 1, 2, 3
 INT_RK3TVD, INT_LSRK3 = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -59,6 +59,11 @@ def lib():
         L.orc_apply_bc_on_nodes.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_rk3_step.argtypes = [M, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_lsrk3_step.argtypes = L.orc_rk3_step.argtypes
+        L.orc_classical_pressure.restype = ctypes.c_double
+        L.orc_classical_pressure.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_openbc_timedev.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_void_p, _dp]
         L.orc_lsrk3_amplification.restype = ctypes.c_double
         L.orc_lsrk3_amplification.argtypes = [ctypes.c_double]
         L.orc_run.restype = ctypes.c_int
@@ -145,6 +150,16 @@ def run(m: Model, nodes, x_map, y_map, dts, t0=0.0):
     dts = np.ascontiguousarray(dts, dtype=np.float64)
     return lib().orc_run(ctypes.byref(m), t0, len(dts), _ptr(dts), _ptr(x_map), _ptr(y_map),
                          _ptr(nodes))
+
+
+def openbc_timedev(m: Model, nodes, i, j, ydir, right, grad_type, dn, pe=None):
+    """Characteristic time derivative of an open boundary point (1-based i, j);
+    pe = [rho, ux, uy, P, c] given explicitly, or None for the point's own."""
+    out = np.zeros(4)
+    pe_arr = None if pe is None else np.ascontiguousarray(pe, dtype=np.float64)
+    lib().orc_openbc_timedev(ctypes.byref(m), _ptr(nodes), i, j, ydir, right, grad_type, dn,
+                             None if pe_arr is None else pe_arr.ctypes.data, _ptr(out))
+    return out
 
 
 def lsrk3_amplification(z):

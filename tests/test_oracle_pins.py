@@ -289,3 +289,129 @@ def test_lsrk3_converges_to_rk3tvd_at_third_order(orc):
         diffs.append(np.abs(a[:, 2:-2, 2:-2] - b[:, 2:-2, 2:-2]).max())
     r1, r2 = diffs[0] / diffs[1], diffs[1] / diffs[2]
     assert 6.0 < r1 < 10.5 and 6.0 < r2 < 10.5, (diffs, r1, r2)
+
+
+# ---- open boundaries: hedstrom_xy ----
+
+def test_oneside_stencils_closed_forms(orc):
+    """The one-sided face stencils of the open edges
+    (sd_operators_{y,x}_oneside_{L0,R0}_class.f) against exact derivatives of
+    polynomials at the points the stencils are centred on: first differences
+    give f' at the half-way point, second differences f'' of a cubic at the
+    middle point; a shifted row or column fails."""
+    m, xm, ym = _grid(orc, 9, 8)
+    dx, dy = xm[1] - xm[0], ym[1] - ym[0]
+    X, Y = np.meshgrid(xm, ym)
+    i, j = 5, 6  # 1-based, away from the array edges
+    xh = lambda a: 0.5 * (xm[a - 1] + xm[a - 2])   # x_{a-1/2} (1-based a)
+    yh = lambda b: 0.5 * (ym[b - 1] + ym[b - 2])
+    x, y = lambda a: xm[a - 1], lambda b: ym[b - 1]
+
+    def st(which, f):
+        nodes = np.zeros((4, m.ny, m.nx))
+        nodes[0] = f
+        return orc.stencil(m, nodes, which, i, j, dx, dy)
+
+    close = lambda a, b: math.isclose(a, b, rel_tol=1e-11, abs_tol=1e-11)
+    # y-one-sided x-face stencils at (i-1/2, j)
+    assert close(st(16, Y ** 2 + 3 * X), 2 * yh(j + 1))          # L0: f' at y_{j+1/2}
+    assert close(st(19, Y ** 2 + 3 * X), 2 * yh(j))              # R0: f' at y_{j-1/2}
+    assert close(st(17, X * Y ** 3), xh(i) * 6 * y(j + 1))       # L0: f'' at y_{j+1}
+    assert close(st(20, X * Y ** 3), xh(i) * 6 * y(j - 1))       # R0: f'' at y_{j-1}
+    f = X ** 2 * Y ** 2  # nl: d/dx of (central d/dx + one-sided d/dy)
+    assert close(st(18, f), 4 * xh(i) * yh(j + 1) + 2 * y(j) ** 2)
+    assert close(st(21, f), 4 * xh(i) * yh(j) + 2 * y(j) ** 2)
+    # x-one-sided y-face stencils at (i, j-1/2)
+    assert close(st(22, X ** 2 + 3 * Y), 2 * xh(i + 1))
+    assert close(st(25, X ** 2 + 3 * Y), 2 * xh(i))
+    assert close(st(23, Y * X ** 3), yh(j) * 6 * x(i + 1))
+    assert close(st(26, Y * X ** 3), yh(j) * 6 * x(i - 1))
+    assert close(st(24, f), 4 * xh(i + 1) * yh(j) + 2 * x(i) ** 2)
+    assert close(st(27, f), 4 * xh(i) * yh(j) + 2 * x(i) ** 2)
+
+
+def test_openbc_timedev_known_answers(orc):
+    """compute_timedev_with_openbc on the reference's 3-point DIM data, left
+    (L0 gradient) and right (R0) boundaries, x direction, dx = 0.1, with the
+    far-field eigen-quantities [rho, ux, uy, P, c] that test checks
+    (test_bc_operators/test_hedstrom_xy.f:54-178)."""
+    pe = [1.46510213931996, 0.1, 0.0, 0.814835776, 2.18134901757932]
+    m = orc.make_model(3, 1)
+    L0 = np.array([[1.46, 1.27, 1.47], [0.146, 0.143, 0.145], [0.01, 0.002, 0.05],
+                   [4.89, 4.87, 4.86]]).reshape(4, 1, 3)
+    R0 = np.array([[1.47, 1.27, 1.46], [-0.145, -0.143, -0.146], [0.05, 0.002, 0.01],
+                   [4.86, 4.87, 4.89]]).reshape(4, 1, 3)
+    got_l = orc.openbc_timedev(m, np.ascontiguousarray(L0), 1, 1, 0, 0, 0, 0.1, pe)
+    got_r = orc.openbc_timedev(m, np.ascontiguousarray(R0), 3, 1, 0, 1, 1, 0.1, pe)
+    exp_l = [-1.18764115727727, 2.471895756, 0.0, -2.70907495789801]
+    exp_r = [-1.38610288419910, -2.97821061646847, -0.007727698696229735, -3.23478486279010]
+    for g, e in zip(list(got_l) + list(got_r), exp_l + exp_r):
+        assert math.isclose(g, e, rel_tol=1e-9, abs_tol=1e-12), (got_l, got_r)
+
+
+def _cons_of_prim(prim, cv=2.5):
+    """U(rho, u, v, P) of the van der Waals fluid: rho e = (P + 3 rho^2)(3 - rho) cv/3 - 3 rho^2."""
+    r, u, v, P = prim
+    return np.array([r, r * u, r * v, (P + 3 * r * r) * (3 - r) * cv / 3 - 3 * r * r + 0.5 * r * (u * u + v * v)])
+
+
+@pytest.mark.parametrize("ydir", [0, 1])
+def test_openbc_supersonic_outflow_is_euler(orc, ydir):
+    """With every wave leaving through the boundary, the characteristic time
+    derivative is the full 1-D Euler one, dU/dt = -(dU/dprim) A dprim/dn with
+    the primitive Jacobian A of the van der Waals Euler equations; with every
+    wave entering it vanishes."""
+    rng = np.random.default_rng(4)
+    shape = (4, 3, 1) if ydir else (4, 1, 3)
+    m = orc.make_model(shape[2], shape[1])
+    prim = np.array([[1.1, 1.25, 1.4], [0.3, 0.35, 0.32], [0.2, 0.1, 0.25], [0.6, 0.7, 0.65]])
+    nodes = np.stack([_cons_of_prim(prim[:, a]) for a in range(3)], axis=1).reshape(shape)
+    nodes = np.ascontiguousarray(nodes)
+    c = 0.5
+    un = 3.0  # normal velocity above c: all eigenvalues positive
+    pe = [1.3, un if not ydir else 0.2, un if ydir else 0.2, 0.7, c]
+    i, j = (1, 3) if ydir else (3, 1)
+    got = orc.openbc_timedev(m, nodes, i, j, ydir, 1, 1, 0.1, pe)
+    # primitive gradient with the R0 one-sided difference
+    pv = np.array([[nodes[0].flat[a], nodes[1].flat[a] / nodes[0].flat[a],
+                    nodes[2].flat[a] / nodes[0].flat[a],
+                    orc.lib().orc_classical_pressure(ctypes.byref(m), orc._ptr(nodes),
+                                                     *( (1, a + 1) if ydir else (a + 1, 1)))]
+                   for a in range(3)])
+    g = (pv[2] - pv[1]) / 0.1
+    r, ux, uy, P = pe[0], pe[1], pe[2], pe[3]
+    if not ydir:
+        A = np.array([[ux, r, 0, 0], [0, ux, 0, 1 / r], [0, 0, ux, 0], [0, r * c * c, 0, ux]])
+    else:
+        A = np.array([[uy, 0, r, 0], [0, uy, 0, 0], [0, 0, uy, 1 / r], [0, 0, r * c * c, uy]])
+    # dU/dprim by central differences of the closed-form U(prim)
+    p0 = np.array([r, ux, uy, P])
+    M = np.zeros((4, 4))
+    for l in range(4):
+        h = 1e-6 * max(1.0, abs(p0[l]))
+        e = np.zeros(4)
+        e[l] = h
+        M[:, l] = (_cons_of_prim(p0 + e) - _cons_of_prim(p0 - e)) / (2 * h)
+    exp = -M @ (A @ g)
+    assert np.allclose(got, exp, rtol=1e-7, atol=1e-9), (got, exp)
+    # the same waves all enter through a left boundary: nothing is imposed
+    i0, j0 = (1, 1)
+    zero = orc.openbc_timedev(m, nodes, i0, j0, ydir, 0, 0, 0.1, pe)
+    assert np.all(zero == 0.0)
+
+
+def test_open_boundaries_keep_a_uniform_flow(orc):
+    """A uniform subsonic flow in a domain with open edges: zero time derivative
+    in every cell including the integrated ghost frame, for both integrators."""
+    m, xm, ym = _grid(orc, 12, 10, bc=(3, 3, 3, 3))
+    nodes = np.empty((4, m.ny, m.nx))
+    r, u, v = 1.3, 0.1, -0.05
+    nodes[0], nodes[1], nodes[2] = r, r * u, r * v
+    nodes[3] = r * (8.0 / 3.0 * 2.5 * 0.95 - 3 * r) + 0.5 * r * (u * u + v * v)
+    td = orc.time_dev(m, nodes, xm, ym)
+    assert np.abs(td).max() < 1e-12
+    for integ in (orc.INT_RK3TVD, orc.INT_LSRK3):
+        m.integrator = integ
+        a = nodes.copy()
+        assert orc.run(m, a, xm, ym, np.full(5, 1e-3)) == 0
+        assert np.abs(a - nodes).max() < 1e-12
