@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("DIM2D_HIP_LIBRARY") or os.path.join(HERE, "lib", "lib
 
 OK = 0
 EINVAL, EDEVICE, ENOMEM, ESTATE, ENONFINITE = -1, -2, -3, -4, -5
-BC_REFLECTION, BC_PERIODIC, BC_WALL = 0, 1, 2
+BC_REFLECTION, BC_PERIODIC, BC_WALL, BC_OPEN = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 INT_RK3TVD, INT_LSRK3 = 0, 1
 

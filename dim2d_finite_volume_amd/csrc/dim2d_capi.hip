@@ -49,9 +49,16 @@ int validate(const dim2d_params* p) {
     return fail(DIM2D_EINVAL, "grid %dx%d too small: need at least 3 interior points per axis",
                 p->nx, p->ny);
   const int b[4] = {p->bc_w, p->bc_e, p->bc_s, p->bc_n};
-  for (int k = 0; k < 4; ++k)
-    if (b[k] < DIM2D_BC_REFLECTION || b[k] > DIM2D_BC_WALL)
+  int nopen = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (b[k] < DIM2D_BC_REFLECTION || b[k] > DIM2D_BC_OPEN)
       return fail(DIM2D_EINVAL, "unknown boundary kind %d", b[k]);
+    nopen += b[k] == DIM2D_BC_OPEN;
+  }
+  if (nopen != 0 && nopen != 4)
+    return fail(DIM2D_EINVAL,
+                "open (Hedstrom) boundaries are supported on all four edges at once "
+                "(hedstrom_xy_choice)");
   if ((p->bc_w == DIM2D_BC_PERIODIC) != (p->bc_e == DIM2D_BC_PERIODIC) ||
       (p->bc_s == DIM2D_BC_PERIODIC) != (p->bc_n == DIM2D_BC_PERIODIC))
     return fail(DIM2D_EINVAL, "periodic boundaries must pair W/E and S/N");
@@ -130,6 +137,8 @@ DevParams make_dev_params(const dim2d_params& q) {
 bool has_wall(const DevParams& p) {
   return p.bc_w == kBcWall || p.bc_e == kBcWall || p.bc_s == kBcWall || p.bc_n == kBcWall;
 }
+
+bool is_open(const DevParams& p) { return p.bc_n == kBcOpen; }
 
 }  // namespace
 
@@ -384,13 +393,16 @@ int dim2d_time_dev(dim2d_ctx* c, double t, double* time_dev) {
   const DevParams& p = c->dp;
   // U2 is free between steps: the derivative goes there, then to planar order
   // with the ghost cells set to debug_real (td_operators_class.f:150-154)
+  // unless the edges are open
   StageArgs a;
   memset(&a, 0, sizeof a);
   a.U = c->U;
   a.S64 = c->U;
   a.O64 = c->U2;
   HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
-  HIP_TRY(launch_unpack(p, c->U2, c->T, 1, kDebugReal, c->stream));
+  // open edges: the ghost frame has a time derivative of its own
+  if (is_open(p)) HIP_TRY(launch_frame(p, c->mixed, 0, a, c->stream));
+  HIP_TRY(launch_unpack(p, c->U2, c->T, is_open(p) ? 0 : 1, kDebugReal, c->stream));
   HIP_TRY(hipMemcpyAsync(time_dev, c->T, (size_t)p.nx * p.ny * 4 * sizeof(double),
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -407,6 +419,14 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
   a.cfl_mode = cfl_mode;
   a.parity = (int)(c->steps & 1);
   const bool wall = has_wall(p);
+  const bool open = is_open(p);
+  // the ghost frame of open edges after each stage kernel (it reads ctl->dt,
+  // which the first stage kernel of the step writes)
+  auto frame = [&](int mixed, int stage) -> int {
+    if (open) HIP_TRY(timed_launch(c, 3, [&] { return launch_frame(p, mixed, stage, a, c->stream); }));
+    return DIM2D_OK;
+  };
+  int st = DIM2D_OK;
   if (c->lsrk) {
     // low-storage RK3: the state ping-pongs U -> U2 -> U -> U2, q stays in Q;
     // the wall root guess is each stage's input state
@@ -417,6 +437,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
       a.S64 = in;
       a.O64 = out;
       HIP_TRY(timed_launch(c, s, [&] { return launch_stage(p, 0, 4 + s, a, c->block_x, c->rows, c->stream); }));
+      if ((st = frame(0, 4 + s))) return st;
       if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, out, nullptr, out, nullptr, in, c->stream); }));
       std::swap(in, out);
     }
@@ -426,28 +447,34 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     a.S64 = c->U;
     a.O64 = c->A;
     HIP_TRY(timed_launch(c, 0, [&] { return launch_stage(p, 0, 1, a, c->block_x, c->rows, c->stream); }));
+    if ((st = frame(0, 1))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->A, nullptr, c->A, nullptr, c->U, c->stream); }));
     a.S64 = c->A;
     a.O64 = c->B;
     HIP_TRY(timed_launch(c, 1, [&] { return launch_stage(p, 0, 2, a, c->block_x, c->rows, c->stream); }));
+    if ((st = frame(0, 2))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->B, nullptr, c->B, nullptr, c->U, c->stream); }));
     a.S64 = c->B;
     a.O64 = c->U2;
     HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 0, 3, a, c->block_x, c->rows, c->stream); }));
+    if ((st = frame(0, 3))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
   } else {
     a.U = c->U;
     a.O32 = c->D1;
     HIP_TRY(timed_launch(c, 0, [&] { return launch_stage(p, 1, 1, a, c->block_x, c->rows, c->stream); }));
+    if ((st = frame(1, 1))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D1, nullptr, c->D1, c->U, c->stream); }));
     a.S32 = c->D1;
     a.O32 = c->D2;
     HIP_TRY(timed_launch(c, 1, [&] { return launch_stage(p, 1, 2, a, c->block_x, c->rows, c->stream); }));
+    if ((st = frame(1, 2))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D2, nullptr, c->D2, c->U, c->stream); }));
     a.S32 = c->D2;
     a.O32 = nullptr;
     a.O64 = c->U2;
     HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 1, 3, a, c->block_x, c->rows, c->stream); }));
+    if ((st = frame(1, 3))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
   }
   std::swap(c->U, c->U2);
@@ -671,6 +698,8 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
       return fail(DIM2D_EINVAL, "all simulations of a batch must share the precision mode");
     if (params[i].integrator != DIM2D_INT_RK3TVD)
       return fail(DIM2D_EINVAL, "batches run the RK3-TVD integrator only");
+    if (params[i].bc_n == DIM2D_BC_OPEN)
+      return fail(DIM2D_EINVAL, "batches do not support open boundaries");
   }
   dim2d_batch* b = new (std::nothrow) dim2d_batch();
   if (!b) return fail(DIM2D_ENOMEM, "host allocation failed");

@@ -17,6 +17,7 @@ namespace dim2d {
 constexpr int kBcReflection = 0;
 constexpr int kBcPeriodic = 1;
 constexpr int kBcWall = 2;
+constexpr int kBcOpen = 3;  // hedstrom_xy, on all four edges at once
 constexpr int kBcSize = 2;
 // the reference's debug_real (NF90_FILL_DOUBLE), written into the ghost cells
 // of a time derivative as td_operators_class.f:150-154 does
@@ -99,6 +100,10 @@ hipError_t launch_wall_batch(const DevParams* h_params, const DevParams* d_param
                              double* O64, float* O32, const double* tmp, hipStream_t s);
 // store reflection/periodic ghost cells of an FP64 field
 hipError_t launch_fill_ghosts(const DevParams& p, double* F, hipStream_t s);
+// open boundaries: time derivative and RK update of the ghost frame for a
+// stage (0: time derivative into a.O64; 1-3 RK3-TVD; 4-6 low-storage RK3)
+hipError_t launch_frame(const DevParams& p, int mixed, int stage, const StageArgs& a,
+                        hipStream_t s);
 // reductions of an FP64 field into ctl slot `slot` (slot cleared first)
 hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int slot,
                          hipStream_t s);

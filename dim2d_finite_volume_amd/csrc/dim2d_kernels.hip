@@ -979,6 +979,403 @@ __global__ void wall_batch_kernel(const DevParams* P, int nsim, int mixed_in, co
 }
 
 // ------------------------------------------------------------------------
+// open boundaries (hedstrom_xy, bc_operators_hedstrom_xy_class.f:182-496)
+// ------------------------------------------------------------------------
+// The 2-cell ghost frame of a domain with open edges is state, advanced by the
+// same RK stages with its own time derivative (borders widened as
+// field_abstract_class.f:244-270 does):
+//   N/S edge cells  (1/dx)(Fx(i-1/2) - Fx(i+1/2)) + characteristic y-term + body
+//   E/W edge cells  characteristic x-term + (1/dy)(Fy(j-1/2) - Fy(j+1/2)) + body
+//   corner cells    characteristic x- and y-terms + body
+// The fluxes of the inner ghost row/column use the interior Mattsson
+// stencils, those of the outer one the one-sided stencils of
+// sd_operators_{y,x}_oneside_{L0,R0}_class.f (bc_operators_openbc_normal_class.f:563-787);
+// the characteristic term is hedstrom_xy_module.f:445-558 with the
+// eigen-quantities of the point itself (obc_eigenqties_bc).  One thread per
+// frame cell evaluates everything pointwise from the stage input, in FP64:
+// O(perimeter) work, written as plainly as the reference.  Indices are 0-based.
+namespace frame {
+
+enum Var { kInt = 0, kYL0, kYR0, kXL0, kXR0 };  // stencil set of a face
+enum Grad { kGInt = 0, kGL0, kGR0 };            // gradient kind
+enum Prim {
+  kRho, kQx, kQy, kUx, kUy, kP, kTcl, kQxTx, kQyTx, kQxTy, kQyTy, kEUx, kEUy, kPxw, kPyw,
+  kCap, kCapx, kCapy
+};
+
+struct View {
+  const DevParams* p;
+  const double* U;
+  const double* S64;
+  const float* S32;
+  int src;  // 0: S64, 1: U, 2: U + S32
+  __device__ void cell(int i, int j, double v[4]) const {
+    const long long o = ((long long)j * p->pitch + i) * 4;
+    for (int k = 0; k < 4; ++k)
+      v[k] = src == 0 ? S64[o + k] : (src == 1 ? U[o + k] : U[o + k] + (double)S32[o + k]);
+  }
+};
+
+// classical_pressure_local (dim2d_prim_module.f:425-453)
+__device__ double pressure_of(const DevParams& p, const double v[4]) {
+  const double r = v[0];
+  return 3.0 / ((3.0 - r) * p.cv) *
+             (v[3] - 0.5 * r * ((v[1] / r) * (v[1] / r) + (v[2] / r) * (v[2] / r)) + 3.0 * r * r) -
+         3.0 * r * r;
+}
+
+// the point functions the fluxes take (dim2d_prim_module.f:99-1150)
+__device__ double prim(const View& f, int i, int j, int pk) {
+  double v[4];
+  f.cell(i, j, v);
+  const double r = v[0];
+  switch (pk) {
+    case kRho: return r;
+    case kQx: return v[1];
+    case kQy: return v[2];
+    case kUx: return v[1] / r;
+    case kUy: return v[2] / r;
+    case kP: return pressure_of(*f.p, v);
+    case kTcl: return 1.0 / r * (v[3] - 0.5 * r * ((v[1] / r) * (v[1] / r) + (v[2] / r) * (v[2] / r)) + 3.0 * r * r);
+    case kQxTx: return v[1] * v[1] / r;
+    case kQyTx: return v[2] * v[1] / r;
+    case kQxTy: return v[1] * v[2] / r;
+    case kQyTy: return v[2] * v[2] / r;
+    case kEUx: return v[3] * v[1] / r;
+    case kEUy: return v[3] * v[2] / r;
+    case kPxw: return pressure_of(*f.p, v) * v[1] / r;
+    case kPyw: return pressure_of(*f.p, v) * v[2] / r;
+    case kCap: return 1.0 / (3.0 - r);
+    case kCapx: return v[1] / (r * (3.0 - r));
+    default: return v[2] / (r * (3.0 - r));
+  }
+}
+
+// gradients of a point function (sd_operators_fd_module.f:71-450)
+__device__ double grad_x(const View& f, int i, int j, int pk, double dx, int g) {
+  if (g == kGL0) return 1.0 / dx * (-prim(f, i, j, pk) + prim(f, i + 1, j, pk));
+  if (g == kGR0) return 1.0 / dx * (-prim(f, i - 1, j, pk) + prim(f, i, j, pk));
+  return 0.5 / dx * (-prim(f, i - 1, j, pk) + prim(f, i + 1, j, pk));
+}
+__device__ double grad_y(const View& f, int i, int j, int pk, double dy, int g) {
+  if (g == kGL0) return 1.0 / dy * (-prim(f, i, j, pk) + prim(f, i, j + 1, pk));
+  if (g == kGR0) return 1.0 / dy * (-prim(f, i, j - 1, pk) + prim(f, i, j, pk));
+  return 0.5 / dy * (-prim(f, i, j - 1, pk) + prim(f, i, j + 1, pk));
+}
+// capillarity_temperature_eff (:582-613) with the given gradient kinds
+__device__ double tcap(const View& f, int i, int j, int gx, int gy) {
+  const double a = grad_x(f, i, j, kRho, f.p->dx, gx);
+  const double b = grad_y(f, i, j, kRho, f.p->dy, gy);
+  return 0.5 / prim(f, i, j, kRho) * (a * a + b * b);
+}
+
+// x-face stencils at (i-1/2, j) (sd_operators_class.f:185-549, one-sided
+// sets sd_operators_y_oneside_{L0,R0}_class.f)
+__device__ double st_f(const View& f, int i, int j, int pk) {
+  return 0.5 * (+prim(f, i - 1, j, pk) + prim(f, i, j, pk));
+}
+__device__ double st_dfdx(const View& f, int i, int j, int pk) {
+  return 1.0 / f.p->dx * (-prim(f, i - 1, j, pk) + prim(f, i, j, pk));
+}
+__device__ double st_dfdy(const View& f, int i, int j, int pk, int var) {
+  const double dy = f.p->dy;
+  if (var == kYL0)
+    return 1.0 / dy * (-0.5 * prim(f, i - 1, j, pk) - 0.5 * prim(f, i, j, pk) +
+                       0.5 * prim(f, i - 1, j + 1, pk) + 0.5 * prim(f, i, j + 1, pk));
+  if (var == kYR0)
+    return 1.0 / dy * (-0.5 * prim(f, i - 1, j - 1, pk) - 0.5 * prim(f, i, j - 1, pk) +
+                       0.5 * prim(f, i - 1, j, pk) + 0.5 * prim(f, i, j, pk));
+  return 1.0 / (2.0 * dy) * (-0.5 * prim(f, i - 1, j - 1, pk) - 0.5 * prim(f, i, j - 1, pk) +
+                             0.5 * prim(f, i - 1, j + 1, pk) + 0.5 * prim(f, i, j + 1, pk));
+}
+__device__ double st_d2fdx2(const View& f, int i, int j, int pk) {
+  const double dx = f.p->dx;
+  return 0.5 / (dx * dx) *
+         (+prim(f, i - 2, j, pk) - prim(f, i - 1, j, pk) - prim(f, i, j, pk) + prim(f, i + 1, j, pk));
+}
+__device__ double st_d2fdy2(const View& f, int i, int j, int pk, int var) {
+  const double dy = f.p->dy;
+  const int j0 = var == kYL0 ? j : (var == kYR0 ? j - 2 : j - 1);  // rows j0, j0+1, j0+2
+  return 1.0 / (dy * dy) *
+         (+0.5 * prim(f, i - 1, j0, pk) + 0.5 * prim(f, i, j0, pk) -
+          2.0 * (+0.5 * prim(f, i - 1, j0 + 1, pk) + 0.5 * prim(f, i, j0 + 1, pk)) +
+          0.5 * prim(f, i - 1, j0 + 2, pk) + 0.5 * prim(f, i, j0 + 2, pk));
+}
+__device__ double st_dfdx_nl(const View& f, int i, int j, int var) {
+  const int gy = var == kYL0 ? kGL0 : (var == kYR0 ? kGR0 : kGInt);
+  return 1.0 / f.p->dx * (-tcap(f, i - 1, j, kGInt, gy) + tcap(f, i, j, kGInt, gy));
+}
+// y-face stencils at (i, j-1/2) (:622-1003, sd_operators_x_oneside_{L0,R0}_class.f)
+__device__ double st_g(const View& f, int i, int j, int pk) {
+  return 0.5 * (+prim(f, i, j - 1, pk) + prim(f, i, j, pk));
+}
+__device__ double st_dgdy(const View& f, int i, int j, int pk) {
+  return 1.0 / f.p->dy * (-prim(f, i, j - 1, pk) + prim(f, i, j, pk));
+}
+__device__ double st_dgdx(const View& f, int i, int j, int pk, int var) {
+  const double dx = f.p->dx;
+  if (var == kXL0)
+    return 1.0 / dx * (-0.5 * prim(f, i, j - 1, pk) - 0.5 * prim(f, i, j, pk) +
+                       0.5 * prim(f, i + 1, j - 1, pk) + 0.5 * prim(f, i + 1, j, pk));
+  if (var == kXR0)
+    return 1.0 / dx * (-0.5 * prim(f, i - 1, j - 1, pk) - 0.5 * prim(f, i - 1, j, pk) +
+                       0.5 * prim(f, i, j - 1, pk) + 0.5 * prim(f, i, j, pk));
+  return 0.25 / dx * (-prim(f, i - 1, j - 1, pk) - prim(f, i - 1, j, pk) +
+                      prim(f, i + 1, j - 1, pk) + prim(f, i + 1, j, pk));
+}
+__device__ double st_d2gdy2(const View& f, int i, int j, int pk) {
+  const double dy = f.p->dy;
+  return 0.5 / (dy * dy) *
+         (+prim(f, i, j - 2, pk) - prim(f, i, j - 1, pk) - prim(f, i, j, pk) + prim(f, i, j + 1, pk));
+}
+__device__ double st_d2gdx2(const View& f, int i, int j, int pk, int var) {
+  const double dx = f.p->dx;
+  if (var == kXL0 || var == kXR0) {
+    const int i0 = var == kXL0 ? i : i - 2;  // columns i0, i0+1, i0+2
+    return 1.0 / (dx * dx) *
+           (0.5 * prim(f, i0, j - 1, pk) + 0.5 * prim(f, i0, j, pk) -
+            2.0 * (+0.5 * prim(f, i0 + 1, j - 1, pk) + 0.5 * prim(f, i0 + 1, j, pk)) +
+            0.5 * prim(f, i0 + 2, j - 1, pk) + 0.5 * prim(f, i0 + 2, j, pk));
+  }
+  return 0.5 / (dx * dx) *
+         (+prim(f, i - 1, j - 1, pk) + prim(f, i - 1, j, pk) -
+          2.0 * (+prim(f, i, j - 1, pk) + prim(f, i, j, pk)) + prim(f, i + 1, j - 1, pk) +
+          prim(f, i + 1, j, pk));
+}
+__device__ double st_dgdy_nl(const View& f, int i, int j, int var) {
+  const int gx = var == kXL0 ? kGL0 : (var == kXR0 ? kGR0 : kGInt);
+  return 1.0 / f.p->dy * (-tcap(f, i, j - 1, gx, kGInt) + tcap(f, i, j, gx, kGInt));
+}
+
+// the four x-face fluxes (dim2d_fluxes_module.f:103-1649): inviscid - visc/Re - cap/We
+__device__ void flux_x(const View& f, int i, int j, int var, double F[4]) {
+  const DevParams& p = *f.p;
+  const double vr = p.vr;
+  F[0] = st_f(f, i, j, kQx);
+  {
+    const double inv = st_f(f, i, j, kQxTx) + st_f(f, i, j, kP);
+    const double vis = (2.0 + vr) * st_dfdx(f, i, j, kUx) + vr * st_dfdy(f, i, j, kUy, var);
+    const double rx = st_dfdx(f, i, j, kRho), ry = st_dfdy(f, i, j, kRho, var);
+    const double cap = 1.5 / p.cv * st_f(f, i, j, kCap) * (rx * rx + ry * ry) +
+                       st_f(f, i, j, kRho) * (st_d2fdx2(f, i, j, kRho) + st_d2fdy2(f, i, j, kRho, var)) +
+                       0.5 * (-rx * rx + ry * ry);
+    F[1] = inv - 1.0 / p.re * vis - 1.0 / p.we * cap;
+  }
+  {
+    const double inv = st_f(f, i, j, kQyTx);
+    const double vis = st_dfdy(f, i, j, kUx, var) + st_dfdx(f, i, j, kUy);
+    const double cap = -st_dfdx(f, i, j, kRho) * st_dfdy(f, i, j, kRho, var);
+    F[2] = inv - 1.0 / p.re * vis - 1.0 / p.we * cap;
+  }
+  {
+    const double inv = st_f(f, i, j, kEUx) + st_f(f, i, j, kPxw);
+    const double ux = st_f(f, i, j, kUx), uy = st_f(f, i, j, kUy);
+    const double vis = ux * ((2.0 + vr) * st_dfdx(f, i, j, kUx) + vr * st_dfdy(f, i, j, kUy, var)) +
+                       uy * (st_dfdy(f, i, j, kUx, var) + st_dfdx(f, i, j, kUy)) +
+                       1.0 / p.pr * st_dfdx(f, i, j, kTcl);
+    const double duxdx = st_dfdx(f, i, j, kUx), duydy = st_dfdy(f, i, j, kUy, var);
+    const double rx = st_dfdx(f, i, j, kRho), ry = st_dfdy(f, i, j, kRho, var);
+    const double cap = 1.5 / p.cv * st_f(f, i, j, kCapx) * (rx * rx + ry * ry) +
+                       st_f(f, i, j, kQx) * (st_d2fdx2(f, i, j, kRho) + st_d2fdy2(f, i, j, kRho, var)) +
+                       0.5 * ux * (-rx * rx + ry * ry) - rx * ry * uy -
+                       st_f(f, i, j, kRho) * rx * (duxdx + duydy) -
+                       1.0 / (p.re * p.pr) * st_dfdx_nl(f, i, j, var);
+    F[3] = inv - 1.0 / p.re * vis - 1.0 / p.we * cap;
+  }
+}
+// the four y-face fluxes
+__device__ void flux_y(const View& f, int i, int j, int var, double G[4]) {
+  const DevParams& p = *f.p;
+  const double vr = p.vr;
+  G[0] = st_g(f, i, j, kQy);
+  {
+    const double inv = st_g(f, i, j, kQxTy);
+    const double vis = st_dgdy(f, i, j, kUx) + st_dgdx(f, i, j, kUy, var);
+    const double cap = -st_dgdx(f, i, j, kRho, var) * st_dgdy(f, i, j, kRho);
+    G[1] = inv - 1.0 / p.re * vis - 1.0 / p.we * cap;
+  }
+  {
+    const double inv = st_g(f, i, j, kQyTy) + st_g(f, i, j, kP);
+    const double vis = vr * st_dgdx(f, i, j, kUx, var) + (2.0 + vr) * st_dgdy(f, i, j, kUy);
+    const double rx = st_dgdx(f, i, j, kRho, var), ry = st_dgdy(f, i, j, kRho);
+    const double cap = 1.5 / p.cv * st_g(f, i, j, kCap) * (rx * rx + ry * ry) +
+                       st_g(f, i, j, kRho) * (st_d2gdx2(f, i, j, kRho, var) + st_d2gdy2(f, i, j, kRho)) +
+                       0.5 * (rx * rx - ry * ry);
+    G[2] = inv - 1.0 / p.re * vis - 1.0 / p.we * cap;
+  }
+  {
+    const double inv = st_g(f, i, j, kEUy) + st_g(f, i, j, kPyw);
+    const double ux = st_g(f, i, j, kUx), uy = st_g(f, i, j, kUy);
+    const double duxdx = st_dgdx(f, i, j, kUx, var), duydy = st_dgdy(f, i, j, kUy);
+    const double vis = ux * (st_dgdy(f, i, j, kUx) + st_dgdx(f, i, j, kUy, var)) +
+                       uy * (vr * duxdx + (2.0 + vr) * duydy) +
+                       1.0 / p.pr * st_dgdy(f, i, j, kTcl);
+    const double rx = st_dgdx(f, i, j, kRho, var), ry = st_dgdy(f, i, j, kRho);
+    const double cap = 1.5 / p.cv * st_g(f, i, j, kCapy) * (rx * rx + ry * ry) +
+                       st_g(f, i, j, kQy) * (st_d2gdx2(f, i, j, kRho, var) + st_d2gdy2(f, i, j, kRho)) +
+                       0.5 * uy * (rx * rx - ry * ry) - rx * ry * ux -
+                       st_g(f, i, j, kRho) * ry * (duxdx + duydy) -
+                       1.0 / (p.re * p.pr) * st_dgdy_nl(f, i, j, var);
+    G[3] = inv - 1.0 / p.re * vis - 1.0 / p.we * cap;
+  }
+}
+
+// compute_timedev_with_openbc (hedstrom_xy_module.f:445-558): amplitudes of
+// the outgoing characteristics from one-sided primitive gradients, incoming
+// ones zero (openbc_operators_module.f:97-136), mapped back through the right
+// eigenvectors and the primitive-to-conservative Jacobian
+// (ns_vdw2d_prim_module.f:249-891); eigen-quantities of the point itself.
+__device__ void openbc(const View& f, int i, int j, int ydir, int right, double out[4]) {
+  const DevParams& p = *f.p;
+  double v[4];
+  f.cell(i, j, v);
+  const double md = v[0], ux = v[1] / v[0], uy = v[2] / v[0];
+  const double P = pressure_of(p, v);
+  // speed_of_sound (dim2d_prim_module.f:1179-1215)
+  const double e = P + md * md * (-3.0 + 2.0 * md);
+  const double b = sqrt((P + 3.0 * md * md) / (p.cv * e));
+  const double a = 1.0 / sqrt(1.0 + b * b);
+  const double d = sqrt(md * md * md) * sqrt((3.0 - md) / e);
+  const double c = sqrt(3.0) * md / (a * d);
+  const double un = ydir ? uy : ux;
+  const double lam[4] = {un, un, un - c, un + c};
+  const double q_c = 0.5 * md * c;
+  const int g = right ? kGR0 : kGL0;
+  const double dn = ydir ? p.dy : p.dx;
+  double gr[4];
+  for (int k = 0; k < 4; ++k) {
+    const int pk = k == 0 ? kRho : (k == 1 ? kUx : (k == 2 ? kUy : kP));
+    gr[k] = ydir ? grad_y(f, i, j, pk, dn, g) : grad_x(f, i, j, pk, dn, g);
+  }
+  // left eigenvectors l_k (columns of the reference's left eigenmatrix)
+  const int t = ydir ? 2 : 1;  // velocity normal to the edge in (rho, ux, uy, P)
+  const int s = ydir ? 1 : 2;  // transverse velocity
+  double amp[4];
+  for (int k = 0; k < 4; ++k) {
+    const bool incoming = right ? (lam[k] <= 0.0) : (lam[k] >= 0.0);
+    double dot;
+    if (k == 0) dot = gr[s];
+    else if (k == 1) dot = gr[0] + (-1.0 / (c * c)) * gr[3];
+    else if (k == 2) dot = -q_c * gr[t] + 0.5 * gr[3];
+    else dot = q_c * gr[t] + 0.5 * gr[3];
+    amp[k] = incoming ? 0.0 : -lam[k] * dot;
+  }
+  // primitive time derivative: sum_k amp_k r_k
+  const double inv_c = 1.0 / (c * c), inv_q_c = 1.0 / (md * c);
+  double tp[4];
+  tp[0] = amp[1] + amp[2] * inv_c + amp[3] * inv_c;
+  tp[t] = amp[2] * -inv_q_c + amp[3] * inv_q_c;
+  tp[s] = amp[0];
+  tp[3] = amp[2] + amp[3];
+  // conservative: J(l, m) = d cons_m / d prim_l
+  const double dEdrho = 0.5 * (ux * ux + uy * uy) - 6.0 * md -
+                        1.0 / 3.0 * p.cv * (P + 9.0 * (md - 2.0) * md);
+  out[0] = tp[0];
+  out[1] = tp[0] * ux + tp[1] * md;
+  out[2] = tp[0] * uy + tp[2] * md;
+  out[3] = tp[0] * dEdrho + tp[1] * (md * ux) + tp[2] * (md * uy) + tp[3] * (p.cv * (1.0 - md / 3.0));
+}
+
+__device__ double body(const DevParams& p, const double v[4], int k) {
+  if (!p.gravity_ac) return 0.0;
+  if (k == 2) return -v[0] * p.gravity;
+  if (k == 3) return -v[2] * p.gravity;
+  return 0.0;
+}
+
+// time derivative of frame cell (i, j)
+__device__ void timedev(const View& f, int i, int j, double L[4]) {
+  const DevParams& p = *f.p;
+  const int nx = p.nx, ny = p.ny;
+  double v[4];
+  f.cell(i, j, v);
+  const bool xg = i < kBcSize || i >= nx - kBcSize;
+  const bool yg = j < kBcSize || j >= ny - kBcSize;
+  double o[4], F[4], F2[4];
+  if (xg && yg) {  // corner
+    double oy[4];
+    openbc(f, i, j, 0, i >= nx - kBcSize, o);
+    openbc(f, i, j, 1, j >= ny - kBcSize, oy);
+    for (int k = 0; k < 4; ++k) L[k] = o[k] + oy[k] + body(p, v, k);
+  } else if (yg) {  // N or S edge
+    const int var = j == 0 ? kYL0 : (j == ny - 1 ? kYR0 : kInt);
+    flux_x(f, i, j, var, F);
+    flux_x(f, i + 1, j, var, F2);
+    openbc(f, i, j, 1, j >= ny - kBcSize, o);
+    for (int k = 0; k < 4; ++k) L[k] = 1.0 / p.dx * (F[k] - F2[k]) + o[k] + body(p, v, k);
+  } else {  // E or W edge
+    const int var = i == 0 ? kXL0 : (i == nx - 1 ? kXR0 : kInt);
+    flux_y(f, i, j, var, F);
+    flux_y(f, i, j + 1, var, F2);
+    openbc(f, i, j, 0, i >= nx - kBcSize, o);
+    for (int k = 0; k < 4; ++k) L[k] = o[k] + 1.0 / p.dy * (F[k] - F2[k]) + body(p, v, k);
+  }
+}
+
+}  // namespace frame
+
+// one thread per frame cell: 4(nx-4) edge cells on N/S, 4(ny-4) on E/W, 16 in corners
+template <int MIXED, int STAGE>
+__global__ void __launch_bounds__(64) frame_kernel(const DevParams p, const StageArgs a) {
+  const int nxi = p.nx - 2 * kBcSize, nyi = p.ny - 2 * kBcSize;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j;
+  if (t < 4 * nxi) {  // N/S edges: rows 0, 1, ny-2, ny-1
+    const int q = t / nxi;
+    i = kBcSize + t % nxi;
+    j = q < 2 ? q : p.ny - 4 + q;
+  } else if ((t -= 4 * nxi) < 4 * nyi) {  // E/W edges
+    const int q = t / nyi;
+    j = kBcSize + t % nyi;
+    i = q < 2 ? q : p.nx - 4 + q;
+  } else if ((t -= 4 * nyi) < 16) {  // corners
+    const int cx = (t >> 1) & 1, cy = (t >> 3) & 1;
+    i = (cx ? p.nx - kBcSize : 0) + (t & 1);
+    j = (cy ? p.ny - kBcSize : 0) + ((t >> 2) & 1);
+  } else {
+    return;
+  }
+  constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
+  const frame::View f{&p, a.U, a.S64, a.S32, SRC};
+  double L[4];
+  frame::timedev(f, i, j, L);
+  const long long o = ((long long)j * p.pitch + i) * 4;
+  if (STAGE == 0) {
+    for (int k = 0; k < 4; ++k) a.O64[o + k] = L[k];
+    return;
+  }
+  const double dt = a.ctl->dt;
+  double S[4];
+  f.cell(i, j, S);
+  if (STAGE >= 4) {
+    constexpr int s = STAGE - 4;
+    constexpr double LA = s == 0 ? 0.0 : (s == 1 ? -5.0 / 9.0 : -153.0 / 128.0);
+    constexpr double LB = s == 0 ? 1.0 / 3.0 : (s == 1 ? 15.0 / 16.0 : 8.0 / 15.0);
+    for (int k = 0; k < 4; ++k) {
+      const double q = (s == 0 ? 0.0 : LA * a.Q[o + k]) + dt * L[k];
+      a.O64[o + k] = S[k] + LB * q;
+      if (s < 2) a.Q[o + k] = q;
+    }
+  } else if (!MIXED) {
+    for (int k = 0; k < 4; ++k) {
+      const double inc = S[k] + dt * L[k];
+      double val;
+      if (STAGE == 1) val = inc;
+      else if (STAGE == 2) val = 0.75 * a.U[o + k] + (1.0 - 0.75) * inc;
+      else val = (1.0 / 3.0) * a.U[o + k] + (1.0 - 1.0 / 3.0) * inc;
+      a.O64[o + k] = val;
+    }
+  } else {
+    for (int k = 0; k < 4; ++k) {
+      const double dl = dt * L[k];
+      if (STAGE == 1) a.O32[o + k] = (float)dl;
+      else if (STAGE == 2) a.O32[o + k] = (float)(0.25 * ((double)a.S32[o + k] + dl));
+      else a.O64[o + k] = S[k] + ((2.0 / 3.0) * dl - (1.0 / 3.0) * (double)a.S32[o + k]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // ghost materialisation, reductions
 // ------------------------------------------------------------------------
 __global__ void fill_ghosts_kernel(const DevParams p, double* F) {
@@ -1141,6 +1538,33 @@ hipError_t launch_wall_batch(const DevParams* h_params, const DevParams* d_param
 hipError_t launch_fill_ghosts(const DevParams& p, double* F, hipStream_t s) {
   const int n = 2 * kBcSize * p.nx + 2 * kBcSize * (p.ny - 2 * kBcSize);
   hipLaunchKernelGGL(fill_ghosts_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame(const DevParams& p, int mixed, int stage, const StageArgs& a,
+                        hipStream_t s) {
+  const int n = 4 * (p.nx - 2 * kBcSize) + 4 * (p.ny - 2 * kBcSize) + 16;
+  const dim3 g((n + 63) / 64), b(64);
+#define DIM2D_FR(M, S) hipLaunchKernelGGL((frame_kernel<M, S>), g, b, 0, s, p, a)
+  if (!mixed) {
+    switch (stage) {
+      case 0: DIM2D_FR(0, 0); break;
+      case 1: DIM2D_FR(0, 1); break;
+      case 2: DIM2D_FR(0, 2); break;
+      case 3: DIM2D_FR(0, 3); break;
+      case 4: DIM2D_FR(0, 4); break;
+      case 5: DIM2D_FR(0, 5); break;
+      default: DIM2D_FR(0, 6); break;
+    }
+  } else {
+    switch (stage) {
+      case 0: DIM2D_FR(1, 0); break;
+      case 1: DIM2D_FR(1, 1); break;
+      case 2: DIM2D_FR(1, 2); break;
+      default: DIM2D_FR(1, 3); break;
+    }
+  }
+#undef DIM2D_FR
   return hipGetLastError();
 }
 

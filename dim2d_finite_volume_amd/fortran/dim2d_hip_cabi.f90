@@ -12,6 +12,7 @@ module dim2d_hip_cabi
   integer(c_int), parameter :: DIM2D_BC_REFLECTION = 0
   integer(c_int), parameter :: DIM2D_BC_PERIODIC = 1
   integer(c_int), parameter :: DIM2D_BC_WALL = 2
+  integer(c_int), parameter :: DIM2D_BC_OPEN = 3
   integer(c_int), parameter :: DIM2D_PREC_FP64 = 0
   integer(c_int), parameter :: DIM2D_PREC_MIXED = 1
   integer(c_int), parameter :: DIM2D_INT_RK3TVD = 0

@@ -39,6 +39,9 @@ extern "C" {
 #define DIM2D_BC_REFLECTION 0 /* reflection_x / reflection_y */
 #define DIM2D_BC_PERIODIC 1   /* periodic_xy */
 #define DIM2D_BC_WALL 2       /* wall_xy (equilibrium wall, constant heat flux) */
+#define DIM2D_BC_OPEN 3       /* hedstrom_xy: characteristic open boundary, all four edges at
+                                 once; the ghost frame is integrated with its own time
+                                 derivative (bc_operators_hedstrom_xy_class.f:182-496) */
 
 /* time integrators */
 #define DIM2D_INT_RK3TVD 0 /* the reference's RK3-TVD (rk3tvd_steps_module.f) */

@@ -13,13 +13,15 @@ from conftest import max_rel_err
 
 pytestmark = pytest.mark.gpu
 
-R, P, W = 0, 1, 2
+R, P, W, O = 0, 1, 2, 3
 BC_CASES = {
     "reflection": (R, R, R, R),
     "periodic": (P, P, P, P),
     "periodic_x_wall_y": (P, P, W, W),
     "wall_s_reflection": (R, R, W, R),
+    "open": (O, O, O, O),
 }
+ALL = (slice(None), slice(None))
 
 
 def _lib():
@@ -69,11 +71,15 @@ def test_time_dev_fp64(orc, case):
     err = max_rel_err(got, ref)
     print(case, "time_dev rel err", err)
     assert max(err) < 1e-11, err
-    # ghost cells carry debug_real like the reference
-    assert np.all(got[:, :2, :] == ref[:, :2, :])
+    if case == "open":
+        # the ghost frame has the characteristic time derivative
+        assert max(max_rel_err(got, ref, region=ALL)) < 1e-11
+    else:
+        # ghost cells carry debug_real like the reference
+        assert np.all(got[:, :2, :] == ref[:, :2, :])
 
 
-@pytest.mark.parametrize("case", ["reflection", "periodic_x_wall_y"])
+@pytest.mark.parametrize("case", ["reflection", "periodic_x_wall_y", "open"])
 def test_time_dev_mixed(orc, case):
     d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], precision=1)
     ref = orc.time_dev(m, nodes, xm, ym)
@@ -172,7 +178,7 @@ def test_cfl_integrate(orc):
     assert max(err) < 1e-10, err
 
 
-@pytest.mark.parametrize("case", ["reflection", "periodic_x_wall_y", "wall_s_reflection"])
+@pytest.mark.parametrize("case", ["reflection", "periodic_x_wall_y", "wall_s_reflection", "open"])
 def test_lsrk3_steps_fp64(orc, case):
     """Low-storage RK3 (2N-storage, Williamson 1980): 20 fixed-dt steps against
     the oracle's restatement of the same scheme, FP64 parity 1e-10."""
@@ -205,6 +211,47 @@ def test_lsrk3_cfl_integrate(orc):
     err = max_rel_err(got, ref)
     print("lsrk3 cfl run rel err", err)
     assert max(err) < 1e-10, err
+
+
+def test_open_boundaries_whole_field(orc):
+    """Hedstrom open edges (hedstrom_xy): the ghost frame is integrated with
+    its characteristic time derivative; 30 RK3 steps of a drop in a mean flow
+    against the oracle over every cell, frame included, FP64 parity 1e-10;
+    then the device CFL time step over 10 more steps."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["open"], nx=70, ny=53)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(30, dt)) == 0
+    f = d.Field(p, nodes.copy())
+    f.run(30, dt=dt)
+    got = f.download()
+    err = max_rel_err(got, ref, region=ALL)
+    print("open 30-step rel err (all cells)", err)
+    assert max(err) < 1e-10, err
+    dts = []
+    for _ in range(10):
+        dtc = orc.cfl_dt(m, ref, *p.spacing(), 0.1)[0]
+        dts.append(dtc)
+        assert orc.run(m, ref, xm, ym, np.array([dtc])) == 0
+    f.run(10, cfl=0.1)
+    got = f.download()
+    err = max_rel_err(got, ref, region=ALL)
+    print("open CFL rel err (all cells)", err)
+    assert max(err) < 1e-10, err
+
+
+def test_open_boundaries_mixed(orc):
+    """Mixed precision with open edges: density and energy within 1e-6 of the
+    FP64 oracle over every cell after 50 steps."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["open"], nx=70, ny=53, precision=1)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(50, dt)) == 0
+    f = d.Field(p, nodes.copy())
+    f.run(50, dt=dt)
+    err = max_rel_err(f.download(), ref, region=ALL)
+    print("open mixed 50-step rel err", err)
+    assert err[0] < 1e-6 and err[3] < 1e-6, err
 
 
 def test_cfl_and_diagnostics(orc):
