@@ -95,6 +95,78 @@ def cpu_baseline(seconds_hint=15.0):
                       + (" (diverged)" if bad else "")}
 
 
+def batch_bench(steps, nsims=256, seed=4):
+    """256 independent drop/bubble simulations, n^2 with n
+    log-uniform in [64, 512], reflection edges, each with its own fixed dt
+    (CFL 0.1 of its initial state); one persistent kernel per RK stage under a
+    HIP graph with tiles chosen by dim2d_batch_autotune; FP64 and mixed."""
+    import dataclasses
+
+    import numpy as np
+
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    sims = ic.batch_configs(nsims=nsims, seed=seed)
+    dts = np.array([ic.cfl_dt(p, n, 0.1) for p, n in sims])
+    cells = sum((p.nx - 4) * (p.ny - 4) for p, _ in sims)
+    out = {"nsims": nsims, "interior_cells": cells, "steps": steps,
+           "sizes": [min(p.nx - 4 for p, _ in sims), max(p.nx - 4 for p, _ in sims)]}
+    for prec, name in ((d.PREC_FP64, "f64"), (d.PREC_MIXED, "mixed")):
+        b = d.Batch([dataclasses.replace(p, precision=prec) for p, _ in sims])
+        for i, (_, nodes) in enumerate(sims):
+            b.upload(i, nodes)
+        mass0 = [b.download(i)[0, 2:-2, 2:-2].sum() for i in (0, nsims - 1)]
+        rows, wg, tuned_ms = b.autotune(dts, 10)
+        b.integrate(dts, 2)
+        t0 = time.perf_counter()
+        b.integrate(dts, steps)
+        el = time.perf_counter() - t0
+        last = [b.download(i) for i in (0, nsims - 1)]
+        b.close()
+        value = cells * steps / el
+        out[name] = {"value": value, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * el / steps,
+                     "hbm_frac_algorithmic": value * sum(ALG_BYTES[name]) / HBM_PEAK,
+                     "tiles": {"rows": rows, "workgroups": wg, "autotune_ms_per_step": tuned_ms},
+                     "finite": bool(all(np.isfinite(x).all() for x in last)),
+                     "mass_drift": [float((x[0, 2:-2, 2:-2].sum() - m0) / m0)
+                                    for x, m0 in zip(last, mass0)]}
+        log(f"[bench] batch {name}: {value:.4e} updates/s, tiles rows={rows} wg={wg}")
+    return out
+
+
+def open_boundary_bench(n, steps):
+    """configs[4] on a fixed domain: a vapour bubble (r = 0.1) in a mean flow
+    (0.2, 0.05) with Hedstrom open edges, mixed precision against FP64 with a
+    common fixed dt (CFL 0.1 of the initial state)."""
+    import dataclasses
+
+    import numpy as np
+
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = ic.open_bubble(n=n)
+    dt = ic.cfl_dt(p, nodes, 0.1)
+    res = {}
+    finals = {}
+    for prec, name in ((d.PREC_FP64, "f64"), (d.PREC_MIXED, "mixed")):
+        f = d.Field(dataclasses.replace(p, precision=prec), nodes)
+        f.run(2, dt=dt)
+        f.synchronize()
+        t0 = time.perf_counter()
+        f.run(steps, dt=dt)
+        el = time.perf_counter() - t0
+        finals[name] = f.download()
+        f.close()
+        res[name] = {"value": n * n * steps / el, "unit": "grid-pt updates/s",
+                     "ms_per_step": 1e3 * el / steps}
+        log(f"[bench] open {n}^2 {name}: {res[name]['value']:.4e} updates/s")
+    del nodes
+    res["mixed"]["error_vs_fp64"] = rel_errors(finals["mixed"], finals["f64"])
+    res["config"] = (f"vapour bubble r=0.1 in liquid, T0=0.95, (u,v)=(0.2,0.05), {n}x{n} "
+                     f"interior, Hedstrom open edges, fixed dt={dt:.4e}, {steps + 2} steps")
+    return res
+
+
 def read_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -117,6 +189,11 @@ def main():
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=500,
+                    help="steps of the 256-simulation batch (0: skip)")
+    ap.add_argument("--open-n", type=int, default=8192)
+    ap.add_argument("--open-steps", type=int, default=2000,
+                    help="steps of the open-boundary case (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run an elementwise kernel reading and writing 1 GiB "
                          "(for HBM counter scaling)")
@@ -221,6 +298,8 @@ def main():
         mixed["dominant_kernel_frac"] = (ALG_BYTES["mixed"][sim] * cells /
                                          (ktm[domm][0] * 1e-3)) / HBM_PEAK
 
+    batch = batch_bench(args.batch_steps) if args.batch_steps > 0 else None
+    open_bc = open_boundary_bench(args.open_n, args.open_steps) if args.open_steps > 0 else None
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": METRIC, "value": value, "unit": "grid-pt updates/s", "n_gpus": 1,
@@ -237,6 +316,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "batch": batch,
+        "open_boundary": open_bc,
         "checks": {"mass_drift": float((diag1[0] - diag0[0]) / diag0[0]),
                    "nonfinite": int(diag1[6]), "t_end": t_end, "dt_last": dt_last},
     }

@@ -28,6 +28,7 @@ EXPORTS = (
     "dim2d_profile_enable", "dim2d_profile_read",
     "dim2d_batch_create", "dim2d_batch_destroy", "dim2d_batch_upload", "dim2d_batch_download",
     "dim2d_batch_integrate", "dim2d_batch_set_tiles", "dim2d_batch_synchronize",
+    "dim2d_batch_autotune",
 )
 
 
@@ -95,6 +96,7 @@ def load() -> ctypes.CDLL:
         "dim2d_batch_integrate": (ctypes.c_int, [vp, dp, i32]),
         "dim2d_batch_set_tiles": (ctypes.c_int, [vp, i32, i32, i32]),
         "dim2d_batch_synchronize": (ctypes.c_int, [vp]),
+        "dim2d_batch_autotune": (ctypes.c_int, [vp, dp, ctypes.c_int32, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -149,6 +149,10 @@ struct dim2d_ctx {
   int mixed = 0;
   int lsrk = 0;          // low-storage RK3 instead of RK3-TVD
   hipStream_t stream = nullptr;
+  // open edges: the ghost-frame kernel of a stage runs on `side`, forked from
+  // and joined to `stream` around the stage kernel
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   double* U = nullptr;   // u^n
   double* U2 = nullptr;  // u^{n+1}
   double* A = nullptr;   // FP64 stage 1 output
@@ -236,6 +240,11 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D1, n * sizeof(float));
   if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D2, n * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&c->T, n * sizeof(double));
+  if (e == hipSuccess && params->bc_n == DIM2D_BC_OPEN) {
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
+  }
   if (e == hipSuccess) e = hipMalloc(&c->ctl, sizeof(StepCtl));
   if (e == hipSuccess) e = hipMemsetAsync(c->U, 0, n * sizeof(double), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->U2, 0, n * sizeof(double), c->stream);
@@ -276,6 +285,10 @@ int dim2d_ctx_destroy(dim2d_ctx* c) {
   (void)hipFree(c->ctl);
   release_timed(c);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DIM2D_OK;
@@ -420,10 +433,19 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
   a.parity = (int)(c->steps & 1);
   const bool wall = has_wall(p);
   const bool open = is_open(p);
-  // the ghost frame of open edges after each stage kernel (it reads ctl->dt,
-  // which the first stage kernel of the step writes)
-  auto frame = [&](int mixed, int stage) -> int {
-    if (open) HIP_TRY(timed_launch(c, 3, [&] { return launch_frame(p, mixed, stage, a, c->stream); }));
+  // the ghost frame of open edges: its kernel reads only the stage input and
+  // writes only frame cells of the stage output, so it runs on the side stream
+  // beside the stage kernel (forked before it, joined after it)
+  auto fork_frame = [&](int mixed, int stage) -> int {
+    if (!open) return DIM2D_OK;
+    HIP_TRY(hipEventRecord(c->fork_ev, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    HIP_TRY(launch_frame(p, mixed, stage, a, c->side));
+    HIP_TRY(hipEventRecord(c->join_ev, c->side));
+    return DIM2D_OK;
+  };
+  auto frame = [&](int, int) -> int {
+    if (open) HIP_TRY(hipStreamWaitEvent(c->stream, c->join_ev, 0));
     return DIM2D_OK;
   };
   int st = DIM2D_OK;
@@ -436,6 +458,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     for (int s = 0; s < 3; ++s) {
       a.S64 = in;
       a.O64 = out;
+      if ((st = fork_frame(0, 4 + s))) return st;
       HIP_TRY(timed_launch(c, s, [&] { return launch_stage(p, 0, 4 + s, a, c->block_x, c->rows, c->stream); }));
       if ((st = frame(0, 4 + s))) return st;
       if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, out, nullptr, out, nullptr, in, c->stream); }));
@@ -446,33 +469,39 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     a.U = c->U;
     a.S64 = c->U;
     a.O64 = c->A;
+    if ((st = fork_frame(0, 1))) return st;
     HIP_TRY(timed_launch(c, 0, [&] { return launch_stage(p, 0, 1, a, c->block_x, c->rows, c->stream); }));
     if ((st = frame(0, 1))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->A, nullptr, c->A, nullptr, c->U, c->stream); }));
     a.S64 = c->A;
     a.O64 = c->B;
+    if ((st = fork_frame(0, 2))) return st;
     HIP_TRY(timed_launch(c, 1, [&] { return launch_stage(p, 0, 2, a, c->block_x, c->rows, c->stream); }));
     if ((st = frame(0, 2))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->B, nullptr, c->B, nullptr, c->U, c->stream); }));
     a.S64 = c->B;
     a.O64 = c->U2;
+    if ((st = fork_frame(0, 3))) return st;
     HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 0, 3, a, c->block_x, c->rows, c->stream); }));
     if ((st = frame(0, 3))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
   } else {
     a.U = c->U;
     a.O32 = c->D1;
+    if ((st = fork_frame(1, 1))) return st;
     HIP_TRY(timed_launch(c, 0, [&] { return launch_stage(p, 1, 1, a, c->block_x, c->rows, c->stream); }));
     if ((st = frame(1, 1))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D1, nullptr, c->D1, c->U, c->stream); }));
     a.S32 = c->D1;
     a.O32 = c->D2;
+    if ((st = fork_frame(1, 2))) return st;
     HIP_TRY(timed_launch(c, 1, [&] { return launch_stage(p, 1, 2, a, c->block_x, c->rows, c->stream); }));
     if ((st = frame(1, 2))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D2, nullptr, c->D2, c->U, c->stream); }));
     a.S32 = c->D2;
     a.O32 = nullptr;
     a.O64 = c->U2;
+    if ((st = fork_frame(1, 3))) return st;
     HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 1, 3, a, c->block_x, c->rows, c->stream); }));
     if ((st = frame(1, 3))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
@@ -628,7 +657,8 @@ static int batch_rebuild_items(dim2d_batch* b) {
   HIP_TRY(hipMalloc(&b->d_items, items.size() * sizeof(WorkItem)));
   HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem),
                     hipMemcpyHostToDevice));
-  b->wg_eff = b->wg > 0 ? b->wg : std::min(b->nitems, max_resident_workgroups(b->block_x));
+  b->wg_eff = b->wg > 0 ? std::min(b->wg, b->nitems)
+                       : std::min(b->nitems, max_resident_workgroups(b->mixed));
   if (b->wg_eff < 1) b->wg_eff = 1;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
@@ -842,6 +872,66 @@ int dim2d_batch_integrate(dim2d_batch* b, const double* dts, int32_t nsteps) {
     }
   }
   b->steps += nsteps;
+  return DIM2D_OK;
+}
+
+int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, double out[3]) {
+  if (!b || !dts || !out || nsteps < 2) return fail(DIM2D_EINVAL, "invalid argument");
+  const int resident = max_resident_workgroups(b->mixed);
+  const int rows_c[] = {8, 12, 16, 24, 32};
+  const int wg_c[] = {resident, 2 * resident};
+  const int steps = nsteps & ~1;  // whole graph replays
+  // keep the state: candidates run real steps
+  double* keep = nullptr;
+  HIP_TRY(hipMalloc(&keep, b->total * sizeof(double)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = DIM2D_OK;
+  double best = INFINITY;
+  int best_rows = b->rows, best_wg = b->wg;
+  auto run = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(keep, b->U, b->total * sizeof(double), hipMemcpyDeviceToDevice,
+                           b->stream));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int r : rows_c)
+      for (int w : wg_c) {
+        b->rows = r;
+        b->wg = w;
+        int s2 = batch_rebuild_items(b);
+        if (s2) return s2;
+        s2 = dim2d_batch_integrate(b, dts, 2);  // warm-up, graph capture
+        if (s2) return s2;
+        HIP_TRY(hipEventRecord(e0, b->stream));
+        s2 = dim2d_batch_integrate(b, dts, steps);
+        if (s2) return s2;
+        HIP_TRY(hipEventRecord(e1, b->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / steps < best) {
+          best = ms / steps;
+          best_rows = r;
+          best_wg = w;
+        }
+      }
+    b->rows = best_rows;
+    b->wg = best_wg;
+    int s2 = batch_rebuild_items(b);
+    if (s2) return s2;
+    HIP_TRY(hipMemcpyAsync(b->U, keep, b->total * sizeof(double), hipMemcpyDeviceToDevice,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DIM2D_OK;
+  };
+  st = run();
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(keep);
+  if (st) return st;
+  out[0] = best_rows;
+  out[1] = best_wg;
+  out[2] = best;
   return DIM2D_OK;
 }
 

@@ -114,6 +114,7 @@ hipError_t launch_pack(const DevParams& p, const double* planar, double* F, hipS
 hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, int fill_ghosts,
                          double ghost_value, hipStream_t s);
 
-int max_resident_workgroups(int block_x);
+// workgroups of the persistent batch kernels that fit the device at once
+int max_resident_workgroups(int mixed);
 
 }  // namespace dim2d

@@ -1344,7 +1344,11 @@ __global__ void __launch_bounds__(64) frame_kernel(const DevParams p, const Stag
     for (int k = 0; k < 4; ++k) a.O64[o + k] = L[k];
     return;
   }
-  const double dt = a.ctl->dt;
+  // the frame of a step's first stage runs beside its stage kernel, which
+  // writes ctl->dt: it evaluates the same rule itself (later stages read it)
+  const double dt = (STAGE == 1 || STAGE == 4)
+                        ? (a.cfl_mode ? cfl_dt_from(a.ctl, a.parity) : a.dt_fixed)
+                        : a.ctl->dt;
   double S[4];
   f.cell(i, j, S);
   if (STAGE >= 4) {
@@ -1592,16 +1596,18 @@ hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int 
   return hipGetLastError();
 }
 
-int max_resident_workgroups(int block_x) {
+int max_resident_workgroups(int mixed) {
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   }
-  (void)block_x;
+  // the stage with the most registers bounds the persistent grid of all three
   int per = 0;
-  hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, stage_batch_kernel<0, 2>, kLanes, 0);
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, stage_batch_kernel<1, 3>, kLanes, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, stage_batch_kernel<0, 2>, kLanes, 0);
   if (e != hipSuccess || per <= 0) per = 4;
   return ncu * per;
 }

@@ -213,6 +213,16 @@ class Batch:
         capi.check(self._lib.dim2d_batch_set_tiles(self._b, block_x, rows, workgroups),
                    "dim2d_batch_set_tiles")
 
+    def autotune(self, dts, nsteps: int = 10):
+        """Pick the tile height and persistent workgroup count by timing
+        (dim2d_batch_autotune); the state is left as it was.  Returns
+        (rows, workgroups, ms_per_step)."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        out = np.zeros(3)
+        capi.check(self._lib.dim2d_batch_autotune(self._b, _dptr(d), nsteps, _dptr(out)),
+                   "dim2d_batch_autotune")
+        return int(out[0]), int(out[1]), float(out[2])
+
     def close(self) -> None:
         if self._b:
             self._lib.dim2d_batch_destroy(self._b)

@@ -180,6 +180,13 @@ int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows_per_bloc
                           int32_t workgroups);
 int dim2d_batch_synchronize(dim2d_batch* b);
 
+/* Tile autotuner of a batch (no counterpart in the reference): times nsteps
+ * (>= 2) steps with the given dts for each candidate tile height (8, 12, 16,
+ * 24, 32 rows) and persistent workgroup count (1x and 2x the workgroups that
+ * fit the device at once), keeps the fastest, and restores the state it
+ * started from.  out[0] = rows, out[1] = workgroups, out[2] = ms per step. */
+int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

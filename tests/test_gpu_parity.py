@@ -277,6 +277,9 @@ def test_batch_matches_single(orc):
     b.set_tiles(64, 16)
     for i, (p, n) in enumerate(sims):
         b.upload(i, n)
+    # the autotuner runs real steps on candidate tilings, then restores the state
+    rows, wg, ms = b.autotune(dts, 4)
+    assert rows in (8, 12, 16, 24, 32) and wg > 0 and ms > 0
     b.integrate(dts, 6)
     for i, (p, n) in enumerate(sims):
         f = d.Field(p, n)
