@@ -140,6 +140,26 @@ bool has_wall(const DevParams& p) {
 
 bool is_open(const DevParams& p) { return p.bc_n == kBcOpen; }
 
+// Field buffers start `shift` elements past their allocation so that the
+// first interior column of every row (column 2; rows start on 1 KiB
+// boundaries) begins a 128-byte line: the 60 output cells of a column strip
+// are then written as whole lines (2 cells for FP64, 6 for FP32 increments).
+constexpr int kShift64 = 2 * 4;
+constexpr int kShift32 = 6 * 4;
+template <typename T>
+hipError_t field_alloc(T** p, size_t n) {
+  constexpr int shift = sizeof(T) == 8 ? kShift64 : kShift32;
+  T* raw = nullptr;
+  const hipError_t e = hipMalloc(&raw, (n + shift) * sizeof(T));
+  *p = raw ? raw + shift : nullptr;
+  return e;
+}
+template <typename T>
+void field_free(T* p) {
+  constexpr int shift = sizeof(T) == 8 ? kShift64 : kShift32;
+  if (p) (void)hipFree(p - shift);
+}
+
 }  // namespace
 
 struct dim2d_ctx {
@@ -232,13 +252,13 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   const size_t n = (size_t)c->dp.ncell * 4;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc(&c->U, n * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&c->U2, n * sizeof(double));
-  if (e == hipSuccess && !c->mixed && !c->lsrk) e = hipMalloc(&c->A, n * sizeof(double));
-  if (e == hipSuccess && !c->mixed && !c->lsrk) e = hipMalloc(&c->B, n * sizeof(double));
-  if (e == hipSuccess && c->lsrk) e = hipMalloc(&c->Q, n * sizeof(double));
-  if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D1, n * sizeof(float));
-  if (e == hipSuccess && c->mixed) e = hipMalloc(&c->D2, n * sizeof(float));
+  if (e == hipSuccess) e = field_alloc(&c->U, n);
+  if (e == hipSuccess) e = field_alloc(&c->U2, n);
+  if (e == hipSuccess && !c->mixed && !c->lsrk) e = field_alloc(&c->A, n);
+  if (e == hipSuccess && !c->mixed && !c->lsrk) e = field_alloc(&c->B, n);
+  if (e == hipSuccess && c->lsrk) e = field_alloc(&c->Q, n);
+  if (e == hipSuccess && c->mixed) e = field_alloc(&c->D1, n);
+  if (e == hipSuccess && c->mixed) e = field_alloc(&c->D2, n);
   if (e == hipSuccess) e = hipMalloc(&c->T, n * sizeof(double));
   if (e == hipSuccess && params->bc_n == DIM2D_BC_OPEN) {
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
@@ -274,13 +294,13 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
 int dim2d_ctx_destroy(dim2d_ctx* c) {
   if (!c) return DIM2D_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(c->U);
-  (void)hipFree(c->U2);
-  (void)hipFree(c->A);
-  (void)hipFree(c->B);
-  (void)hipFree(c->Q);
-  (void)hipFree(c->D1);
-  (void)hipFree(c->D2);
+  field_free(c->U);
+  field_free(c->U2);
+  field_free(c->A);
+  field_free(c->B);
+  field_free(c->Q);
+  field_free(c->D1);
+  field_free(c->D2);
   (void)hipFree(c->T);
   (void)hipFree(c->ctl);
   release_timed(c);
@@ -753,12 +773,12 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
   if (e == hipSuccess) e = hipMalloc(&b->dp, n * sizeof(DevParams));
   if (e == hipSuccess)
     e = hipMemcpy(b->dp, b->hp.data(), n * sizeof(DevParams), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc(&b->U, off * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&b->U2, off * sizeof(double));
-  if (e == hipSuccess && !b->mixed) e = hipMalloc(&b->A, off * sizeof(double));
-  if (e == hipSuccess && !b->mixed) e = hipMalloc(&b->B, off * sizeof(double));
-  if (e == hipSuccess && b->mixed) e = hipMalloc(&b->D1, off * sizeof(float));
-  if (e == hipSuccess && b->mixed) e = hipMalloc(&b->D2, off * sizeof(float));
+  if (e == hipSuccess) e = field_alloc(&b->U, off);
+  if (e == hipSuccess) e = field_alloc(&b->U2, off);
+  if (e == hipSuccess && !b->mixed) e = field_alloc(&b->A, off);
+  if (e == hipSuccess && !b->mixed) e = field_alloc(&b->B, off);
+  if (e == hipSuccess && b->mixed) e = field_alloc(&b->D1, off);
+  if (e == hipSuccess && b->mixed) e = field_alloc(&b->D2, off);
   if (e == hipSuccess) e = hipMalloc(&b->d_dts, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&b->stage, b->stage_n * sizeof(double));
   if (e == hipSuccess) e = hipMemset(b->U, 0, off * sizeof(double));
@@ -787,12 +807,12 @@ int dim2d_batch_destroy(dim2d_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->dp);
-  (void)hipFree(b->U);
-  (void)hipFree(b->U2);
-  (void)hipFree(b->A);
-  (void)hipFree(b->B);
-  (void)hipFree(b->D1);
-  (void)hipFree(b->D2);
+  field_free(b->U);
+  field_free(b->U2);
+  field_free(b->A);
+  field_free(b->B);
+  field_free(b->D1);
+  field_free(b->D2);
   (void)hipFree(b->d_dts);
   (void)hipFree(b->stage);
   (void)hipFree(b->d_items);
