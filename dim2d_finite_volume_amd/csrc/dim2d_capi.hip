@@ -186,8 +186,16 @@ struct dim2d_ctx {
   int rows = 16;
   bool uploaded = false;
   long long steps = 0;
-  // kernel timing: event pairs around launches, tagged by kernel kind
+  // kernel timing: event pairs around launches of the kinds in profile_mask
+  // (bit k: kind k = stage 1, 2, 3, boundary kernels)
   bool profiling = false;
+  int profile_mask = 0;
+  // dim2d_integrate replays two captured steps per launch when not profiling;
+  // one graph per step parity (which buffer holds u^n and which reduction slot
+  // a step reads alternate with it), valid for the dt/cfl/tiles it was made with
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double g_dt[2] = {0.0, 0.0};
+  int g_cfl[2] = {0, 0};
   struct Timed {
     hipEvent_t a, b;
     int kind;
@@ -212,7 +220,7 @@ hipEvent_t take_event(dim2d_ctx* c) {
 // record an event pair around `launch` when profiling (kind 0..3)
 template <typename F>
 hipError_t timed_launch(dim2d_ctx* c, int kind, F&& launch) {
-  if (!c->profiling) return launch();
+  if (!c->profiling || !(c->profile_mask & (1 << kind))) return launch();
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return hipErrorOutOfMemory;
   hipError_t e = hipEventRecord(a, c->stream);
@@ -220,6 +228,14 @@ hipError_t timed_launch(dim2d_ctx* c, int kind, F&& launch) {
   if (e == hipSuccess) e = hipEventRecord(b, c->stream);
   if (e == hipSuccess) c->timed.push_back({a, b, kind});
   return e;
+}
+
+void drop_graphs(dim2d_ctx* c) {
+  for (int k = 0; k < 2; ++k)
+    if (c->gexec[k]) {
+      (void)hipGraphExecDestroy(c->gexec[k]);
+      c->gexec[k] = nullptr;
+    }
 }
 
 void release_timed(dim2d_ctx* c) {
@@ -304,6 +320,7 @@ int dim2d_ctx_destroy(dim2d_ctx* c) {
   (void)hipFree(c->T);
   (void)hipFree(c->ctl);
   release_timed(c);
+  drop_graphs(c);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->side) (void)hipStreamSynchronize(c->side);
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
@@ -321,14 +338,17 @@ int dim2d_set_tiles(dim2d_ctx* c, int32_t block_x, int32_t rows) {
   if (rows < 0 || rows > 4096) return fail(DIM2D_EINVAL, "rows_per_block out of range: %d", rows);
   if (block_x) c->block_x = block_x;
   if (rows) c->rows = rows;
+  drop_graphs(c);
   return DIM2D_OK;
 }
 
 int dim2d_profile_enable(dim2d_ctx* c, int32_t enable) {
   if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (enable < 0 || enable > 15) return fail(DIM2D_EINVAL, "profile mask out of range: %d", enable);
   HIP_TRY(hipStreamSynchronize(c->stream));
   release_timed(c);
   c->profiling = enable != 0;
+  c->profile_mask = enable;
   return DIM2D_OK;
 }
 
@@ -561,7 +581,46 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
     HIP_TRY(hipMemcpyAsync(&c->ctl->cfl, &cfl, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(launch_reduce(c->dp, c->U, c->ctl, (int)(c->steps & 1), c->stream));
   }
-  for (int s = 0; s < nsteps; ++s) {
+  // pairs of steps from a captured graph: no host launch gaps inside a pair.
+  // Not while profiling (event pairs around launches) nor with open edges
+  // (the frame kernel runs on a second stream).
+  int s = 0;
+  const bool graphs = !c->profiling && !is_open(c->dp);
+  for (; graphs && s + 1 < nsteps; s += 2) {
+    const int k = (int)(c->steps & 1);
+    if (c->gexec[k] && (c->g_dt[k] != dt_fixed || c->g_cfl[k] != cfl_mode)) {
+      (void)hipGraphExecDestroy(c->gexec[k]);
+      c->gexec[k] = nullptr;
+    }
+    if (!c->gexec[k]) {
+      // capture two steps; run_step's bookkeeping (buffer swap, step count)
+      // is undone afterwards, the graph launch below makes the steps happen
+      double* u = c->U;
+      double* u2 = c->U2;
+      const long long steps0 = c->steps;
+      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      st = run_step(c, dt_fixed, cfl_mode);
+      if (!st) st = run_step(c, dt_fixed, cfl_mode);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      c->U = u;
+      c->U2 = u2;
+      c->steps = steps0;
+      if (st || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (st) return st;
+        HIP_TRY(e);
+      }
+      const hipError_t ei = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ei);
+      c->g_dt[k] = dt_fixed;
+      c->g_cfl[k] = cfl_mode;
+    }
+    HIP_TRY(hipGraphLaunch(c->gexec[k], c->stream));
+    c->steps += 2;  // two buffer swaps: U and U2 are where they were
+  }
+  for (; s < nsteps; ++s) {
     st = run_step(c, dt_fixed, cfl_mode);
     if (st) return st;
   }

@@ -153,9 +153,15 @@ class Field:
     def set_tiles(self, block_x: int = 0, rows: int = 0) -> None:
         capi.check(self._lib.dim2d_set_tiles(self._ctx, block_x, rows), "dim2d_set_tiles")
 
-    def profile(self, enable: bool) -> None:
-        """Record HIP events around every kernel launch of the following steps."""
-        capi.check(self._lib.dim2d_profile_enable(self._ctx, int(enable)), "dim2d_profile_enable")
+    KIND = {"stage1": 1, "stage2": 2, "stage3": 4, "wall": 8}
+
+    def profile(self, enable, kinds=None) -> None:
+        """Record HIP events around the kernel launches of the following steps:
+        all kinds (enable True), the named ones (kinds, e.g. ["stage3"]), or none."""
+        mask = 0
+        if enable:
+            mask = 15 if kinds is None else sum(self.KIND[k] for k in kinds)
+        capi.check(self._lib.dim2d_profile_enable(self._ctx, mask), "dim2d_profile_enable")
 
     def kernel_times(self) -> dict:
         """Mean duration (ms) and count per kernel since profile(True)."""

@@ -115,7 +115,9 @@ int dim2d_apply_bc(dim2d_ctx* ctx, double t);
  * for DIM2D_INT_RK3TVD. */
 int dim2d_rk3_step(dim2d_ctx* ctx, double t, double dt);
 
-/* nsteps steps of the context's integrator without host synchronisation.
+/* nsteps steps of the context's integrator without host synchronisation
+ * (pairs of steps are replayed from a captured HIP graph, one per step parity,
+ * unless profiling is on or the edges are open).
  * cfl > 0: dt of each step from the CFL rule below, evaluated on the device
  *          (fused into the third stage of the previous step); dt_fixed ignored.
  * cfl <= 0: every step uses dt_fixed.
@@ -147,10 +149,14 @@ int dim2d_synchronize(dim2d_ctx* ctx);
  * the current value. */
 int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
 
-/* Kernel timing with HIP events recorded on the context's stream around every
- * kernel launch of the following steps (enable = 1), or off (enable = 0, which
- * also discards what was recorded).  No counterpart in the reference, which
- * only reports CPU_TIME totals (sim_dim2d.f:46,64,106). */
+/* Kernel timing with HIP events recorded on the context's stream around the
+ * kernel launches of the following steps whose kind is in the bit mask
+ * `enable` (1 stage 1, 2 stage 2, 4 stage 3, 8 wall ghost kernels; 15 all),
+ * or off (enable = 0, which also discards what was recorded).  Every event
+ * pair costs a few microseconds of step time, and while profiling,
+ * dim2d_integrate launches the steps one by one instead of replaying them
+ * from a graph.  No counterpart in the reference, which only reports CPU_TIME
+ * totals (sim_dim2d.f:46,64,106). */
 int dim2d_profile_enable(dim2d_ctx* ctx, int32_t enable);
 
 /* Mean duration (ms) and launch count per kernel since enabling (blocking):

@@ -290,6 +290,27 @@ def test_batch_matches_single(orc):
         assert np.array_equal(got[:, 2:-2, 2:-2], single[:, 2:-2, 2:-2]), i
 
 
+@pytest.mark.parametrize("case,prec", [("periodic_x_wall_y", 0), ("reflection", 1)])
+def test_graph_replay_matches_eager(orc, case, prec):
+    """dim2d_integrate replays pairs of steps from captured HIP graphs (one per
+    step parity); profiling makes it launch every step eagerly.  Several
+    replays with host work in between, odd step counts and a dt change give
+    bitwise the eager results, fixed-dt and CFL."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], precision=prec)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    g = d.Field(p, nodes.copy())
+    e = d.Field(p, nodes.copy())
+    e.profile(True, kinds=["stage1"])
+    for n, kw in ((4, {"dt": dt}), (5, {"dt": dt}), (2, {"dt": 0.5 * dt}), (7, {"cfl": 0.1}),
+                  (3, {"cfl": 0.1})):
+        g.run(n, **kw)
+        a = g.download()
+        e.run(n, **kw)
+        b = e.download()
+        assert np.array_equal(a, b), (n, kw)
+    assert np.array_equal(g.device_time(), e.device_time())
+
+
 def test_tile_invariance(orc):
     d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["periodic"], nx=131, ny=70)
     outs = []
