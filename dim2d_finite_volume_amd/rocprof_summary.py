@@ -4,12 +4,14 @@ Two reports, each written as a short Markdown file (and, for traffic, one JSON
 file per FP64 stage kernel that bench.py reads for the roofline's `traffic`):
 
   stats:    per-kernel call count and durations from `--kernel-trace --stats`
-            (<prefix>_kernel_stats.csv)
+            (<prefix>_kernel_stats.csv) and, with --trace, the same per kernel
+            and launch grid from <prefix>_kernel_trace.csv, so that one kernel
+            run on two grid sizes reads as two rows
   traffic:  HBM bytes per dispatch from two `--pmc` runs (FETCH_SIZE, then
             WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
             reads for a calibration kernel of known size in the same run
 
-    python -m dim2d_finite_volume_amd.rocprof_summary stats STATS.csv OUT.md --command "..."
+    python -m dim2d_finite_volume_amd.rocprof_summary stats STATS.csv OUT.md [--trace TRACE.csv] --command "..."
     python -m dim2d_finite_volume_amd.rocprof_summary traffic FETCH.csv WRITE.csv OUTDIR \
         --calib SUBSTRING:READ_BYTES:WRITE_BYTES --workload two_bubbles_4096 --command "..."
 """
@@ -45,6 +47,33 @@ def stats(args):
             lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | "
                          f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
                          f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    if args.trace:
+        lines += ["", "Per launch grid (from the kernel trace):", "",
+                  "| kernel | grid (work-items x, y) | workgroup | VGPRs | calls | mean us | min us | max us |",
+                  "|---|---|---|---:|---:|---:|---:|---:|"]
+        groups = defaultdict(list)
+        meta = {}
+        with open(args.trace) as f:
+            for r in csv.DictReader(f):
+                if not (args.all or "dim2d::" in r["Kernel_Name"]):
+                    continue
+                key = (short_name(r["Kernel_Name"]), int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]))
+                groups[key].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+                meta[key] = (f"{r['Workgroup_Size_X']}x{r['Workgroup_Size_Y']}",
+                             int(r["VGPR_Count"]) + int(r.get("Accum_VGPR_Count", 0) or 0))
+        grand = sum(sum(d) for d in groups.values()) or 1.0
+        keys = sorted(groups, key=lambda k: -sum(groups[k]))
+        small = [k for k in keys if sum(groups[k]) < 0.001 * grand]
+        for key in keys:
+            if key in small:
+                continue
+            d = groups[key]
+            lines.append(f"| {key[0]} | {key[1]} x {key[2]} | {meta[key][0]} | {meta[key][1]} | "
+                         f"{len(d)} | {sum(d) / len(d):.1f} | {min(d):.1f} | {max(d):.1f} |")
+        if small:
+            lines += ["", f"{len(small)} smaller launch groups (each under 0.1% of the time above, "
+                          f"{sum(sum(groups[k]) for k in small) / grand * 100:.2f}% together) "
+                          f"are left out."]
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -111,6 +140,7 @@ def main():
     a.add_argument("csv")
     a.add_argument("out")
     a.add_argument("--command", required=True)
+    a.add_argument("--trace", help="the run's kernel_trace.csv: adds a table per launch grid")
     a.add_argument("--all", action="store_true", help="list every kernel, not only dim2d's")
     b = sp.add_parser("traffic")
     b.add_argument("fetch")
