@@ -530,8 +530,14 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
     if (r < kBcSize || r >= p.ny - kBcSize) rm = map_row(p, min(max(r, 0), p.ny - 1), fly);
     load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, flx, fly, v, inc);
   };
-  // u^n at (i, r); r stays within [r0 - 1, r1 - 1] (inside the array)
-  auto load_u = [&](int r, double u[4]) { ld_cell(rk_src + (long long)r * p.pitch * 4, uc, u); };
+  // u^n at (i, r); r stays within [r0 - 1, r1 - 1] (inside the array).  Halo
+  // lanes read the nearest output column of the strip instead of their own
+  // (their value is never used): no lines beyond the output columns' are
+  // fetched, and no lane is masked.
+  const unsigned ucu = (unsigned)min(max(c, bx * kOutCols + kBcSize),
+                                     min(bx * kOutCols + kLanes - kBcSize - 1,
+                                         p.nx - kBcSize - 1));
+  auto load_u = [&](int r, double u[4]) { ld_cell(rk_src + (long long)r * p.pitch * 4, ucu, u); };
 
   // rings: slot (q - (r0 - 2)) % kRing holds row q
   double raw[kRing][4];
