@@ -48,6 +48,9 @@ int validate(const dim2d_params* p) {
   if (p->nx < 2 * kBcSize + 3 || p->ny < 2 * kBcSize + 3)
     return fail(DIM2D_EINVAL, "grid %dx%d too small: need at least 3 interior points per axis",
                 p->nx, p->ny);
+  if (p->nx > kMaxExtent || p->ny > kMaxExtent)
+    return fail(DIM2D_EINVAL, "grid %dx%d too large: at most %d points per axis", p->nx, p->ny,
+                kMaxExtent);
   const int b[4] = {p->bc_w, p->bc_e, p->bc_s, p->bc_n};
   int nopen = 0;
   for (int k = 0; k < 4; ++k) {
@@ -625,6 +628,92 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
     if (st) return st;
   }
   return DIM2D_OK;
+}
+
+int dim2d_get_params(dim2d_ctx* c, dim2d_params* out) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  *out = c->prm;
+  return DIM2D_OK;
+}
+
+int dim2d_openbc_undermined(dim2d_ctx* c, int32_t distance, double md_threshold,
+                            int32_t out[4]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "openbc_undermined before upload");
+  if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "the edges are not open (DIM2D_BC_OPEN)");
+  if (distance < 1) return fail(DIM2D_EINVAL, "distance must be >= 1, got %d", distance);
+  // the planar staging buffer is free between calls: four counters in it
+  int* counts = reinterpret_cast<int*>(c->T);
+  HIP_TRY(hipMemsetAsync(counts, 0, 4 * sizeof(int), c->stream));
+  if (!(md_threshold < 0.0) && !(md_threshold <= 1.0))
+    return fail(DIM2D_EINVAL, "md_threshold must be < 0 (off) or in [0, 1], got %g", md_threshold);
+  HIP_TRY(launch_undermined(c->dp, distance, md_threshold, c->U, counts, c->stream));
+  HIP_TRY(hipMemcpyAsync(out, counts, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DIM2D_OK;
+}
+
+int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int32_t add_n) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "extend before upload");
+  if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "only open (DIM2D_BC_OPEN) edges are extended");
+  if (add_w < 0 || add_e < 0 || add_s < 0 || add_n < 0)
+    return fail(DIM2D_EINVAL, "grid points to add must be >= 0");
+  const long long nx = (long long)c->prm.nx + add_w + add_e;
+  const long long ny = (long long)c->prm.ny + add_s + add_n;
+  if (nx > kMaxExtent || ny > kMaxExtent)
+    return fail(DIM2D_EINVAL, "extended grid %lldx%lld exceeds %d points per axis", nx, ny,
+                kMaxExtent);
+  if (nx == c->prm.nx && ny == c->prm.ny) return DIM2D_OK;
+  // the new grid keeps dx and dy: x_map(bc+1) = x_min moves add_w points west
+  // (field_abstract_class.f:286-315 spacing)
+  dim2d_params q = c->prm;
+  q.nx = (int32_t)nx;
+  q.ny = (int32_t)ny;
+  q.x_min = c->prm.x_min - add_w * c->dp.dx;
+  q.x_max = c->prm.x_max + add_e * c->dp.dx;
+  q.y_min = c->prm.y_min - add_s * c->dp.dy;
+  q.y_max = c->prm.y_max + add_n * c->dp.dy;
+  dim2d_ctx* n = nullptr;
+  int st = dim2d_ctx_create(&q, &n);
+  if (st) return st;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // state and step control (t, dt, CFL number, reduction slots) move over
+  hipError_t e = launch_extend(n->dp, c->dp, add_w, add_s, c->U, n->U, n->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(n->ctl, c->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, n->stream);
+  const double h = std::min(n->dp.dx, n->dp.dy);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&n->ctl->h, &h, sizeof h, hipMemcpyHostToDevice, n->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+  if (e != hipSuccess) {
+    dim2d_ctx_destroy(n);
+    return fail(DIM2D_EDEVICE, "domain extension failed: %s", hipGetErrorString(e));
+  }
+  n->block_x = c->block_x;
+  n->rows = c->rows;
+  n->uploaded = true;
+  n->steps = c->steps;
+  n->profiling = c->profiling;
+  n->profile_mask = c->profile_mask;
+  // the caller's handle keeps pointing at `c`: swap the contents, then free
+  // the old buffers (now held by `n`)
+  std::swap(*c, *n);
+  dim2d_ctx_destroy(n);
+  return DIM2D_OK;
+}
+
+int dim2d_adapt_domain(dim2d_ctx* c, int32_t adapt_mask, int32_t distance, double md_threshold,
+                       int32_t grow, int32_t out[4]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  if (adapt_mask < 0 || adapt_mask > 15)
+    return fail(DIM2D_EINVAL, "adapt_mask out of range: %d", adapt_mask);
+  if (grow < 1) return fail(DIM2D_EINVAL, "grow must be >= 1, got %d", grow);
+  int32_t cnt[4];
+  int st = dim2d_openbc_undermined(c, distance, md_threshold, cnt);
+  if (st) return st;
+  for (int k = 0; k < 4; ++k) out[k] = ((adapt_mask >> k) & 1) && cnt[k] > 0 ? grow : 0;
+  return dim2d_extend(c, out[0], out[1], out[2], out[3]);
 }
 
 int dim2d_synchronize(dim2d_ctx* c) {

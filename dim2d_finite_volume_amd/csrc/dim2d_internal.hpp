@@ -19,6 +19,8 @@ constexpr int kBcPeriodic = 1;
 constexpr int kBcWall = 2;
 constexpr int kBcOpen = 3;  // hedstrom_xy, on all four edges at once
 constexpr int kBcSize = 2;
+// largest grid extent per axis (points including ghosts) a context accepts
+constexpr int kMaxExtent = 1 << 16;
 // the reference's debug_real (NF90_FILL_DOUBLE), written into the ghost cells
 // of a time derivative as td_operators_class.f:150-154 does
 constexpr double kDebugReal = 9.9692099683868690e+36;
@@ -113,6 +115,17 @@ hipError_t launch_pack(const DevParams& p, const double* planar, double* F, hipS
 // receive `ghost_value` instead of the field's values
 hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, int fill_ghosts,
                          double ghost_value, hipStream_t s);
+
+// open edges: number of undermined points (imaginary sound speed; with
+// md_thr >= 0 also a density inside the two-phase band) in the `dist`
+// interior rows/columns nearest each edge, added into counts[0..3] (W, E, S, N)
+hipError_t launch_undermined(const DevParams& p, int dist, double md_thr, const double* U,
+                             int* counts, hipStream_t s);
+// state of grid `op` copied into grid `np`, grown by add_w columns west and
+// add_s rows south (any number east and north); new points repeat the
+// nearest old point
+hipError_t launch_extend(const DevParams& np, const DevParams& op, int add_w, int add_s,
+                         const double* oldU, double* newU, hipStream_t s);
 
 // workgroups of the persistent batch kernels that fit the device at once
 int max_resident_workgroups(int mixed);

@@ -705,6 +705,9 @@ __attribute__((amdgpu_waves_per_eu((MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64
                                        (STAGE == 3 || STAGE == 6 ? 1 : 0),
                                    8)))
 stage_kernel(const DevParams p, const StageArgs a, int rows) {
+  // density and its differences stay FP64 in both modes: the capillary
+  // second differences of O(1) densities are scaled by 1/dx^2, so FP32
+  // densities (absolute rounding ~1e-7) would leave errors of order 1e-7/dx^2
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
   constexpr bool FIRST = STAGE == 1 || STAGE == 4;  // takes dt, clears the next slot
@@ -736,6 +739,9 @@ __global__ void __launch_bounds__(kLanes)
 __attribute__((amdgpu_waves_per_eu(MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64, 8)))
 stage_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restrict__ items, int nitems,
                    const StageArgs a, const double* __restrict__ dts, int rows) {
+  // density and its differences stay FP64 in both modes: the capillary
+  // second differences of O(1) densities are scaled by 1/dx^2, so FP32
+  // densities (absolute rounding ~1e-7) would leave errors of order 1e-7/dx^2
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
   RedAcc<false> red;
@@ -1465,6 +1471,97 @@ __global__ void unpack_kernel(const DevParams p, const double* __restrict__ F,
 }
 
 // ------------------------------------------------------------------------
+// domain extension (field_extended_class.f:427-452, bf_interface_icr_class.f:251-386)
+// ------------------------------------------------------------------------
+// are_openbc_undermined (pmodel_eq_class.f:1991-2070): the sound speed of the
+// point is imaginary, P + rho^2 (2 rho - 3) < 0; with md_thr >= 0 (the
+// reference's bf_openbc_md_threshold_ac with bf_openbc_md_threshold = md_thr,
+// parameters_input.f:833-846) also a density inside the two-phase band
+// [rho_vap + eps (rho_mid - rho_vap), rho_liq - eps (rho_liq - rho_mid)] of the
+// saturation densities at the point's temperature 3/(8 cv) temperature_eff
+// (dim2d_prim_module.f:652-691, central density gradients; fits of
+// dim2d_state_eq_module.f:145-215).  Counted over the `dist` interior rows or
+// columns nearest each edge (the band up to the increase detectors, which sit
+// obc_dct_distance points inside the edge, parameters_bf_layer.f:138-142);
+// counts[0..3] = W, E, S, N.  Corner points count for both of their edges.
+__device__ bool openbc_undermined_at(const DevParams& p, const double* __restrict__ U, int i,
+                                     int j, double md_thr) {
+  auto cell = [&](int ii, int jj) { return U + ((long long)jj * p.pitch + ii) * 4; };
+  const double* c = cell(i, j);
+  const double v[4] = {c[0], c[1], c[2], c[3]};
+  const double P = frame::pressure_of(p, v);
+  if (P + v[0] * v[0] * (-3.0 + 2.0 * v[0]) < 0.0) return true;
+  if (md_thr < 0.0) return false;
+  const double gx = (cell(i + 1, j)[0] - cell(i - 1, j)[0]) / (2.0 * p.dx);
+  const double gy = (cell(i, j + 1)[0] - cell(i, j - 1)[0]) / (2.0 * p.dy);
+  const double r = v[0];
+  const double Teff = 1.0 / r *
+                      (v[3] - 0.5 * r * ((v[1] / r) * (v[1] / r) + (v[2] / r) * (v[2] / r)) -
+                       0.5 / p.we * (gx * gx + gy * gy) + 3.0 * r * r);
+  const double T = 3.0 / (8.0 * p.cv) * Teff;
+  const double md_liq = 1.0 + 2.06 * sqrt(1.0 - T);
+  const double md_vap = 1.0 - 1.91 * sqrt(1.0 - T);
+  const double md_mid = 0.5 * (md_vap + md_liq);
+  const double vap_thr = md_vap + md_thr * (md_mid - md_vap);
+  const double liq_thr = md_liq - md_thr * (md_liq - md_mid);
+  return r >= vap_thr && r <= liq_thr;
+}
+
+__global__ void undermined_kernel(const DevParams p, int dist, double md_thr,
+                                  const double* __restrict__ U, int* __restrict__ counts) {
+  const int nxi = p.nx - 2 * kBcSize, nyi = p.ny - 2 * kBcSize;
+  const int dx = min(dist, nxi), dy = min(dist, nyi);
+  const long long nx_band = (long long)dx * nyi;  // one x-band (W or E)
+  const long long n = 2 * nx_band + 2LL * dy * nxi;
+  int local[4] = {0, 0, 0, 0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    int edge, i, j;
+    if (t < 2 * nx_band) {
+      edge = t < nx_band ? 0 : 1;
+      const long long q = t - edge * nx_band;
+      const int off = (int)(q % dx);
+      j = kBcSize + (int)(q / dx);
+      i = edge == 0 ? kBcSize + off : p.nx - kBcSize - 1 - off;
+    } else {
+      const long long q0 = t - 2 * nx_band;
+      const long long y_band = (long long)dy * nxi;
+      edge = q0 < y_band ? 2 : 3;
+      const long long q = q0 - (edge - 2) * y_band;
+      const int off = (int)(q / nxi);
+      i = kBcSize + (int)(q % nxi);
+      j = edge == 2 ? kBcSize + off : p.ny - kBcSize - 1 - off;
+    }
+    if (openbc_undermined_at(p, U, i, j, md_thr)) local[edge] += 1;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const unsigned c = wave_sum_u((unsigned)local[e]);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&counts[e], (int)c);
+  }
+}
+
+// copy of the state into a grid grown by add_w columns on the west and add_s
+// rows on the south (and any number on the east and north): every point of
+// the new grid takes the value of the nearest point of the old grid (the old
+// points keep theirs, new points repeat the old edge point along the normal,
+// new corner points the old corner point)
+__global__ void extend_kernel(const DevParams np, const DevParams op, int add_w, int add_s,
+                              const double* __restrict__ oldU, double* __restrict__ newU) {
+  const long long n = (long long)np.nx * np.ny;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % np.nx), j = (int)(t / np.nx);
+    const int io = min(max(i - add_w, 0), op.nx - 1);
+    const int jo = min(max(j - add_s, 0), op.ny - 1);
+    const double2* src = reinterpret_cast<const double2*>(oldU + ((long long)jo * op.pitch + io) * 4);
+    double2* dst = reinterpret_cast<double2*>(newU + ((long long)j * np.pitch + i) * 4);
+    dst[0] = src[0];
+    dst[1] = src[1];
+  }
+}
+
+// ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
 template <int MIXED, int STAGE>
@@ -1599,6 +1696,22 @@ hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int 
   long long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, F, ctl, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_undermined(const DevParams& p, int dist, double md_thr, const double* U,
+                             int* counts, hipStream_t s) {
+  hipLaunchKernelGGL(undermined_kernel, dim3(1024), dim3(256), 0, s, p, dist, md_thr, U, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend(const DevParams& np, const DevParams& op, int add_w, int add_s,
+                         const double* oldU, double* newU, hipStream_t s) {
+  const long long n = (long long)np.nx * np.ny;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(extend_kernel, dim3((unsigned)blocks), dim3(256), 0, s, np, op, add_w, add_s,
+                     oldU, newU);
   return hipGetLastError();
 }
 

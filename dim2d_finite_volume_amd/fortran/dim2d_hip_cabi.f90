@@ -128,6 +128,40 @@ module dim2d_hip_cabi
       integer(c_int) :: st
     end function dim2d_set_tiles
 
+    function dim2d_get_params(ctx, out) bind(C, name="dim2d_get_params") result(st)
+      import :: c_int, c_ptr, dim2d_params
+      type(c_ptr), value :: ctx
+      type(dim2d_params), intent(out) :: out
+      integer(c_int) :: st
+    end function dim2d_get_params
+
+    function dim2d_openbc_undermined(ctx, distance, md_threshold, out) &
+        bind(C, name="dim2d_openbc_undermined") result(st)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: distance
+      real(c_double), value :: md_threshold
+      integer(c_int32_t), intent(out) :: out(4)
+      integer(c_int) :: st
+    end function dim2d_openbc_undermined
+
+    function dim2d_extend(ctx, add_w, add_e, add_s, add_n) bind(C, name="dim2d_extend") result(st)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: add_w, add_e, add_s, add_n
+      integer(c_int) :: st
+    end function dim2d_extend
+
+    function dim2d_adapt_domain(ctx, adapt_mask, distance, md_threshold, grow, out) &
+        bind(C, name="dim2d_adapt_domain") result(st)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: adapt_mask, distance, grow
+      real(c_double), value :: md_threshold
+      integer(c_int32_t), intent(out) :: out(4)
+      integer(c_int) :: st
+    end function dim2d_adapt_domain
+
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s

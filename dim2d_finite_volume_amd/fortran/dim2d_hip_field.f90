@@ -7,6 +7,8 @@
 !>     -> td_integrator%integrate           td_integrator/rungekutta3rdtvd/td_integrator_class.f:103-161
 !>   td_operators%compute_time_dev          td_operators/finitevolume/td_operators_class.f:119-235
 !>   bc_operators_gen%apply_bc_on_nodes     boundary_conditions/bc_operators_gen_class.f:238-381
+!>   field_extended%adapt_domain          field/field_extended_class.f:427-452
+!>     (rectangular growth of open edges; the grid, maps and mirror follow)
 !> The host array nodes(nx,ny,ne) is a mirror: it is refreshed by sync_nodes
 !> (before output, as field_abstract%write_data would need) and pushed by
 !> set_nodes; integrate keeps it current after every step when
@@ -44,6 +46,7 @@ module dim2d_hip_field
     procedure :: apply_bc_on_nodes
     procedure :: cfl_dt
     procedure :: diagnostics
+    procedure :: adapt_domain
     procedure :: finalize
   end type hip_field
 
@@ -53,10 +56,23 @@ contains
   subroutine ini(this, params)
     class(hip_field), intent(inout) :: this
     type(dim2d_params), intent(in) :: params
+    call set_grid(this, params)
+    this%nodes = 0
+    call dim2d_check(dim2d_ctx_create(this%params, this%ctx), 'dim2d_ctx_create')
+  end subroutine ini
+
+  !> grid size, coordinates and borders of `params`; the host mirror is
+  !> (re)allocated
+  subroutine set_grid(this, params)
+    class(hip_field), intent(inout) :: this
+    type(dim2d_params), intent(in) :: params
     integer :: i, j
     this%params = params
     this%nx = params%nx
     this%ny = params%ny
+    if (allocated(this%nodes)) deallocate(this%nodes)
+    if (allocated(this%x_map)) deallocate(this%x_map)
+    if (allocated(this%y_map)) deallocate(this%y_map)
     allocate(this%nodes(this%nx, this%ny, ne))
     allocate(this%x_map(this%nx), this%y_map(this%ny))
     this%dx = (params%x_max - params%x_min) / (this%nx - 1 - 2*bc_size)
@@ -69,9 +85,7 @@ contains
     end do
     this%x_borders = [bc_size + 1, this%nx - bc_size]
     this%y_borders = [bc_size + 1, this%ny - bc_size]
-    this%nodes = 0
-    call dim2d_check(dim2d_ctx_create(this%params, this%ctx), 'dim2d_ctx_create')
-  end subroutine ini
+  end subroutine set_grid
 
   subroutine set_nodes(this, nodes)
     class(hip_field), intent(inout) :: this
@@ -132,6 +146,26 @@ contains
     real(rkind) :: out(7)
     call dim2d_check(dim2d_diagnostics(this%ctx, out), 'dim2d_diagnostics')
   end function diagnostics
+
+  !> field_extended%adapt_domain: each open edge in adapt_mask (1 W, 2 E, 4 S,
+  !> 8 N) whose band of `distance` points is undermined moves out by `grow`
+  !> points (md_threshold < 0: imaginary sound speed only); the host grid,
+  !> maps and mirror follow, and the mirror is refreshed when the grid grew
+  function adapt_domain(this, adapt_mask, distance, md_threshold, grow) result(added)
+    class(hip_field), intent(inout) :: this
+    integer, intent(in) :: adapt_mask, distance, grow
+    real(rkind), intent(in) :: md_threshold
+    integer(c_int32_t) :: added(4)
+    type(dim2d_params) :: q
+    call dim2d_check(dim2d_adapt_domain(this%ctx, int(adapt_mask, c_int32_t), &
+                     int(distance, c_int32_t), md_threshold, int(grow, c_int32_t), added), &
+                     'dim2d_adapt_domain')
+    if (any(added > 0)) then
+      call dim2d_check(dim2d_get_params(this%ctx, q), 'dim2d_get_params')
+      call set_grid(this, q)
+      call this%sync_nodes()
+    end if
+  end function adapt_domain
 
   subroutine finalize(this)
     class(hip_field), intent(inout) :: this

@@ -84,14 +84,15 @@ def two_bubbles(n: int = 4096, precision: int = capi.PREC_FP64, seed: int = 2,
     return p, nodes
 
 
-def open_bubble(n: int = 8192, precision: int = capi.PREC_MIXED):
-    """A single vapour bubble of radius 0.1 in liquid at T0 = 0.95 transported
-    by the mean flow (u, v) = (0.2, 0.05), Hedstrom open boundaries on all four
-    edges (bc_choice hedstrom_xy_choice)."""
+def open_bubble(n: int = 8192, precision: int = capi.PREC_MIXED, center=(0.0, 0.0),
+                radius: float = 0.1, flow=(0.2, 0.05)):
+    """A single vapour bubble (default radius 0.1, centred) in liquid at
+    T0 = 0.95 transported by the mean flow (default (u, v) = (0.2, 0.05)),
+    Hedstrom open boundaries on all four edges (bc_choice hedstrom_xy_choice)."""
     p = ModelParams(nx=n + 4, ny=n + 4, bc=(capi.BC_OPEN,) * 4, precision=precision)
     dx, _ = p.spacing()
-    nodes = circles_field(p, 0.95, [(0.0, 0.0)], 0.1, "vapor", 2.0 * math.sqrt(p.we) * dx,
-                          flow=(0.2, 0.05))
+    nodes = circles_field(p, 0.95, [tuple(center)], radius, "vapor", 2.0 * math.sqrt(p.we) * dx,
+                          flow=tuple(flow))
     return p, nodes
 
 

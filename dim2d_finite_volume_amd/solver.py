@@ -15,7 +15,7 @@ capi.Dim2dError (the reference prints and stops).
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass, field as dc_field
+from dataclasses import dataclass, field as dc_field, replace
 
 import numpy as np
 
@@ -118,6 +118,47 @@ class Field:
 
     def apply_bc_on_nodes(self) -> None:
         capi.check(self._lib.dim2d_apply_bc(self._ctx, self.time), "dim2d_apply_bc")
+
+    # -- domain extension at open edges -------------------------------------
+    EDGES = {"W": 1, "E": 2, "S": 4, "N": 8}
+
+    def _refresh_params(self) -> None:
+        q = capi.Params()
+        capi.check(self._lib.dim2d_get_params(self._ctx, ctypes.byref(q)), "dim2d_get_params")
+        self.params = replace(self.params, nx=q.nx, ny=q.ny, x_min=q.x_min, x_max=q.x_max,
+                              y_min=q.y_min, y_max=q.y_max)
+        self.x_map, self.y_map = self.params.coordinates()
+
+    def openbc_undermined(self, distance: int = 5, md_threshold: float | None = None) -> np.ndarray:
+        """Undermined points within `distance` points of each open edge (W, E,
+        S, N): are_openbc_undermined, pmodel_eq_class.f:1991-2070; an imaginary
+        sound speed, or with md_threshold (the reference's
+        bf_openbc_md_threshold) a density inside the two-phase band."""
+        out = (ctypes.c_int32 * 4)()
+        thr = -1.0 if md_threshold is None else float(md_threshold)
+        capi.check(self._lib.dim2d_openbc_undermined(self._ctx, distance, thr, out),
+                   "dim2d_openbc_undermined")
+        return np.array(out[:], dtype=np.int32)
+
+    def extend(self, w: int = 0, e: int = 0, s: int = 0, n: int = 0) -> None:
+        """Grow the grid at open edges by whole columns/rows (dx, dy kept)."""
+        capi.check(self._lib.dim2d_extend(self._ctx, w, e, s, n), "dim2d_extend")
+        self._refresh_params()
+
+    def adapt_domain(self, edges: str = "WESN", distance: int = 5, grow: int = 32,
+                     md_threshold: float | None = None) -> np.ndarray:
+        """field_extended%adapt_domain (field_extended_class.f:427-452): each
+        listed edge whose band of `distance` points is undermined moves out by
+        `grow` points.  Returns the points added per edge (W, E, S, N)."""
+        mask = sum(self.EDGES[c] for c in set(edges.upper()))
+        out = (ctypes.c_int32 * 4)()
+        thr = -1.0 if md_threshold is None else float(md_threshold)
+        capi.check(self._lib.dim2d_adapt_domain(self._ctx, mask, distance, thr, grow, out),
+                   "dim2d_adapt_domain")
+        added = np.array(out[:], dtype=np.int32)
+        if added.any():
+            self._refresh_params()
+        return added
 
     def integrate(self, dt: float) -> None:
         """field%integrate: one RK3-TVD step, then time += dt (field_class.f:61-72)."""

@@ -165,6 +165,53 @@ int dim2d_profile_enable(dim2d_ctx* ctx, int32_t enable);
 int dim2d_profile_read(dim2d_ctx* ctx, double out[8]);
 
 /* ------------------------------------------------------------------ */
+/* Domain extension at open edges (host-driven).                        */
+/* ------------------------------------------------------------------ */
+
+/* The context's current parameters: nx, ny and the coordinate range change
+ * when the domain is extended. */
+int dim2d_get_params(dim2d_ctx* ctx, dim2d_params* out);
+
+/* Open edges only (blocking).  out[0..3] = number of points, among the
+ * `distance` interior columns (W, E) or rows (S, N) nearest each edge, where
+ * the open boundary conditions are undermined (are_openbc_undermined,
+ * pmodel_eq_class.f:1991-2070): the speed of sound is imaginary,
+ * P + rho^2 (2 rho - 3) < 0, or, when md_threshold >= 0 (the reference's
+ * bf_openbc_md_threshold_ac = .true. with bf_openbc_md_threshold =
+ * md_threshold, parameters_input.f:833-846; its default is off: pass a
+ * negative value), the density lies in the two-phase band
+ * [rho_vap + eps (rho_mid - rho_vap), rho_liq - eps (rho_liq - rho_mid)] of the
+ * saturation densities at the point's temperature 3/(8 cv) temperature_eff.
+ * The reference checks the points of its increase detectors,
+ * obc_dct_distance = 5 points inside each edge (parameters_input.f:679,
+ * parameters_bf_layer.f:138-142); this checks every point of the band between
+ * the edge and that line. */
+int dim2d_openbc_undermined(dim2d_ctx* ctx, int32_t distance, double md_threshold,
+                            int32_t out[4]);
+
+/* Open edges only (blocking).  Grow the grid by add_w/add_e columns and
+ * add_s/add_n rows, keeping dx and dy: x_min moves add_w*dx west, x_max
+ * add_e*dx east, and so on.  The state moves to its place in the larger
+ * grid; every new point takes the value of the nearest point of the old grid
+ * (the old edge point repeated along the edge normal, the old corner point in
+ * the new corners).  Time, dt and the step count carry over; the ctx handle
+ * stays valid and its device buffers are reallocated.  The reference grows
+ * buffer layers (bf_interface_icr%adapt_domain_extension,
+ * bf_interface_icr_class.f:251-386) whose new points come from a
+ * characteristic extrapolation in time (bf_layer_newgrdpt); here the whole
+ * edge moves, and new points repeat the far-field state at the edge. */
+int dim2d_extend(dim2d_ctx* ctx, int32_t add_w, int32_t add_e, int32_t add_s, int32_t add_n);
+
+/* field_extended%adapt_domain (field_extended_class.f:427-452), called by the
+ * host between steps: for each edge whose bit is set in adapt_mask (1 W, 2 E,
+ * 4 S, 8 N: the reference's adapt_*_choice, parameters_input.f:796-817)
+ * where dim2d_openbc_undermined(distance, md_threshold) finds points, the
+ * edge moves out by `grow` points (dim2d_extend).  out[0..3] = points added
+ * per edge. */
+int dim2d_adapt_domain(dim2d_ctx* ctx, int32_t adapt_mask, int32_t distance, double md_threshold,
+                       int32_t grow, int32_t out[4]);
+
+/* ------------------------------------------------------------------ */
 /* Batches of independent simulations (uneven grid sizes).              */
 /* ------------------------------------------------------------------ */
 typedef struct dim2d_batch dim2d_batch;

@@ -1505,6 +1505,59 @@ void orc_cfl_dt(const orc_model* m, const double* nodes, double dx, double dy, d
   out[3] = rmax;
 }
 
+/* are_openbc_undermined (pmodel_eq_class.f:1991-2070) with its default
+ * criterion (bf_openbc_md_threshold_ac = .false., parameters_input.f:833):
+ * the speed of sound of the point is imaginary, P + rho^2 (-3 + 2 rho) < 0,
+ * with P = classical_pressure_local. */
+int orc_openbc_undermined_point(const orc_model* m, const double v[4]) {
+  const double P = classical_pressure_local(m, v);
+  const double coeff = P + sq(v[0]) * (-3.0 + 2.0 * v[0]);
+  return coeff < 0.0;
+}
+
+/* are_openbc_undermined at the interior point (i, j) (1-based) of a field:
+ * the default criterion, then, when md_thr >= 0 (bf_openbc_md_threshold_ac
+ * with bf_openbc_md_threshold = md_thr, parameters_input.f:833-846), a density
+ * inside [md_vap_thr, md_liq_thr] at T = 3/(8 cv) temperature_eff
+ * (dim2d_prim_module.f:652-691) with central density gradients
+ * (gradient_x/y_interior) and the saturation fits of dim2d_state_eq_module.f:145-227 */
+int orc_openbc_undermined_at(const orc_model* m, const double* nodes, int i, int j, double dx,
+                             double dy, double md_thr) {
+  const double v[4] = {ND(nodes, i, j, 1), ND(nodes, i, j, 2), ND(nodes, i, j, 3), ND(nodes, i, j, 4)};
+  if (orc_openbc_undermined_point(m, v)) return 1;
+  if (md_thr < 0.0) return 0;
+  const double gx = (ND(nodes, i + 1, j, 1) - ND(nodes, i - 1, j, 1)) / (2.0 * dx);
+  const double gy = (ND(nodes, i, j + 1, 1) - ND(nodes, i, j - 1, 1)) / (2.0 * dy);
+  const double teff = 1.0 / v[0] *
+                      (v[3] - 0.5 * v[0] * (sq(v[1] / v[0]) + sq(v[2] / v[0])) -
+                       0.5 / m->we * (sq(gx) + sq(gy)) + 3.0 * sq(v[0]));
+  const double T = 3.0 / (8.0 * m->cv_r) * teff;
+  const double md_liq = orc_mass_density_liquid(T);
+  const double md_vap = orc_mass_density_vapor(T);
+  const double md_mid = 0.5 * (md_vap + md_liq);
+  const double md_vap_thr = md_vap + md_thr * (md_mid - md_vap);
+  const double md_liq_thr = md_liq - md_thr * (md_liq - md_mid);
+  return v[0] >= md_vap_thr && v[0] <= md_liq_thr;
+}
+
+/* undermined points among the `dist` interior columns nearest the W and E
+ * edges and the `dist` interior rows nearest the S and N edges
+ * (out[0..3] = W, E, S, N; a corner point counts for both of its edges) */
+void orc_openbc_undermined(const orc_model* m, const double* nodes, int dist, double dx, double dy,
+                           double md_thr, int out[4]) {
+  const int i0 = 1 + BC_SIZE, i1 = m->nx - BC_SIZE, j0 = 1 + BC_SIZE, j1 = m->ny - BC_SIZE;
+  for (int k = 0; k < 4; ++k) out[k] = 0;
+  for (int j = j0; j <= j1; ++j)
+    for (int i = i0; i <= i1; ++i) {
+      const int near = (i - i0 < dist) || (i1 - i < dist) || (j - j0 < dist) || (j1 - j < dist);
+      if (!near || !orc_openbc_undermined_at(m, nodes, i, j, dx, dy, md_thr)) continue;
+      if (i - i0 < dist) out[0] += 1;
+      if (i1 - i < dist) out[1] += 1;
+      if (j - j0 < dist) out[2] += 1;
+      if (j1 - j < dist) out[3] += 1;
+    }
+}
+
 void orc_sums(const orc_model* m, const double* nodes, double out[4]) {
   for (int k = 1; k <= 4; ++k) {
     double s = 0.0;

@@ -124,6 +124,18 @@ void orc_coordinates(int nx, int ny, double x_min, double x_max, double y_min,
 void orc_cfl_dt(const orc_model*, const double* nodes, double dx, double dy,
                 double cfl, double out[4]);
 
+/* are_openbc_undermined (pmodel_eq_class.f:1991-2070), default criterion:
+ * 1 if P + rho^2 (2 rho - 3) < 0 (imaginary speed of sound) at the point */
+int orc_openbc_undermined_point(const orc_model*, const double v[4]);
+/* the criterion at interior point (i, j) (1-based) with neighbours; with
+ * md_thr >= 0 also the mass-density band test (bf_openbc_md_threshold) */
+int orc_openbc_undermined_at(const orc_model*, const double* nodes, int i, int j, double dx,
+                             double dy, double md_thr);
+/* undermined points within `dist` interior columns/rows of each edge,
+ * out[0..3] = W, E, S, N */
+void orc_openbc_undermined(const orc_model*, const double* nodes, int dist, double dx, double dy,
+                           double md_thr, int out[4]);
+
 /* conserved sums over interior cells: out[0]=sum rho, [1]=sum qx, [2]=sum qy, [3]=sum E */
 void orc_sums(const orc_model*, const double* nodes, double out[4]);
 

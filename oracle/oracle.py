@@ -71,6 +71,13 @@ def lib():
         L.orc_coordinates.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp, _dp]
         L.orc_cfl_dt.argtypes = [M, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_sums.argtypes = [M, _dp, _dp]
+        L.orc_openbc_undermined_point.restype = ctypes.c_int
+        L.orc_openbc_undermined_point.argtypes = [M, _dp]
+        L.orc_openbc_undermined_at.restype = ctypes.c_int
+        L.orc_openbc_undermined_at.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double]
+        L.orc_openbc_undermined.argtypes = [M, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
         for name in ("orc_mass_density_liquid", "orc_mass_density_vapor"):
             getattr(L, name).restype = ctypes.c_double
             getattr(L, name).argtypes = [ctypes.c_double]
@@ -170,6 +177,31 @@ def cfl_dt(m: Model, nodes, dx, dy, cfl):
     out = np.zeros(4)
     lib().orc_cfl_dt(ctypes.byref(m), _ptr(nodes), dx, dy, cfl, _ptr(out))
     return out
+
+
+def openbc_undermined_point(m: Model, v) -> bool:
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return bool(lib().orc_openbc_undermined_point(ctypes.byref(m), _ptr(v)))
+
+
+def openbc_undermined_at(m: Model, nodes, i: int, j: int, dx, dy, md_threshold=-1.0) -> bool:
+    """The criterion at interior point (i, j), 1-based."""
+    return bool(lib().orc_openbc_undermined_at(ctypes.byref(m), _ptr(nodes), i, j, dx, dy,
+                                               md_threshold))
+
+
+def openbc_undermined(m: Model, nodes, dist: int, dx=1.0, dy=1.0, md_threshold=-1.0):
+    """Undermined points within `dist` interior columns/rows of each edge (W, E,
+    S, N); md_threshold < 0: the default criterion only."""
+    out = (ctypes.c_int * 4)()
+    lib().orc_openbc_undermined(ctypes.byref(m), _ptr(nodes), dist, dx, dy, md_threshold, out)
+    return np.array(out[:], dtype=np.int32)
+
+
+def extend(nodes, w: int, e: int, s: int, n: int):
+    """The grid grown by w/e columns and s/n rows, every new point repeating
+    the nearest old point (the domain extension's copy rule)."""
+    return np.ascontiguousarray(np.pad(nodes, ((0, 0), (s, n), (w, e)), mode="edge"))
 
 
 def sums(m: Model, nodes):

@@ -39,3 +39,33 @@ def test_fortran_driver_matches_oracle(orc, tmp_path, bc, nsync):
     assert orc.run(m, ref, xm, ym, np.full(nt, dt)) == 0
     err = max_rel_err(got, ref)
     assert max(err) < 1e-10, err
+
+
+def test_fortran_adapt_domain_matches_python(tmp_path):
+    """The Fortran field's adapt_domain loop (chunks of steps, then
+    field_extended%adapt_domain) gives the same grid and the same bits as the
+    same calls made from Python."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (make -C dim2d_finite_volume_amd/fortran)")
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = ic.open_bubble(n=60, precision=0, center=(0.25, 0.0), radius=0.1, flow=(0.4, 0.0))
+    dt = ic.cfl_dt(p, nodes, 0.1)
+    nt, every, mask, grow, thr = 600, 50, 2, 16, 0.3
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    nodes.tofile(fin)
+    r = subprocess.run([DRIVER, str(fin), str(fout), str(p.nx), str(p.ny), "3", "3", "3", "3",
+                        str(nt), f"{dt:.17g}", "0", "0", "0", str(every), str(mask), str(grow),
+                        f"{thr:.17g}"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    grid = [ln for ln in r.stdout.splitlines() if ln.startswith("grid:")][0].split()
+    nx, ny = int(grid[1]), int(grid[2])
+    f = d.Field(p, nodes)
+    for _ in range(nt // every):
+        f.run(every, dt=dt)
+        f.adapt_domain("E", distance=5, grow=grow, md_threshold=thr)
+    assert (nx, ny) == (f.params.nx, f.params.ny) and nx > p.nx
+    got = np.fromfile(fout, dtype=np.float64).reshape(4, ny, nx)
+    assert np.array_equal(got, f.download())
+    f.close()

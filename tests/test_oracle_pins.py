@@ -415,3 +415,117 @@ def test_open_boundaries_keep_a_uniform_flow(orc):
         a = nodes.copy()
         assert orc.run(m, a, xm, ym, np.full(5, 1e-3)) == 0
         assert np.abs(a - nodes).max() < 1e-12
+
+
+# ---- domain extension at open edges ----
+
+def _rest_state(cv_r, rho, T, u=0.0, v=0.0):
+    """conservative variables of a point with density rho, temperature T and
+    velocity (u, v): T_cl = (E - rho|u|^2/2 + 3 rho^2)/rho = 8 cv T / 3
+    (dim2d_prim_module.f:515-544 with T = 3/(8 cv) T_cl)."""
+    E = rho * (8.0 * cv_r / 3.0) * T - 3.0 * rho ** 2 + 0.5 * rho * (u * u + v * v)
+    return np.array([rho, rho * u, rho * v, E])
+
+
+def test_openbc_undermined_closed_form(orc):
+    """With P the van der Waals pressure 8 rho T/(3 - rho) - 3 rho^2, the
+    criterion P + rho^2 (2 rho - 3) < 0 reduces to T < rho (3 - rho)^2 / 4,
+    whatever the velocity; the curve peaks at the critical point (1, 1)."""
+    m = orc.make_model(12, 12, bc=(orc.BC_OPEN,) * 4)
+    for rho in np.linspace(0.05, 2.9, 58):
+        edge = rho * (3.0 - rho) ** 2 / 4.0
+        for T in (0.5, 0.8, 0.9, 0.95, 0.99, 1.05):
+            if abs(T - edge) < 1e-9:
+                continue
+            for u, v in ((0.0, 0.0), (0.3, -0.2)):
+                got = orc.openbc_undermined_point(m, _rest_state(m.cv_r, rho, T, u, v))
+                assert got == (T < edge), (rho, T, u, v)
+    assert not orc.openbc_undermined_point(m, _rest_state(m.cv_r, 1.0, 1.0 + 1e-6))
+    assert orc.openbc_undermined_point(m, _rest_state(m.cv_r, 1.0, 1.0 - 1e-6))
+
+
+def test_openbc_undermined_is_imaginary_speed_of_sound(orc):
+    """The criterion holds exactly where the reference's speed_of_sound
+    (dim2d_prim_module.f:1179-1215) has no real value: on random states
+    across the two-phase region, and on a drop interface (none there: its
+    capillary energy keeps every point's classical temperature above the curve)."""
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, m, xm, ym, nodes = _drop(orc, 40, (orc.BC_OPEN,) * 4)
+    rng = np.random.default_rng(11)
+    pts = [_rest_state(m.cv_r, r, T, u, v) for r, T, u, v in zip(
+        rng.uniform(0.1, 2.9, 3000), rng.uniform(0.5, 1.1, 3000),
+        rng.uniform(-0.5, 0.5, 3000), rng.uniform(-0.5, 0.5, 3000))]
+    it = np.concatenate([np.array(pts).T, nodes[:, 2:-2, 2:-2].reshape(4, -1)], axis=1)
+    with np.errstate(invalid="ignore"):
+        imaginary = ~np.isfinite(ic.sound_speed(p, it))
+    flags = np.array([orc.openbc_undermined_point(m, np.ascontiguousarray(it[:, k]))
+                      for k in range(it.shape[1])])
+    assert flags[:3000].any() and not flags[:3000].all() and not flags[3000:].any()
+    assert np.array_equal(flags, imaginary)
+
+
+def test_openbc_undermined_band_counts(orc):
+    """Counts per edge of a uniform liquid with planted two-phase points at
+    known distances from the edges, against a count by hand."""
+    n = 20
+    m = orc.make_model(n + 4, n + 4, bc=(orc.BC_OPEN,) * 4)
+    liquid = _rest_state(m.cv_r, 1.9, 0.95, 0.1, 0.0)
+    bad = _rest_state(m.cv_r, 1.0, 0.9)
+    nodes = np.ascontiguousarray(np.broadcast_to(liquid[:, None, None], (4, n + 4, n + 4)))
+    # interior 0-based offsets (i, j) from the first interior point
+    planted = [(0, 10), (4, 12), (5, 7), (19, 3), (15, 15), (0, 0), (10, 19), (10, 14)]
+    for i, j in planted:
+        nodes[:, 2 + j, 2 + i] = bad
+    for dist in (1, 3, 5, 6, 20):
+        want = np.zeros(4, dtype=np.int32)
+        for i, j in planted:
+            want += [i < dist, n - 1 - i < dist, j < dist, n - 1 - j < dist]
+        assert np.array_equal(orc.openbc_undermined(m, nodes, dist), want), dist
+
+
+def test_extend_copy_rule():
+    """New points repeat the nearest old point; old points keep their values."""
+    from oracle import oracle as orc_mod
+    a = np.arange(4 * 5 * 6, dtype=np.float64).reshape(4, 5, 6)
+    b = orc_mod.extend(a, 2, 1, 3, 0)
+    assert b.shape == (4, 8, 9)
+    assert np.array_equal(b[:, 3:, 2:8], a)
+    assert np.array_equal(b[:, 0, 0], a[:, 0, 0]) and np.array_equal(b[:, 3:, 0], a[:, :, 0])
+    assert np.array_equal(b[:, 3:, 8], a[:, :, 5]) and np.array_equal(b[:, 1, 2:8], a[:, 0, :])
+
+
+def test_openbc_undermined_md_threshold_closed_form(orc):
+    """bf_openbc_md_threshold criterion on a density ramp rho = r0 + a x whose
+    points have the classical temperature T_cl: the central gradient is a, so
+    the point's temperature is T = 3/(8 cv) (8 cv/3 T_cl - a^2/(2 We r0)), and
+    the band comes from the published saturation fits
+    rho_liq = 1 + 2.06 sqrt(1 - T), rho_vap = 1 - 1.91 sqrt(1 - T)
+    (dim2d_state_eq_module.f:145-227, dim2d_lowTemperature = .false.)."""
+    n = 8
+    m = orc.make_model(n + 4, n + 4, bc=(orc.BC_OPEN,) * 4)
+    dx = dy = 0.01
+    checked = [0, 0]
+    for r0 in np.linspace(0.3, 1.7, 29):
+        for Tcl in (0.9, 0.95, 0.97):
+            for a in (0.0, 3.0, -5.0):
+                for eps in (0.0, 0.2, 0.5):
+                    nodes = np.zeros((4, n + 4, n + 4))
+                    for i in range(n + 4):
+                        rho = r0 + a * (i - 6) * dx
+                        nodes[:, :, i] = _rest_state(m.cv_r, rho, Tcl)[:, None]
+                    # point (7, 7) 1-based is column 6 of the array: density r0
+                    v = nodes[:, 6, 6]
+                    if orc.openbc_undermined_point(m, np.ascontiguousarray(v)):
+                        continue
+                    T = 3.0 / (8.0 * m.cv_r) * (8.0 * m.cv_r / 3.0 * Tcl - 0.5 / m.we * a * a / r0)
+                    liq, vap = 1.0 + 2.06 * math.sqrt(1.0 - T), 1.0 - 1.91 * math.sqrt(1.0 - T)
+                    mid = 0.5 * (liq + vap)
+                    lo, hi = vap + eps * (mid - vap), liq - eps * (liq - mid)
+                    if min(abs(r0 - lo), abs(r0 - hi)) < 1e-9:
+                        continue
+                    want = bool(lo <= r0 <= hi)
+                    got = orc.openbc_undermined_at(m, nodes, 7, 7, dx, dy, eps)
+                    assert got == want, (r0, Tcl, a, eps)
+                    assert not orc.openbc_undermined_at(m, nodes, 7, 7, dx, dy, -1.0)
+                    checked[int(want)] += 1
+    assert min(checked) > 20
