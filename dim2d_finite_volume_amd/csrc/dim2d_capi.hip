@@ -465,6 +465,36 @@ int dim2d_time_dev(dim2d_ctx* c, double t, double* time_dev) {
   return DIM2D_OK;
 }
 
+int dim2d_check_steady_state(dim2d_ctx* c, double limit, double out[4], int32_t* steady) {
+  if (!c || !steady) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "check_steady_state before upload");
+  if (!(limit >= 0.0)) return fail(DIM2D_EINVAL, "limit must be >= 0, got %g", limit);
+  const DevParams& p = c->dp;
+  // the time derivative into U2 (free between steps), as dim2d_time_dev
+  StageArgs a;
+  memset(&a, 0, sizeof a);
+  a.U = c->U;
+  a.S64 = c->U;
+  a.O64 = c->U2;
+  HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
+  if (is_open(p)) HIP_TRY(launch_frame(p, c->mixed, 0, a, c->stream));
+  unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->T);
+  HIP_TRY(hipMemsetAsync(bits, 0, 4 * sizeof(unsigned long long), c->stream));
+  HIP_TRY(launch_absmax(p, is_open(p) ? 1 : 0, c->U2, bits, c->stream));
+  unsigned long long h[4];
+  HIP_TRY(hipMemcpyAsync(h, bits, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  bool ok = true;
+  for (int k = 0; k < 4; ++k) {
+    double v;
+    memcpy(&v, &h[k], sizeof v);
+    if (out) out[k] = v;
+    ok = ok && v <= limit;
+  }
+  *steady = ok ? 1 : 0;
+  return DIM2D_OK;
+}
+
 // the three stages of one step; dt from `dt_fixed` or from the CFL reductions
 static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
   const DevParams& p = c->dp;

@@ -116,6 +116,11 @@ hipError_t launch_pack(const DevParams& p, const double* planar, double* F, hipS
 hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, int fill_ghosts,
                          double ghost_value, hipStream_t s);
 
+// max |F| per variable over the interior (and the ghost frame when `frame`)
+// into out[0..3] as bits of non-negative doubles (atomicMax; clear out first);
+// non-finite values count as +inf
+hipError_t launch_absmax(const DevParams& p, int frame, const double* F, unsigned long long* out,
+                         hipStream_t s);
 // open edges: number of undermined points (imaginary sound speed; with
 // md_thr >= 0 also a density inside the two-phase band) in the `dist`
 // interior rows/columns nearest each edge, added into counts[0..3] (W, E, S, N)

@@ -1471,6 +1471,37 @@ __global__ void unpack_kernel(const DevParams p, const double* __restrict__ F,
 }
 
 // ------------------------------------------------------------------------
+// steady-state check (field_abstract_class.f:629-667)
+// ------------------------------------------------------------------------
+// max |F| per variable over the interior cells, and with `frame` the ghost
+// frame too (open edges, where it carries a time derivative); a non-finite
+// value counts as +inf.  out[0..3]: bits of non-negative doubles, combined
+// with atomicMax.
+__global__ void absmax_kernel(const DevParams p, int frame, const double* __restrict__ F,
+                              unsigned long long* __restrict__ out) {
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long n = (long long)p.nx * p.ny;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % p.nx), j = (int)(t / p.nx);
+    const bool ghost = i < kBcSize || i >= p.nx - kBcSize || j < kBcSize || j >= p.ny - kBcSize;
+    if (ghost && !frame) continue;
+    const double* c = F + ((long long)j * p.pitch + i) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double a = fabs(c[k]);
+      if (!(a <= m[k])) m[k] = isnan(a) ? INFINITY : a;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double w = wave_max(m[k]);
+    if ((threadIdx.x & 63) == 0 && w > 0.0)
+      atomicMax(&out[k], (unsigned long long)__double_as_longlong(w));
+  }
+}
+
+// ------------------------------------------------------------------------
 // domain extension (field_extended_class.f:427-452, bf_interface_icr_class.f:251-386)
 // ------------------------------------------------------------------------
 // are_openbc_undermined (pmodel_eq_class.f:1991-2070): the sound speed of the
@@ -1696,6 +1727,15 @@ hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int 
   long long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, F, ctl, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_absmax(const DevParams& p, int frame, const double* F, unsigned long long* out,
+                         hipStream_t s) {
+  const long long n = (long long)p.nx * p.ny;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, frame, F, out);
   return hipGetLastError();
 }
 

@@ -128,6 +128,16 @@ module dim2d_hip_cabi
       integer(c_int) :: st
     end function dim2d_set_tiles
 
+    function dim2d_check_steady_state(ctx, limit, out, steady) &
+        bind(C, name="dim2d_check_steady_state") result(st)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: limit
+      real(c_double), intent(out) :: out(4)
+      integer(c_int32_t), intent(out) :: steady
+      integer(c_int) :: st
+    end function dim2d_check_steady_state
+
     function dim2d_get_params(ctx, out) bind(C, name="dim2d_get_params") result(st)
       import :: c_int, c_ptr, dim2d_params
       type(c_ptr), value :: ctx

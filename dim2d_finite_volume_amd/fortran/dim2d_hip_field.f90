@@ -7,6 +7,7 @@
 !>     -> td_integrator%integrate           td_integrator/rungekutta3rdtvd/td_integrator_class.f:103-161
 !>   td_operators%compute_time_dev          td_operators/finitevolume/td_operators_class.f:119-235
 !>   bc_operators_gen%apply_bc_on_nodes     boundary_conditions/bc_operators_gen_class.f:238-381
+!>   field%check_steady_state              field/field_abstract_class.f:629-667
 !>   field_extended%adapt_domain          field/field_extended_class.f:427-452
 !>     (rectangular growth of open edges; the grid, maps and mirror follow)
 !> The host array nodes(nx,ny,ne) is a mirror: it is refreshed by sync_nodes
@@ -46,6 +47,7 @@ module dim2d_hip_field
     procedure :: apply_bc_on_nodes
     procedure :: cfl_dt
     procedure :: diagnostics
+    procedure :: check_steady_state
     procedure :: adapt_domain
     procedure :: finalize
   end type hip_field
@@ -146,6 +148,19 @@ contains
     real(rkind) :: out(7)
     call dim2d_check(dim2d_diagnostics(this%ctx, out), 'dim2d_diagnostics')
   end function diagnostics
+
+  !> field%check_steady_state (field_abstract_class.f:629-667): max |time_dev|
+  !> over the cells with a derivative below steady_state_limit
+  function check_steady_state(this, limit) result(steady)
+    class(hip_field), intent(inout) :: this
+    real(rkind), intent(in) :: limit
+    logical :: steady
+    real(c_double) :: out(4)
+    integer(c_int32_t) :: st
+    call dim2d_check(dim2d_check_steady_state(this%ctx, limit, out, st), &
+                     'dim2d_check_steady_state')
+    steady = st /= 0
+  end function check_steady_state
 
   !> field_extended%adapt_domain: each open edge in adapt_mask (1 W, 2 E, 4 S,
   !> 8 N) whose band of `distance` points is undermined moves out by `grow`

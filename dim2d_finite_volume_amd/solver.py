@@ -119,6 +119,16 @@ class Field:
     def apply_bc_on_nodes(self) -> None:
         capi.check(self._lib.dim2d_apply_bc(self._ctx, self.time), "dim2d_apply_bc")
 
+    def check_steady_state(self, limit: float = 1e-12):
+        """field%check_steady_state (field_abstract_class.f:629-667): returns
+        (steady, max |time_dev| per variable over the cells with a derivative)."""
+        out = np.zeros(4)
+        steady = ctypes.c_int32()
+        capi.check(self._lib.dim2d_check_steady_state(self._ctx, limit, _dptr(out),
+                                                      ctypes.byref(steady)),
+                   "dim2d_check_steady_state")
+        return bool(steady.value), out
+
     # -- domain extension at open edges -------------------------------------
     EDGES = {"W": 1, "E": 2, "S": 4, "N": 8}
 

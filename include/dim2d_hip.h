@@ -137,6 +137,18 @@ int dim2d_cfl_dt(dim2d_ctx* ctx, double cfl, double out[4]);
  * out[6] = number of non-finite values.  (No counterpart in the reference.) */
 int dim2d_diagnostics(dim2d_ctx* ctx, double out[7]);
 
+/* field%check_steady_state (field_abstract_class.f:629-667), used by the time
+ * loop of steady-state runs (sim_dim2d.f:75-101, steady_state_limit = 1e-12 in
+ * parameters_input.f:913) (blocking).  Computes the time derivative of the
+ * current state and its max |.| per variable over the cells that carry one:
+ * the interior, and the ghost frame when the edges are open.  out[0..3] (may
+ * be NULL) receives the maxima, non-finite values counting as +inf;
+ * *steady = 1 when every maximum is <= limit.  The reference scans every
+ * cell of time_dev, whose ghost cells hold debug_real (td_operators_class.f:
+ * 150-154) unless the edges are open, and it passes NaN (abs(NaN) > limit is
+ * false); this checks the cells with a derivative and fails on NaN. */
+int dim2d_check_steady_state(dim2d_ctx* ctx, double limit, double out[4], int32_t* steady);
+
 /* Simulated time and the dt of the last step after dim2d_integrate
  * (blocking): out[0] = t, out[1] = dt. */
 int dim2d_time(dim2d_ctx* ctx, double out[2]);

@@ -327,3 +327,42 @@ def test_smallest_grid(orc):
     ref = orc.time_dev(m, nodes, xm, ym)
     got = d.Field(p, nodes).compute_time_dev()
     assert max(max_rel_err(got, ref)) < 1e-11
+
+
+@pytest.mark.parametrize("case", ["reflection", "periodic_x_wall_y", "open"])
+def test_check_steady_state_max_matches_oracle(orc, case):
+    """field%check_steady_state: the max |time_dev| per variable over the cells
+    with a derivative (interior; the frame too with open edges) against the
+    oracle's time derivative, 1e-11 relative; limits on either side of it."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case])
+    ref = orc.time_dev(m, nodes, xm, ym)
+    region = ALL if case == "open" else (slice(2, -2), slice(2, -2))
+    want = np.array([np.abs(ref[k][region]).max() for k in range(4)])
+    f = d.Field(p, nodes)
+    steady, got = f.check_steady_state(1e-12)
+    assert not steady
+    assert np.allclose(got, want, rtol=1e-11, atol=0), (got, want)
+    assert f.check_steady_state(1.01 * want.max())[0]
+    assert not f.check_steady_state(0.99 * want.max())[0]
+    # the state is unchanged by the check
+    assert np.array_equal(f.download()[:, 2:-2, 2:-2], nodes[:, 2:-2, 2:-2])
+    f.close()
+
+
+def test_check_steady_state_uniform_and_nonfinite():
+    """A uniform fluid at rest has an exactly zero time derivative; a NaN in the
+    state is never steady (the maximum reads +inf)."""
+    d = _lib()
+    p = d.ModelParams(nx=40, ny=36, bc=(R, R, R, R))
+    rho, T = 1.2, 0.97
+    E = rho * 8.0 * p.cv_r / 3.0 * T - 3.0 * rho ** 2
+    nodes = np.zeros((4, p.ny, p.nx))
+    nodes[0], nodes[3] = rho, E
+    f = d.Field(p, nodes)
+    steady, got = f.check_steady_state(0.0)
+    assert steady and np.array_equal(got, np.zeros(4))
+    nodes[3, 10, 12] = np.nan
+    f.upload(nodes)
+    steady, got = f.check_steady_state(1e30)
+    assert not steady and np.isinf(got).any()
+    f.close()
