@@ -119,6 +119,30 @@ class Field:
     def apply_bc_on_nodes(self) -> None:
         capi.check(self._lib.dim2d_apply_bc(self._ctx, self.time), "dim2d_apply_bc")
 
+    def write_data(self, path: str, dt: float | None = None, t_max: float | None = None,
+                   attrs: dict | None = None) -> None:
+        """field%write_data (field_abstract_class.f:597-610): the current state
+        as one output file in the reference's NetCDF layout (io_netcdf)."""
+        from . import io_netcdf
+        io_netcdf.write_data(path, self.download(), self.x_map, self.y_map, self.time,
+                             self.params, dt=dt, t_max=t_max, attrs=attrs)
+
+    @classmethod
+    def restart(cls, params: ModelParams, path: str) -> "Field":
+        """A field whose state and time come from an output file: the
+        reference's restart (-r, io_operators_class.f:183-236).  The file's
+        grid must be the one params describe."""
+        from . import io_netcdf
+        nodes, x_map, y_map, time, _ = io_netcdf.read_data(path)
+        if nodes.shape != (4, params.ny, params.nx):
+            raise ValueError(f"{path}: grid {nodes.shape[2]}x{nodes.shape[1]} is not "
+                             f"{params.nx}x{params.ny}")
+        xm, ym = params.coordinates()
+        h = min(params.spacing())
+        if np.abs(xm - x_map).max() > 1e-9 * h or np.abs(ym - y_map).max() > 1e-9 * h:
+            raise ValueError(f"{path}: coordinates differ from the parameters' domain")
+        return cls(params, nodes, time=time)
+
     def check_steady_state(self, limit: float = 1e-12):
         """field%check_steady_state (field_abstract_class.f:629-667): returns
         (steady, max |time_dev| per variable over the cells with a derivative)."""

@@ -366,3 +366,23 @@ def test_check_steady_state_uniform_and_nonfinite():
     steady, got = f.check_steady_state(1e30)
     assert not steady and np.isinf(got).any()
     f.close()
+
+
+@pytest.mark.parametrize("case", ["periodic_x_wall_y", "open"])
+def test_restart_from_output_file_continues_exactly(orc, tmp_path, case):
+    """field%write_data then a restart from that file (-r) continues the run
+    bit for bit: the file holds the state, ghost cells included, and the time."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case])
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    f = d.Field(p, nodes)
+    f.run(5, dt=dt)
+    path = str(tmp_path / "data1.nc")
+    f.write_data(path, dt=dt)
+    g = d.Field.restart(p, path)
+    assert g.time == f.time
+    f.run(7, dt=dt)
+    g.run(7, dt=dt)
+    assert np.array_equal(f.download(), g.download())
+    assert np.array_equal(f.device_time(), g.device_time())
+    f.close()
+    g.close()
