@@ -18,6 +18,7 @@ EINVAL, EDEVICE, ENOMEM, ESTATE, ENONFINITE = -1, -2, -3, -4, -5
 BC_REFLECTION, BC_PERIODIC, BC_WALL, BC_OPEN = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 INT_RK3TVD, INT_LSRK3 = 0, 1
+OPS_MATTSSON, OPS_COCKBURN_GAU = 0, 1
 
 # every symbol declared in include/dim2d_hip.h
 EXPORTS = (
@@ -45,6 +46,7 @@ class Params(ctypes.Structure):
         ("pr", ctypes.c_double), ("viscous_r", ctypes.c_double), ("cv_r", ctypes.c_double),
         ("gravity", ctypes.c_double), ("wall_contact_angle", ctypes.c_double),
         ("wall_heat_flux", ctypes.c_double), ("integrator", ctypes.c_int32),
+        ("operators", ctypes.c_int32),
     ]
 
 

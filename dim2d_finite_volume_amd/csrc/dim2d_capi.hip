@@ -77,6 +77,16 @@ int validate(const dim2d_params* p) {
     return fail(DIM2D_EINVAL, "unknown integrator %d", p->integrator);
   if (p->integrator == DIM2D_INT_LSRK3 && p->precision != DIM2D_PREC_FP64)
     return fail(DIM2D_EINVAL, "the low-storage RK3 runs in FP64 only");
+  if (p->operators != DIM2D_OPS_MATTSSON && p->operators != DIM2D_OPS_COCKBURN_GAU)
+    return fail(DIM2D_EINVAL, "unknown operator set %d", p->operators);
+  if (p->operators == DIM2D_OPS_COCKBURN_GAU) {
+    if (p->precision != DIM2D_PREC_FP64 || p->integrator != DIM2D_INT_RK3TVD)
+      return fail(DIM2D_EINVAL, "the Cockburn-Gau operators run in FP64 with RK3-TVD only");
+    for (int k = 0; k < 4; ++k)
+      if (b[k] == DIM2D_BC_WALL || b[k] == DIM2D_BC_OPEN)
+        return fail(DIM2D_EINVAL,
+                    "the Cockburn-Gau operators support reflection and periodic edges only");
+  }
   if (!(p->x_max > p->x_min) || !(p->y_max > p->y_min))
     return fail(DIM2D_EINVAL, "empty coordinate range");
   if (!(p->re > 0) || !(p->we > 0) || !(p->pr > 0) || !(p->cv_r > 0))
@@ -171,6 +181,7 @@ struct dim2d_ctx {
   int device = 0;
   int mixed = 0;
   int lsrk = 0;          // low-storage RK3 instead of RK3-TVD
+  int cg = 0;            // Cockburn-Gau face stencils (FP64 RK3-TVD, own kernel)
   hipStream_t stream = nullptr;
   // open edges: the ghost-frame kernel of a stage runs on `side`, forked from
   // and joined to `stream` around the stage kernel
@@ -268,6 +279,7 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   c->dp = make_dev_params(*params);
   c->mixed = params->precision == DIM2D_PREC_MIXED;
   c->lsrk = params->integrator == DIM2D_INT_LSRK3;
+  c->cg = params->operators == DIM2D_OPS_COCKBURN_GAU;
   const size_t n = (size_t)c->dp.ncell * 4;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -455,7 +467,8 @@ int dim2d_time_dev(dim2d_ctx* c, double t, double* time_dev) {
   a.U = c->U;
   a.S64 = c->U;
   a.O64 = c->U2;
-  HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
+  if (c->cg) HIP_TRY(launch_cg_stage(p, 0, a, c->stream));
+  else HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
   // open edges: the ghost frame has a time derivative of its own
   if (is_open(p)) HIP_TRY(launch_frame(p, c->mixed, 0, a, c->stream));
   HIP_TRY(launch_unpack(p, c->U2, c->T, is_open(p) ? 0 : 1, kDebugReal, c->stream));
@@ -476,7 +489,8 @@ int dim2d_check_steady_state(dim2d_ctx* c, double limit, double out[4], int32_t*
   a.U = c->U;
   a.S64 = c->U;
   a.O64 = c->U2;
-  HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
+  if (c->cg) HIP_TRY(launch_cg_stage(p, 0, a, c->stream));
+  else HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
   if (is_open(p)) HIP_TRY(launch_frame(p, c->mixed, 0, a, c->stream));
   unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->T);
   HIP_TRY(hipMemsetAsync(bits, 0, 4 * sizeof(unsigned long long), c->stream));
@@ -522,7 +536,17 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     return DIM2D_OK;
   };
   int st = DIM2D_OK;
-  if (c->lsrk) {
+  if (c->cg) {
+    // Cockburn-Gau stencils: FP64 RK3-TVD stages u^n -> A -> B -> U2, no walls
+    a.U = c->U;
+    double* in[3] = {c->U, c->A, c->B};
+    double* out[3] = {c->A, c->B, c->U2};
+    for (int s = 0; s < 3; ++s) {
+      a.S64 = in[s];
+      a.O64 = out[s];
+      HIP_TRY(timed_launch(c, s, [&] { return launch_cg_stage(p, 1 + s, a, c->stream); }));
+    }
+  } else if (c->lsrk) {
     // low-storage RK3: the state ping-pongs U -> U2 -> U -> U2, q stays in Q;
     // the wall root guess is each stage's input state
     double* in = c->U;
@@ -926,6 +950,8 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
       return fail(DIM2D_EINVAL, "all simulations of a batch must share the precision mode");
     if (params[i].integrator != DIM2D_INT_RK3TVD)
       return fail(DIM2D_EINVAL, "batches run the RK3-TVD integrator only");
+    if (params[i].operators != DIM2D_OPS_MATTSSON)
+      return fail(DIM2D_EINVAL, "batches run the Mattsson operators only");
     if (params[i].bc_n == DIM2D_BC_OPEN)
       return fail(DIM2D_EINVAL, "batches do not support open boundaries");
   }

@@ -116,6 +116,10 @@ hipError_t launch_pack(const DevParams& p, const double* planar, double* F, hipS
 hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, int fill_ghosts,
                          double ghost_value, hipStream_t s);
 
+// Cockburn and Gau operators, FP64: time derivative (stage 0) or RK3-TVD
+// stage 1-3 from a.S64 (and a.U), into a.O64; stage 3 also reduces for the CFL
+// step, as launch_stage does
+hipError_t launch_cg_stage(const DevParams& p, int stage, const StageArgs& a, hipStream_t s);
 // max |F| per variable over the interior (and the ghost frame when `frame`)
 // into out[0..3] as bits of non-negative doubles (atomicMax; clear out first);
 // non-finite values count as +inf

@@ -1471,6 +1471,232 @@ __global__ void unpack_kernel(const DevParams p, const double* __restrict__ F,
 }
 
 // ------------------------------------------------------------------------
+// Cockburn and Gau operators (sd_operators/cg_operators/sd_operators_class.f)
+// ------------------------------------------------------------------------
+// FP64 time derivative and RK3-TVD stages with the fourth-order face stencils
+// of Cockburn and Gau: f = (-p[i-2] + 7p[i-1] + 7p[i] - p[i+1])/12, dfdy over
+// four columns of rows j+-1, d2fdy2 over the f of three rows, dfdx and d2fdx2
+// as Mattsson's; g, dgdx, dgdy, d2gdx2, d2gdy2 the same across y.  An x-face
+// reads a 4x3 block of cells and a y-face a 3x4 block, so this kernel does not
+// use the register pipeline of the Mattsson stencils (whose faces read a cross
+// of cells): a 16x16 tile of output cells stages the per-cell quantities of
+// the tile and its 2-cell halo in LDS (state, velocities, pressure, classical
+// and capillary temperatures, 1/(3 - rho)), computes each face of the tile
+// once into LDS, then the divergence and the RK update.  The DIM fluxes follow
+// dim2d_fluxes_module.f:103-1649 term by term (F = inv - visc/Re - cap/We).
+namespace cg {
+
+constexpr int kT = 16;            // output cells per tile side
+constexpr int kH = 2;             // halo (bc_size)
+constexpr int kW = kT + 2 * kH;   // staged side, 20
+constexpr int kQ = 10;            // staged quantities per cell
+enum { Q_R = 0, Q_QX, Q_QY, Q_E, Q_U, Q_V, Q_P, Q_TCL, Q_TCAP, Q_W };
+
+struct Tile {
+  double q[kQ][kW * kW];
+  double fx[4][(kT + 1) * kT];  // x-faces: (kT+1) per row, kT rows
+  double fy[4][kT * (kT + 1)];  // y-faces: kT per row, kT+1 rows
+};
+
+// staged value of quantity Q at tile-local cell (x, y) (0..kW-1, halo included)
+#define CG_AT(Q, x, y) (t.q[(Q)][(y) * kW + (x)])
+
+// the four-point face average of a product A*B (B = 1 when absent)
+__device__ __forceinline__ double fx4(const Tile& t, int qa, int qb, int x, int y) {
+  auto v = [&](int xx) { return qb < 0 ? CG_AT(qa, xx, y) : CG_AT(qa, xx, y) * CG_AT(qb, xx, y); };
+  return 1.0 / 12.0 * (-v(x - 2) + 7.0 * v(x - 1) + 7.0 * v(x) - v(x + 1));
+}
+__device__ __forceinline__ double gy4(const Tile& t, int qa, int qb, int x, int y) {
+  auto v = [&](int yy) { return qb < 0 ? CG_AT(qa, x, yy) : CG_AT(qa, x, yy) * CG_AT(qb, x, yy); };
+  return 1.0 / 12.0 * (-v(y - 2) + 7.0 * v(y - 1) + 7.0 * v(y) - v(y + 1));
+}
+
+// x-face (i-1/2, j) at tile-local (x, y) = cell i's position
+__device__ void x_face(const DevParams& p, const Tile& t, int x, int y, double F[4]) {
+  const double idx = p.idx, idy = p.idy;
+  auto dfdx = [&](int Q) { return idx * (-CG_AT(Q, x - 1, y) + CG_AT(Q, x, y)); };
+  auto dfdy = [&](int Q) {
+    return 1.0 / 24.0 * idy *
+           (+CG_AT(Q, x - 2, y - 1) - 7.0 * CG_AT(Q, x - 1, y - 1) - 7.0 * CG_AT(Q, x, y - 1) +
+            CG_AT(Q, x + 1, y - 1) - CG_AT(Q, x - 2, y + 1) + 7.0 * CG_AT(Q, x - 1, y + 1) +
+            7.0 * CG_AT(Q, x, y + 1) - CG_AT(Q, x + 1, y + 1));
+  };
+  auto row4 = [&](int Q, int yy) {
+    return -CG_AT(Q, x - 2, yy) + 7.0 * CG_AT(Q, x - 1, yy) + 7.0 * CG_AT(Q, x, yy) -
+           CG_AT(Q, x + 1, yy);
+  };
+  const double d2x = 0.5 * idx * idx *
+                     (CG_AT(Q_R, x - 2, y) - CG_AT(Q_R, x - 1, y) - CG_AT(Q_R, x, y) + CG_AT(Q_R, x + 1, y));
+  const double d2y = 1.0 / 12.0 * idy * idy * (row4(Q_R, y - 1) - 2.0 * row4(Q_R, y) + row4(Q_R, y + 1));
+  const double rx = dfdx(Q_R), ry = dfdy(Q_R);
+  const double dux = dfdx(Q_U), duy = dfdy(Q_U), dvx = dfdx(Q_V), dvy = dfdy(Q_V);
+  const double fu = fx4(t, Q_U, -1, x, y), fv = fx4(t, Q_V, -1, x, y);
+  const double fr = fx4(t, Q_R, -1, x, y), fqx = fx4(t, Q_QX, -1, x, y);
+  const double lap = d2x + d2y;
+  const double visc = p.two_vr * dux + p.vr * dvy;
+  const double shear = duy + dvx;
+  F[0] = fqx;
+  F[1] = fx4(t, Q_QX, Q_U, x, y) + fx4(t, Q_P, -1, x, y) - p.inv_re * visc -
+         p.inv_we * (p.c15cv * fx4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + fr * lap +
+                     0.5 * (-rx * rx + ry * ry));
+  F[2] = fx4(t, Q_QY, Q_U, x, y) - p.inv_re * shear - p.inv_we * (-rx * ry);
+  const double visc_e = fu * visc + fv * shear + p.inv_pr * dfdx(Q_TCL);
+  const double cap_e = p.c15cv * fx4(t, Q_W, Q_U, x, y) * (rx * rx + ry * ry) + fqx * lap +
+                       0.5 * fu * (-rx * rx + ry * ry) - rx * ry * fv - fr * rx * (dux + dvy) -
+                       p.inv_repr * dfdx(Q_TCAP);
+  F[3] = fx4(t, Q_E, Q_U, x, y) + fx4(t, Q_P, Q_U, x, y) - p.inv_re * visc_e - p.inv_we * cap_e;
+}
+
+// y-face (i, j-1/2) at tile-local (x, y) = cell j's position
+__device__ void y_face(const DevParams& p, const Tile& t, int x, int y, double G[4]) {
+  const double idx = p.idx, idy = p.idy;
+  auto dgdy = [&](int Q) { return idy * (-CG_AT(Q, x, y - 1) + CG_AT(Q, x, y)); };
+  auto dgdx = [&](int Q) {
+    return 1.0 / 24.0 * idx *
+           (+CG_AT(Q, x - 1, y - 2) - 7.0 * CG_AT(Q, x - 1, y - 1) - 7.0 * CG_AT(Q, x - 1, y) +
+            CG_AT(Q, x - 1, y + 1) - CG_AT(Q, x + 1, y - 2) + 7.0 * CG_AT(Q, x + 1, y - 1) +
+            7.0 * CG_AT(Q, x + 1, y) - CG_AT(Q, x + 1, y + 1));
+  };
+  auto col4 = [&](int Q, int xx) {
+    return -CG_AT(Q, xx, y - 2) + 7.0 * CG_AT(Q, xx, y - 1) + 7.0 * CG_AT(Q, xx, y) -
+           CG_AT(Q, xx, y + 1);
+  };
+  const double d2x = 1.0 / 12.0 * idx * idx * (col4(Q_R, x - 1) - 2.0 * col4(Q_R, x) + col4(Q_R, x + 1));
+  const double d2y = 0.5 * idy * idy *
+                     (CG_AT(Q_R, x, y - 2) - CG_AT(Q_R, x, y - 1) - CG_AT(Q_R, x, y) + CG_AT(Q_R, x, y + 1));
+  const double rx = dgdx(Q_R), ry = dgdy(Q_R);
+  const double dux = dgdx(Q_U), duy = dgdy(Q_U), dvx = dgdx(Q_V), dvy = dgdy(Q_V);
+  const double gu = gy4(t, Q_U, -1, x, y), gv = gy4(t, Q_V, -1, x, y);
+  const double gr = gy4(t, Q_R, -1, x, y), gqy = gy4(t, Q_QY, -1, x, y);
+  const double lap = d2x + d2y;
+  const double shear = duy + dvx;
+  const double visc = p.vr * dux + p.two_vr * dvy;
+  G[0] = gqy;
+  G[1] = gy4(t, Q_QX, Q_V, x, y) - p.inv_re * shear - p.inv_we * (-rx * ry);
+  G[2] = gy4(t, Q_QY, Q_V, x, y) + gy4(t, Q_P, -1, x, y) - p.inv_re * visc -
+         p.inv_we * (p.c15cv * gy4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + gr * lap +
+                     0.5 * (rx * rx - ry * ry));
+  const double visc_e = gu * shear + gv * visc + p.inv_pr * dgdy(Q_TCL);
+  const double cap_e = p.c15cv * gy4(t, Q_W, Q_V, x, y) * (rx * rx + ry * ry) + gqy * lap +
+                       0.5 * gv * (rx * rx - ry * ry) - rx * ry * gu - gr * ry * (dux + dvy) -
+                       p.inv_repr * dgdy(Q_TCAP);
+  G[3] = gy4(t, Q_E, Q_V, x, y) + gy4(t, Q_P, Q_V, x, y) - p.inv_re * visc_e - p.inv_we * cap_e;
+}
+
+#undef CG_AT
+
+}  // namespace cg
+
+// STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages from S64 (stage
+// input) and U (u^n), as stage_kernel's FP64 stages, stage 3 with the CFL
+// reduction.  Grid: one 256-thread workgroup per 16x16 tile.
+template <int STAGE>
+__global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const StageArgs a) {
+  using namespace cg;
+  __shared__ Tile t;
+  constexpr bool FIRST = STAGE == 1, LAST = STAGE == 3;
+  double dt = a.dt_fixed;
+  if (FIRST) {
+    if (a.cfl_mode) dt = cfl_dt_from(a.ctl, a.parity);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      a.ctl->dt = dt;
+      clear_slot(a.ctl, a.parity ^ 1);
+    }
+  } else if (STAGE != 0) {
+    dt = a.ctl->dt;
+  }
+  const int tid = threadIdx.x;
+  const int i0 = kBcSize + blockIdx.x * kT, j0 = kBcSize + blockIdx.y * kT;  // first output cell
+  // 1. state of the tile and its halo (virtual ghosts through the index maps)
+  for (int c = tid; c < kW * kW; c += 256) {
+    const int x = c % kW, y = c / kW;
+    const int i = min(max(i0 - kH + x, 0), p.nx - 1), j = min(max(j0 - kH + y, 0), p.ny - 1);
+    bool fx, fy;
+    const int cm = map_col(p, i, fx), rm = map_row(p, j, fy);
+    const double* s = a.S64 + ((long long)rm * p.pitch + cm) * 4;
+    const double r = s[0], qx = fx ? -s[1] : s[1], qy = fy ? -s[2] : s[2], E = s[3];
+    const double u = qx / r, v = qy / r;
+    const double Tn = E - 0.5 * r * (u * u + v * v) + 3.0 * r * r;
+    t.q[Q_R][c] = r;
+    t.q[Q_QX][c] = qx;
+    t.q[Q_QY][c] = qy;
+    t.q[Q_E][c] = E;
+    t.q[Q_U][c] = u;
+    t.q[Q_V][c] = v;
+    t.q[Q_P][c] = 3.0 / ((3.0 - r) * p.cv) * Tn - 3.0 * r * r;  // classical_pressure_local
+    t.q[Q_TCL][c] = Tn / r;                                      // classical_temperature_eff
+    t.q[Q_W][c] = 1.0 / (3.0 - r);                               // capillarity_pressure
+  }
+  __syncthreads();
+  // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell halo
+  for (int c = tid; c < (kW - 2) * (kW - 2); c += 256) {
+    const int x = 1 + c % (kW - 2), y = 1 + c / (kW - 2);
+    const int k = y * kW + x;
+    const double gx = 0.5 * p.idx * (-t.q[Q_R][k - 1] + t.q[Q_R][k + 1]);
+    const double gy = 0.5 * p.idy * (-t.q[Q_R][k - kW] + t.q[Q_R][k + kW]);
+    t.q[Q_TCAP][k] = 0.5 / t.q[Q_R][k] * (gx * gx + gy * gy);
+  }
+  __syncthreads();
+  // 3. the faces of the tile: x-faces i-1/2 for i = i0..i0+kT, y-faces j-1/2
+  for (int c = tid; c < (kT + 1) * kT; c += 256) {
+    double F[4];
+    const int fi = c % (kT + 1), fj = c / (kT + 1);
+    cg::x_face(p, t, kH + fi, kH + fj, F);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t.fx[k][c] = F[k];
+  }
+  for (int c = tid; c < kT * (kT + 1); c += 256) {
+    double G[4];
+    const int fi = c % kT, fj = c / kT;
+    cg::y_face(p, t, kH + fi, kH + fj, G);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t.fy[k][c] = G[k];
+  }
+  __syncthreads();
+  // 4. divergence (td_operators_class.f:181-194), body force and RK update
+  const int ti = tid % kT, tj = tid / kT;
+  const int i = i0 + ti, j = j0 + tj;
+  RedAcc<false> red;
+  red.init();
+  if (i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1) {
+    double L[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      L[k] = p.idx * (t.fx[k][tj * (kT + 1) + ti] - t.fx[k][tj * (kT + 1) + ti + 1]) +
+             p.idy * (t.fy[k][tj * kT + ti] - t.fy[k][(tj + 1) * kT + ti]);
+    const int c = (kH + tj) * kW + kH + ti;
+    if (p.gravity_ac) {
+      // compute_body_forces (pmodel_eq_class.f:1840-1918)
+      L[2] -= t.q[Q_R][c] * p.gravity;
+      L[3] -= t.q[Q_QY][c] * p.gravity;
+    }
+    const long long o = ((long long)j * p.pitch + i) * 4;
+    double out[4];
+    if (STAGE == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out[k] = L[k];
+    } else {
+      // the stage input at an output cell is interior: no map or sign applies
+      const double S[4] = {t.q[Q_R][c], t.q[Q_QX][c], t.q[Q_QY][c], t.q[Q_E][c]};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double inc = S[k] + dt * L[k];
+        if (STAGE == 1) out[k] = inc;
+        else if (STAGE == 2) out[k] = 0.75 * a.U[o + k] + (1.0 - 0.75) * inc;
+        else out[k] = (1.0 / 3.0) * a.U[o + k] + (1.0 - 1.0 / 3.0) * inc;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
+    if (LAST) red.add(p, out);
+  }
+  if (LAST) {
+    red.flush(a.ctl, a.parity ^ 1);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctl->t += dt;
+  }
+}
+
+// ------------------------------------------------------------------------
 // steady-state check (field_abstract_class.f:629-667)
 // ------------------------------------------------------------------------
 // max |F| per variable over the interior cells, and with `frame` the ghost
@@ -1736,6 +1962,18 @@ hipError_t launch_absmax(const DevParams& p, int frame, const double* F, unsigne
   long long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, frame, F, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_stage(const DevParams& p, int stage, const StageArgs& a, hipStream_t s) {
+  const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
+  const dim3 grid((ni + cg::kT - 1) / cg::kT, (nj + cg::kT - 1) / cg::kT);
+  switch (stage) {
+    case 0: hipLaunchKernelGGL(cg_stage_kernel<0>, grid, dim3(256), 0, s, p, a); break;
+    case 1: hipLaunchKernelGGL(cg_stage_kernel<1>, grid, dim3(256), 0, s, p, a); break;
+    case 2: hipLaunchKernelGGL(cg_stage_kernel<2>, grid, dim3(256), 0, s, p, a); break;
+    default: hipLaunchKernelGGL(cg_stage_kernel<3>, grid, dim3(256), 0, s, p, a); break;
+  }
   return hipGetLastError();
 }
 

@@ -17,6 +17,8 @@ module dim2d_hip_cabi
   integer(c_int), parameter :: DIM2D_PREC_MIXED = 1
   integer(c_int), parameter :: DIM2D_INT_RK3TVD = 0
   integer(c_int), parameter :: DIM2D_INT_LSRK3 = 1
+  integer(c_int), parameter :: DIM2D_OPS_MATTSSON = 0
+  integer(c_int), parameter :: DIM2D_OPS_COCKBURN_GAU = 1
 
   !> dim2d_params
   type, bind(C) :: dim2d_params
@@ -27,6 +29,7 @@ module dim2d_hip_cabi
     real(c_double) :: re, we, pr, viscous_r, cv_r
     real(c_double) :: gravity, wall_contact_angle, wall_heat_flux
     integer(c_int32_t) :: integrator = 0
+    integer(c_int32_t) :: operators = 0
   end type dim2d_params
 
   interface

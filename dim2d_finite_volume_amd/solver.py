@@ -46,13 +46,14 @@ class ModelParams:
     wall_contact_angle: float = 90.0
     wall_heat_flux: float = 0.0
     integrator: int = capi.INT_RK3TVD
+    operators: int = capi.OPS_MATTSSON  # sd_choice: Mattsson or Cockburn-Gau face stencils
 
     def to_c(self) -> capi.Params:
         return capi.Params(
             self.nx, self.ny, BC_SIZE, self.bc[0], self.bc[1], self.bc[2], self.bc[3],
             self.precision, int(self.gravity != 0.0), self.x_min, self.x_max, self.y_min,
             self.y_max, self.re, self.we, self.pr, self.viscous_r, self.cv_r, self.gravity,
-            self.wall_contact_angle, self.wall_heat_flux, self.integrator)
+            self.wall_contact_angle, self.wall_heat_flux, self.integrator, self.operators)
 
     def coordinates(self):
         """x_map, y_map (field_abstract_class.f:286-315)."""

@@ -49,6 +49,12 @@ extern "C" {
                               in the reference (its only integrator is RK3-TVD); FP64 only,
                               single contexts only; three state-sized buffers instead of four */
 
+/* interior face stencils (the reference's compile-time sd_choice,
+ * config/code_config/makefile_sd.mk) */
+#define DIM2D_OPS_MATTSSON 0      /* mattsson_operators (sd_operators_class.f:185-1039) */
+#define DIM2D_OPS_COCKBURN_GAU 1  /* cg_operators (cg_operators/sd_operators_class.f): FP64,
+                                     RK3-TVD, reflection and periodic edges, single contexts */
+
 /* arithmetic of the stage kernels */
 #define DIM2D_PREC_FP64 0  /* everything in FP64 */
 #define DIM2D_PREC_MIXED 1 /* FP64 u^n and RK accumulation, FP32 stage increments and stencils */
@@ -65,6 +71,7 @@ typedef struct dim2d_params {
   double wall_contact_angle; /* degrees (wall_micro_contact_angle) */
   double wall_heat_flux;   /* constant wall heat flux, 0 = no_heat_source */
   int32_t integrator;      /* DIM2D_INT_* (ti_choice of the reference's makefile_ti.mk) */
+  int32_t operators;       /* DIM2D_OPS_* (sd_choice); 0 = Mattsson */
 } dim2d_params;
 
 typedef struct dim2d_ctx dim2d_ctx;

@@ -260,6 +260,91 @@ typedef struct sd_ops {
 static const sd_ops MATTSSON = {s_f,    s_dfdx, s_dfdx_nl, s_dfdy,    s_d2fdx2, s_d2fdy2,
                                 s_g,    s_dgdx, s_dgdy,    s_dgdy_nl, s_d2gdx2, s_d2gdy2};
 
+/* Cockburn and Gau operators (sd_operators/cg_operators/sd_operators_class.f:
+ * f :151, dfdx :209, dfdx_nl :270, dfdy :329, d2fdx2 :399, d2fdy2, d2fdxdy,
+ * g, dgdx, dgdy, dgdy_nl, d2gdx2, d2gdy2, d2gdxdy; bc_size 2, :118-121) */
+static double cg_f(const orc_model* m, const double* n, int i, int j, prim_fn p) {
+  return 1.0 / 12.0 *
+         (-p(m, n, i - 2, j) + 7.0 * p(m, n, i - 1, j) + 7.0 * p(m, n, i, j) - p(m, n, i + 1, j));
+}
+static double cg_dfdx(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx) {
+  return 1.0 / dx * (-p(m, n, i - 1, j) + p(m, n, i, j));
+}
+static double cg_dfdx_nl(const orc_model* m, const double* n, int i, int j, sec_fn p, double dx,
+                         double dy) {
+  return 1.0 / dx *
+         (-p(m, n, i - 1, j, dx, dy, gradient_x_interior, gradient_y_interior) +
+          p(m, n, i, j, dx, dy, gradient_x_interior, gradient_y_interior));
+}
+static double cg_dfdy(const orc_model* m, const double* n, int i, int j, prim_fn p, double dy) {
+  return 1.0 / (24.0 * dy) *
+         (+p(m, n, i - 2, j - 1) - 7.0 * p(m, n, i - 1, j - 1) - 7.0 * p(m, n, i, j - 1) +
+          p(m, n, i + 1, j - 1) - p(m, n, i - 2, j + 1) + 7.0 * p(m, n, i - 1, j + 1) +
+          7.0 * p(m, n, i, j + 1) - p(m, n, i + 1, j + 1));
+}
+static double cg_d2fdx2(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx) {
+  return 0.5 / (dx * dx) *
+         (+p(m, n, i - 2, j) - p(m, n, i - 1, j) - p(m, n, i, j) + p(m, n, i + 1, j));
+}
+static double cg_d2fdy2(const orc_model* m, const double* n, int i, int j, prim_fn p, double dy) {
+  return (1.0 / (12.0 * dy * dy)) *
+         ((-p(m, n, i - 2, j - 1) + 7.0 * p(m, n, i - 1, j - 1) + 7.0 * p(m, n, i, j - 1) -
+           p(m, n, i + 1, j - 1)) -
+          2.0 * (-p(m, n, i - 2, j) + 7.0 * p(m, n, i - 1, j) + 7.0 * p(m, n, i, j) -
+                 p(m, n, i + 1, j)) +
+          (-p(m, n, i - 2, j + 1) + 7.0 * p(m, n, i - 1, j + 1) + 7.0 * p(m, n, i, j + 1) -
+           p(m, n, i + 1, j + 1)));
+}
+static double cg_d2fdxdy(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx,
+                         double dy) {
+  return (p(m, n, i - 1, j - 1) - p(m, n, i, j - 1) - p(m, n, i - 1, j + 1) + p(m, n, i, j + 1)) *
+         0.5 / (dy * dx);
+}
+static double cg_g(const orc_model* m, const double* n, int i, int j, prim_fn p) {
+  return 1.0 / 12.0 *
+         (-p(m, n, i, j - 2) + 7.0 * p(m, n, i, j - 1) + 7.0 * p(m, n, i, j) - p(m, n, i, j + 1));
+}
+static double cg_dgdx(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx) {
+  return 1.0 / (24.0 * dx) *
+         (+p(m, n, i - 1, j - 2) - 7.0 * p(m, n, i - 1, j - 1) - 7.0 * p(m, n, i - 1, j) +
+          p(m, n, i - 1, j + 1) - p(m, n, i + 1, j - 2) + 7.0 * p(m, n, i + 1, j - 1) +
+          7.0 * p(m, n, i + 1, j) - p(m, n, i + 1, j + 1));
+}
+static double cg_dgdy(const orc_model* m, const double* n, int i, int j, prim_fn p, double dy) {
+  return 1.0 / dy * (-p(m, n, i, j - 1) + p(m, n, i, j));
+}
+static double cg_dgdy_nl(const orc_model* m, const double* n, int i, int j, sec_fn p, double dx,
+                         double dy) {
+  return 1.0 / dy *
+         (-p(m, n, i, j - 1, dx, dy, gradient_x_interior, gradient_y_interior) +
+          p(m, n, i, j, dx, dy, gradient_x_interior, gradient_y_interior));
+}
+static double cg_d2gdx2(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx) {
+  return 1.0 / (12.0 * dx * dx) *
+         (-p(m, n, i - 1, j - 2) + 7.0 * p(m, n, i - 1, j - 1) + 7.0 * p(m, n, i - 1, j) -
+          p(m, n, i - 1, j + 1) -
+          2.0 * (-p(m, n, i, j - 2) + 7.0 * p(m, n, i, j - 1) + 7.0 * p(m, n, i, j) -
+                 p(m, n, i, j + 1)) -
+          p(m, n, i + 1, j - 2) + 7.0 * p(m, n, i + 1, j - 1) + 7.0 * p(m, n, i + 1, j) -
+          p(m, n, i + 1, j + 1));
+}
+static double cg_d2gdy2(const orc_model* m, const double* n, int i, int j, prim_fn p, double dy) {
+  return 0.5 / (dy * dy) *
+         (+p(m, n, i, j - 2) - p(m, n, i, j - 1) - p(m, n, i, j) + p(m, n, i, j + 1));
+}
+static double cg_d2gdxdy(const orc_model* m, const double* n, int i, int j, prim_fn p, double dx,
+                         double dy) {
+  return (p(m, n, i - 1, j - 1) - p(m, n, i - 1, j) - p(m, n, i + 1, j - 1) + p(m, n, i + 1, j)) *
+         0.5 / (dx * dy);
+}
+static const sd_ops COCKBURN_GAU = {cg_f, cg_dfdx, cg_dfdx_nl, cg_dfdy,    cg_d2fdx2, cg_d2fdy2,
+                                    cg_g, cg_dgdx, cg_dgdy,    cg_dgdy_nl, cg_d2gdx2, cg_d2gdy2};
+
+/* the interior operator set of a model (sd_choice: mattsson or cockburnandgau) */
+static const sd_ops* interior_ops(const orc_model* m) {
+  return m->operators == ORC_OPS_COCKBURN_GAU ? &COCKBURN_GAU : &MATTSSON;
+}
+
 /* gradient_av of the reference stencil test (test_mattsson_operators.f:1219-1238) */
 static double gradient_av(const orc_model* m, const double* n, int i, int j, double dx, double dy,
                           grad_fn gx, grad_fn gy) {
@@ -290,6 +375,32 @@ double orc_mattsson_density_stencil(const orc_model* m, const double* n, int whi
     case 13: return gradient_y_interior(m, n, i, j, p, dy);
     case 14: return s_dfdx_nl(m, n, i, j, gradient_av, dx, dy);
     case 15: return s_dgdy_nl(m, n, i, j, gradient_av, dx, dy);
+    default: return NAN;
+  }
+}
+
+/* the Cockburn and Gau stencils of the mass density at (i, j), numbered as
+ * orc_mattsson_density_stencil's cases 0-15 */
+double orc_cg_density_stencil(const orc_model* m, const double* n, int which, int i, int j,
+                              double dx, double dy) {
+  prim_fn p = orc_mass_density;
+  switch (which) {
+    case 0: return cg_f(m, n, i, j, p);
+    case 1: return cg_dfdx(m, n, i, j, p, dx);
+    case 2: return cg_dfdy(m, n, i, j, p, dy);
+    case 3: return cg_d2fdx2(m, n, i, j, p, dx);
+    case 4: return cg_d2fdy2(m, n, i, j, p, dy);
+    case 5: return cg_d2fdxdy(m, n, i, j, p, dx, dy);
+    case 6: return gradient_x_interior(m, n, i, j, p, dx);
+    case 7: return cg_g(m, n, i, j, p);
+    case 8: return cg_dgdx(m, n, i, j, p, dx);
+    case 9: return cg_dgdy(m, n, i, j, p, dy);
+    case 10: return cg_d2gdx2(m, n, i, j, p, dx);
+    case 11: return cg_d2gdy2(m, n, i, j, p, dy);
+    case 12: return cg_d2gdxdy(m, n, i, j, p, dx, dy);
+    case 13: return gradient_y_interior(m, n, i, j, p, dy);
+    case 14: return cg_dfdx_nl(m, n, i, j, gradient_av, dx, dy);
+    case 15: return cg_dgdy_nl(m, n, i, j, gradient_av, dx, dy);
     default: return NAN;
   }
 }
@@ -1081,6 +1192,7 @@ void orc_compute_time_dev(const orc_model* m, double t, const double* nodes, con
   const double dx = x_map[1] - x_map[0];
   const double dy = y_map[1] - y_map[0];
 
+  const sd_ops* ops = interior_ops(m);
   double* flux_x = (double*)calloc((size_t)(nx + 1) * ny * 4, sizeof(double));
   double* flux_y = (double*)calloc((size_t)nx * (ny + 1) * 4, sizeof(double));
 #define FX(i, j, k) flux_x[((size_t)((k) - 1) * ny + ((j) - 1)) * (nx + 1) + ((i) - 1)]
@@ -1089,18 +1201,18 @@ void orc_compute_time_dev(const orc_model* m, double t, const double* nodes, con
   /* compute_flux_x, pmodel_eq_class.f:1073-1139 */
   for (int j = 1 + BC_SIZE; j <= ny - BC_SIZE; ++j)
     for (int i = 1 + BC_SIZE; i <= nx + 1 - BC_SIZE; ++i) {
-      FX(i, j, 1) = flux_x_mass_density(m, nodes, i, j, &MATTSSON);
-      FX(i, j, 2) = flux_x_momentum_x(m, nodes, i, j, dx, dy, &MATTSSON);
-      FX(i, j, 3) = flux_x_momentum_y(m, nodes, i, j, dx, dy, &MATTSSON);
-      FX(i, j, 4) = flux_x_total_energy(m, nodes, i, j, dx, dy, &MATTSSON);
+      FX(i, j, 1) = flux_x_mass_density(m, nodes, i, j, ops);
+      FX(i, j, 2) = flux_x_momentum_x(m, nodes, i, j, dx, dy, ops);
+      FX(i, j, 3) = flux_x_momentum_y(m, nodes, i, j, dx, dy, ops);
+      FX(i, j, 4) = flux_x_total_energy(m, nodes, i, j, dx, dy, ops);
     }
   /* compute_flux_y, :1140-1214 */
   for (int j = 1 + BC_SIZE; j <= ny + 1 - BC_SIZE; ++j)
     for (int i = 1 + BC_SIZE; i <= nx - BC_SIZE; ++i) {
-      FY(i, j, 1) = flux_y_mass_density(m, nodes, i, j, &MATTSSON);
-      FY(i, j, 2) = flux_y_momentum_x(m, nodes, i, j, dx, dy, &MATTSSON);
-      FY(i, j, 3) = flux_y_momentum_y(m, nodes, i, j, dx, dy, &MATTSSON);
-      FY(i, j, 4) = flux_y_total_energy(m, nodes, i, j, dx, dy, &MATTSSON);
+      FY(i, j, 1) = flux_y_mass_density(m, nodes, i, j, ops);
+      FY(i, j, 2) = flux_y_momentum_x(m, nodes, i, j, dx, dy, ops);
+      FY(i, j, 3) = flux_y_momentum_y(m, nodes, i, j, dx, dy, ops);
+      FY(i, j, 4) = flux_y_total_energy(m, nodes, i, j, dx, dy, ops);
     }
 
   /* apply_bc_on_fluxes for wall edges (bc_operators_wall_xy_class.f:702-854),

@@ -44,7 +44,11 @@ typedef struct orc_model {
   double wall_contact_angle;   /* micro contact angle in degrees */
   double wall_heat_flux;       /* constant heat flux (0: no_heat_source) */
   int integrator;              /* ORC_INT_RK3TVD or ORC_INT_LSRK3 */
+  int operators;               /* ORC_OPS_MATTSSON or ORC_OPS_COCKBURN_GAU (interior faces) */
 } orc_model;
+
+/* interior face stencils (sd_choice): Mattsson, or Cockburn and Gau */
+enum { ORC_OPS_MATTSSON = 0, ORC_OPS_COCKBURN_GAU = 1 };
 
 /* time integrators: the reference's RK3-TVD, and a low-storage RK3 that the
  * reference does not have (see orc_lsrk3_step) */
@@ -149,6 +153,9 @@ double orc_wall_root_f(const orc_model*, double md, const double md_x[2],
                        const double md_y[2], double velocity_x, double velocity_y,
                        double Ed, double delta_x, double delta_y);
 double orc_mass_density_liquid(double T);
+/* Cockburn and Gau stencils of the mass density (cases as orc_mattsson_density_stencil) */
+double orc_cg_density_stencil(const orc_model*, const double* nodes, int which, int i, int j,
+                              double dx, double dy);
 double orc_mass_density_vapor(double T);
 double orc_surface_tension(const orc_model*, double T);
 double orc_dwall_internal_energy_dmd(const orc_model*, double md, double temperature,

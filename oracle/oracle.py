@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "build", "libdim2d_oracle.so")
 
 BC_REFLECTION, BC_PERIODIC, BC_WALL, BC_OPEN = 0, 1, 2, 3
 INT_RK3TVD, INT_LSRK3 = 0, 1
+OPS_MATTSSON, OPS_COCKBURN_GAU = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -31,7 +32,7 @@ class Model(ctypes.Structure):
         ("bc_w", ctypes.c_int), ("bc_e", ctypes.c_int), ("bc_s", ctypes.c_int),
         ("bc_n", ctypes.c_int),
         ("wall_contact_angle", ctypes.c_double), ("wall_heat_flux", ctypes.c_double),
-        ("integrator", ctypes.c_int),
+        ("integrator", ctypes.c_int), ("operators", ctypes.c_int),
     ]
 
 
@@ -55,6 +56,8 @@ def lib():
                                     ctypes.c_double, ctypes.c_double]
         L.orc_mattsson_density_stencil.restype = ctypes.c_double
         L.orc_mattsson_density_stencil.argtypes = L.orc_flux_term.argtypes
+        L.orc_cg_density_stencil.restype = ctypes.c_double
+        L.orc_cg_density_stencil.argtypes = L.orc_flux_term.argtypes
         L.orc_compute_time_dev.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_apply_bc_on_nodes.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_rk3_step.argtypes = [M, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
@@ -115,11 +118,12 @@ def _ptr(a: np.ndarray):
 
 def make_model(nx, ny, re=5.0, we=10.0, pr=20.0, viscous_r=-1.5, cv_r=2.5, gravity=0.0,
                bc=(BC_REFLECTION,) * 4, wall_contact_angle=90.0, wall_heat_flux=0.0,
-               integrator=INT_RK3TVD) -> Model:
+               integrator=INT_RK3TVD, operators=OPS_MATTSSON) -> Model:
     """bc = (W, E, S, N).  Default physics is the reference's test set
     (dim2d_parameters.f:182-187)."""
     return Model(nx, ny, re, we, pr, viscous_r, cv_r, gravity, int(gravity != 0.0),
-                 bc[0], bc[1], bc[2], bc[3], wall_contact_angle, wall_heat_flux, integrator)
+                 bc[0], bc[1], bc[2], bc[3], wall_contact_angle, wall_heat_flux, integrator,
+                 operators)
 
 
 def flux_term(m: Model, nodes: np.ndarray, which: int, i: int, j: int, dx: float, dy: float):
@@ -128,6 +132,13 @@ def flux_term(m: Model, nodes: np.ndarray, which: int, i: int, j: int, dx: float
 
 def stencil(m: Model, nodes: np.ndarray, which: int, i: int, j: int, dx: float, dy: float):
     return lib().orc_mattsson_density_stencil(ctypes.byref(m), _ptr(nodes), which, i, j, dx, dy)
+
+
+def cg_density_stencil(m: Model, nodes: np.ndarray, which: int, i: int, j: int, dx: float,
+                       dy: float):
+    """Cockburn and Gau stencil `which` (cases as mattsson_density_stencil) of
+    the mass density at (i, j), 1-based."""
+    return lib().orc_cg_density_stencil(ctypes.byref(m), _ptr(nodes), which, i, j, dx, dy)
 
 
 def coordinates(nx, ny, x_min, x_max, y_min, y_max):

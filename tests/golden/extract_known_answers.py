@@ -6,6 +6,7 @@ Sources (jdesmarais/dim2D_finite_volume, src/test_files/):
   test_lernaeanhydra/test_dim2d_fluxes.f:303-365        26 DIM flux values at (3,3)
   test_augeanstables/test_mattsson_operators.f:135-152   interior Mattsson stencil values
   test_augeanstables/test_mattsson_operators.f:1580-1637 the 7x5 density field of that test
+  test_augeanstables/test_cg_operators.f:113-187          Cockburn-Gau stencil values and field
   test_bc_operators/test_wall_xy_equilibrium.f           wall equilibrium values
   test_bc_operators/test_ridders_method.f                Ridders roots
 
@@ -73,6 +74,33 @@ def stencil_test():
                        "dgdy_nl"]}
 
 
+def cg_stencil_test():
+    lines = read("test_augeanstables/test_cg_operators.f")
+    text = "\n".join(lines)
+    start = text.index("test interior")
+    seg = text[start:text.index("call test_operator", start)]
+    exp = [num(m) for m in re.findall(r"test_data\(\d+\)\s*=\s*([-\d.]+d0)", seg)]
+    assert len(exp) == 16
+    call = re.search(r"call test_operator\(\s*\$\s*sd_interior,[^)]*?(\d),(\d),\s*(\d),(\d),", text[start:],
+                     re.S)
+    i_f, j_f, i_g, j_g = (int(call.group(k)) for k in range(1, 5))
+    assert (i_f, j_f) == (i_g, j_g)
+    init = text[:start]
+    field = {}
+    for m in re.finditer(r"nodes\((\d),(\d),1\)=([-\d.]+)", init):
+        field[(int(m.group(1)), int(m.group(2)))] = float(m.group(3))
+    rows = [[field[(i, j)] for i in range(1, 8)] for j in range(1, 6)]
+    dx = num(re.search(r"dx\s*=\s*([-\d.]+)", init).group(1))
+    dy = num(re.search(r"dy\s*=\s*([-\d.]+)", init).group(1))
+    return {"source": "test_augeanstables/test_cg_operators.f:113-187",
+            "note": "default-kind (single precision) literals: inputs and dx, dy are float32 values",
+            "rho_rows_j1_to_j5": rows, "dx": dx, "dy": dy, "i": i_f, "j": j_f,
+            "expected": exp,
+            "labels": ["f", "dfdx", "dfdy", "d2fdx2", "d2fdy2", "d2fdxdy", "gradient_x", "g",
+                       "dgdx", "dgdy", "d2gdx2", "d2gdy2", "d2gdxdy", "gradient_y", "dfdx_nl",
+                       "dgdy_nl"]}
+
+
 def wall_test():
     text = "\n".join(read("test_bc_operators/test_wall_xy_equilibrium.f"))
 
@@ -107,8 +135,8 @@ def ridders_test():
 
 
 if __name__ == "__main__":
-    out = {"fluxes": flux_test(), "mattsson": stencil_test(), "wall": wall_test(),
-           "ridders": ridders_test()}
+    out = {"fluxes": flux_test(), "mattsson": stencil_test(), "cockburn_gau": cg_stencil_test(),
+           "wall": wall_test(), "ridders": ridders_test()}
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", os.path.join(HERE, "known_answers.json"))

@@ -108,3 +108,27 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def test_cockburn_gau_limits():
+    """The Cockburn-Gau operators run in FP64 RK3-TVD single contexts with
+    reflection or periodic edges; anything else is rejected before the device."""
+    import dim2d_finite_volume_amd as d
+    L = d.capi.load()
+    ctx = ctypes.c_void_p()
+    cases = [
+        (dict(operators=7), b"unknown operator set"),
+        (dict(operators=1, precision=1), b"FP64 with RK3-TVD"),
+        (dict(operators=1, integrator=1), b"FP64 with RK3-TVD"),
+        (dict(operators=1, bc=(1, 1, 2, 2)), b"reflection and periodic"),
+        (dict(operators=1, bc=(3, 3, 3, 3)), b"reflection and periodic"),
+    ]
+    for kw, msg in cases:
+        p = d.ModelParams(nx=40, ny=40, **kw).to_c()
+        assert L.dim2d_ctx_create(ctypes.byref(p), ctypes.byref(ctx)) == d.capi.EINVAL, kw
+        assert msg in L.dim2d_last_error(), (kw, L.dim2d_last_error())
+    n = 1
+    p = (d.capi.Params * 1)(d.ModelParams(nx=40, ny=40, operators=1).to_c())
+    b = ctypes.c_void_p()
+    assert L.dim2d_batch_create(p, n, ctypes.byref(b)) == d.capi.EINVAL
+    assert b"Mattsson" in L.dim2d_last_error()

@@ -48,12 +48,14 @@ def perturbed_state(params, seed=3):
 def oracle_model(orc, p):
     return orc.make_model(p.nx, p.ny, re=p.re, we=p.we, pr=p.pr, viscous_r=p.viscous_r,
                           cv_r=p.cv_r, bc=p.bc, wall_contact_angle=p.wall_contact_angle,
-                          wall_heat_flux=p.wall_heat_flux, integrator=p.integrator)
+                          wall_heat_flux=p.wall_heat_flux, integrator=p.integrator,
+                          operators=p.operators)
 
 
-def setup_case(orc, bc, nx=74, ny=41, precision=0, integrator=0):
+def setup_case(orc, bc, nx=74, ny=41, precision=0, integrator=0, operators=0):
     d = _lib()
-    p = d.ModelParams(nx=nx, ny=ny, bc=bc, precision=precision, integrator=integrator)
+    p = d.ModelParams(nx=nx, ny=ny, bc=bc, precision=precision, integrator=integrator,
+                      operators=operators)
     m = oracle_model(orc, p)
     x_map, y_map = p.coordinates()
     nodes = perturbed_state(p)
@@ -386,3 +388,48 @@ def test_restart_from_output_file_continues_exactly(orc, tmp_path, case):
     assert np.array_equal(f.device_time(), g.device_time())
     f.close()
     g.close()
+
+
+# ---- Cockburn and Gau operators ----
+
+CG_CASES = ["reflection", "periodic"]
+
+
+@pytest.mark.parametrize("case", CG_CASES)
+@pytest.mark.parametrize("shape", [(74, 41), (37, 52)])
+def test_cockburn_gau_time_dev(orc, case, shape):
+    """The Cockburn-Gau time derivative (sd_choice cockburnandgau) against the
+    oracle with the same operator set, 1e-11, on grids that are and are not
+    multiples of the 16x16 tile."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], nx=shape[0], ny=shape[1],
+                                        operators=1)
+    ref = orc.time_dev(m, nodes, xm, ym)
+    f = d.Field(p, nodes)
+    got = f.compute_time_dev()
+    err = max_rel_err(got, ref)
+    print(case, shape, "CG time_dev rel err", err)
+    assert max(err) < 1e-11, err
+    f.close()
+
+
+@pytest.mark.parametrize("case", CG_CASES)
+def test_cockburn_gau_steps(orc, case):
+    """30 RK3-TVD steps with the Cockburn-Gau operators, fixed dt, then 10 steps
+    with the device CFL rule, against the oracle: 1e-10."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], operators=1)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(30, dt)) == 0
+    f = d.Field(p, nodes.copy())
+    f.run(30, dt=dt)
+    err = max_rel_err(f.download(), ref)
+    print(case, "CG 30-step rel err", err)
+    assert max(err) < 1e-10, err
+    for _ in range(10):
+        dtc = orc.cfl_dt(m, ref, *p.spacing(), 0.1)[0]
+        assert orc.run(m, ref, xm, ym, np.array([dtc])) == 0
+    f.run(10, cfl=0.1)
+    err = max_rel_err(f.download(), ref)
+    print(case, "CG CFL rel err", err)
+    assert max(err) < 1e-10, err
+    f.close()

@@ -41,6 +41,50 @@ def test_mattsson_known_answers(orc):
         assert abs(got - exp) <= 1e-6 * abs(exp), (label, got, exp)
 
 
+def test_cockburn_gau_known_answers(orc):
+    """Interior Cockburn and Gau stencil values of test_cg_operators.f:162-187
+    at (3, 3) of the same 7x5 field (quoted to 6-8 significant digits; the
+    test's literals are single precision, so the inputs are float32 values)."""
+    d = KA["cockburn_gau"]
+    nodes = np.ones((4, 5, 7))
+    nodes[0] = np.array(d["rho_rows_j1_to_j5"], dtype=np.float32).astype(np.float64)
+    m = orc.make_model(7, 5, operators=orc.OPS_COCKBURN_GAU)
+    dx, dy = float(np.float32(d["dx"])), float(np.float32(d["dy"]))
+    for k, (exp, label) in enumerate(zip(d["expected"], d["labels"])):
+        got = orc.cg_density_stencil(m, nodes, k, d["i"], d["j"], dx, dy)
+        assert abs(got - exp) <= 1e-6 * abs(exp), (label, got, exp)
+
+
+def test_cockburn_gau_run_symmetries_and_conservation(orc):
+    """With the Cockburn and Gau operators a centred drop in a reflecting box
+    keeps its mirror and transpose symmetries and conserves mass and energy,
+    and its time derivative differs from the Mattsson one (the operator set
+    is in use)."""
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p = d.ModelParams(nx=28, ny=28)
+    dx, _ = p.spacing()
+    nodes = ic.circles_field(p, 0.95, [(0.0, 0.0)], 0.2, "liquid", 2 * math.sqrt(p.we) * dx)
+    xm, ym = p.coordinates()
+    m = orc.make_model(p.nx, p.ny, operators=orc.OPS_COCKBURN_GAU)
+    mm = orc.make_model(p.nx, p.ny)
+    orc.apply_bc(m, nodes, xm, ym)
+    td_cg = orc.time_dev(m, nodes, xm, ym)[:, 2:-2, 2:-2]
+    td_m = orc.time_dev(mm, nodes, xm, ym)[:, 2:-2, 2:-2]
+    assert np.abs(td_cg - td_m).max() > 1e-6 * np.abs(td_m).max()
+    s0 = orc.sums(m, nodes)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    assert orc.run(m, nodes, xm, ym, np.full(30, dt)) == 0
+    it = nodes[:, 2:-2, 2:-2]
+    scale = np.abs(it).max(axis=(1, 2))
+    assert np.abs(it[0] - it[0][:, ::-1]).max() < 1e-12 * scale[0]
+    assert np.abs(it[1] + it[1][:, ::-1]).max() < 1e-12 * scale[0]
+    assert np.abs(it[0] - it[0].T).max() < 1e-12 * scale[0]
+    assert np.abs(it[1] - it[2].T).max() < 1e-12 * scale[0]
+    s1 = orc.sums(m, nodes)
+    assert abs(s1[0] - s0[0]) < 1e-12 * abs(s0[0]) and abs(s1[3] - s0[3]) < 1e-11 * abs(s0[3])
+
+
 def test_wall_known_answers(orc):
     L = orc.lib()
     w = KA["wall"]
