@@ -183,6 +183,37 @@ def open_boundary_bench(n, steps):
     return res
 
 
+def variants_bench(n, steps):
+    """The other integrator and operator choices on the headline's two-bubble
+    state at n^2, CFL 0.1 device-side dt, FP64: the low-storage RK3 (periodic
+    x / wall y, as the headline) and the Cockburn-Gau stencils (periodic
+    edges: they take reflection and periodic edges only), with the Mattsson
+    RK3-TVD on the same periodic case beside them."""
+    import dataclasses
+
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = ic.two_bubbles(n=n)
+    per = (d.BC_PERIODIC,) * 4
+    cases = (("lsrk3", dict(integrator=d.INT_LSRK3)),
+             ("rk3tvd_periodic", dict(bc=per)),
+             ("cockburn_gau_periodic", dict(bc=per, operators=d.OPS_COCKBURN_GAU)))
+    out = {}
+    for name, kw in cases:
+        f = d.Field(dataclasses.replace(p, **kw), nodes)
+        f.run(5, cfl=0.1)
+        f.synchronize()
+        t0 = time.perf_counter()
+        f.run(steps, cfl=0.1)
+        el = time.perf_counter() - t0
+        f.close()
+        out[name] = {"value": n * n * steps / el, "unit": "grid-pt updates/s",
+                     "ms_per_step": 1e3 * el / steps}
+        log(f"[bench] {name} {n}^2: {out[name]['value']:.4e} updates/s")
+    out["config"] = f"two-bubble state, {n}x{n} interior, FP64, CFL 0.1, {steps} timed steps"
+    return out
+
+
 def read_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -210,6 +241,8 @@ def main():
     ap.add_argument("--open-n", type=int, default=8192)
     ap.add_argument("--open-steps", type=int, default=2000,
                     help="steps of the open-boundary case (0: skip)")
+    ap.add_argument("--variant-steps", type=int, default=200,
+                    help="steps of the LSRK3 / Cockburn-Gau variants (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run an elementwise kernel reading and writing 1 GiB "
                          "(for HBM counter scaling)")
@@ -314,6 +347,7 @@ def main():
 
     batch = batch_bench(args.batch_steps) if args.batch_steps > 0 else None
     open_bc = open_boundary_bench(args.open_n, args.open_steps) if args.open_steps > 0 else None
+    variants = variants_bench(n, args.variant_steps) if args.variant_steps > 0 else None
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": METRIC, "value": value, "unit": "grid-pt updates/s", "n_gpus": 1,
@@ -332,6 +366,7 @@ def main():
         "mixed": mixed,
         "batch": batch,
         "open_boundary": open_bc,
+        "variants": variants,
         "checks": {"mass_drift": float((diag1[0] - diag0[0]) / diag0[0]),
                    "nonfinite": int(diag1[6]), "t_end": t_end, "dt_last": dt_last},
     }

@@ -1479,27 +1479,30 @@ __global__ void unpack_kernel(const DevParams p, const double* __restrict__ F,
 // as Mattsson's; g, dgdx, dgdy, d2gdx2, d2gdy2 the same across y.  An x-face
 // reads a 4x3 block of cells and a y-face a 3x4 block, so this kernel does not
 // use the register pipeline of the Mattsson stencils (whose faces read a cross
-// of cells): a 16x16 tile of output cells stages the per-cell quantities of
+// of cells): a 16x15 tile of output cells stages the per-cell quantities of
 // the tile and its 2-cell halo in LDS (state, velocities, pressure, classical
 // and capillary temperatures, 1/(3 - rho)), computes each face of the tile
 // once into LDS, then the divergence and the RK update.  The DIM fluxes follow
 // dim2d_fluxes_module.f:103-1649 term by term (F = inv - visc/Re - cap/We).
 namespace cg {
 
-constexpr int kT = 16;            // output cells per tile side
-constexpr int kH = 2;             // halo (bc_size)
-constexpr int kW = kT + 2 * kH;   // staged side, 20
-constexpr int kQ = 10;            // staged quantities per cell
+// 16 x 15 output cells: the tile's 17 x 15 x-faces and 16 x 16 y-faces each
+// take one pass of the 256 threads
+constexpr int kTX = 16, kTY = 15;  // output cells per tile along x and y
+constexpr int kH = 2;              // halo (bc_size)
+constexpr int kWX = kTX + 2 * kH;  // staged cells along x (20) and y (19)
+constexpr int kWY = kTY + 2 * kH;
+constexpr int kQ = 10;             // staged quantities per cell
 enum { Q_R = 0, Q_QX, Q_QY, Q_E, Q_U, Q_V, Q_P, Q_TCL, Q_TCAP, Q_W };
 
 struct Tile {
-  double q[kQ][kW * kW];
-  double fx[4][(kT + 1) * kT];  // x-faces: (kT+1) per row, kT rows
-  double fy[4][kT * (kT + 1)];  // y-faces: kT per row, kT+1 rows
+  double q[kQ][kWX * kWY];
+  double fx[4][(kTX + 1) * kTY];  // x-faces: kTX+1 per row, kTY rows
+  double fy[4][kTX * (kTY + 1)];  // y-faces: kTX per row, kTY+1 rows
 };
 
-// staged value of quantity Q at tile-local cell (x, y) (0..kW-1, halo included)
-#define CG_AT(Q, x, y) (t.q[(Q)][(y) * kW + (x)])
+// staged value of quantity Q at tile-local cell (x, y) (halo included)
+#define CG_AT(Q, x, y) (t.q[(Q)][(y) * kWX + (x)])
 
 // the four-point face average of a product A*B (B = 1 when absent)
 __device__ __forceinline__ double fx4(const Tile& t, int qa, int qb, int x, int y) {
@@ -1589,7 +1592,7 @@ __device__ void y_face(const DevParams& p, const Tile& t, int x, int y, double G
 
 // STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages from S64 (stage
 // input) and U (u^n), as stage_kernel's FP64 stages, stage 3 with the CFL
-// reduction.  Grid: one 256-thread workgroup per 16x16 tile.
+// reduction.  Grid: one 256-thread workgroup per 16x15 tile.
 template <int STAGE>
 __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const StageArgs a) {
   using namespace cg;
@@ -1606,10 +1609,10 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
     dt = a.ctl->dt;
   }
   const int tid = threadIdx.x;
-  const int i0 = kBcSize + blockIdx.x * kT, j0 = kBcSize + blockIdx.y * kT;  // first output cell
+  const int i0 = kBcSize + blockIdx.x * kTX, j0 = kBcSize + blockIdx.y * kTY;  // first output cell
   // 1. state of the tile and its halo (virtual ghosts through the index maps)
-  for (int c = tid; c < kW * kW; c += 256) {
-    const int x = c % kW, y = c / kW;
+  for (int c = tid; c < kWX * kWY; c += 256) {
+    const int x = c % kWX, y = c / kWX;
     const int i = min(max(i0 - kH + x, 0), p.nx - 1), j = min(max(j0 - kH + y, 0), p.ny - 1);
     bool fx, fy;
     const int cm = map_col(p, i, fx), rm = map_row(p, j, fy);
@@ -1629,42 +1632,43 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
   }
   __syncthreads();
   // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell halo
-  for (int c = tid; c < (kW - 2) * (kW - 2); c += 256) {
-    const int x = 1 + c % (kW - 2), y = 1 + c / (kW - 2);
-    const int k = y * kW + x;
+  for (int c = tid; c < (kWX - 2) * (kWY - 2); c += 256) {
+    const int x = 1 + c % (kWX - 2), y = 1 + c / (kWX - 2);
+    const int k = y * kWX + x;
     const double gx = 0.5 * p.idx * (-t.q[Q_R][k - 1] + t.q[Q_R][k + 1]);
-    const double gy = 0.5 * p.idy * (-t.q[Q_R][k - kW] + t.q[Q_R][k + kW]);
+    const double gy = 0.5 * p.idy * (-t.q[Q_R][k - kWX] + t.q[Q_R][k + kWX]);
     t.q[Q_TCAP][k] = 0.5 / t.q[Q_R][k] * (gx * gx + gy * gy);
   }
   __syncthreads();
-  // 3. the faces of the tile: x-faces i-1/2 for i = i0..i0+kT, y-faces j-1/2
-  for (int c = tid; c < (kT + 1) * kT; c += 256) {
+  // 3. the faces of the tile, one of each kind per thread: x-faces i-1/2 for
+  // i = i0..i0+kTX, y-faces j-1/2 for j = j0..j0+kTY
+  if (tid < (kTX + 1) * kTY) {
     double F[4];
-    const int fi = c % (kT + 1), fj = c / (kT + 1);
+    const int fi = tid % (kTX + 1), fj = tid / (kTX + 1);
     cg::x_face(p, t, kH + fi, kH + fj, F);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t.fx[k][c] = F[k];
+    for (int k = 0; k < 4; ++k) t.fx[k][tid] = F[k];
   }
-  for (int c = tid; c < kT * (kT + 1); c += 256) {
+  {
     double G[4];
-    const int fi = c % kT, fj = c / kT;
+    const int fi = tid % kTX, fj = tid / kTX;
     cg::y_face(p, t, kH + fi, kH + fj, G);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t.fy[k][c] = G[k];
+    for (int k = 0; k < 4; ++k) t.fy[k][tid] = G[k];
   }
   __syncthreads();
   // 4. divergence (td_operators_class.f:181-194), body force and RK update
-  const int ti = tid % kT, tj = tid / kT;
+  const int ti = tid % kTX, tj = tid / kTX;
   const int i = i0 + ti, j = j0 + tj;
   RedAcc<false> red;
   red.init();
-  if (i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1) {
+  if (tj < kTY && i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1) {
     double L[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      L[k] = p.idx * (t.fx[k][tj * (kT + 1) + ti] - t.fx[k][tj * (kT + 1) + ti + 1]) +
-             p.idy * (t.fy[k][tj * kT + ti] - t.fy[k][(tj + 1) * kT + ti]);
-    const int c = (kH + tj) * kW + kH + ti;
+      L[k] = p.idx * (t.fx[k][tj * (kTX + 1) + ti] - t.fx[k][tj * (kTX + 1) + ti + 1]) +
+             p.idy * (t.fy[k][tj * kTX + ti] - t.fy[k][(tj + 1) * kTX + ti]);
+    const int c = (kH + tj) * kWX + kH + ti;
     if (p.gravity_ac) {
       // compute_body_forces (pmodel_eq_class.f:1840-1918)
       L[2] -= t.q[Q_R][c] * p.gravity;
@@ -1967,7 +1971,7 @@ hipError_t launch_absmax(const DevParams& p, int frame, const double* F, unsigne
 
 hipError_t launch_cg_stage(const DevParams& p, int stage, const StageArgs& a, hipStream_t s) {
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
-  const dim3 grid((ni + cg::kT - 1) / cg::kT, (nj + cg::kT - 1) / cg::kT);
+  const dim3 grid((ni + cg::kTX - 1) / cg::kTX, (nj + cg::kTY - 1) / cg::kTY);
   switch (stage) {
     case 0: hipLaunchKernelGGL(cg_stage_kernel<0>, grid, dim3(256), 0, s, p, a); break;
     case 1: hipLaunchKernelGGL(cg_stage_kernel<1>, grid, dim3(256), 0, s, p, a); break;

@@ -28,7 +28,11 @@ STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
 def short_name(name):
     m = STAGE_RE.search(name)
     if m:
-        return f"stage_kernel<{m.group(1)}, {m.group(2)}> ({'mixed' if m.group(1) == '1' else 'FP64'} stage {m.group(2)})"
+        prec = "mixed" if m.group(1) == "1" else "FP64"
+        st = int(m.group(2))
+        what = ("time derivative" if st == 0 else
+                f"stage {st}" if st <= 3 else f"low-storage RK3 stage {st - 3}")
+        return f"stage_kernel<{m.group(1)}, {m.group(2)}> ({prec} {what})"
     name = name.replace("void ", "")
     return name.split("(")[0][:90]
 
