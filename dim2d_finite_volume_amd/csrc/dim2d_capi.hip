@@ -639,10 +639,11 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
     HIP_TRY(launch_reduce(c->dp, c->U, c->ctl, (int)(c->steps & 1), c->stream));
   }
   // pairs of steps from a captured graph: no host launch gaps inside a pair.
-  // Not while profiling (event pairs around launches) nor with open edges
-  // (the frame kernel runs on a second stream).
+  // Not while profiling (event pairs around launches).  With open edges the
+  // frame kernels' fork and join events take the side stream into the
+  // capture, so the graph holds them as parallel branches.
   int s = 0;
-  const bool graphs = !c->profiling && !is_open(c->dp);
+  const bool graphs = !c->profiling;
   for (; graphs && s + 1 < nsteps; s += 2) {
     const int k = (int)(c->steps & 1);
     if (c->gexec[k] && (c->g_dt[k] != dt_fixed || c->g_cfl[k] != cfl_mode)) {

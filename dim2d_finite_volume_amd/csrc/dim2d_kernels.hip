@@ -1004,8 +1004,11 @@ __global__ void wall_batch_kernel(const DevParams* P, int nsim, int mixed_in, co
 // sd_operators_{y,x}_oneside_{L0,R0}_class.f (bc_operators_openbc_normal_class.f:563-787);
 // the characteristic term is hedstrom_xy_module.f:445-558 with the
 // eigen-quantities of the point itself (obc_eigenqties_bc).  One thread per
-// frame cell evaluates everything pointwise from the stage input, in FP64:
-// O(perimeter) work, written as plainly as the reference.  Indices are 0-based.
+// frame cell evaluates everything pointwise, in FP64, written as plainly as
+// the reference; every stencil point is read from a window of the stage input
+// that the workgroup stages in LDS first (a frame cell reads ~250 stencil
+// points, so reading them from HBM leaves each thread a chain of dependent
+// loads).  Indices are 0-based.
 namespace frame {
 
 enum Var { kInt = 0, kYL0, kYR0, kXL0, kXR0 };  // stencil set of a face
@@ -1015,16 +1018,22 @@ enum Prim {
   kCap, kCapx, kCapy
 };
 
+// stage input of cell (i, j): S64, U, or U + S32
+__device__ __forceinline__ void load_input(const DevParams& p, const StageArgs& a, int src, int i,
+                                           int j, double v[4]) {
+  const long long o = ((long long)j * p.pitch + i) * 4;
+  for (int k = 0; k < 4; ++k)
+    v[k] = src == 0 ? a.S64[o + k] : (src == 1 ? a.U[o + k] : a.U[o + k] + (double)a.S32[o + k]);
+}
+
+// the stage input over the window [ci0, ci0 + w) x [rj0, ...) held in LDS
 struct View {
   const DevParams* p;
-  const double* U;
-  const double* S64;
-  const float* S32;
-  int src;  // 0: S64, 1: U, 2: U + S32
+  const double* win;
+  int ci0, rj0, w;
   __device__ void cell(int i, int j, double v[4]) const {
-    const long long o = ((long long)j * p->pitch + i) * 4;
-    for (int k = 0; k < 4; ++k)
-      v[k] = src == 0 ? S64[o + k] : (src == 1 ? U[o + k] : U[o + k] + (double)S32[o + k]);
+    const double* c = win + ((j - rj0) * w + (i - ci0)) * 4;
+    for (int k = 0; k < 4; ++k) v[k] = c[k];
   }
 };
 
@@ -1326,29 +1335,77 @@ __device__ void timedev(const View& f, int i, int j, double L[4]) {
 
 }  // namespace frame
 
-// one thread per frame cell: 4(nx-4) edge cells on N/S, 4(ny-4) on E/W, 16 in corners
+// Workgroups of 128 threads, each two frame lines of 64 cells:
+//   S, N: rows {0, 1} / {ny-2, ny-1}, columns i0 .. i0+63 of [2, nx-3];
+//         window columns i0-2 .. i0+65, rows 0..2 / ny-3..ny-1
+//   W, E: columns {0, 1} / {nx-2, nx-1}, rows j0 .. j0+63 of [2, ny-3];
+//         window rows j0-2 .. j0+65, columns 0..2 / nx-3..nx-1
+//   the last workgroup: the 16 corner cells, four 3x3 windows
+// The stencils of a frame cell reach two cells along its edge and stay within
+// the three outer rows (columns) across it, so every point it reads lies in
+// its window.
+constexpr int kFrameSeg = 64;
+constexpr int kFrameWin = (kFrameSeg + 4) * 3;  // cells per window
+
 template <int MIXED, int STAGE>
-__global__ void __launch_bounds__(64) frame_kernel(const DevParams p, const StageArgs a) {
-  const int nxi = p.nx - 2 * kBcSize, nyi = p.ny - 2 * kBcSize;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j;
-  if (t < 4 * nxi) {  // N/S edges: rows 0, 1, ny-2, ny-1
-    const int q = t / nxi;
-    i = kBcSize + t % nxi;
-    j = q < 2 ? q : p.ny - 4 + q;
-  } else if ((t -= 4 * nxi) < 4 * nyi) {  // E/W edges
-    const int q = t / nyi;
-    j = kBcSize + t % nyi;
-    i = q < 2 ? q : p.nx - 4 + q;
-  } else if ((t -= 4 * nyi) < 16) {  // corners
-    const int cx = (t >> 1) & 1, cy = (t >> 3) & 1;
-    i = (cx ? p.nx - kBcSize : 0) + (t & 1);
-    j = (cy ? p.ny - kBcSize : 0) + ((t >> 2) & 1);
-  } else {
-    return;
-  }
+__global__ void __launch_bounds__(2 * kFrameSeg) frame_kernel(const DevParams p, const StageArgs a) {
+  __shared__ double win[kFrameWin * 4];
   constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
-  const frame::View f{&p, a.U, a.S64, a.S32, SRC};
+  const int nxi = p.nx - 2 * kBcSize, nyi = p.ny - 2 * kBcSize;
+  const int segx = (nxi + kFrameSeg - 1) / kFrameSeg, segy = (nyi + kFrameSeg - 1) / kFrameSeg;
+  const int tid = threadIdx.x, line = tid / kFrameSeg, lt = tid % kFrameSeg;
+  int b = blockIdx.x;
+  int i, j, ci0, rj0, w, h;
+  bool active;
+  const bool corners = b == 2 * segx + 2 * segy;
+  if (b < 2 * segx) {  // S (b < segx) or N
+    const bool north = b >= segx;
+    const int i0 = kBcSize + (north ? b - segx : b) * kFrameSeg;
+    i = i0 + lt;
+    j = north ? p.ny - kBcSize + line : line;
+    active = i <= p.nx - kBcSize - 1;
+    ci0 = i0 - 2;
+    rj0 = north ? p.ny - 3 : 0;
+    w = kFrameSeg + 4;
+    h = 3;
+  } else if ((b -= 2 * segx) < 2 * segy) {  // W or E
+    const bool east = b >= segy;
+    const int j0 = kBcSize + (east ? b - segy : b) * kFrameSeg;
+    j = j0 + lt;
+    i = east ? p.nx - kBcSize + line : line;
+    active = j <= p.ny - kBcSize - 1;
+    ci0 = east ? p.nx - 3 : 0;
+    rj0 = j0 - 2;
+    w = 3;
+    h = kFrameSeg + 4;
+  } else {  // corners: thread t < 16, corner q = t / 4
+    const int q = (tid >> 2) & 3, cx = q & 1, cy = q >> 1;
+    i = (cx ? p.nx - kBcSize : 0) + (tid & 1);
+    j = (cy ? p.ny - kBcSize : 0) + ((tid >> 1) & 1);
+    active = tid < 16;
+    ci0 = cx ? p.nx - 3 : 0;
+    rj0 = cy ? p.ny - 3 : 0;
+    w = 3;
+    h = 3;
+  }
+  // stage the window(s) (cells outside the array are never read)
+  if (corners) {
+    for (int t = tid; t < 36; t += blockDim.x) {
+      const int q = t / 9, cx = q & 1, cy = q >> 1;
+      const int ci = (cx ? p.nx - 3 : 0) + t % 3, rj = (cy ? p.ny - 3 : 0) + (t % 9) / 3;
+      frame::load_input(p, a, SRC, ci, rj, win + t * 4);
+    }
+  } else {
+    for (int t = tid; t < w * h; t += blockDim.x) {
+      const int ci = ci0 + t % w, rj = rj0 + t / w;
+      if (ci >= 0 && ci < p.nx && rj >= 0 && rj < p.ny)
+        frame::load_input(p, a, SRC, ci, rj, win + t * 4);
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const double* mywin = corners ? win + ((tid >> 2) & 3) * 9 * 4 : win;
+  const frame::View f{&p, mywin, ci0, rj0, w};
   double L[4];
   frame::timedev(f, i, j, L);
   const long long o = ((long long)j * p.pitch + i) * 4;
@@ -1911,8 +1968,9 @@ hipError_t launch_fill_ghosts(const DevParams& p, double* F, hipStream_t s) {
 
 hipError_t launch_frame(const DevParams& p, int mixed, int stage, const StageArgs& a,
                         hipStream_t s) {
-  const int n = 4 * (p.nx - 2 * kBcSize) + 4 * (p.ny - 2 * kBcSize) + 16;
-  const dim3 g((n + 63) / 64), b(64);
+  const int segx = (p.nx - 2 * kBcSize + kFrameSeg - 1) / kFrameSeg;
+  const int segy = (p.ny - 2 * kBcSize + kFrameSeg - 1) / kFrameSeg;
+  const dim3 g(2 * segx + 2 * segy + 1), b(2 * kFrameSeg);
 #define DIM2D_FR(M, S) hipLaunchKernelGGL((frame_kernel<M, S>), g, b, 0, s, p, a)
   if (!mixed) {
     switch (stage) {

@@ -256,6 +256,44 @@ def test_open_boundaries_mixed(orc):
     assert err[0] < 1e-6 and err[3] < 1e-6, err
 
 
+@pytest.mark.parametrize("shape", [(7, 7), (200, 9), (9, 150), (132, 68)])
+def test_open_frame_segments(orc, shape):
+    """Open edges on the smallest grid and on grids whose frame lines hold
+    one, several, partial and exactly filled segments of 64 cells (the frame
+    kernel's workgroups): time derivative over every cell (1e-11) and 5 RK3
+    steps (1e-10) against the oracle.  Liquid at T0 = 0.95 with the
+    perturbations of perturbed_state, so the sound speed is real everywhere.
+    On the 7x7 grid (dx = 1/3) the derivatives are ~6e-4, differences of
+    fluxes ~1e3 times larger, so the time derivative's tolerance there is
+    1e-11 x 1e2."""
+    d = _lib()
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p = d.ModelParams(nx=shape[0], ny=shape[1], bc=BC_CASES["open"])
+    m = oracle_model(orc, p)
+    dx, _ = p.spacing()
+    nodes = ic.circles_field(p, 0.95, [(9.0, 9.0)], 0.1, "vapor", 2.0 * math.sqrt(p.we) * dx,
+                             flow=(0.1, -0.05), noise=1e-4, seed=3)
+    xm, ym = p.coordinates()
+    X, Y = np.meshgrid(xm, ym)
+    nodes[1] += 0.02 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    nodes[2] += 0.03 * np.cos(2 * np.pi * X + 0.3) * np.sin(2 * np.pi * Y)
+    nodes[3] += 0.01 * np.sin(2 * np.pi * (X + Y))
+    orc.apply_bc(m, nodes, xm, ym)
+    ref = orc.time_dev(m, nodes, xm, ym)
+    f = d.Field(p, nodes.copy())
+    err = max_rel_err(f.compute_time_dev(), ref, region=ALL)
+    print(shape, "open time_dev rel err (all cells)", err)
+    assert max(err) < (1e-9 if shape == (7, 7) else 1e-11), err
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(5, dt)) == 0
+    f.run(5, dt=dt)
+    err = max_rel_err(f.download(), ref, region=ALL)
+    print(shape, "open 5-step rel err (all cells)", err)
+    assert max(err) < 1e-10, err
+    f.close()
+
+
 def test_cfl_and_diagnostics(orc):
     d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["reflection"])
     f = d.Field(p, nodes)
@@ -292,7 +330,8 @@ def test_batch_matches_single(orc):
         assert np.array_equal(got[:, 2:-2, 2:-2], single[:, 2:-2, 2:-2]), i
 
 
-@pytest.mark.parametrize("case,prec", [("periodic_x_wall_y", 0), ("reflection", 1)])
+@pytest.mark.parametrize("case,prec", [("periodic_x_wall_y", 0), ("reflection", 1), ("open", 0),
+                                       ("open", 1)])
 def test_graph_replay_matches_eager(orc, case, prec):
     """dim2d_integrate replays pairs of steps from captured HIP graphs (one per
     step parity); profiling makes it launch every step eagerly.  Several
