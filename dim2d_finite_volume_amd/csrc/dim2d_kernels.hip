@@ -428,19 +428,28 @@ struct RedAcc {
 #pragma unroll
         for (int k = 0; k < 4; ++k) atomicAdd(&ctl->sum[slot][k], t[k]);
       }
-      // The stored max (min) only grows (shrinks) while the kernel runs, so a
-      // wave whose value cannot change it skips the device-scope atomic: most
-      // waves do, which keeps thousands of waves off one cache line.
-      const unsigned long long ab = (unsigned long long)__double_as_longlong(am);
-      if (ab > peek(&ctl->amax[slot])) atomicMax(&ctl->amax[slot], ab);
-      if (rn >= 0.0 && rn < INFINITY) {
-        const unsigned long long nb = (unsigned long long)__double_as_longlong(rn);
-        if (nb < peek(&ctl->rmin[slot])) atomicMin(&ctl->rmin[slot], nb);
-      }
-      const unsigned long long xb = (unsigned long long)__double_as_longlong(fmax(rx, 0.0));
-      if (xb > peek(&ctl->rmax[slot])) atomicMax(&ctl->rmax[slot], xb);
-      if (nf) atomicAdd(&ctl->nonfinite[slot], nf);
+      const unsigned long long pk[3] = {peek(&ctl->amax[slot]), peek(&ctl->rmin[slot]),
+                                        peek(&ctl->rmax[slot])};
+      commit(ctl, slot, am, rn, rx, nf, pk);
     }
+  }
+  // Combined max |u|+c, min/max rho and non-finite count into slot `slot`.
+  // The stored max (min) only grows (shrinks) while the kernel runs, so a
+  // value that cannot change it skips the device-scope atomic: most do, which
+  // keeps thousands of waves off one cache line.  pk: the stored amax, rmin,
+  // rmax read at any time since the slot was cleared (earlier reads are only
+  // weaker bounds).
+  static __device__ void commit(StepCtl* ctl, int slot, double am, double rn, double rx,
+                                unsigned nf, const unsigned long long pk[3]) {
+    const unsigned long long ab = (unsigned long long)__double_as_longlong(am);
+    if (ab > pk[0]) atomicMax(&ctl->amax[slot], ab);
+    if (rn >= 0.0 && rn < INFINITY) {
+      const unsigned long long nb = (unsigned long long)__double_as_longlong(rn);
+      if (nb < pk[1]) atomicMin(&ctl->rmin[slot], nb);
+    }
+    const unsigned long long xb = (unsigned long long)__double_as_longlong(fmax(rx, 0.0));
+    if (xb > pk[2]) atomicMax(&ctl->rmax[slot], xb);
+    if (nf) atomicAdd(&ctl->nonfinite[slot], nf);
   }
 };
 
@@ -1667,6 +1676,14 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
   }
   const int tid = threadIdx.x;
   const int i0 = kBcSize + blockIdx.x * kTX, j0 = kBcSize + blockIdx.y * kTY;  // first output cell
+  // stage 3: the reduction words, read now so that the read's latency hides
+  // behind the tile's work (the slot was cleared by stage 1)
+  unsigned long long pk[3] = {0ull, 0ull, 0ull};
+  if (LAST && tid == 0) {
+    pk[0] = peek(&a.ctl->amax[a.parity ^ 1]);
+    pk[1] = peek(&a.ctl->rmin[a.parity ^ 1]);
+    pk[2] = peek(&a.ctl->rmax[a.parity ^ 1]);
+  }
   // 1. state of the tile and its halo (virtual ghosts through the index maps)
   for (int c = tid; c < kWX * kWY; c += 256) {
     const int x = c % kWX, y = c / kWX;
@@ -1752,8 +1769,30 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
     if (LAST) red.add(p, out);
   }
   if (LAST) {
-    red.flush(a.ctl, a.parity ^ 1);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctl->t += dt;
+    // the four waves combine in LDS; one thread commits the workgroup's values
+    __shared__ double rl[3][4];
+    __shared__ unsigned rnf[4];
+    const double am = wave_max(red.amax), rn = wave_min(red.rmin), rx = wave_max(red.rmax);
+    const unsigned nf = wave_sum_u(red.nonfinite);
+    if ((tid & 63) == 0) {
+      rl[0][tid >> 6] = am;
+      rl[1][tid >> 6] = rn;
+      rl[2][tid >> 6] = rx;
+      rnf[tid >> 6] = nf;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double A = rl[0][0], N = rl[1][0], X = rl[2][0];
+      unsigned F = rnf[0];
+      for (int w = 1; w < 4; ++w) {
+        A = fmax(A, rl[0][w]);
+        N = fmin(N, rl[1][w]);
+        X = fmax(X, rl[2][w]);
+        F += rnf[w];
+      }
+      RedAcc<false>::commit(a.ctl, a.parity ^ 1, A, N, X, F, pk);
+      if (blockIdx.x == 0 && blockIdx.y == 0) a.ctl->t += dt;
+    }
   }
 }
 
