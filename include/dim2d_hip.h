@@ -124,7 +124,8 @@ int dim2d_rk3_step(dim2d_ctx* ctx, double t, double dt);
 
 /* nsteps steps of the context's integrator without host synchronisation
  * (pairs of steps are replayed from a captured HIP graph, one per step parity,
- * unless profiling is on or the edges are open).
+ * unless profiling is on; with open edges the frame kernels are branches of
+ * the graph beside the stage kernels).
  * cfl > 0: dt of each step from the CFL rule below, evaluated on the device
  *          (fused into the third stage of the previous step); dt_fixed ignored.
  * cfl <= 0: every step uses dt_fixed.
