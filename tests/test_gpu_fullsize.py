@@ -1,0 +1,194 @@
+"""Parity at the full sizes of the measured workloads: the 4096^2 two-bubble
+case (periodic x, walls y) and the 8192^2 bubble with open edges.
+
+The oracle cannot run these grids in seconds, so it checks windows.  A cell's
+time derivative depends on the cells within 2 of it, and one RK3 step on those
+within 6.  The oracle runs on a window of the full state whose edges are the
+domain's own boundary where they lie on it and reflection elsewhere; the GPU
+result on the full grid must then equal the oracle's on the window's inner
+cells.  Errors are max |gpu - oracle| over those cells divided by the largest
+|value| of the variable over the whole GPU field: 1e-11 for time
+derivatives, 1e-10 for a step.  Plus conservation over 50 steps, which no
+window can show.
+
+Away from the bubbles these states are uniform liquid in a uniform flow,
+whose derivatives vanish; a smooth perturbation of amplitude 1e-3 (relative
+for the density) over the whole field gives the windows at the walls and at
+the open corners something to compute.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+R, P, W, O = 0, 1, 2, 3
+HALO_TD, HALO_STEP = 2, 6
+
+
+def _model(orc, p, nx, ny, bc):
+    return orc.make_model(nx, ny, re=p.re, we=p.we, pr=p.pr, viscous_r=p.viscous_r,
+                          cv_r=p.cv_r, bc=bc, wall_contact_angle=p.wall_contact_angle,
+                          wall_heat_flux=p.wall_heat_flux)
+
+
+def _perturb(p, nodes):
+    """Smooth waves of amplitude 1e-3 on every variable (in place)."""
+    xm, ym = p.coordinates()
+    sx = [np.sin(2 * np.pi * f * xm + ph) for f, ph in ((3, 0.0), (2, 0.3), (1, 0.7), (1, 0.0))]
+    sy = [np.sin(2 * np.pi * f * ym + ph) for f, ph in ((2, 0.0), (3, 1.9), (4, 0.1), (2, 0.5))]
+    for j in range(nodes.shape[1]):
+        nodes[0, j] *= 1.0 + 1e-3 * sx[0] * sy[0][j]
+        for k in range(1, 4):
+            nodes[k, j] += 1e-3 * sx[k] * sy[k][j]
+    return nodes
+
+
+def _interface_column(nodes, j0, j1):
+    """Column of the largest x-difference of the density in rows j0..j1 (a
+    window there crosses a bubble's interface), kept 100 cells from the x-edges."""
+    rho = nodes[0, j0:j1, 100:-100]
+    g = np.abs(rho[:, 2:] - rho[:, :-2]).max(axis=0)
+    return 100 + 1 + int(np.argmax(g))
+
+
+def _interface_row(nodes):
+    rho = nodes[0, 100:-100, 100:-100]
+    g = np.abs(rho[2:, :] - rho[:-2, :]).max(axis=1)
+    return 100 + 1 + int(np.argmax(g))
+
+
+def _window_error(got, ref_win, scale, rows, cols, j0, i0):
+    """max |got - ref| / scale per variable over window rows/cols (slices of
+    the window), got being the full GPU field and the window at (j0, i0)."""
+    out = []
+    for k in range(4):
+        g = got[k, j0 + rows.start:j0 + rows.stop, i0 + cols.start:i0 + cols.stop]
+        r = ref_win[k, rows, cols]
+        out.append(float(np.abs(g - r).max() / scale[k]))
+    return out
+
+
+@pytest.fixture(scope="module")
+def two_bubbles(orc):
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = ic.two_bubbles(n=4096)
+    _perturb(p, nodes)
+    m = _model(orc, p, p.nx, p.ny, p.bc)
+    xm, ym = p.coordinates()
+    orc.apply_bc(m, nodes, xm, ym)
+    return d, p, nodes, xm, ym
+
+
+def _windows_4096(p, nodes, h):
+    """(j0, i0, bc) of three windows h cells high and wide: across a bubble
+    interface, and at the S and N walls."""
+    jc = _interface_row(nodes)
+    ic_ = _interface_column(nodes, jc - 2, jc + 3)
+    mid = (jc - h // 2, ic_ - h // 2, (R, R, R, R))
+    south = (0, _interface_column(nodes, 2, 8) - h // 2, (R, R, W, R))
+    north = (p.ny - h, _interface_column(nodes, p.ny - 8, p.ny - 2) - h // 2, (R, R, R, W))
+    return [mid, south, north]
+
+
+def test_time_dev_4096_windows(orc, two_bubbles):
+    """The time derivative of the full 4096^2 field against the oracle on
+    72x72 windows (interface, S wall, N wall): 1e-11."""
+    d, p, nodes, xm, ym = two_bubbles
+    f = d.Field(p, nodes)
+    got = f.compute_time_dev()
+    f.close()
+    interior = got[:, 2:-2, 2:-2]
+    scale = [np.abs(interior[k]).max() for k in range(4)]
+    h = 72
+    for j0, i0, bc in _windows_4096(p, nodes, h):
+        win = np.ascontiguousarray(nodes[:, j0:j0 + h, i0:i0 + h])
+        m = _model(orc, p, h, h, bc)
+        ref = orc.time_dev(m, win, xm[i0:i0 + h].copy(), ym[j0:j0 + h].copy())
+        # rows: inner rows of the window; at a wall all interior rows
+        r0 = 2 if bc[2] == W else HALO_TD + 2
+        r1 = h - 2 if bc[3] == W else h - HALO_TD - 2
+        err = _window_error(got, ref, scale, slice(r0, r1), slice(HALO_TD + 2, h - HALO_TD - 2),
+                            j0, i0)
+        print("4096^2 time_dev window", (j0, i0, bc), err)
+        assert max(err) < 1e-11, (j0, i0, err)
+
+
+def test_one_step_4096_windows(orc, two_bubbles):
+    """One RK3-TVD step of the full 4096^2 field (the CFL dt of the device)
+    against the oracle on 80x80 windows: 1e-10 on the cells 6 or more from a
+    window edge that is not the domain's, ghost rows of the walls included
+    (density and momenta of the inner ghost row)."""
+    d, p, nodes, xm, ym = two_bubbles
+    f = d.Field(p, nodes)
+    dt = f.cfl_dt(0.1)[0]
+    f.run(1, dt=dt)
+    got = f.download()
+    f.close()
+    scale = [np.abs(got[k, 2:-2, 2:-2]).max() for k in range(4)]
+    h = 80
+    for j0, i0, bc in _windows_4096(p, nodes, h):
+        win = np.ascontiguousarray(nodes[:, j0:j0 + h, i0:i0 + h])
+        m = _model(orc, p, h, h, bc)
+        assert orc.run(m, win, xm[i0:i0 + h].copy(), ym[j0:j0 + h].copy(), np.array([dt])) == 0
+        r0 = 2 if bc[2] == W else HALO_STEP
+        r1 = h - 2 if bc[3] == W else h - HALO_STEP
+        cols = slice(HALO_STEP, h - HALO_STEP)
+        err = _window_error(got, win, scale, slice(r0, r1), cols, j0, i0)
+        print("4096^2 one-step window", (j0, i0, bc), err)
+        assert max(err) < 1e-10, (j0, i0, err)
+        # the inner wall ghost row: density and momenta (the reference leaves
+        # the ghost energy unset)
+        g = 1 if bc[2] == W else (h - 2 if bc[3] == W else None)
+        if g is not None:
+            e = max(float(np.abs(got[k, j0 + g, i0 + cols.start:i0 + cols.stop] -
+                                 win[k, g, cols]).max() / scale[k]) for k in range(3))
+            assert e < 1e-10, (j0, i0, e)
+
+
+def test_conservation_4096(two_bubbles):
+    """50 steps of the full 4096^2 field with the device CFL dt: the mass sum
+    (periodic in x, no mass flux through the walls) keeps 1e-12 relative;
+    every cell stays finite."""
+    d, p, nodes, xm, ym = two_bubbles
+    f = d.Field(p, nodes)
+    s0 = f.diagnostics()
+    f.run(50, cfl=0.1)
+    s1 = f.diagnostics()
+    f.close()
+    assert s1[6] == 0
+    assert abs(s1[0] - s0[0]) < 1e-12 * abs(s0[0])
+    print("4096^2 50-step drift: mass", (s1[0] - s0[0]) / s0[0])
+
+
+def test_one_step_8192_open_windows(orc):
+    """One RK3-TVD step of the 8192^2 bubble with Hedstrom open edges against
+    the oracle on 80x80 windows at the SW and NE corners (frame cells
+    included) and across the bubble's interface: 1e-10."""
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import capi
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = ic.open_bubble(n=8192, precision=capi.PREC_FP64)
+    _perturb(p, nodes)
+    xm, ym = p.coordinates()
+    f = d.Field(p, nodes)
+    dt = f.cfl_dt(0.1)[0]
+    f.run(1, dt=dt)
+    got = f.download()
+    f.close()
+    scale = [np.abs(got[k]).max() for k in range(4)]
+    h = 80
+    jc = _interface_row(nodes)
+    wins = [
+        (0, 0, (O, O, O, O), slice(0, h - HALO_STEP), slice(0, h - HALO_STEP)),
+        (p.ny - h, p.nx - h, (O, O, O, O), slice(HALO_STEP, h), slice(HALO_STEP, h)),
+        (jc - h // 2, _interface_column(nodes, jc - 2, jc + 3) - h // 2, (R, R, R, R),
+         slice(HALO_STEP, h - HALO_STEP), slice(HALO_STEP, h - HALO_STEP)),
+    ]
+    for j0, i0, bc, rows, cols in wins:
+        win = np.ascontiguousarray(nodes[:, j0:j0 + h, i0:i0 + h])
+        m = _model(orc, p, h, h, bc)
+        assert orc.run(m, win, xm[i0:i0 + h].copy(), ym[j0:j0 + h].copy(), np.array([dt])) == 0
+        err = _window_error(got, win, scale, rows, cols, j0, i0)
+        print("8192^2 open one-step window", (j0, i0, bc), err)
+        assert max(err) < 1e-10, (j0, i0, err)
