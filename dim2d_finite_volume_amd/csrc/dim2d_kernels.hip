@@ -353,25 +353,49 @@ __device__ __forceinline__ double sound_speed(const DevParams& p, const double v
   return c2 > 0.0 ? sqrt(c2) : 0.0;
 }
 
+// Reductions over the 64 lanes of a wave (all lanes active), result in every
+// lane.  Within each row of 16 lanes by DPP moves (VALU, no LDS round trip):
+// quad_perm [1,0,3,2] and [2,3,0,1] combine the 4 lanes of a quad, row_ror:4
+// and row_ror:8 the 4 quads of a row; then the four row results are read
+// from lanes 0, 16, 32 and 48.
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_move<CTRL>((int)b), hi = dpp_move<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ int lane_value(int x, int lane) {
+  return __builtin_amdgcn_readlane(x, lane);
+}
+__device__ __forceinline__ double lane_value(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce(T x, Op op) {
+  x = op(x, dpp_move<0xB1>(x));   // quad_perm [1,0,3,2]
+  x = op(x, dpp_move<0x4E>(x));   // quad_perm [2,3,0,1]
+  x = op(x, dpp_move<0x124>(x));  // row_ror:4
+  x = op(x, dpp_move<0x128>(x));  // row_ror:8
+  return op(op(lane_value(x, 0), lane_value(x, 16)), op(lane_value(x, 32), lane_value(x, 48)));
+}
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  return wave_reduce(x, [](double a, double b) { return a + b; });
 }
 __device__ __forceinline__ double wave_max(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
-  return x;
+  return wave_reduce(x, [](double a, double b) { return fmax(a, b); });
 }
 __device__ __forceinline__ double wave_min(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
-  return x;
+  return wave_reduce(x, [](double a, double b) { return fmin(a, b); });
 }
 __device__ __forceinline__ unsigned wave_sum_u(unsigned x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  return (unsigned)wave_reduce((int)x, [](int a, int b) { return (int)((unsigned)a + (unsigned)b); });
 }
 
 // relaxed device-scope read of a reduction word
@@ -1561,10 +1585,12 @@ constexpr int kWY = kTY + 2 * kH;
 constexpr int kQ = 10;             // staged quantities per cell
 enum { Q_R = 0, Q_QX, Q_QY, Q_E, Q_U, Q_V, Q_P, Q_TCL, Q_TCAP, Q_W };
 
+// one face buffer, for the x-faces (kTX+1 per row, kTY rows) and then the
+// y-faces (kTX per row, kTY+1 rows): 38.6 KB per tile, four tiles per CU
+constexpr int kFaces = (kTX + 1) * kTY > kTX * (kTY + 1) ? (kTX + 1) * kTY : kTX * (kTY + 1);
 struct Tile {
   double q[kQ][kWX * kWY];
-  double fx[4][(kTX + 1) * kTY];  // x-faces: kTX+1 per row, kTY rows
-  double fy[4][kTX * (kTY + 1)];  // y-faces: kTX per row, kTY+1 rows
+  double f[4][kFaces];
 };
 
 // staged value of quantity Q at tile-local cell (x, y) (halo included)
@@ -1692,7 +1718,10 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
     const int cm = map_col(p, i, fx), rm = map_row(p, j, fy);
     const double* s = a.S64 + ((long long)rm * p.pitch + cm) * 4;
     const double r = s[0], qx = fx ? -s[1] : s[1], qy = fy ? -s[2] : s[2], E = s[3];
-    const double u = qx / r, v = qy / r;
+    // reciprocals as the Mattsson stage kernels take them (hardware estimate
+    // and two Newton steps, ~1 ulp) instead of five divisions
+    const double ir = fast_rcp(r), w = fast_rcp(3.0 - r);
+    const double u = qx * ir, v = qy * ir;
     const double Tn = E - 0.5 * r * (u * u + v * v) + 3.0 * r * r;
     t.q[Q_R][c] = r;
     t.q[Q_QX][c] = qx;
@@ -1700,9 +1729,9 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
     t.q[Q_E][c] = E;
     t.q[Q_U][c] = u;
     t.q[Q_V][c] = v;
-    t.q[Q_P][c] = 3.0 / ((3.0 - r) * p.cv) * Tn - 3.0 * r * r;  // classical_pressure_local
-    t.q[Q_TCL][c] = Tn / r;                                      // classical_temperature_eff
-    t.q[Q_W][c] = 1.0 / (3.0 - r);                               // capillarity_pressure
+    t.q[Q_P][c] = p.c3cv * w * Tn - 3.0 * r * r;  // classical_pressure_local
+    t.q[Q_TCL][c] = Tn * ir;                      // classical_temperature_eff
+    t.q[Q_W][c] = w;                              // capillarity_pressure
   }
   __syncthreads();
   // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell halo
@@ -1711,37 +1740,44 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
     const int k = y * kWX + x;
     const double gx = 0.5 * p.idx * (-t.q[Q_R][k - 1] + t.q[Q_R][k + 1]);
     const double gy = 0.5 * p.idy * (-t.q[Q_R][k - kWX] + t.q[Q_R][k + kWX]);
-    t.q[Q_TCAP][k] = 0.5 / t.q[Q_R][k] * (gx * gx + gy * gy);
+    t.q[Q_TCAP][k] = 0.5 * fast_rcp(t.q[Q_R][k]) * (gx * gx + gy * gy);
   }
   __syncthreads();
-  // 3. the faces of the tile, one of each kind per thread: x-faces i-1/2 for
-  // i = i0..i0+kTX, y-faces j-1/2 for j = j0..j0+kTY
+  // 3. the faces of the tile, one of each kind per thread, and their
+  // divergence (td_operators_class.f:181-194): x-faces i-1/2 for
+  // i = i0..i0+kTX, then y-faces j-1/2 for j = j0..j0+kTY, through one buffer
+  const int ti = tid % kTX, tj = tid / kTX;
+  const int i = i0 + ti, j = j0 + tj;
+  const bool out_cell = tj < kTY && i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1;
   if (tid < (kTX + 1) * kTY) {
     double F[4];
     const int fi = tid % (kTX + 1), fj = tid / (kTX + 1);
     cg::x_face(p, t, kH + fi, kH + fj, F);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t.fx[k][tid] = F[k];
+    for (int k = 0; k < 4; ++k) t.f[k][tid] = F[k];
   }
+  __syncthreads();
+  double L[4];
+  if (out_cell) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      L[k] = p.idx * (t.f[k][tj * (kTX + 1) + ti] - t.f[k][tj * (kTX + 1) + ti + 1]);
+  }
+  __syncthreads();
   {
     double G[4];
     const int fi = tid % kTX, fj = tid / kTX;
     cg::y_face(p, t, kH + fi, kH + fj, G);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t.fy[k][tid] = G[k];
+    for (int k = 0; k < 4; ++k) t.f[k][tid] = G[k];
   }
   __syncthreads();
-  // 4. divergence (td_operators_class.f:181-194), body force and RK update
-  const int ti = tid % kTX, tj = tid / kTX;
-  const int i = i0 + ti, j = j0 + tj;
+  // 4. body force and RK update
   RedAcc<false> red;
   red.init();
-  if (tj < kTY && i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1) {
-    double L[4];
+  if (out_cell) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      L[k] = p.idx * (t.fx[k][tj * (kTX + 1) + ti] - t.fx[k][tj * (kTX + 1) + ti + 1]) +
-             p.idy * (t.fy[k][tj * kTX + ti] - t.fy[k][(tj + 1) * kTX + ti]);
+    for (int k = 0; k < 4; ++k) L[k] += p.idy * (t.f[k][tj * kTX + ti] - t.f[k][(tj + 1) * kTX + ti]);
     const int c = (kH + tj) * kWX + kH + ti;
     if (p.gravity_ac) {
       // compute_body_forces (pmodel_eq_class.f:1840-1918)
