@@ -47,15 +47,16 @@ def perturbed_state(params, seed=3):
 
 def oracle_model(orc, p):
     return orc.make_model(p.nx, p.ny, re=p.re, we=p.we, pr=p.pr, viscous_r=p.viscous_r,
-                          cv_r=p.cv_r, bc=p.bc, wall_contact_angle=p.wall_contact_angle,
+                          cv_r=p.cv_r, gravity=p.gravity, bc=p.bc,
+                          wall_contact_angle=p.wall_contact_angle,
                           wall_heat_flux=p.wall_heat_flux, integrator=p.integrator,
                           operators=p.operators)
 
 
-def setup_case(orc, bc, nx=74, ny=41, precision=0, integrator=0, operators=0):
+def setup_case(orc, bc, nx=74, ny=41, precision=0, integrator=0, operators=0, **physics):
     d = _lib()
     p = d.ModelParams(nx=nx, ny=ny, bc=bc, precision=precision, integrator=integrator,
-                      operators=operators)
+                      operators=operators, **physics)
     m = oracle_model(orc, p)
     x_map, y_map = p.coordinates()
     nodes = perturbed_state(p)
@@ -291,6 +292,57 @@ def test_open_frame_segments(orc, shape):
     err = max_rel_err(f.download(), ref, region=ALL)
     print(shape, "open 5-step rel err (all cells)", err)
     assert max(err) < 1e-10, err
+    f.close()
+
+
+@pytest.mark.parametrize("case,prec,integ,ops,physics", [
+    ("reflection", 0, 0, 0, dict(gravity=0.5)),
+    ("periodic_x_wall_y", 0, 0, 0, dict(gravity=0.5)),
+    ("open", 0, 0, 0, dict(gravity=0.5)),
+    ("periodic_x_wall_y", 1, 0, 0, dict(gravity=0.5)),
+    ("periodic_x_wall_y", 0, 1, 0, dict(gravity=0.5)),
+    ("periodic", 0, 0, 1, dict(gravity=0.5)),
+    ("periodic_x_wall_y", 0, 0, 0, dict(wall_contact_angle=60.0)),
+    ("wall_s_reflection", 0, 0, 0, dict(wall_contact_angle=120.0, wall_heat_flux=0.01)),
+    ("periodic_x_wall_y", 1, 0, 0, dict(wall_contact_angle=60.0, wall_heat_flux=0.01)),
+    ("wall_s_reflection", 0, 1, 0, dict(wall_contact_angle=45.0, wall_heat_flux=-0.01,
+                                        gravity=0.5)),
+])
+def test_body_force_and_wall_physics(orc, case, prec, integ, ops, physics):
+    """Gravity (compute_body_forces, pmodel_eq_class.f:1840-1918), contact
+    angles other than 90 degrees (the wetting term of the wall root function,
+    wall_xy_equilibrium_module.f:2238-2293) and a wall heat flux
+    (get_wall_heat_flux, :1552-1569, in the wall temperature and the wall
+    energy flux), across boundary sets, precisions, both integrators and the
+    Cockburn-Gau operators: the time derivative (FP64: 1e-11) and 10 steps
+    (FP64 1e-10 on every variable; mixed 1e-6 on density and energy) against
+    the oracle, ghost rows of the walls included (density and momenta of the
+    inner one)."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], precision=prec, integrator=integ,
+                                        operators=ops, **physics)
+    if prec == 0:
+        ref = orc.time_dev(m, nodes, xm, ym)
+        got = d.Field(p, nodes).compute_time_dev()
+        region = ALL if case == "open" else (slice(2, -2), slice(2, -2))
+        err = max_rel_err(got, ref, region=region)
+        print(case, physics, "time_dev rel err", err)
+        assert max(err) < 1e-11, err
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(10, dt)) == 0
+    f = d.Field(p, nodes.copy())
+    f.run(10, dt=dt)
+    got = f.download()
+    err = max_rel_err(got, ref, region=ALL if case == "open" else (slice(2, -2), slice(2, -2)))
+    print(case, prec, integ, ops, physics, "10-step rel err", err)
+    if prec == 0:
+        assert max(err) < 1e-10, err
+    else:
+        assert err[0] < 1e-6 and err[3] < 1e-6, err
+    if BC_CASES[case][2] == W:
+        tol = 1e-10 if prec == 0 else 1e-6
+        e = max_rel_err(got, ref, region=(slice(1, 2), slice(2, -2)), ks=range(3))
+        assert max(e) < tol, e
     f.close()
 
 
