@@ -1576,9 +1576,14 @@ __global__ void unpack_kernel(const DevParams p, const double* __restrict__ F,
 // dim2d_fluxes_module.f:103-1649 term by term (F = inv - visc/Re - cap/We).
 namespace cg {
 
-// 16 x 15 output cells: the tile's 17 x 15 x-faces and 16 x 16 y-faces each
-// take one pass of the 256 threads
-constexpr int kTX = 16, kTY = 15;  // output cells per tile along x and y
+// kTX x 15 output cells (16 x 15 by default): the tile's (kTX+1) x 15 x-faces
+// and kTX x 16 y-faces each take one pass of the kTX * 16 threads.  A 32-wide
+// tile (512 threads, two tiles per CU) measured the same as the 16-wide one.
+#ifndef DIM2D_CG_TX
+#define DIM2D_CG_TX 16
+#endif
+constexpr int kTX = DIM2D_CG_TX, kTY = 15;  // output cells per tile along x and y
+constexpr int kThreads = kTX * (kTY + 1);   // one y-face per thread
 constexpr int kH = 2;              // halo (bc_size)
 constexpr int kWX = kTX + 2 * kH;  // staged cells along x (20) and y (19)
 constexpr int kWY = kTY + 2 * kH;
@@ -1684,9 +1689,9 @@ __device__ void y_face(const DevParams& p, const Tile& t, int x, int y, double G
 
 // STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages from S64 (stage
 // input) and U (u^n), as stage_kernel's FP64 stages, stage 3 with the CFL
-// reduction.  Grid: one 256-thread workgroup per 16x15 tile.
+// reduction.  Grid: one workgroup of kThreads threads per tile.
 template <int STAGE>
-__global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const StageArgs a) {
+__global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams p, const StageArgs a) {
   using namespace cg;
   __shared__ Tile t;
   constexpr bool FIRST = STAGE == 1, LAST = STAGE == 3;
@@ -1711,7 +1716,7 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
     pk[2] = peek(&a.ctl->rmax[a.parity ^ 1]);
   }
   // 1. state of the tile and its halo (virtual ghosts through the index maps)
-  for (int c = tid; c < kWX * kWY; c += 256) {
+  for (int c = tid; c < kWX * kWY; c += kThreads) {
     const int x = c % kWX, y = c / kWX;
     const int i = min(max(i0 - kH + x, 0), p.nx - 1), j = min(max(j0 - kH + y, 0), p.ny - 1);
     bool fx, fy;
@@ -1735,7 +1740,7 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
   }
   __syncthreads();
   // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell halo
-  for (int c = tid; c < (kWX - 2) * (kWY - 2); c += 256) {
+  for (int c = tid; c < (kWX - 2) * (kWY - 2); c += kThreads) {
     const int x = 1 + c % (kWX - 2), y = 1 + c / (kWX - 2);
     const int k = y * kWX + x;
     const double gx = 0.5 * p.idx * (-t.q[Q_R][k - 1] + t.q[Q_R][k + 1]);
@@ -1806,8 +1811,9 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
   }
   if (LAST) {
     // the four waves combine in LDS; one thread commits the workgroup's values
-    __shared__ double rl[3][4];
-    __shared__ unsigned rnf[4];
+    constexpr int kWaves = kThreads / 64;
+    __shared__ double rl[3][kWaves];
+    __shared__ unsigned rnf[kWaves];
     const double am = wave_max(red.amax), rn = wave_min(red.rmin), rx = wave_max(red.rmax);
     const unsigned nf = wave_sum_u(red.nonfinite);
     if ((tid & 63) == 0) {
@@ -1820,7 +1826,7 @@ __global__ void __launch_bounds__(256) cg_stage_kernel(const DevParams p, const 
     if (tid == 0) {
       double A = rl[0][0], N = rl[1][0], X = rl[2][0];
       unsigned F = rnf[0];
-      for (int w = 1; w < 4; ++w) {
+      for (int w = 1; w < kWaves; ++w) {
         A = fmax(A, rl[0][w]);
         N = fmin(N, rl[1][w]);
         X = fmax(X, rl[2][w]);
@@ -2106,10 +2112,10 @@ hipError_t launch_cg_stage(const DevParams& p, int stage, const StageArgs& a, hi
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
   const dim3 grid((ni + cg::kTX - 1) / cg::kTX, (nj + cg::kTY - 1) / cg::kTY);
   switch (stage) {
-    case 0: hipLaunchKernelGGL(cg_stage_kernel<0>, grid, dim3(256), 0, s, p, a); break;
-    case 1: hipLaunchKernelGGL(cg_stage_kernel<1>, grid, dim3(256), 0, s, p, a); break;
-    case 2: hipLaunchKernelGGL(cg_stage_kernel<2>, grid, dim3(256), 0, s, p, a); break;
-    default: hipLaunchKernelGGL(cg_stage_kernel<3>, grid, dim3(256), 0, s, p, a); break;
+    case 0: hipLaunchKernelGGL(cg_stage_kernel<0>, grid, dim3(cg::kThreads), 0, s, p, a); break;
+    case 1: hipLaunchKernelGGL(cg_stage_kernel<1>, grid, dim3(cg::kThreads), 0, s, p, a); break;
+    case 2: hipLaunchKernelGGL(cg_stage_kernel<2>, grid, dim3(cg::kThreads), 0, s, p, a); break;
+    default: hipLaunchKernelGGL(cg_stage_kernel<3>, grid, dim3(cg::kThreads), 0, s, p, a); break;
   }
   return hipGetLastError();
 }
