@@ -198,6 +198,7 @@ struct dim2d_ctx {
   StepCtl* ctl = nullptr;
   int block_x = 64;
   int rows = 16;
+  bool rows_set = false;  // rows chosen by dim2d_set_tiles, else by default_rows()
   bool uploaded = false;
   long long steps = 0;
   // kernel timing: event pairs around launches of the kinds in profile_mask
@@ -268,6 +269,19 @@ const char* dim2d_last_error(void) { return g_err.c_str(); }
 
 const char* dim2d_version(void) { return "dim2d-mi355x 0.1 (gfx950)"; }
 
+// Default tile height from the grid: a wave runs two warm-up rows before its
+// first output row, so tall tiles waste least, but a small grid cut into
+// 16-row tiles leaves most of the GPU idle and every wave's row latency on the
+// critical path.  Measured on MI355X (two-bubble state, FP64 and mixed):
+// 128^2 is fastest with 2 rows (1.8x the 16-row step rate), 256^2 with 4
+// (1.37x), 512^2 and 1024^2 with 8 (2-7 %), 2048^2 and up with 16.  The rule
+// counts the waves of 16-row tiles: fewer than 50, 200, 2000 take 2, 4, 8.
+static int default_rows(const DevParams& p) {
+  const int strips = (p.nx - 2 * kBcSize + 59) / 60, nj = p.ny - 2 * kBcSize;
+  const long long waves16 = (long long)strips * ((nj + 15) / 16);
+  return waves16 < 50 ? 2 : waves16 < 200 ? 4 : waves16 < 2000 ? 8 : 16;
+}
+
 int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (!out) return fail(DIM2D_EINVAL, "out is NULL");
   *out = nullptr;
@@ -280,6 +294,7 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   c->mixed = params->precision == DIM2D_PREC_MIXED;
   c->lsrk = params->integrator == DIM2D_INT_LSRK3;
   c->cg = params->operators == DIM2D_OPS_COCKBURN_GAU;
+  c->rows = default_rows(c->dp);
   const size_t n = (size_t)c->dp.ncell * 4;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -352,7 +367,10 @@ int dim2d_set_tiles(dim2d_ctx* c, int32_t block_x, int32_t rows) {
     return fail(DIM2D_EINVAL, "block_x must be 64 (one wave per column strip), got %d", block_x);
   if (rows < 0 || rows > 4096) return fail(DIM2D_EINVAL, "rows_per_block out of range: %d", rows);
   if (block_x) c->block_x = block_x;
-  if (rows) c->rows = rows;
+  if (rows) {
+    c->rows = rows;
+    c->rows_set = true;
+  }
   drop_graphs(c);
   return DIM2D_OK;
 }
@@ -746,7 +764,10 @@ int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int3
     return fail(DIM2D_EDEVICE, "domain extension failed: %s", hipGetErrorString(e));
   }
   n->block_x = c->block_x;
-  n->rows = c->rows;
+  if (c->rows_set) {  // else the new grid keeps the default of its own size
+    n->rows = c->rows;
+    n->rows_set = true;
+  }
   n->uploaded = true;
   n->steps = c->steps;
   n->profiling = c->profiling;
