@@ -1059,13 +1059,18 @@ __device__ __forceinline__ void load_input(const DevParams& p, const StageArgs& 
     v[k] = src == 0 ? a.S64[o + k] : (src == 1 ? a.U[o + k] : a.U[o + k] + (double)a.S32[o + k]);
 }
 
-// the stage input over the window [ci0, ci0 + w) x [rj0, ...) held in LDS
+constexpr int kNumPrim = kCapy + 1;  // point functions per cell
+
+// the stage input over the window [ci0, ci0 + w) x [rj0, ...) held in LDS, and
+// the point functions of its cells (tab[pk * stride + cell])
 struct View {
   const DevParams* p;
   const double* win;
-  int ci0, rj0, w;
+  const double* tab;
+  int ci0, rj0, w, stride;
+  __device__ int at(int i, int j) const { return (j - rj0) * w + (i - ci0); }
   __device__ void cell(int i, int j, double v[4]) const {
-    const double* c = win + ((j - rj0) * w + (i - ci0)) * 4;
+    const double* c = win + at(i, j) * 4;
     for (int k = 0; k < 4; ++k) v[k] = c[k];
   }
 };
@@ -1079,9 +1084,7 @@ __device__ double pressure_of(const DevParams& p, const double v[4]) {
 }
 
 // the point functions the fluxes take (dim2d_prim_module.f:99-1150)
-__device__ double prim(const View& f, int i, int j, int pk) {
-  double v[4];
-  f.cell(i, j, v);
+__device__ double point_function(const DevParams& p, const double v[4], int pk) {
   const double r = v[0];
   switch (pk) {
     case kRho: return r;
@@ -1089,7 +1092,7 @@ __device__ double prim(const View& f, int i, int j, int pk) {
     case kQy: return v[2];
     case kUx: return v[1] / r;
     case kUy: return v[2] / r;
-    case kP: return pressure_of(*f.p, v);
+    case kP: return pressure_of(p, v);
     case kTcl: return 1.0 / r * (v[3] - 0.5 * r * ((v[1] / r) * (v[1] / r) + (v[2] / r) * (v[2] / r)) + 3.0 * r * r);
     case kQxTx: return v[1] * v[1] / r;
     case kQyTx: return v[2] * v[1] / r;
@@ -1097,12 +1100,17 @@ __device__ double prim(const View& f, int i, int j, int pk) {
     case kQyTy: return v[2] * v[2] / r;
     case kEUx: return v[3] * v[1] / r;
     case kEUy: return v[3] * v[2] / r;
-    case kPxw: return pressure_of(*f.p, v) * v[1] / r;
-    case kPyw: return pressure_of(*f.p, v) * v[2] / r;
+    case kPxw: return pressure_of(p, v) * v[1] / r;
+    case kPyw: return pressure_of(p, v) * v[2] / r;
     case kCap: return 1.0 / (3.0 - r);
     case kCapx: return v[1] / (r * (3.0 - r));
     default: return v[2] / (r * (3.0 - r));
   }
+}
+// point function pk at (i, j), from the window's table (each one computed
+// once per cell, where the stencils read it ~250 times per frame cell)
+__device__ __forceinline__ double prim(const View& f, int i, int j, int pk) {
+  return f.tab[pk * f.stride + f.at(i, j)];
 }
 
 // gradients of a point function (sd_operators_fd_module.f:71-450)
@@ -1383,6 +1391,7 @@ constexpr int kFrameWin = (kFrameSeg + 4) * 3;  // cells per window
 template <int MIXED, int STAGE>
 __global__ void __launch_bounds__(2 * kFrameSeg) frame_kernel(const DevParams p, const StageArgs a) {
   __shared__ double win[kFrameWin * 4];
+  __shared__ double tab[frame::kNumPrim * kFrameWin];
   constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
   const int nxi = p.nx - 2 * kBcSize, nyi = p.ny - 2 * kBcSize;
   const int segx = (nxi + kFrameSeg - 1) / kFrameSeg, segy = (nyi + kFrameSeg - 1) / kFrameSeg;
@@ -1436,9 +1445,16 @@ __global__ void __launch_bounds__(2 * kFrameSeg) frame_kernel(const DevParams p,
     }
   }
   __syncthreads();
+  // the point functions of every window cell (cells outside the array hold
+  // unused values)
+  const int ncell = corners ? 36 : w * h;
+  for (int t = tid; t < ncell; t += blockDim.x)
+    for (int pk = 0; pk < frame::kNumPrim; ++pk)
+      tab[pk * kFrameWin + t] = frame::point_function(p, win + t * 4, pk);
+  __syncthreads();
   if (!active) return;
-  const double* mywin = corners ? win + ((tid >> 2) & 3) * 9 * 4 : win;
-  const frame::View f{&p, mywin, ci0, rj0, w};
+  const int wo = corners ? ((tid >> 2) & 3) * 9 : 0;  // first cell of this thread's window
+  const frame::View f{&p, win + wo * 4, tab + wo, ci0, rj0, w, kFrameWin};
   double L[4];
   frame::timedev(f, i, j, L);
   const long long o = ((long long)j * p.pitch + i) * 4;
