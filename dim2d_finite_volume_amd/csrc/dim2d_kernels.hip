@@ -800,19 +800,32 @@ struct WallRoot {
   double ys0, ys1;     // md_y: transverse neighbours of the first interior cell
   double ux, uy, Ed;
   double dn, dt;       // normal and transverse spacings
+  // the parts of the root function that do not depend on the unknown, set
+  // by wall_root_prepare: 1/dn, the transverse gradient squared, the
+  // temperature terms without the normal gradient and the wall heat flux term
+  double idn, gt2, Ta, Tb, Tq;
 };
 
-__device__ double wall_root_f(const DevParams& p, const WallRoot& w, double x) {
-  // wall_x_equilibrium_root_fct(md = first, md_x = [far, x]) (:2151-2236)
+__device__ void wall_root_prepare(const DevParams& p, WallRoot& w) {
   const double md = w.first;
-  const double dmddx_half = (-md + x) / w.dn;
-  const double md_av = 0.5 * (md + x);
-  const double gn = (x - w.far) / (2.0 * w.dn);
   const double gt = (w.ys1 - w.ys0) / (2.0 * w.dt);
-  const double g2 = gn * gn + gt * gt;
-  const double T1 = p.c38cv * (w.Ed / md - 0.5 * (w.ux * w.ux + w.uy * w.uy) -
-                               0.5 / p.we * g2 / md + 3.0 * md);
-  const double Tav = T1 + 0.5 * w.dn * p.re * p.pr * p.c38cv * p.wall_q;
+  w.idn = 1.0 / w.dn;
+  w.gt2 = gt * gt;
+  w.Ta = w.Ed / md - 0.5 * (w.ux * w.ux + w.uy * w.uy) + 3.0 * md;
+  w.Tb = 0.5 / p.we / md;
+  w.Tq = 0.5 * w.dn * p.re * p.pr * p.c38cv * p.wall_q;
+}
+
+__device__ double wall_root_f(const DevParams& p, const WallRoot& w, double x) {
+  // wall_x_equilibrium_root_fct(md = first, md_x = [far, x]) (:2151-2236),
+  // its terms regrouped around the unknown x (wall_root_prepare)
+  const double md = w.first;
+  const double dmddx_half = (-md + x) * w.idn;
+  const double md_av = 0.5 * (md + x);
+  const double gn = (x - w.far) * (0.5 * w.idn);
+  const double g2 = gn * gn + w.gt2;
+  const double T1 = p.c38cv * (w.Ta - w.Tb * g2);
+  const double Tav = T1 + w.Tq;
   // dwallInternalEnergy_dmd (:2238-2293)
   const double sq = sqrt(1.0 - Tav);
   const double ml = 1.0 + 2.06 * sq;
@@ -950,6 +963,7 @@ __device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const 
   }
   w.ys0 = vy0[0];
   w.ys1 = vy1[0];
+  wall_root_prepare(p, w);
   // compute_wall_xy_guess from u^n (bc_operators_wall_xy_class.f:898-930)
   const double T_guess = p.c38cv * temperature_eff_at(p, tmp, c1, r1);
   const double md_guess = tmp[((long long)r * p.pitch + c) * 4];
