@@ -352,6 +352,28 @@ __device__ __forceinline__ double sound_speed(const DevParams& p, const double v
   *speed = sqrt(u * u + vv * vv);
   return c2 > 0.0 ? sqrt(c2) : 0.0;
 }
+// The same sound speed for the time step of mixed-precision runs: 1/rho, the
+// velocities and the temperature recovered from the total energy (a sum of
+// terms that nearly cancel) in FP64; the EOS terms, the reciprocal of 3 - rho
+// and both square roots in FP32.  The time step then differs from the FP64
+// rule by a few FP32 roundings (~1e-7 relative), far below what the FP32
+// stage increments already carry.
+__device__ __forceinline__ float sound_speed_f32(const DevParams& p, const double v[4],
+                                                 float* speed) {
+  const double rho = v[0];
+  const double ir = fast_rcp(rho);
+  const double u = v[1] * ir, vv = v[2] * ir;
+  const double rho2 = rho * rho;
+  const double Tn = v[3] - 0.5 * rho * (u * u + vv * vv) + 3.0 * rho2;
+  const float rf = (float)rho, r2 = (float)rho2, uf = (float)u, vf = (float)vv;
+  const float w = __builtin_amdgcn_rcpf(3.0f - rf);
+  const float c3cv = (float)p.c3cv;
+  const float P = c3cv * w * (float)Tn - 3.0f * r2;
+  const float e = P + r2 * (2.0f * rf - 3.0f);
+  const float c2 = 3.0f * (e + (P + 3.0f * r2) * (c3cv / 3.0f)) * w * (float)ir;
+  *speed = __builtin_amdgcn_sqrtf(uf * uf + vf * vf);
+  return c2 > 0.0f ? __builtin_amdgcn_sqrtf(c2) : 0.0f;
+}
 
 // Reductions over the 64 lanes of a wave (all lanes active), result in every
 // lane.  Within each row of 16 lanes by DPP moves (VALU, no LDS round trip):
@@ -418,6 +440,8 @@ struct RedAcc {
     rmax = 0.0;
     nonfinite = 0;
   }
+  // F32: the sound speed of sound_speed_f32 (mixed-precision stages)
+  template <bool F32 = false>
   __device__ void add(const DevParams& p, const double v[4]) {
     bool bad = false;
 #pragma unroll
@@ -425,9 +449,16 @@ struct RedAcc {
       if (SUMS) s[k] += v[k];
       bad |= !isfinite(v[k]);
     }
-    double sp;
-    const double c = sound_speed(p, v, &sp);
-    const double a = sp + c;
+    double a;
+    if (F32) {
+      float sp;
+      const float c = sound_speed_f32(p, v, &sp);
+      a = (double)(sp + c);
+    } else {
+      double sp;
+      const double c = sound_speed(p, v, &sp);
+      a = sp + c;
+    }
     bad |= !isfinite(a);
     nonfinite += bad ? 1u : 0u;
     if (!bad) {
@@ -704,7 +735,7 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
           vout[k] = S[k] + ((2.0 / 3.0) * (dt * (double)L[k]) - (1.0 / 3.0) * (double)inc[K][k]);
         st_cell(a.O64 + row, uc, vout);
       }
-      if (REDUCE) red.add(p, vout);
+      if (REDUCE) red.template add<MIXED != 0>(p, vout);
     }
   };
 

@@ -128,6 +128,8 @@ int dim2d_rk3_step(dim2d_ctx* ctx, double t, double dt);
  * the graph beside the stage kernels).
  * cfl > 0: dt of each step from the CFL rule below, evaluated on the device
  *          (fused into the third stage of the previous step); dt_fixed ignored.
+ *          In DIM2D_PREC_MIXED that evaluation takes c and |u| in FP32 after
+ *          an FP64 temperature, so its dt is within ~1e-7 relative of the rule.
  * cfl <= 0: every step uses dt_fixed.
  * Replaces the time loop of sim_dim2d.f:81-92 for the steps between outputs. */
 int dim2d_integrate(dim2d_ctx* ctx, double t0, double dt_fixed, double cfl, int32_t nsteps);

@@ -181,6 +181,31 @@ def test_cfl_integrate(orc):
     assert max(err) < 1e-10, err
 
 
+@pytest.mark.parametrize("case", ["periodic_x_wall_y", "open"])
+def test_cfl_integrate_mixed(orc, case):
+    """Mixed precision with the device CFL time step (its sound speed in FP32
+    after an FP64 temperature) against the FP64 oracle stepping with the FP64
+    rule: elapsed time and last dt within 1e-6 relative; density and energy
+    within 1e-6 of the oracle."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], precision=1)
+    f = d.Field(p, nodes.copy())
+    ref = nodes.copy()
+    dts = []
+    for _ in range(12):
+        dt = orc.cfl_dt(m, ref, *p.spacing(), 0.1)[0]
+        dts.append(dt)
+        assert orc.run(m, ref, xm, ym, np.array([dt])) == 0
+    f.run(12, cfl=0.1)
+    got = f.download()
+    t, last_dt = f.device_time()
+    print(case, "mixed cfl: t", t / sum(dts) - 1, "last dt", last_dt / dts[-1] - 1)
+    assert abs(t - sum(dts)) < 1e-6 * sum(dts)
+    assert abs(last_dt - dts[-1]) < 1e-6 * dts[-1]
+    err = max_rel_err(got, ref)
+    print(case, "mixed cfl run rel err", err)
+    assert err[0] < 1e-6 and err[3] < 1e-6, err
+
+
 @pytest.mark.parametrize("case", ["reflection", "periodic_x_wall_y", "wall_s_reflection", "open"])
 def test_lsrk3_steps_fp64(orc, case):
     """Low-storage RK3 (2N-storage, Williamson 1980): 20 fixed-dt steps against
