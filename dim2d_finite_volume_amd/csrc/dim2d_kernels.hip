@@ -969,6 +969,16 @@ __device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const 
     if (e == 2) { c = kBcSize - 1; c1 = c + 1; c2 = c + 2; co = c - 1; }
     else { c = p.nx - kBcSize; c1 = c - 1; c2 = c - 2; co = c + 1; }
   }
+  const long long ig = ((long long)r * p.pitch + c) * 4, io = ((long long)ro * p.pitch + co) * 4;
+  // u^n at the two ghost cells, for the increments of a mixed-precision
+  // output: loaded with the stencil values, not after the root
+  double ug[4] = {0.0, 0.0, 0.0, 0.0}, uo[4] = {0.0, 0.0, 0.0, 0.0};
+  if (O32) {
+    for (int k = 0; k < 4; ++k) {
+      ug[k] = U[ig + k];
+      uo[k] = U[io + k];
+    }
+  }
   double v1[4], v2[4];
   wall_load(p, U, S64, S32, mixed_in, c1, r1, v1);
   wall_load(p, U, S64, S32, mixed_in, c2, r2, v2);
@@ -1016,11 +1026,10 @@ __device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const 
   go[1] = g[1];
   go[2] = g[2];
   go[3] = v1[3];
-  const long long ig = ((long long)r * p.pitch + c) * 4, io = ((long long)ro * p.pitch + co) * 4;
   for (int k = 0; k < 4; ++k) {
     if (O32) {
-      O32[ig + k] = (float)(g[k] - U[ig + k]);
-      O32[io + k] = (float)(go[k] - U[io + k]);
+      O32[ig + k] = (float)(g[k] - ug[k]);
+      O32[io + k] = (float)(go[k] - uo[k]);
     } else {
       O64[ig + k] = g[k];
       O64[io + k] = go[k];
