@@ -763,12 +763,18 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
 #ifndef DIM2D_WAVES_MIXED
 #define DIM2D_WAVES_MIXED 2
 #endif
+// A workgroup holds one strip (64 lanes) or two neighbouring strips (128
+// lanes): the two waves of a pair run on one CU, so the 128-byte line that
+// holds both strips' shared halo columns is fetched from L2 once.  Waves have
+// no barrier between them; a wave past the last strip exits at once.
 template <int MIXED, int STAGE>
-__global__ void __launch_bounds__(kLanes)
+__global__ void __launch_bounds__(2 * kLanes)
 __attribute__((amdgpu_waves_per_eu((MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64) -
                                        (STAGE == 3 || STAGE == 6 ? 1 : 0),
                                    8)))
 stage_kernel(const DevParams p, const StageArgs a, int rows) {
+  const int bx = blockIdx.x * (int)(blockDim.x / kLanes) + (int)(threadIdx.x / kLanes);
+  if (bx * kOutCols >= p.nx - 2 * kBcSize) return;
   // density and its differences stay FP64 in both modes: the capillary
   // second differences of O(1) densities are scaled by 1/dx^2, so FP32
   // densities (absolute rounding ~1e-7) would leave errors of order 1e-7/dx^2
@@ -788,7 +794,7 @@ stage_kernel(const DevParams p, const StageArgs a, int rows) {
   }
   RedAcc<false> red;
   red.init();
-  stage_tile<RD, R, MIXED, STAGE, LAST>(p, a, dt, blockIdx.x, blockIdx.y, rows, red);
+  stage_tile<RD, R, MIXED, STAGE, LAST>(p, a, dt, bx, blockIdx.y, rows, red);
   if (LAST) {
     red.flush(a.ctl, a.parity ^ 1);
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.ctl->t += dt;
@@ -2034,32 +2040,34 @@ __global__ void extend_kernel(const DevParams np, const DevParams op, int add_w,
 // launchers
 // ------------------------------------------------------------------------
 template <int MIXED, int STAGE>
-static hipError_t launch_one(const DevParams& p, const StageArgs& a, int rows, hipStream_t s) {
+static hipError_t launch_one(const DevParams& p, const StageArgs& a, int block_x, int rows,
+                             hipStream_t s) {
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
-  dim3 grid((ni + kOutCols - 1) / kOutCols, (nj + rows - 1) / rows);
-  hipLaunchKernelGGL((stage_kernel<MIXED, STAGE>), grid, dim3(kLanes), 0, s, p, a, rows);
+  const int per_wg = block_x == 2 * kLanes ? 2 : 1;  // strips per workgroup
+  const int strips = (ni + kOutCols - 1) / kOutCols;
+  dim3 grid((strips + per_wg - 1) / per_wg, (nj + rows - 1) / rows);
+  hipLaunchKernelGGL((stage_kernel<MIXED, STAGE>), grid, dim3(per_wg * kLanes), 0, s, p, a, rows);
   return hipGetLastError();
 }
 
 hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArgs& a, int block_x,
                         int rows, hipStream_t s) {
-  (void)block_x;
   if (!mixed) {
     switch (stage) {
-      case 0: return launch_one<0, 0>(p, a, rows, s);
-      case 1: return launch_one<0, 1>(p, a, rows, s);
-      case 2: return launch_one<0, 2>(p, a, rows, s);
-      case 3: return launch_one<0, 3>(p, a, rows, s);
-      case 4: return launch_one<0, 4>(p, a, rows, s);
-      case 5: return launch_one<0, 5>(p, a, rows, s);
-      default: return launch_one<0, 6>(p, a, rows, s);
+      case 0: return launch_one<0, 0>(p, a, block_x, rows, s);
+      case 1: return launch_one<0, 1>(p, a, block_x, rows, s);
+      case 2: return launch_one<0, 2>(p, a, block_x, rows, s);
+      case 3: return launch_one<0, 3>(p, a, block_x, rows, s);
+      case 4: return launch_one<0, 4>(p, a, block_x, rows, s);
+      case 5: return launch_one<0, 5>(p, a, block_x, rows, s);
+      default: return launch_one<0, 6>(p, a, block_x, rows, s);
     }
   }
   switch (stage) {
-    case 0: return launch_one<1, 0>(p, a, rows, s);
-    case 1: return launch_one<1, 1>(p, a, rows, s);
-    case 2: return launch_one<1, 2>(p, a, rows, s);
-    default: return launch_one<1, 3>(p, a, rows, s);
+    case 0: return launch_one<1, 0>(p, a, block_x, rows, s);
+    case 1: return launch_one<1, 1>(p, a, block_x, rows, s);
+    case 2: return launch_one<1, 2>(p, a, block_x, rows, s);
+    default: return launch_one<1, 3>(p, a, block_x, rows, s);
   }
 }
 

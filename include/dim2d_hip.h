@@ -167,11 +167,12 @@ int dim2d_time(dim2d_ctx* ctx, double out[2]);
 int dim2d_synchronize(dim2d_ctx* ctx);
 
 /* Kernel tile selection: lanes per workgroup along x (64: one wave per strip
- * of 60 output columns) and rows per workgroup along y; 0 keeps the current
- * value.  The default height depends on the grid: 16 rows from about 1500^2
- * interior points up, fewer on smaller grids (8, 4, 2), which would leave
- * most of the GPU idle with 16-row tiles.  Results do not depend on the
- * tiling. */
+ * of 60 output columns; 128: two neighbouring strips per workgroup) and rows
+ * per workgroup along y; 0 keeps the current value.  The defaults depend on
+ * the grid: 16 rows from about 1500^2 interior points up, fewer on smaller
+ * grids (8, 4, 2), which would leave most of the GPU idle with 16-row tiles;
+ * two strips per workgroup on large grids only.  Results do not depend on
+ * the tiling. */
 int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
 
 /* Kernel timing with HIP events recorded on the context's stream around the

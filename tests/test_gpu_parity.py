@@ -94,7 +94,7 @@ def test_time_dev_mixed(orc, case):
 
 
 @pytest.mark.parametrize("case", list(BC_CASES))
-@pytest.mark.parametrize("tiles", [(64, 16), (64, 7)])
+@pytest.mark.parametrize("tiles", [(64, 16), (64, 7), (128, 9)])
 def test_steps_fp64(orc, case, tiles):
     d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case])
     dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
@@ -470,15 +470,36 @@ def test_graph_replay_matches_eager(orc, case, prec):
     assert np.array_equal(g.device_time(), e.device_time())
 
 
-def test_tile_invariance(orc):
-    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["periodic"], nx=131, ny=70)
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tile_invariance(orc, precision):
+    """Results are bitwise independent of the tile height and of one or two
+    column strips per workgroup (131 columns: 3 strips, so the second
+    workgroup of a pair holds one strip and one wave that exits at once)."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["periodic"], nx=131, ny=70,
+                                        precision=precision)
     outs = []
-    for tiles in [(64, 64), (64, 5), (64, 33)]:
+    for tiles in [(64, 64), (64, 5), (64, 33), (128, 16), (128, 5), (128, 70)]:
         f = d.Field(p, nodes.copy())
         f.set_tiles(*tiles)
         f.run(3, dt=1e-6)
         outs.append(f.download())
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+        f.close()
+    for k, o in enumerate(outs[1:]):
+        assert np.array_equal(outs[0], o), k
+
+
+def test_set_tiles_rejects_other_widths():
+    """block_x is 64 or 128 (one or two strips per workgroup); 0 keeps it."""
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    d = _lib()
+    p, nodes = ic.drop64()
+    f = d.Field(p, nodes)
+    for bad in (32, 96, 192, 256):
+        with pytest.raises(RuntimeError):
+            f.set_tiles(bad, 0)
+    f.set_tiles(128, 0)
+    f.set_tiles(0, 8)
+    f.close()
 
 
 def test_smallest_grid(orc):
