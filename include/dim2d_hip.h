@@ -253,8 +253,10 @@ int dim2d_batch_upload(dim2d_batch* b, int32_t sim, const double* nodes);
 int dim2d_batch_download(dim2d_batch* b, int32_t sim, double* nodes);
 /* nsteps RK3 steps of every simulation, each with its own fixed dt (dts[n]). */
 int dim2d_batch_integrate(dim2d_batch* b, const double* dts, int32_t nsteps);
-/* Tile selection for the batch (same meaning as dim2d_set_tiles) and the
- * number of persistent workgroups (0: one per work item, capped by residency). */
+/* Tile selection for the batch: block_x 64 (one strip per workgroup; the
+ * persistent kernel does not pair strips) or 0, rows per tile as in
+ * dim2d_set_tiles, and the number of persistent workgroups (0: one per work
+ * item, capped by residency). */
 int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows_per_block,
                           int32_t workgroups);
 int dim2d_batch_synchronize(dim2d_batch* b);
