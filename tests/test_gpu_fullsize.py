@@ -9,7 +9,8 @@ result on the full grid must then equal the oracle's on the window's inner
 cells.  Errors are max |gpu - oracle| over those cells divided by the largest
 |value| of the variable over the whole GPU field: 1e-11 for time
 derivatives, 1e-10 for a step.  Plus conservation over 50 steps, which no
-window can show.
+window can show, and the mixed-precision criterion (density and energy within
+1e-6 of FP64) at both sizes.
 
 Away from the bubbles these states are uniform liquid in a uniform flow,
 whose derivatives vanish; a smooth perturbation of amplitude 1e-3 (relative
@@ -227,3 +228,46 @@ def test_batch_config_matches_single(prec):
         assert np.isfinite(got).all(), i
         m0 = n[0, 2:-2, 2:-2].sum()
         assert abs(got[0].sum() - m0) < 1e-12 * m0, i
+
+
+def _mixed_against_fp64(p, nodes, steps, dt):
+    """Density and energy: max |mixed - FP64| / max |FP64| over the interior
+    after `steps` steps of both precisions with the same fixed dt."""
+    import dataclasses
+    import dim2d_finite_volume_amd as d
+    out = []
+    for prec in (0, 1):
+        f = d.Field(dataclasses.replace(p, precision=prec), nodes)
+        f.run(steps, dt=dt)
+        out.append(f.download())
+        f.close()
+    ref, got = out
+    err = [float(np.abs(got[k, 2:-2, 2:-2] - ref[k, 2:-2, 2:-2]).max() /
+                 np.abs(ref[k, 2:-2, 2:-2]).max()) for k in (0, 3)]
+    mass = [got[0, 2:-2, 2:-2].sum(), ref[0, 2:-2, 2:-2].sum()]
+    return err, mass
+
+
+def test_mixed_against_fp64_4096(two_bubbles):
+    """The mixed-precision criterion at the measured size: 200 steps of the
+    4096^2 two-bubble case in mixed precision and in FP64 (same dt); density
+    and total energy within 1e-6 relative of FP64, and the mass sums equal to
+    1e-12."""
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    d, p, nodes, xm, ym = two_bubbles
+    dt = ic.cfl_dt(p, nodes, 0.1)
+    (erho, eE), (m_mixed, m_fp64) = _mixed_against_fp64(p, nodes, 200, dt)
+    print("4096^2 mixed vs FP64 after 200 steps: rho", erho, "E", eE)
+    assert erho < 1e-6 and eE < 1e-6, (erho, eE)
+    assert abs(m_mixed - m_fp64) < 1e-12 * abs(m_fp64)
+
+
+def test_mixed_against_fp64_8192_open():
+    """The same criterion on the 8192^2 bubble with Hedstrom open edges
+    (50 steps with the CFL-0.1 dt of the initial state, as the measured run)."""
+    from dim2d_finite_volume_amd import capi
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = ic.open_bubble(n=8192, precision=capi.PREC_FP64)
+    (erho, eE), _ = _mixed_against_fp64(p, nodes, 50, ic.cfl_dt(p, nodes, 0.1))
+    print("8192^2 open mixed vs FP64 after 50 steps: rho", erho, "E", eE)
+    assert erho < 1e-6 and eE < 1e-6, (erho, eE)
