@@ -48,26 +48,30 @@ def rel_errors(a, b):
     return out
 
 
-def timed_run(field, steps, warmup, cfl, dt=0.0):
-    """`warmup` untimed steps, then `steps` timed steps.
+def clock_warmup(field, cfl, dt=0.0, seconds=1.0, chunk=50):
+    """Untimed steps until `seconds` of GPU work have run, so the timed steps
+    start at the clocks a sustained run reaches (a short run from an idle GPU
+    otherwise times the clock ramp).  Returns (steps, seconds)."""
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        field.run(chunk, dt=dt, cfl=cfl)
+        field.synchronize()
+        n += chunk
+    return n, time.perf_counter() - t0
 
-    The last steps of the warmup (up to 50) record HIP events around every
-    kernel launch: they give every kernel's mean duration and name the
-    dominant stage kernel (largest total time). The timed steps record events
-    around that kernel only, on the context's stream: events around all four
-    kernel kinds add about 5% to a step, around one kind well under 1%.
-    Returns (elapsed s, dominant kind, its (mean ms, launches) over the timed
-    steps, per-kind means from the warmup)."""
+
+def timed_run(field, steps, warmup, cfl, dt=0.0, clock_s=1.0):
+    """The clock warm-up, `warmup` untimed steps, then `steps` timed steps on
+    the production path: profiling off, so dim2d_integrate replays pairs of
+    steps from captured HIP graphs.  Afterwards a separate window of
+    max(steps, 20) eager steps records HIP events on the context's stream
+    around every kernel launch, for each kernel's mean duration (events make
+    the launches eager and add a few microseconds per launch, so they stay out
+    of the timed window).  Returns (elapsed s, dominant kind, its (mean ms,
+    launches), per-kind means, clock warm-up (steps, s))."""
     import torch
-    probe = min(50, warmup // 2) if warmup >= 2 else 0
-    field.run(warmup - probe, dt=dt, cfl=cfl)
-    field.profile(True)
-    field.run(probe, dt=dt, cfl=cfl)
-    probe_kt = field.kernel_times()
-    stages = ("stage1", "stage2", "stage3")
-    dom = (max(stages, key=lambda k: probe_kt[k][0] * probe_kt[k][1])
-           if probe else "stage3")
-    field.profile(True, kinds=[dom])
+    clock = clock_warmup(field, cfl, dt, clock_s)
+    field.run(warmup, dt=dt, cfl=cfl)
     torch.cuda.synchronize()
     field.synchronize()
     t0 = time.perf_counter()
@@ -75,10 +79,14 @@ def timed_run(field, steps, warmup, cfl, dt=0.0):
     field.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    field.profile(True)
+    field.run(max(steps, 20), dt=dt, cfl=cfl)
     kt = field.kernel_times()
     field.profile(False)
+    stages = ("stage1", "stage2", "stage3")
+    dom = max(stages, key=lambda k: kt[k][0] * kt[k][1])
     field.time = field.device_time()[0]
-    return el, dom, kt[dom], probe_kt
+    return el, dom, kt[dom], kt, clock
 
 
 def cpu_baseline(seconds_hint=15.0):
@@ -243,6 +251,8 @@ def main():
                     help="steps of the open-boundary case (0: skip)")
     ap.add_argument("--variant-steps", type=int, default=200,
                     help="steps of the LSRK3 / Cockburn-Gau variants (0: skip)")
+    ap.add_argument("--clock-warmup-s", type=float, default=1.0,
+                    help="seconds of untimed steps before the warmup steps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run an elementwise kernel reading and writing 1 GiB "
                          "(for HBM counter scaling)")
@@ -281,14 +291,16 @@ def main():
         f.set_tiles(args.block_x, args.rows)
     f.apply_bc_on_nodes()
     diag0 = f.diagnostics()
-    log(f"[bench] FP64: {args.warmup} warmup + {args.steps} timed steps")
-    el, dom, dom_kt, kt = timed_run(f, args.steps, args.warmup, cfl=0.1)
+    log(f"[bench] FP64: {args.clock_warmup_s} s clock warm-up, {args.warmup} warmup + "
+        f"{args.steps} timed steps")
+    el, dom, dom_kt, kt, clock = timed_run(f, args.steps, args.warmup, cfl=0.1,
+                                           clock_s=args.clock_warmup_s)
     diag1 = f.diagnostics()
     t_end, dt_last = f.device_time()
     value = cells * args.steps / el
     ms_step = 1e3 * el / args.steps
     log(f"[bench] FP64 {value:.4e} updates/s, {ms_step:.4f} ms/step, dominant {dom} "
-        f"{dom_kt}, warmup kernel means {kt}")
+        f"{dom_kt}, profiled-window kernel means {kt}")
     si = ("stage1", "stage2", "stage3").index(dom)
     alg = ALG_BYTES["f64"][si] * cells
     achieved = alg / (dom_kt[0] * 1e-3)
@@ -297,7 +309,9 @@ def main():
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
             "traffic": (read_traffic(f"stage{si + 1}_fp64_two_bubbles_{n}_traffic.json")),
             "algorithmic_bytes_per_launch": alg, "mean_launch_ms": dom_kt[0],
-            "timed_launches": dom_kt[1]}
+            "profiled_launches": dom_kt[1],
+            "timing": "HIP events on the context's stream around every launch of a "
+                      "window of eager steps run right after the timed (graph-replay) steps"}
     f.close()
     del f
 
@@ -310,36 +324,42 @@ def main():
             fm.set_tiles(args.block_x, args.rows)
         fm.apply_bc_on_nodes()
         log(f"[bench] mixed: {args.warmup} warmup + {args.steps} timed steps")
-        elm, domm, domm_kt, ktm = timed_run(fm, args.steps, args.warmup, cfl=0.1)
+        elm, domm, domm_kt, ktm, clockm = timed_run(fm, args.steps, args.warmup, cfl=0.1,
+                                                    clock_s=args.clock_warmup_s)
         vm = cells * args.steps / elm
         fm.close()
         del fm
-        # error against FP64 with a common fixed dt, at step 100 and at `steps`
+        # error against FP64 with a common fixed dt (CFL 0.1 of the
+        # initial state) at steps 100 and 1000, whatever --steps is
         dt0 = ic.cfl_dt(p, nodes, 0.1)
-        errs = {}
-        for nst in sorted({min(100, args.steps), args.steps}):
-            a = d.Field(p, nodes)
+        marks = (100, 1000)
+        outs = {}
+        for prec, pp in (("f64", p), ("mixed", pm)):
+            a = d.Field(pp, nodes)
             a.apply_bc_on_nodes()
-            a.run(nst, dt=dt0)
-            ra = a.download()
-            da = a.diagnostics()
+            done = 0
+            for nst in marks:
+                a.run(nst - done, dt=dt0)
+                done = nst
+                outs[(prec, nst)] = (a.download(), a.diagnostics())
             a.close()
-            b = d.Field(pm, nodes)
-            b.apply_bc_on_nodes()
-            b.run(nst, dt=dt0)
-            rb = b.download()
-            db = b.diagnostics()
-            b.close()
+        errs = {}
+        for nst in marks:
+            (ra, da), (rb, db) = outs[("f64", nst)], outs[("mixed", nst)]
             e = rel_errors(rb, ra)
             e["mass_drift_mixed"] = float((db[0] - da[0]) / da[0])
             errs[f"step_{nst}"] = e
             log(f"[bench] mixed vs FP64 after {nst} steps: {e}")
+        del outs
         mixed = {"value": vm, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * elm / args.steps,
                  "speedup_vs_fp64": vm / value,
                  "hbm_frac_algorithmic": vm * sum(ALG_BYTES["mixed"]) / HBM_PEAK,
-                 "kernel_ms_warmup": {k: ktm[k][0] for k in ktm},
+                 "kernel_ms": {k: ktm[k][0] for k in ktm},
+                 "clock_warmup": {"steps": clockm[0], "seconds": clockm[1]},
                  "dominant_kernel_frac": None,
-                 "error_vs_fp64": errs}
+                 "error_vs_fp64": errs,
+                 "error_config": f"same initial state, fixed dt={dt0:.6e} (CFL 0.1 of the "
+                                 f"initial state) in both precisions"}
         sim = ("stage1", "stage2", "stage3").index(domm)
         mixed["dominant_kernel"] = f"stage_kernel<1, {sim + 1}> (mixed stage {sim + 1})"
         mixed["dominant_kernel_frac"] = (ALG_BYTES["mixed"][sim] * cells /
@@ -360,7 +380,11 @@ def main():
                    "grid": [n + 4, n + 4], "physics": "Re=5 We=10 Pr=20 viscous_r=-1.5 cv_r=2.5",
                    "device": dev},
         "hbm_frac_algorithmic": value * sum(ALG_BYTES["f64"]) / HBM_PEAK,
-        "kernel_ms_warmup": {k: kt[k][0] for k in kt},
+        "kernel_ms": {k: kt[k][0] for k in kt},
+        "clock_warmup": {"steps": clock[0], "seconds": clock[1],
+                         "note": "untimed steps before the warmup steps, until the GPU "
+                                 "clocks reach their sustained level"},
+        "timed_path": "graph replay (dim2d_integrate, profiling off)",
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,

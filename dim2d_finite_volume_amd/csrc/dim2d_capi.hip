@@ -202,6 +202,9 @@ struct dim2d_ctx {
   bool rows_set = false;     // rows chosen by dim2d_set_tiles, else by default_rows()
   bool uploaded = false;
   long long steps = 0;
+  // the reductions of u^n (CFL quantities) sit in slot (steps & 1): written by
+  // the last stage of the step that produced u^n, cleared by an upload
+  bool red_valid = false;
   // kernel timing: event pairs around launches of the kinds in profile_mask
   // (bit k: kind k = stage 1, 2, 3, boundary kernels)
   bool profiling = false;
@@ -429,6 +432,7 @@ static int copy_in(dim2d_ctx* c, const double* src, hipMemcpyKind kind) {
   }
   HIP_TRY(launch_pack(p, planar, c->U, c->stream));
   c->uploaded = true;
+  c->red_valid = false;
   return DIM2D_OK;
 }
 
@@ -488,6 +492,25 @@ int dim2d_apply_bc(dim2d_ctx* c, double t) {
   return DIM2D_OK;
 }
 
+// The time derivative of u^n into U2 (free between steps), in FP64 whatever
+// the context's precision: the host-facing derivative replaces
+// compute_time_dev, and the steady-state check compares it with limits such
+// as 1e-12 (parameters_input.f:913) that FP32 fluxes, ~1e-7 |F|/dx, could
+// never meet.
+static hipError_t launch_time_dev_fp64(dim2d_ctx* c) {
+  const DevParams& p = c->dp;
+  StageArgs a;
+  memset(&a, 0, sizeof a);
+  a.U = c->U;
+  a.S64 = c->U;
+  a.O64 = c->U2;
+  hipError_t e = c->cg ? launch_cg_stage(p, 0, a, c->stream)
+                       : launch_stage(p, 0, 0, a, c->block_x, c->rows, c->stream);
+  // open edges: the ghost frame has a time derivative of its own
+  if (e == hipSuccess && is_open(p)) e = launch_frame(p, 0, 0, a, c->stream);
+  return e;
+}
+
 int dim2d_time_dev(dim2d_ctx* c, double t, double* time_dev) {
   (void)t;
   if (!c || !time_dev) return fail(DIM2D_EINVAL, "NULL argument");
@@ -496,15 +519,7 @@ int dim2d_time_dev(dim2d_ctx* c, double t, double* time_dev) {
   // U2 is free between steps: the derivative goes there, then to planar order
   // with the ghost cells set to debug_real (td_operators_class.f:150-154)
   // unless the edges are open
-  StageArgs a;
-  memset(&a, 0, sizeof a);
-  a.U = c->U;
-  a.S64 = c->U;
-  a.O64 = c->U2;
-  if (c->cg) HIP_TRY(launch_cg_stage(p, 0, a, c->stream));
-  else HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
-  // open edges: the ghost frame has a time derivative of its own
-  if (is_open(p)) HIP_TRY(launch_frame(p, c->mixed, 0, a, c->stream));
+  HIP_TRY(launch_time_dev_fp64(c));
   HIP_TRY(launch_unpack(p, c->U2, c->T, is_open(p) ? 0 : 1, kDebugReal, c->stream));
   HIP_TRY(hipMemcpyAsync(time_dev, c->T, (size_t)p.nx * p.ny * 4 * sizeof(double),
                          hipMemcpyDeviceToHost, c->stream));
@@ -518,14 +533,7 @@ int dim2d_check_steady_state(dim2d_ctx* c, double limit, double out[4], int32_t*
   if (!(limit >= 0.0)) return fail(DIM2D_EINVAL, "limit must be >= 0, got %g", limit);
   const DevParams& p = c->dp;
   // the time derivative into U2 (free between steps), as dim2d_time_dev
-  StageArgs a;
-  memset(&a, 0, sizeof a);
-  a.U = c->U;
-  a.S64 = c->U;
-  a.O64 = c->U2;
-  if (c->cg) HIP_TRY(launch_cg_stage(p, 0, a, c->stream));
-  else HIP_TRY(launch_stage(p, c->mixed, 0, a, c->block_x, c->rows, c->stream));
-  if (is_open(p)) HIP_TRY(launch_frame(p, c->mixed, 0, a, c->stream));
+  HIP_TRY(launch_time_dev_fp64(c));
   unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->T);
   HIP_TRY(hipMemsetAsync(bits, 0, 4 * sizeof(unsigned long long), c->stream));
   HIP_TRY(launch_absmax(p, is_open(p) ? 1 : 0, c->U2, bits, c->stream));
@@ -639,6 +647,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
   }
   std::swap(c->U, c->U2);
   c->steps++;
+  c->red_valid = true;
   return DIM2D_OK;
 }
 
@@ -670,7 +679,12 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
   if (st) return st;
   if (cfl_mode) {
     HIP_TRY(hipMemcpyAsync(&c->ctl->cfl, &cfl, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_reduce(c->dp, c->U, c->ctl, (int)(c->steps & 1), c->stream));
+    // the first step's dt needs the reductions of u^n; the last stage of the
+    // previous step left them in its slot unless the state was uploaded since
+    if (!c->red_valid) {
+      HIP_TRY(launch_reduce(c->dp, c->U, c->ctl, (int)(c->steps & 1), c->stream));
+      c->red_valid = true;
+    }
   }
   // pairs of steps from a captured graph: no host launch gaps inside a pair.
   // Not while profiling (event pairs around launches).  With open edges the
@@ -690,6 +704,7 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
       double* u = c->U;
       double* u2 = c->U2;
       const long long steps0 = c->steps;
+      const bool rv0 = c->red_valid;
       HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       st = run_step(c, dt_fixed, cfl_mode);
       if (!st) st = run_step(c, dt_fixed, cfl_mode);
@@ -698,6 +713,7 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
       c->U = u;
       c->U2 = u2;
       c->steps = steps0;
+      c->red_valid = rv0;
       if (st || e != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         if (st) return st;
@@ -711,6 +727,7 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
     }
     HIP_TRY(hipGraphLaunch(c->gexec[k], c->stream));
     c->steps += 2;  // two buffer swaps: U and U2 are where they were
+    c->red_valid = true;
   }
   for (; s < nsteps; ++s) {
     st = run_step(c, dt_fixed, cfl_mode);
@@ -1153,9 +1170,11 @@ int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, doub
   int st = DIM2D_OK;
   double best = INFINITY;
   int best_rows = b->rows, best_wg = b->wg;
+  bool kept = false;
   auto run = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(keep, b->U, b->total * sizeof(double), hipMemcpyDeviceToDevice,
                            b->stream));
+    kept = true;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     for (int r : rows_c)
@@ -1188,7 +1207,16 @@ int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, doub
     HIP_TRY(hipStreamSynchronize(b->stream));
     return DIM2D_OK;
   };
+  const long long steps0 = b->steps;
   st = run();
+  // the candidates' steps are not the caller's: the state they started from
+  // comes back and the step count with it, also when a candidate failed
+  if (st && kept) {
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipMemcpyAsync(b->U, keep, b->total * sizeof(double), hipMemcpyDeviceToDevice,
+                         b->stream);
+  }
+  b->steps = steps0;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(b->stream);

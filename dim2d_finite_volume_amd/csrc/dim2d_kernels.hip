@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
 #include <type_traits>
 
 namespace dim2d {
@@ -486,6 +487,53 @@ struct RedAcc {
       const unsigned long long pk[3] = {peek(&ctl->amax[slot]), peek(&ctl->rmin[slot]),
                                         peek(&ctl->rmax[slot])};
       commit(ctl, slot, am, rn, rx, nf, pk);
+    }
+  }
+  // The contribution of a whole workgroup of up to 4 waves (all lanes
+  // active): wave reductions, then the waves' results combined through LDS
+  // and committed by one lane, so a launch makes one set of atomics per
+  // workgroup instead of per wave (the conserved sums cannot skip theirs).
+  __device__ void flush_block(StepCtl* ctl, int slot) {
+    __shared__ double sh[4][8];
+    __shared__ unsigned shn[4];
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    if (SUMS) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t[k] = wave_sum(s[k]);
+    }
+    const double am = wave_max(amax);
+    const double rn = wave_min(rmin);
+    const double rx = wave_max(rmax);
+    const unsigned nf = wave_sum_u(nonfinite);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sh[w][k] = t[k];
+      sh[w][4] = am;
+      sh[w][5] = rn;
+      sh[w][6] = rx;
+      shn[w] = nf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = (int)(blockDim.x >> 6);
+      double tt[4] = {sh[0][0], sh[0][1], sh[0][2], sh[0][3]};
+      double bam = sh[0][4], brn = sh[0][5], brx = sh[0][6];
+      unsigned bnf = shn[0];
+      for (int q = 1; q < nw; ++q) {
+        for (int k = 0; k < 4; ++k) tt[k] += sh[q][k];
+        bam = fmax(bam, sh[q][4]);
+        brn = fmin(brn, sh[q][5]);
+        brx = fmax(brx, sh[q][6]);
+        bnf += shn[q];
+      }
+      if (SUMS) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) atomicAdd(&ctl->sum[slot][k], tt[k]);
+      }
+      const unsigned long long pk[3] = {peek(&ctl->amax[slot]), peek(&ctl->rmin[slot]),
+                                        peek(&ctl->rmax[slot])};
+      commit(ctl, slot, bam, brn, brx, bnf, pk);
     }
   }
   // Combined max |u|+c, min/max rho and non-finite count into slot `slot`.
@@ -1590,20 +1638,27 @@ __global__ void fill_ghosts_kernel(const DevParams p, double* F) {
 
 __global__ void clear_slot_kernel(StepCtl* ctl, int slot) { clear_slot(ctl, slot); }
 
-__global__ void __launch_bounds__(256) reduce_kernel(const DevParams p, const double* F,
-                                                     StepCtl* ctl, int slot) {
+// Reductions of the interior of F into slot `slot` (cleared before).  A
+// workgroup of 256 lanes takes segments of 256 consecutive cells of a row
+// (32-bit index arithmetic, uniform per segment) and commits once.
+constexpr int kRedThreads = 256;
+__global__ void __launch_bounds__(kRedThreads) reduce_kernel(const DevParams p, const double* F,
+                                                             StepCtl* ctl, int slot) {
   RedAcc<true> red;
   red.init();
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
-  const long long n = (long long)ni * nj;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = kBcSize + (int)(t % ni), j = kBcSize + (int)(t / ni);
-    double v[4];
-    ld_cell(F + (long long)j * p.pitch * 4, (unsigned)i, v);
-    red.add(p, v);
+  const int segs = (ni + kRedThreads - 1) / kRedThreads;
+  const int total = segs * nj;
+  for (int sg = blockIdx.x; sg < total; sg += gridDim.x) {
+    const int j = kBcSize + sg / segs;
+    const int i = kBcSize + (sg % segs) * kRedThreads + (int)threadIdx.x;
+    if (i < kBcSize + ni) {
+      double v[4];
+      ld_cell(F + (long long)j * p.pitch * 4, (unsigned)i, v);
+      red.add(p, v);
+    }
   }
-  red.flush(ctl, slot);
+  red.flush_block(ctl, slot);
 }
 
 // planar nodes(nx,ny,4) <-> interleaved cells, one thread per cell
@@ -1921,15 +1976,18 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
 // frame too (open edges, where it carries a time derivative); a non-finite
 // value counts as +inf.  out[0..3]: bits of non-negative doubles, combined
 // with atomicMax.
-__global__ void absmax_kernel(const DevParams p, int frame, const double* __restrict__ F,
-                              unsigned long long* __restrict__ out) {
+__global__ void __launch_bounds__(kRedThreads) absmax_kernel(const DevParams p, int frame,
+                                                             const double* __restrict__ F,
+                                                             unsigned long long* __restrict__ out) {
   double m[4] = {0.0, 0.0, 0.0, 0.0};
-  const long long n = (long long)p.nx * p.ny;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % p.nx), j = (int)(t / p.nx);
-    const bool ghost = i < kBcSize || i >= p.nx - kBcSize || j < kBcSize || j >= p.ny - kBcSize;
-    if (ghost && !frame) continue;
+  const int i0 = frame ? 0 : kBcSize, j0 = frame ? 0 : kBcSize;
+  const int ni = p.nx - 2 * i0, nj = p.ny - 2 * j0;
+  const int segs = (ni + kRedThreads - 1) / kRedThreads;
+  const int total = segs * nj;
+  for (int sg = blockIdx.x; sg < total; sg += gridDim.x) {
+    const int j = j0 + sg / segs;
+    const int i = i0 + (sg % segs) * kRedThreads + (int)threadIdx.x;
+    if (i >= i0 + ni) continue;
     const double* c = F + ((long long)j * p.pitch + i) * 4;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1940,8 +1998,8 @@ __global__ void absmax_kernel(const DevParams p, int frame, const double* __rest
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const double w = wave_max(m[k]);
-    if ((threadIdx.x & 63) == 0 && w > 0.0)
-      atomicMax(&out[k], (unsigned long long)__double_as_longlong(w));
+    const unsigned long long b = (unsigned long long)__double_as_longlong(w);
+    if ((threadIdx.x & 63) == 0 && w > 0.0 && b > peek(&out[k])) atomicMax(&out[k], b);
   }
 }
 
@@ -2170,19 +2228,19 @@ hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, in
 hipError_t launch_reduce(const DevParams& p, const double* F, StepCtl* ctl, int slot,
                          hipStream_t s) {
   hipLaunchKernelGGL(clear_slot_kernel, dim3(1), dim3(1), 0, s, ctl, slot);
-  const long long n = (long long)(p.nx - 2 * kBcSize) * (p.ny - 2 * kBcSize);
-  long long blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, F, ctl, slot);
+  const int segs = (p.nx - 2 * kBcSize + kRedThreads - 1) / kRedThreads;
+  const int blocks = std::min(segs * (p.ny - 2 * kBcSize), 2048);
+  hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)blocks), dim3(kRedThreads), 0, s, p, F, ctl,
+                     slot);
   return hipGetLastError();
 }
 
 hipError_t launch_absmax(const DevParams& p, int frame, const double* F, unsigned long long* out,
                          hipStream_t s) {
-  const long long n = (long long)p.nx * p.ny;
-  long long blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, frame, F, out);
+  const int ni = frame ? p.nx : p.nx - 2 * kBcSize, nj = frame ? p.ny : p.ny - 2 * kBcSize;
+  const int blocks = std::min((ni + kRedThreads - 1) / kRedThreads * nj, 2048);
+  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(kRedThreads), 0, s, p, frame, F,
+                     out);
   return hipGetLastError();
 }
 

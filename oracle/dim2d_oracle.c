@@ -1413,7 +1413,9 @@ static void wall_edge(const orc_model* m, double* n, const double* ntmp, int edg
  *  - periodic in x with walls in y (no counterpart in the reference's active
  *    dispatcher): x-periodic fill of the interior rows, the wall rows, then an
  *    x-periodic fill of the ghost rows, so corners are periodic images of the
- *    wall ghost cells. */
+ *    wall ghost cells;
+ *  - walls on W and/or E with reflection or periodic edges in y: the
+ *    transposes of the two cases above. */
 void orc_apply_bc_on_nodes(const orc_model* m, double t, const double* x_map, const double* y_map,
                            const double* nodes_tmp, double* nodes) {
   (void)t;
@@ -1448,6 +1450,33 @@ void orc_apply_bc_on_nodes(const orc_model* m, double t, const double* x_map, co
     else reflect_y(m, nodes, 0, ny - B + 1, B + 1, nx - B);
     periodic_x(m, nodes, 1, B);
     periodic_x(m, nodes, ny - B + 1, ny);
+    return;
+  }
+  if (m->bc_w == ORC_BC_WALL || m->bc_e == ORC_BC_WALL) {
+    /* walls on W and/or E: the W/E sections of the wall class
+     * (bc_operators_wall_xy_class.f:232-330, compute_wall_{W,E}_ghost_cell of
+     * wall_xy_equilibrium_module.f:1211-1437), which no choice of
+     * library_config.py selects; composed as the transpose of the S/N cases */
+    if (m->bc_s == ORC_BC_PERIODIC) {
+      periodic_y(m, nodes, 1 + B, nx - B);
+      if (m->bc_w == ORC_BC_WALL) wall_edge(m, nodes, nodes_tmp, 2, dx, dy);
+      else reflect_x(m, nodes, 1, 1, B + 1, ny - B);
+      if (m->bc_e == ORC_BC_WALL) wall_edge(m, nodes, nodes_tmp, 3, dx, dy);
+      else reflect_x(m, nodes, 0, nx - B + 1, B + 1, ny - B);
+      periodic_y(m, nodes, 1, B);
+      periodic_y(m, nodes, nx - B + 1, nx);
+      return;
+    }
+    reflect_y(m, nodes, 1, 1, B + 1, nx - B);          /* S */
+    reflect_y(m, nodes, 0, ny - B + 1, B + 1, nx - B); /* N */
+    if (m->bc_w == ORC_BC_WALL) wall_edge(m, nodes, nodes_tmp, 2, dx, dy);
+    else reflect_x(m, nodes, 1, 1, B + 1, ny - B);
+    if (m->bc_e == ORC_BC_WALL) wall_edge(m, nodes, nodes_tmp, 3, dx, dy);
+    else reflect_x(m, nodes, 0, nx - B + 1, B + 1, ny - B);
+    reflect_y(m, nodes, 1, 1, 1, B);                   /* SW */
+    reflect_y(m, nodes, 1, 1, nx - B + 1, nx);         /* SE */
+    reflect_y(m, nodes, 0, ny - B + 1, 1, B);          /* NW */
+    reflect_y(m, nodes, 0, ny - B + 1, nx - B + 1, nx); /* NE */
     return;
   }
   /* reflection in x; S and N are wall or reflection; corners reflection_x */

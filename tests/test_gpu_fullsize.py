@@ -230,6 +230,41 @@ def test_batch_config_matches_single(prec):
         assert abs(got[0].sum() - m0) < 1e-12 * m0, i
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_config_against_oracle(orc, prec):
+    """The full-size batch workload against the oracle directly: 3 steps of
+    the 256-simulation batch, then the smallest (64^2) and the largest
+    simulation (509^2) compared whole with the oracle's run from the same
+    state and dt (FP64 1e-10 on every variable; mixed 1e-6 on density and
+    energy)."""
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    sims = ic.batch_configs(precision=prec)
+    dts = [ic.cfl_dt(p, n, 0.1) for p, n in sims]
+    b = d.Batch([p for p, _ in sims])
+    for i, (p, n) in enumerate(sims):
+        b.upload(i, n)
+    b.integrate(dts, 3)
+    sizes = [p.nx for p, _ in sims]
+    order = np.argsort(sizes)
+    for i in (int(order[0]), int(order[-1])):
+        p, n = sims[i]
+        m = _model(orc, p, p.nx, p.ny, p.bc)
+        xm, ym = p.coordinates()
+        ref = n.copy()
+        orc.apply_bc(m, ref, xm, ym)
+        assert orc.run(m, ref, xm, ym, np.full(3, dts[i])) == 0
+        got = b.download(i)
+        err = [float(np.abs(got[k, 2:-2, 2:-2] - ref[k, 2:-2, 2:-2]).max() /
+                     np.abs(ref[k, 2:-2, 2:-2]).max()) for k in range(4)]
+        print("batch simulation", i, f"{p.nx - 4}^2", "vs oracle", err)
+        if prec == 0:
+            assert max(err) < 1e-10, (i, err)
+        else:
+            assert err[0] < 1e-6 and err[3] < 1e-6, (i, err)
+    b.close()
+
+
 def _mixed_against_fp64(p, nodes, steps, dt):
     """Density and energy: max |mixed - FP64| / max |FP64| over the interior
     after `steps` steps of both precisions with the same fixed dt."""
