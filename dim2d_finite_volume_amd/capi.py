@@ -26,7 +26,7 @@ EXPORTS = (
     "dim2d_upload", "dim2d_download", "dim2d_upload_device", "dim2d_download_device",
     "dim2d_time_dev", "dim2d_apply_bc", "dim2d_rk3_step", "dim2d_integrate", "dim2d_cfl_dt",
     "dim2d_diagnostics", "dim2d_time", "dim2d_synchronize", "dim2d_set_tiles",
-    "dim2d_profile_enable", "dim2d_profile_read",
+    "dim2d_set_stage_fusion", "dim2d_profile_enable", "dim2d_profile_read",
     "dim2d_get_params", "dim2d_openbc_undermined", "dim2d_extend", "dim2d_adapt_domain",
     "dim2d_check_steady_state",
     "dim2d_batch_create", "dim2d_batch_destroy", "dim2d_batch_upload", "dim2d_batch_download",
@@ -91,6 +91,7 @@ def load() -> ctypes.CDLL:
         "dim2d_time": (ctypes.c_int, [vp, dp]),
         "dim2d_synchronize": (ctypes.c_int, [vp]),
         "dim2d_set_tiles": (ctypes.c_int, [vp, i32, i32]),
+        "dim2d_set_stage_fusion": (ctypes.c_int, [vp, i32, i32]),
         "dim2d_profile_enable": (ctypes.c_int, [vp, i32]),
         "dim2d_profile_read": (ctypes.c_int, [vp, dp]),
         "dim2d_get_params": (ctypes.c_int, [vp, ctypes.POINTER(Params)]),

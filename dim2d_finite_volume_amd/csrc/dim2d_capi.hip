@@ -198,6 +198,10 @@ struct dim2d_ctx {
   StepCtl* ctl = nullptr;
   int block_x = 64;
   int rows = 16;
+  // RK3-TVD stages 1 and 2 in one launch (stage pair) where the edges allow
+  int fuse = 0;
+  int pair_rows = 32;
+  bool pair_rows_set = false;
   bool block_x_set = false;  // block_x chosen by dim2d_set_tiles, else by default_block_x()
   bool rows_set = false;     // rows chosen by dim2d_set_tiles, else by default_rows()
   bool uploaded = false;
@@ -294,6 +298,15 @@ static int default_rows(const DevParams& p) {
 // (+5 to +13 % at 128^2 to 512^2).
 constexpr long long kPairStripsWaves = 8000;
 static int default_block_x(const DevParams& p) { return waves16(p) >= kPairStripsWaves ? 128 : 64; }
+// Tile height of the stage pair: its producer runs 4 rows past each end of the
+// tile, so tall tiles waste least; small grids take shorter tiles to fill the
+// GPU (workgroups of 32-row tiles: fewer than 500 take 8 rows, fewer than 2000
+// take 16).
+static int default_pair_rows(const DevParams& p) {
+  const long long strips = (p.nx - 2 * kBcSize + 55) / 56, nj = p.ny - 2 * kBcSize;
+  const long long wg32 = strips * ((nj + 31) / 32);
+  return wg32 < 500 ? 8 : wg32 < 2000 ? 16 : 32;
+}
 
 int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (!out) return fail(DIM2D_EINVAL, "out is NULL");
@@ -309,6 +322,8 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   c->cg = params->operators == DIM2D_OPS_COCKBURN_GAU;
   c->rows = default_rows(c->dp);
   c->block_x = default_block_x(c->dp);
+  c->fuse = !c->lsrk && !c->cg && pair_supported(c->dp);
+  c->pair_rows = default_pair_rows(c->dp);
   const size_t n = (size_t)c->dp.ncell * 4;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -389,6 +404,24 @@ int dim2d_set_tiles(dim2d_ctx* c, int32_t block_x, int32_t rows) {
   if (rows) {
     c->rows = rows;
     c->rows_set = true;
+  }
+  drop_graphs(c);
+  return DIM2D_OK;
+}
+
+int dim2d_set_stage_fusion(dim2d_ctx* c, int32_t fuse, int32_t rows_per_block) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (fuse != 0 && fuse != 1) return fail(DIM2D_EINVAL, "fuse must be 0 or 1, got %d", fuse);
+  if (rows_per_block != 0 && (rows_per_block < 4 || rows_per_block > 4096))
+    return fail(DIM2D_EINVAL, "rows_per_block must be 0 or in [4, 4096], got %d", rows_per_block);
+  if (fuse && (c->lsrk || c->cg || !pair_supported(c->dp)))
+    return fail(DIM2D_EINVAL,
+                "stage fusion needs the RK3-TVD integrator with the Mattsson operators, "
+                "reflection, periodic or S/N wall edges and at least 8 interior points per axis");
+  c->fuse = fuse;
+  if (rows_per_block) {
+    c->pair_rows = rows_per_block;
+    c->pair_rows_set = true;
   }
   drop_graphs(c);
   return DIM2D_OK;
@@ -604,6 +637,27 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
       std::swap(in, out);
     }
     // three swaps: the new state is in U2, as for RK3-TVD
+  } else if (c->fuse) {
+    // stages 1 and 2 in one launch: u^n -> stage-2 state (B, or its increment
+    // D2 in mixed precision), the stage-1 state and its wall ghosts inside
+    a.U = c->U;
+    if (!c->mixed) a.O64 = c->B;
+    else a.O32 = c->D2;
+    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage_pair(p, c->mixed, a, c->pair_rows, c->stream); }));
+    if (!c->mixed) {
+      if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->B, nullptr, c->B, nullptr, c->U, c->stream); }));
+      a.O64 = c->U2;
+      a.S64 = c->B;
+      HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 0, 3, a, c->block_x, c->rows, c->stream); }));
+      if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
+    } else {
+      if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D2, nullptr, c->D2, c->U, c->stream); }));
+      a.S32 = c->D2;
+      a.O32 = nullptr;
+      a.O64 = c->U2;
+      HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 1, 3, a, c->block_x, c->rows, c->stream); }));
+      if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
+    }
   } else if (!c->mixed) {
     a.U = c->U;
     a.S64 = c->U;
@@ -804,6 +858,11 @@ int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int3
     n->rows = c->rows;
     n->rows_set = true;
   }
+  if (c->pair_rows_set) {
+    n->pair_rows = c->pair_rows;
+    n->pair_rows_set = true;
+  }
+  n->fuse = c->fuse && pair_supported(n->dp);
   n->uploaded = true;
   n->steps = c->steps;
   n->profiling = c->profiling;

@@ -89,6 +89,13 @@ struct StageArgs {
 
 hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArgs& a, int block_x,
                         int rows, hipStream_t s);
+// RK3-TVD stages 1 and 2 in one launch (stage pair): u^n (a.U) -> stage-2
+// state in a.O64 (FP64) or its increment in a.O32 (mixed); dt as stage 1.
+// pair_supported: reflection, periodic and S/N wall edges, grids of at least
+// 8 interior points per axis.
+bool pair_supported(const DevParams& p);
+hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, int rows,
+                             hipStream_t s);
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
                               int mixed, int stage, const StageArgs& a, int block_x, int rows,
                               int workgroups, hipStream_t s);

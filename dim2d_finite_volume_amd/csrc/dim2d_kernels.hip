@@ -613,6 +613,95 @@ struct Slot {
   static constexpr int k = K;
 };
 
+// The pipeline rings of one strip: slot (q - q0) % kRing holds row q, q0 the
+// first row the wave processes.
+template <typename RD, typename R>
+struct StripRings {
+  double raw[kRing][4];    // stencil rows (FP64 state)
+  Cell<RD, R> cell[kRing];
+  XVals<R> xv[kRing];
+  Nb<RD, R> wst[kRing], est[kRing];
+  R fy[kRing][4];          // y-face fluxes (doubled) of the rows' upper faces
+  float inc[kRing][4];     // stage increments of the stencil rows (mixed stages 2, 3)
+  double rkU[kRing][4];    // RK inputs loaded apart from the stencil rows
+};
+
+// the first cell of a strip's pipeline (row q0, slot 0), with no capillary
+// temperature: the warm-up rows that use it produce no output
+template <typename RD, typename R>
+__device__ __forceinline__ void strip_start(const DevParams& p, StripRings<RD, R>& s) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    s.fy[kRing - 1][k] = (R)0;
+    s.rkU[0][k] = 0.0;
+  }
+  make_cell<RD, R>(p, s.raw[0], s.cell[0], s.xv[0]);
+  s.cell[0].Tcap = (R)0;
+  s.cell[kRing - 1] = s.cell[0];
+  s.wst[0] = nb_shr(s.cell[0]);
+  s.est[0] = nb_shl(s.cell[0]);
+  s.wst[kRing - 1] = s.wst[0];
+}
+
+// Time derivative L of row r (slot K) of a strip: builds the cell of row r+1
+// from raw[K1] (its capillary temperature takes the density of row r+2 from
+// raw[K2]), exchanges the neighbouring columns by DPP, and forms the x-face
+// flux at (i-1/2, r), the y-face flux at (i, r+1/2) and the flux-difference
+// divergence plus body force (td_operators_class.f:181-194).
+template <typename RD, typename R, int K>
+__device__ __forceinline__ void strip_time_dev(const DevParams& p, StripRings<RD, R>& s, int r,
+                                               bool wall_w, bool wall_e, R L[4]) {
+  constexpr int KM = (K + kRing - 1) % kRing;  // row r-1
+  constexpr int K1 = (K + 1) % kRing;          // row r+1
+  constexpr int K2 = (K + 2) % kRing;          // row r+2
+  // fluxes are doubled (face sums), hence the 1/2
+  const R hidx = (R)p.half_idx, hidy = (R)p.half_idy;
+  Cell<RD, R>& cm1 = s.cell[KM];
+  Cell<RD, R>& c0 = s.cell[K];
+  Cell<RD, R>& c1 = s.cell[K1];
+  const XVals<R>& x0 = s.xv[K];
+  // cell of row r+1 and its neighbours
+  make_cell<RD, R>(p, s.raw[K1], c1, s.xv[K1]);
+  s.wst[K1] = nb_shr(c1);
+  s.est[K1] = nb_shl(c1);
+  const Nb<RD, R>& c1W = s.wst[K1];
+  const Nb<RD, R>& c1E = s.est[K1];
+  const Nb<RD, R>& c0W = s.wst[K];
+  const Nb<RD, R>& c0E = s.est[K];
+  const Nb<RD, R>& cm1W = s.wst[KM];
+  c1.Tcap = cap_temperature<RD, R>(p, c1.irho, c1W.rho, c1E.rho, c0.rho, (RD)s.raw[K2][0]);
+
+  // the west cell of row r for the x-face
+  Cell<RD, R> Lc;
+  XVals<R> Lx;
+  Lc.w = shr(c0.w);
+  Lc.Tcl = shr(c0.Tcl);
+  Lc.Tcap = shr(c0.Tcap);
+  Lc.qxv = shr(c0.qxv);
+  Lx.qx = shr(x0.qx);
+  Lx.Pxx = shr(x0.Pxx);
+  Lx.Hu = shr(x0.Hu);
+  Lx.uw = shr(x0.uw);
+  const RD rhoLL = shr(c0W.rho);
+
+  R F[4];
+  R* G = s.fy[K];
+  const R* fy_prev = s.fy[KM];
+  x_face<RD, R>(p, c0W, Lc, Lx, rhoLL, c0E.rho, cm1W, c1W, c0, x0,
+                Nb<RD, R>{cm1.rho, cm1.u, cm1.v}, Nb<RD, R>{c1.rho, c1.u, c1.v}, wall_w, wall_e,
+                F);
+  y_face<RD, R>(p, c0, c1, c0W, c0E, c1W, c1E, cm1.rho, (RD)s.raw[K2][0],
+                p.bc_s == kBcWall && r == kBcSize - 1,
+                p.bc_n == kBcWall && r == p.ny - kBcSize - 1, G);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) L[k] = (F[k] - shl(F[k])) * hidx + (fy_prev[k] - G[k]) * hidy;
+  if (p.gravity_ac) {
+    // compute_body_forces (pmodel_eq_class.f:1840-1918)
+    L[2] -= (R)((double)c0.rho * p.gravity);
+    L[3] -= (R)((double)c0.qy * p.gravity);
+  }
+}
+
 template <typename RD, typename R, int MIXED, int STAGE, bool REDUCE>
 __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& a, double dt,
                                            int bx, int by, int rows, RedAcc<false>& red) {
@@ -652,90 +741,28 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   auto load_u = [&](int r, double u[4]) { ld_cell(rk_src + (long long)r * p.pitch * 4, ucu, u); };
 
   // rings: slot (q - (r0 - 2)) % kRing holds row q
-  double raw[kRing][4];
-  Cell<RD, R> cell[kRing];
-  XVals<R> xv[kRing];
-  Nb<RD, R> wst[kRing], est[kRing];
-  R fy[kRing][4];
-  float inc[kRing][4];  // S32 of the raw rows (mixed stages 2 and 3)
-  double rkU[kRing][4];
+  StripRings<RD, R> sr;
+  auto& raw = sr.raw;
+  auto& inc = sr.inc;
+  auto& rkU = sr.rkU;
 
   load_raw(r0 - 2, raw[0], inc[0]);
   load_raw(r0 - 1, raw[1], inc[1]);
   load_raw(r0, raw[2], inc[2]);
   load_raw(r0 + 1, raw[3], inc[3]);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    fy[kRing - 1][k] = (R)0;
-    rkU[0][k] = 0.0;
-  }
-  make_cell<RD, R>(p, raw[0], cell[0], xv[0]);
-  cell[0].Tcap = (R)0;
-  cell[kRing - 1] = cell[0];
-  wst[0] = nb_shr(cell[0]);
-  est[0] = nb_shl(cell[0]);
-  wst[kRing - 1] = wst[0];
-
-  // fluxes are doubled (face sums), hence the 1/2
-  const R hidx = (R)p.half_idx, hidy = (R)p.half_idy;
+  strip_start(p, sr);
 
   auto row = [&](auto slot, int r) __attribute__((always_inline)) {
     constexpr int K = decltype(slot)::k;
-    constexpr int KM = (K + kRing - 1) % kRing;  // row r-1
     constexpr int K1 = (K + 1) % kRing;          // row r+1
-    constexpr int K2 = (K + 2) % kRing;          // row r+2
     constexpr int K4 = (K + 4) % kRing;          // row r+4
     // prefetch: stencil row r+4 and u^n at (i, r+1).  Rows past the last one
     // the tile reads (r1 + 1, resp. r1 - 1) are clamped to it: the repeated
     // load hits in cache and its value is never used.
     load_raw(min(r + 4, r1 + 1), raw[K4], inc[K4]);
     if (LOAD_U) load_u(min(r + 1, r1 - 1), rkU[K1]);
-
-    Cell<RD, R>& cm1 = cell[KM];
-    Cell<RD, R>& c0 = cell[K];
-    Cell<RD, R>& c1 = cell[K1];
-    const XVals<R>& x0 = xv[K];
-    // cell of row r+1 and its neighbours
-    make_cell<RD, R>(p, raw[K1], c1, xv[K1]);
-    wst[K1] = nb_shr(c1);
-    est[K1] = nb_shl(c1);
-    const Nb<RD, R>& c1W = wst[K1];
-    const Nb<RD, R>& c1E = est[K1];
-    const Nb<RD, R>& c0W = wst[K];
-    const Nb<RD, R>& c0E = est[K];
-    const Nb<RD, R>& cm1W = wst[KM];
-    c1.Tcap = cap_temperature<RD, R>(p, c1.irho, c1W.rho, c1E.rho, c0.rho, (RD)raw[K2][0]);
-
-    // the west cell of row r for the x-face
-    Cell<RD, R> Lc;
-    XVals<R> Lx;
-    Lc.w = shr(c0.w);
-    Lc.Tcl = shr(c0.Tcl);
-    Lc.Tcap = shr(c0.Tcap);
-    Lc.qxv = shr(c0.qxv);
-    Lx.qx = shr(x0.qx);
-    Lx.Pxx = shr(x0.Pxx);
-    Lx.Hu = shr(x0.Hu);
-    Lx.uw = shr(x0.uw);
-    const RD rhoLL = shr(c0W.rho);
-
-    R F[4];
-    R* G = fy[K];
-    const R* fy_prev = fy[KM];
-    x_face<RD, R>(p, c0W, Lc, Lx, rhoLL, c0E.rho, cm1W, c1W, c0, x0,
-                  Nb<RD, R>{cm1.rho, cm1.u, cm1.v}, Nb<RD, R>{c1.rho, c1.u, c1.v}, wall_w,
-                  wall_e, F);
-    y_face<RD, R>(p, c0, c1, c0W, c0E, c1W, c1E, cm1.rho, (RD)raw[K2][0],
-                  p.bc_s == kBcWall && r == kBcSize - 1,
-                  p.bc_n == kBcWall && r == p.ny - kBcSize - 1, G);
     R L[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) L[k] = (F[k] - shl(F[k])) * hidx + (fy_prev[k] - G[k]) * hidy;
-    if (p.gravity_ac) {
-      // compute_body_forces (pmodel_eq_class.f:1840-1918)
-      L[2] -= (R)((double)c0.rho * p.gravity);
-      L[3] -= (R)((double)c0.qy * p.gravity);
-    }
+    strip_time_dev<RD, R, K>(p, sr, r, wall_w, wall_e, L);
 
     const long long row = (long long)r * p.pitch * 4;  // uniform row offset
     if (out_col && r >= r0) {
@@ -1009,6 +1036,53 @@ __device__ double temperature_eff_at(const DevParams& p, const double* F, int c,
           0.5 / p.we * (gx * gx + gy * gy) + 3.0 * rho * rho);
 }
 
+// The ghost and outer ghost cells of one wall cell (edge e: 0 S, 1 N, 2 W,
+// 3 E) from the stage state of the first (v1) and far (v2) interior cells
+// along the wall normal, the densities ys0, ys1 of the first interior cell's
+// two transverse neighbours, and the bracket guess from u^n:
+// compute_wall_{S,N,W,E}_ghost_cell (wall_xy_equilibrium_module.f:948-1437)
+// with the outer density of get_wall_md_extra_ghost_cell (:1439-1465).
+__device__ __forceinline__ void wall_ghost_values(const DevParams& p, int e, const double v1[4],
+                                                  const double v2[4], double ys0, double ys1,
+                                                  double T_guess, double md_guess, double g[4],
+                                                  double go[4]) {
+  WallRoot w;
+  w.far = v2[0];
+  w.first = v1[0];
+  w.Ed = v1[3];
+  if (e <= 1) {
+    w.ux = (e == 0 ? -v1[2] : v1[2]) / v1[0];  // normal velocity (sign of the S/N cell)
+    w.uy = v1[1] / v1[0];
+    w.dn = p.dy;
+    w.dt = p.dx;
+  } else {
+    w.ux = (e == 2 ? -v1[1] : v1[1]) / v1[0];
+    w.uy = v1[2] / v1[0];
+    w.dn = p.dx;
+    w.dt = p.dy;
+  }
+  w.ys0 = ys0;
+  w.ys1 = ys1;
+  wall_root_prepare(p, w);
+  const double rg = wall_ghost_density(p, w, T_guess, md_guess);
+  // sign_normal = +1, sign_trans = -1 (wall_xy_equilibrium_module.f:136-140)
+  g[0] = rg;
+  go[0] = v2[0] + 3.0 * (-v1[0] + rg);  // wall_dmddx_high_order
+  if (e <= 1) {
+    g[1] = -1.0 * v1[1] / v1[0] * rg;
+    g[2] = 1.0 * v1[2] / v1[0] * rg;
+  } else {
+    g[1] = 1.0 * v1[1] / v1[0] * rg;
+    g[2] = -1.0 * v1[2] / v1[0] * rg;
+  }
+  // the reference leaves the ghost energy and the outer momenta unset; they are
+  // never read by an interior stencil.  Keep finite values there.
+  g[3] = v1[3];
+  go[1] = g[1];
+  go[2] = g[2];
+  go[3] = v1[3];
+}
+
 // one thread per wall cell; edge e: 0 S, 1 N, 2 W, 3 E
 __device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const double* U,
                           const double* S64, const float* S32, double* O64, float* O32,
@@ -1036,50 +1110,19 @@ __device__ void wall_cell(const DevParams& p, int e, int t, int mixed_in, const 
   double v1[4], v2[4];
   wall_load(p, U, S64, S32, mixed_in, c1, r1, v1);
   wall_load(p, U, S64, S32, mixed_in, c2, r2, v2);
-  WallRoot w;
-  w.far = v2[0];
-  w.first = v1[0];
-  w.Ed = v1[3];
   double vy0[4], vy1[4];
   if (e <= 1) {
     wall_load(p, U, S64, S32, mixed_in, c1 - 1, r1, vy0);
     wall_load(p, U, S64, S32, mixed_in, c1 + 1, r1, vy1);
-    w.ux = (e == 0 ? -v1[2] : v1[2]) / v1[0];  // normal velocity (sign of the S/N cell)
-    w.uy = v1[1] / v1[0];
-    w.dn = p.dy;
-    w.dt = p.dx;
   } else {
     wall_load(p, U, S64, S32, mixed_in, c1, r1 - 1, vy0);
     wall_load(p, U, S64, S32, mixed_in, c1, r1 + 1, vy1);
-    w.ux = (e == 2 ? -v1[1] : v1[1]) / v1[0];
-    w.uy = v1[2] / v1[0];
-    w.dn = p.dx;
-    w.dt = p.dy;
   }
-  w.ys0 = vy0[0];
-  w.ys1 = vy1[0];
-  wall_root_prepare(p, w);
   // compute_wall_xy_guess from u^n (bc_operators_wall_xy_class.f:898-930)
   const double T_guess = p.c38cv * temperature_eff_at(p, tmp, c1, r1);
   const double md_guess = tmp[((long long)r * p.pitch + c) * 4];
-  const double rg = wall_ghost_density(p, w, T_guess, md_guess);
-  // sign_normal = +1, sign_trans = -1 (wall_xy_equilibrium_module.f:136-140)
   double g[4], go[4];
-  g[0] = rg;
-  go[0] = v2[0] + 3.0 * (-v1[0] + rg);  // wall_dmddx_high_order
-  if (e <= 1) {
-    g[1] = -1.0 * v1[1] / v1[0] * rg;
-    g[2] = 1.0 * v1[2] / v1[0] * rg;
-  } else {
-    g[1] = 1.0 * v1[1] / v1[0] * rg;
-    g[2] = -1.0 * v1[2] / v1[0] * rg;
-  }
-  // the reference leaves the ghost energy and the outer momenta unset; they are
-  // never read by an interior stencil.  Keep finite values there.
-  g[3] = v1[3];
-  go[1] = g[1];
-  go[2] = g[2];
-  go[3] = v1[3];
+  wall_ghost_values(p, e, v1, v2, vy0[0], vy1[0], T_guess, md_guess, g, go);
   for (int k = 0; k < 4; ++k) {
     if (O32) {
       O32[ig + k] = (float)(g[k] - ug[k]);
@@ -1129,6 +1172,308 @@ __global__ void wall_batch_kernel(const DevParams* P, int nsim, int mixed_in, co
               S32 ? S32 + o : nullptr, O64 ? O64 + o : nullptr, O32 ? O32 + o : nullptr,
               tmp + o);
   }
+}
+
+// ------------------------------------------------------------------------
+// stage pair: RK3-TVD stages 1 and 2 fused (rk3tvd_steps_module.f:52-100,
+// 205-269), so the stage-1 state never goes through HBM
+// ------------------------------------------------------------------------
+// A workgroup of two waves owns a strip of 64 columns, of which the 56 centre
+// columns are outputs, and `rows` output rows.  Wave 0, the producer, marches
+// up the strip with the stage-1 pipeline from u^n and writes the stage-1 state
+// s1 of its 60 valid columns (lanes 2..61) row by row into a ring of LDS rows.
+// Wave 1, the consumer, marches kPairLag rows behind with the stage-2
+// pipeline, reading s1 from the ring instead of HBM, and writes the stage-2
+// state.  One barrier per row keeps them in step; the two waves run on
+// different SIMDs of the CU.
+// Ghost cells of s1 that the consumer reads:
+//  * periodic edges: the producer computes them, its loads reaching two more
+//    rows/columns out through the periodic maps, so they are the images'
+//    values (same inputs, same instruction stream);
+//  * reflection edges: read from the mirror lane/row of the ring with the
+//    normal momentum negated, as the stage kernels' virtual ghosts;
+//  * S/N walls: the wall root of each ghost cell (wall_ghost_values, as
+//    wall_kernel) from s1 of the first and far interior rows, in the tile that
+//    holds the wall: the consumer before its first row at the south wall, the
+//    producer after its last row at the north wall.
+// Walls on W/E edges and open edges take the separate stage kernels.
+// Every output cell sees the same operands in the same order as with the
+// separate kernels and the wall kernel between them.
+constexpr int kPairOut = kLanes - 8;  // output columns per strip (lanes 4..59)
+constexpr int kPairLag = 6;           // rows the consumer runs behind the producer
+constexpr int kPairSlots = 8;         // LDS ring rows (a power of two)
+
+template <int MIXED>
+struct PairRing {
+  double s[kPairSlots][4][kLanes];                   // s1, one plane per variable
+  float d[MIXED ? kPairSlots : 1][4][kLanes];        // mixed: the stage-1 increments
+};
+
+// map_col / map_row reaching two more columns/rows out (c in [-2, nx+1]),
+// clamped into the array (the clamped lanes' values are never used)
+__device__ __forceinline__ int pair_map_col(const DevParams& p, int c, bool& flip) {
+  flip = false;
+  int m = c;
+  if (c < kBcSize) {
+    if (p.bc_w == kBcReflection) { flip = true; m = 2 * kBcSize - 1 - c; }
+    else if (p.bc_w == kBcPeriodic) m = c + p.nx - 2 * kBcSize;
+  } else if (c >= p.nx - kBcSize) {
+    if (p.bc_e == kBcReflection) { flip = true; m = 2 * p.nx - 2 * kBcSize - 1 - c; }
+    else if (p.bc_e == kBcPeriodic) m = c - (p.nx - 2 * kBcSize);
+  }
+  return min(max(m, 0), p.nx - 1);
+}
+__device__ __forceinline__ int pair_map_row(const DevParams& p, int r, bool& flip) {
+  flip = false;
+  int m = r;
+  if (r < kBcSize) {
+    if (p.bc_s == kBcReflection) { flip = true; m = 2 * kBcSize - 1 - r; }
+    else if (p.bc_s == kBcPeriodic) m = r + p.ny - 2 * kBcSize;
+  } else if (r >= p.ny - kBcSize) {
+    if (p.bc_n == kBcReflection) { flip = true; m = 2 * p.ny - 2 * kBcSize - 1 - r; }
+    else if (p.bc_n == kBcPeriodic) m = r - (p.ny - 2 * kBcSize);
+  }
+  return min(max(m, 0), p.ny - 1);
+}
+
+// s1 of (row r, lane l) of strip bx from the ring: ghost cells of reflection
+// edges come from the mirror lane/row with the normal momentum negated
+template <int MIXED>
+__device__ __forceinline__ void pair_gather(const DevParams& p, const PairRing<MIXED>& ring,
+                                            int bx, int r, int l, double v[4], float inc[4]) {
+  const int cc = kBcSize + bx * kPairOut - 4 + l;
+  int ll = l;
+  bool fx = false, fy = false;
+  if (cc < kBcSize && p.bc_w == kBcReflection) {
+    ll = l + (2 * kBcSize - 1 - 2 * cc);
+    fx = true;
+  } else if (cc >= p.nx - kBcSize && p.bc_e == kBcReflection) {
+    ll = l + (2 * p.nx - 2 * kBcSize - 1 - 2 * cc);
+    fx = true;
+  }
+  ll = min(max(ll, 0), kLanes - 1);
+  int rr = r;
+  if (r < kBcSize && p.bc_s == kBcReflection) {
+    rr = 2 * kBcSize - 1 - r;
+    fy = true;
+  } else if (r >= p.ny - kBcSize && p.bc_n == kBcReflection) {
+    rr = 2 * p.ny - 2 * kBcSize - 1 - r;
+    fy = true;
+  }
+  const int sl = rr & (kPairSlots - 1);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = ring.s[sl][k][ll];
+    inc[k] = MIXED ? ring.d[MIXED ? sl : 0][k][ll] : 0.f;
+  }
+  if (fx) v[1] = -v[1];
+  if (fy) v[2] = -v[2];
+}
+
+// The wall ghost rows of s1 of a S (north = false) or N wall into the ring,
+// for the lanes whose column lies in [1, nx-2] (the ghost cells the
+// consumer's outputs read, and those a reflecting W/E edge mirrors).  The
+// values are those wall_kernel stores between the separate stage kernels: in
+// mixed precision the ghost state goes through its FP32 increment against
+// u^n, as the stage-2 kernel would read it.
+template <int MIXED>
+__device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<MIXED>& ring,
+                               int bx, bool north) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int c = kBcSize + bx * kPairOut - 4 + lane;
+  const int r = north ? p.ny - kBcSize : kBcSize - 1;  // inner ghost row
+  const int r1 = north ? r - 1 : r + 1, r2 = north ? r - 2 : r + 2, ro = north ? r + 1 : r - 1;
+  double g[4], go[4];
+  if (c >= 1 && c <= p.nx - 2) {
+    double v1[4], v2[4], y0[4], y1[4];
+    float t[4];
+    pair_gather<MIXED>(p, ring, bx, r1, lane, v1, t);
+    pair_gather<MIXED>(p, ring, bx, r2, lane, v2, t);
+    pair_gather<MIXED>(p, ring, bx, r1, lane - 1, y0, t);
+    pair_gather<MIXED>(p, ring, bx, r1, lane + 1, y1, t);
+    bool fx;
+    const int cm = map_col(p, c, fx);
+    const double T_guess = p.c38cv * temperature_eff_at(p, a.U, c, r1);
+    const double md_guess = a.U[((long long)r * p.pitch + cm) * 4];
+    wall_ghost_values(p, north ? 1 : 0, v1, v2, y0[0], y1[0], T_guess, md_guess, g, go);
+    if (MIXED) {
+      const double* ug = a.U + ((long long)r * p.pitch + cm) * 4;
+      const double* uo = a.U + ((long long)ro * p.pitch + cm) * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float dg = (float)(g[k] - ug[k]), dgo = (float)(go[k] - uo[k]);
+        g[k] = ug[k] + (double)dg;
+        go[k] = uo[k] + (double)dgo;
+        ring.d[MIXED ? (r & (kPairSlots - 1)) : 0][k][lane] = dg;
+        ring.d[MIXED ? (ro & (kPairSlots - 1)) : 0][k][lane] = dgo;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ring.s[r & (kPairSlots - 1)][k][lane] = g[k];
+      ring.s[ro & (kPairSlots - 1)][k][lane] = go[k];
+    }
+  }
+}
+
+// the producer wave: stage 1 of rows r0-2 .. rlast into the ring, then the
+// north wall rows when the tile holds them, then idle barriers up to `steps`
+template <typename RD, typename R, int MIXED>
+__device__ __forceinline__ void pair_producer(const DevParams& p, const StageArgs& a, double dt,
+                                              int bx, int r0, int r1, int steps,
+                                              PairRing<MIXED>& ring) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int c = kBcSize + bx * kPairOut - 4 + lane;
+  bool flx;
+  const int cm = pair_map_col(p, c, flx);
+  // The consumer reads s1 up to row r1 + 1, which is a ghost row when
+  // r1 + 1 >= ny - 2 (the top tile, and a tile ending one row below it).  The
+  // ghost rows of a reflecting or wall edge are the consumer's mirrors and the
+  // wall step's; a periodic edge's are the images this wave computes.
+  const bool top = r1 + 1 >= p.ny - kBcSize;
+  const int rlast = p.bc_n == kBcPeriodic ? r1 + 1 : min(r1 + 1, p.ny - kBcSize - 1);
+  const int q0 = r0 - 4;  // two warm-up rows before the first output row r0 - 2
+  auto load_raw = [&](int r, double v[4]) {
+    bool fly;
+    const int rm = pair_map_row(p, r, fly);
+    ld_cell(a.U + (long long)rm * p.pitch * 4, (unsigned)cm, v);
+    if (flx) v[1] = -v[1];
+    if (fly) v[2] = -v[2];
+  };
+  StripRings<RD, R> sr;
+  load_raw(q0, sr.raw[0]);
+  load_raw(q0 + 1, sr.raw[1]);
+  load_raw(q0 + 2, sr.raw[2]);
+  load_raw(q0 + 3, sr.raw[3]);
+  strip_start(p, sr);
+  const int nrows = rlast - q0 + 1;
+  int g = 0;
+  auto step = [&](auto slot) __attribute__((always_inline)) {
+    constexpr int K = decltype(slot)::k;
+    constexpr int K4 = (K + 4) % kRing;
+    const int r = q0 + g;
+    load_raw(min(r + 4, rlast + 2), sr.raw[K4]);
+    R L[4];
+    strip_time_dev<RD, R, K>(p, sr, r, false, false, L);
+    if (r >= r0 - 2) {
+      const int sl = r & (kPairSlots - 1);
+      const double* S = sr.raw[K];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double dl = dt * (double)L[k];
+        if (MIXED) {
+          const float o = (float)dl;
+          ring.d[MIXED ? sl : 0][k][lane] = o;
+          ring.s[sl][k][lane] = S[k] + (double)o;
+        } else {
+          ring.s[sl][k][lane] = S[k] + dl;
+        }
+      }
+    }
+    __syncthreads();
+    return ++g < nrows;
+  };
+  for (;;) {
+    if (!step(Slot<0>{})) break;
+    if (!step(Slot<1>{})) break;
+    if (!step(Slot<2>{})) break;
+    if (!step(Slot<3>{})) break;
+    if (!step(Slot<4>{})) break;
+  }
+  if (top && p.bc_n == kBcWall) {
+    pair_wall_rows<MIXED>(p, a, ring, bx, true);
+    __syncthreads();
+    ++g;
+  }
+  for (; g < steps; ++g) __syncthreads();
+}
+
+// the consumer wave: kPairLag idle barriers, the south wall rows when the
+// tile holds them, then stage 2 of rows r0 .. r1-1 from the ring
+template <typename RD, typename R, int MIXED>
+__device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArgs& a, double dt,
+                                              int bx, int r0, int r1, PairRing<MIXED>& ring) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int c0 = kBcSize + bx * kPairOut;  // first output column of the strip
+  const int c = c0 - 4 + lane;
+  const bool out_col = lane >= 4 && lane < kLanes - 4 && c <= p.nx - kBcSize - 1;
+  const unsigned uc = (unsigned)min(max(c, 0), p.nx - 1);
+  // u^n at (i, r) for the RK combination (FP64); halo lanes read an output
+  // column's address
+  const unsigned ucu = (unsigned)min(max(c, c0), min(c0 + kPairOut - 1, p.nx - kBcSize - 1));
+  for (int g = 0; g < kPairLag; ++g) __syncthreads();
+  if (r0 == kBcSize && p.bc_s == kBcWall) {
+    pair_wall_rows<MIXED>(p, a, ring, bx, false);
+    __threadfence_block();
+  }
+  const int q0 = r0 - 2;
+  StripRings<RD, R> sr;
+  pair_gather<MIXED>(p, ring, bx, q0, lane, sr.raw[0], sr.inc[0]);
+  pair_gather<MIXED>(p, ring, bx, q0 + 1, lane, sr.raw[1], sr.inc[1]);
+  strip_start(p, sr);
+  int r = q0;
+  auto row = [&](auto slot) __attribute__((always_inline)) {
+    constexpr int K = decltype(slot)::k;
+    constexpr int K1 = (K + 1) % kRing;
+    constexpr int K2 = (K + 2) % kRing;
+    // s1 of row r+2 from the ring (written at least one barrier ago)
+    pair_gather<MIXED>(p, ring, bx, r + 2, lane, sr.raw[K2], sr.inc[K2]);
+    if (!MIXED) ld_cell(a.U + (long long)min(r + 1, r1 - 1) * p.pitch * 4, ucu, sr.rkU[K1]);
+    R L[4];
+    strip_time_dev<RD, R, K>(p, sr, r, false, false, L);
+    if (out_col && r >= r0) {
+      const long long row = (long long)r * p.pitch * 4;
+      const double* S = sr.raw[K];
+      if (!MIXED) {
+        double vout[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double inc = S[k] + dt * (double)L[k];
+          vout[k] = 0.75 * sr.rkU[K][k] + (1.0 - 0.75) * inc;
+        }
+        st_cell(a.O64 + row, uc, vout);
+      } else {
+        float o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double dl = dt * (double)L[k];
+          o[k] = (float)(0.25 * ((double)sr.inc[K][k] + dl));
+        }
+        st_cell(a.O32 + row, uc, o);
+      }
+    }
+    __syncthreads();
+    return ++r < r1;
+  };
+  for (;;) {
+    if (!row(Slot<0>{})) break;
+    if (!row(Slot<1>{})) break;
+    if (!row(Slot<2>{})) break;
+    if (!row(Slot<3>{})) break;
+    if (!row(Slot<4>{})) break;
+  }
+}
+
+// minimum waves per SIMD: 2 (up to 256 VGPRs), as the stage kernels
+template <int MIXED>
+__global__ void __launch_bounds__(2 * kLanes) __attribute__((amdgpu_waves_per_eu(2, 8)))
+stage_pair_kernel(const DevParams p, const StageArgs a, int rows) {
+  __shared__ PairRing<MIXED> ring;
+  using RD = double;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  double dt = a.dt_fixed;
+  if (a.cfl_mode) dt = cfl_dt_from(a.ctl, a.parity);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    a.ctl->dt = dt;
+    clear_slot(a.ctl, a.parity ^ 1);
+  }
+  const int bx = blockIdx.x;
+  const int r0 = kBcSize + blockIdx.y * rows;
+  const int r1 = min(r0 + rows, p.ny - kBcSize);
+  // the consumer's rows r0-2 .. r1-1 after kPairLag idle barriers
+  const int steps = kPairLag + (r1 - r0 + 2);
+  if (threadIdx.x < kLanes) pair_producer<RD, R, MIXED>(p, a, dt, bx, r0, r1, steps, ring);
+  else pair_consumer<RD, R, MIXED>(p, a, dt, bx, r0, r1, ring);
 }
 
 // ------------------------------------------------------------------------
@@ -2127,6 +2472,27 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
     case 2: return launch_one<1, 2>(p, a, block_x, rows, s);
     default: return launch_one<1, 3>(p, a, block_x, rows, s);
   }
+}
+
+bool pair_supported(const DevParams& p) {
+  const int b[4] = {p.bc_w, p.bc_e, p.bc_s, p.bc_n};
+  for (int k = 0; k < 4; ++k)
+    if (b[k] == kBcOpen) return false;
+  if (p.bc_w == kBcWall || p.bc_e == kBcWall) return false;
+  // the producer's maps reach 4 rows/columns past an edge: the images must be
+  // interior cells
+  return p.nx >= 2 * kBcSize + 8 && p.ny >= 2 * kBcSize + 8;
+}
+
+hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, int rows,
+                             hipStream_t s) {
+  const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
+  dim3 grid((ni + kPairOut - 1) / kPairOut, (nj + rows - 1) / rows);
+  if (mixed)
+    hipLaunchKernelGGL(stage_pair_kernel<1>, grid, dim3(2 * kLanes), 0, s, p, a, rows);
+  else
+    hipLaunchKernelGGL(stage_pair_kernel<0>, grid, dim3(2 * kLanes), 0, s, p, a, rows);
+  return hipGetLastError();
 }
 
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,

@@ -229,6 +229,12 @@ class Field:
     def set_tiles(self, block_x: int = 0, rows: int = 0) -> None:
         capi.check(self._lib.dim2d_set_tiles(self._ctx, block_x, rows), "dim2d_set_tiles")
 
+    def set_stage_fusion(self, fuse: bool, rows: int = 0) -> None:
+        """RK3-TVD stages 1 and 2 in one launch (on by default where the edges
+        allow it) or apart; rows: tile height of the fused launch (0 keeps it)."""
+        capi.check(self._lib.dim2d_set_stage_fusion(self._ctx, int(bool(fuse)), rows),
+                   "dim2d_set_stage_fusion")
+
     KIND = {"stage1": 1, "stage2": 2, "stage3": 4, "wall": 8}
 
     def profile(self, enable, kinds=None) -> None:

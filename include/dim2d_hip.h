@@ -175,6 +175,17 @@ int dim2d_synchronize(dim2d_ctx* ctx);
  * the tiling. */
 int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
 
+/* RK3-TVD stage fusion (no counterpart in the reference, whose stages each
+ * write a full field, rk3tvd_steps_module.f:52-490): fuse = 1 computes stages
+ * 1 and 2 in one launch whose stage-1 state stays on chip (64 fewer HBM bytes
+ * per cell and step in FP64, 32 in mixed precision); fuse = 0 launches the
+ * stages apart.  rows_per_block: tile height of the fused launch (0 keeps the
+ * current value, default from the grid size).  On by default where it
+ * applies: RK3-TVD with the Mattsson operators, reflection, periodic or S/N
+ * wall edges, at least 8 interior points per axis; DIM2D_EINVAL otherwise.
+ * Results equal those of the separate launches. */
+int dim2d_set_stage_fusion(dim2d_ctx* ctx, int32_t fuse, int32_t rows_per_block);
+
 /* Kernel timing with HIP events recorded on the context's stream around the
  * kernel launches of the following steps whose kind is in the bit mask
  * `enable` (1 stage 1, 2 stage 2, 4 stage 3, 8 wall ghost kernels; 15 all),
@@ -186,7 +197,8 @@ int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
 int dim2d_profile_enable(dim2d_ctx* ctx, int32_t enable);
 
 /* Mean duration (ms) and launch count per kernel since enabling (blocking):
- * out[0],out[1] stage 1; out[2],out[3] stage 2; out[4],out[5] stage 3;
+ * out[0],out[1] stage 1, or the fused stages 1 and 2 with stage fusion on
+ * (out[2], out[3] then stay 0); out[2],out[3] stage 2; out[4],out[5] stage 3;
  * out[6],out[7] wall ghost kernel. */
 int dim2d_profile_read(dim2d_ctx* ctx, double out[8]);
 

@@ -497,6 +497,7 @@ def test_batch_matches_single(orc):
     for i, (p, n) in enumerate(sims):
         f = d.Field(p, n)
         f.set_tiles(64, 16)
+        f.set_stage_fusion(False)  # the batch kernels launch the stages apart
         f.run(6, dt=dts[i])
         single = f.download()
         got = b.download(i)
@@ -535,6 +536,7 @@ def test_batch_mixed_edges_matches_single(orc, prec):
     b.integrate(dts, 7)
     for i, (p, n, dt, m, xm, ym) in enumerate(sims):
         f = d.Field(p, n.copy())
+        f.set_stage_fusion(False)  # the batch kernels launch the stages apart
         f.run(7, dt=dt)
         single = f.download()
         f.close()
@@ -596,6 +598,91 @@ def test_tile_invariance(orc, precision):
         f.close()
     for k, o in enumerate(outs[1:]):
         assert np.array_equal(outs[0], o), k
+
+
+FUSED_CASES = ["reflection", "periodic", "periodic_x_wall_y", "wall_s_reflection"]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES)
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("shape", [(131, 70), (76, 45), (12, 12)])
+def test_stage_fusion_matches_separate_stages(orc, case, prec, shape):
+    """The fused stages 1 and 2 (one launch, stage-1 state on chip, its ghost
+    cells from periodic images, reflection mirrors and in-kernel wall roots)
+    against the separate stage kernels with the wall kernel between them:
+    equal results after 6 steps (FP64 to 1e-14 of each variable's scale,
+    mixed to 1e-7), for tile heights from 4 rows to one tile for the whole
+    grid, on grids whose last strip is partial (131 and 76 columns) and on
+    the smallest grid the fused launch takes (8 interior points), and with
+    last tiles of one row (41 interior rows in 4- and 8-row tiles), whose
+    neighbour tile's stencil reaches the north ghost rows; the fused run
+    against the oracle too (FP64 1e-10, mixed 1e-6 on density and energy)."""
+    nx, ny = shape
+    kw = dict(wall_contact_angle=70.0, wall_heat_flux=0.01) if W in BC_CASES[case] else {}
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], nx=nx, ny=ny, precision=prec, **kw)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    f = d.Field(p, nodes.copy())
+    f.set_stage_fusion(False)
+    f.run(6, dt=dt)
+    sep = f.download()
+    f.close()
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(6, dt)) == 0
+    tol = 1e-14 if prec == 0 else 1e-7
+    first = None
+    for rows in (4, 7, 8, 32, ny):
+        f = d.Field(p, nodes.copy())
+        f.set_stage_fusion(True, rows)
+        f.run(6, dt=dt)
+        got = f.download()
+        f.close()
+        # every output row runs the same instruction stream whatever the tiling
+        if first is None:
+            first = got
+        assert np.array_equal(got[:, 2:-2, 2:-2], first[:, 2:-2, 2:-2]), rows
+        err = max_rel_err(got, sep, region=ALL if W not in BC_CASES[case] else INT)
+        print(case, prec, shape, rows, "fused vs separate", err)
+        assert max(err) < tol, (rows, err)
+        if W in BC_CASES[case]:
+            assert max(ghost_errors(got, sep, BC_CASES[case])) < tol, rows
+        e = max_rel_err(got, ref)
+        if prec == 0:
+            assert max(e) < 1e-10, (rows, e)
+        else:
+            assert e[0] < 1e-6 and e[3] < 1e-6, (rows, e)
+
+
+def test_stage_fusion_with_cfl_and_graphs(orc):
+    """The fused launch under graph replay with the device CFL time step
+    (the stage pair takes dt from the reductions and clears the next slot)
+    equals the separate launches: elapsed time, last dt and state."""
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["periodic_x_wall_y"], nx=200, ny=150)
+    out = []
+    for fuse in (False, True):
+        f = d.Field(p, nodes.copy())
+        f.set_stage_fusion(fuse)
+        f.run(9, cfl=0.1)
+        out.append((f.download(), f.device_time()))
+        f.close()
+    (a, ta), (b, tb) = out
+    assert abs(ta[0] - tb[0]) <= 1e-14 * ta[0] and abs(ta[1] - tb[1]) <= 1e-14 * ta[1]
+    assert max(max_rel_err(b, a)) < 1e-14
+
+
+def test_stage_fusion_rejects_unsupported_edges():
+    """Open edges, walls on W/E, the low-storage RK3 and tiny grids keep the
+    separate stage kernels: turning fusion on there is an error."""
+    d = _lib()
+    for kw in (dict(bc=BC_CASES["open"]), dict(bc=BC_CASES["wall_x_reflection_y"]),
+               dict(bc=BC_CASES["reflection"], integrator=1),
+               dict(bc=BC_CASES["reflection"], nx=11, ny=40)):
+        kw = dict(kw)
+        p = d.ModelParams(nx=kw.pop("nx", 40), ny=kw.pop("ny", 36), **kw)
+        f = d.Field(p, np.ones((4, p.ny, p.nx)))
+        with pytest.raises(RuntimeError):
+            f.set_stage_fusion(True)
+        f.set_stage_fusion(False)
+        f.close()
 
 
 def test_set_tiles_rejects_other_widths():
