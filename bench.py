@@ -27,10 +27,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grid-pt updates/s + %HBM peak; FP64 vs mixed FP32 at ≤1e-6 rel err"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# compulsory HBM bytes per interior cell and launch (FP64): stage 1 reads u^n and
-# writes u1 (64 B); stages 2 and 3 read the stage state and u^n and write (96 B).
-# Mixed mode: u1/u2 increments in FP32 -> 48, 64, 80 B.
-ALG_BYTES = {"f64": (64, 96, 96), "mixed": (48, 64, 80)}
+# MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
+# every 4 cycles (16 lanes per cycle; FP64 and non-packed FP32 alike), 2.4 GHz
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
+# compulsory HBM bytes per interior cell and launch.  FP64: stage 1 reads u^n
+# and writes u1 (64 B); stages 2 and 3 read the stage state and u^n and write
+# (96 B); the fused stages 1+2 read u^n and write u2 (64 B).  Mixed: u1/u2 are
+# FP32 increments -> 48, 64, 80 B, and 48 B for the fused pair.
+ALG_BYTES = {"f64": {"stage1": 64, "stage2": 96, "stage3": 96, "pair": 64},
+             "mixed": {"stage1": 48, "stage2": 64, "stage3": 80, "pair": 48}}
+KERNEL = {"stage1": "stage_kernel<{m}, 1> ({p} stage 1)", "stage2": "stage_kernel<{m}, 2> ({p} stage 2)",
+          "stage3": "stage_kernel<{m}, 3> ({p} stage 3, with the CFL reduction)",
+          "pair": "stage_pair_kernel<{m}> ({p} stages 1+2 fused)"}
 
 
 def log(*a):
@@ -83,7 +91,10 @@ def timed_run(field, steps, warmup, cfl, dt=0.0, clock_s=1.0):
     field.run(max(steps, 20), dt=dt, cfl=cfl)
     kt = field.kernel_times()
     field.profile(False)
-    stages = ("stage1", "stage2", "stage3")
+    if kt["stage2"][1] == 0 and kt["stage1"][1] > 0:
+        # fused stages 1 and 2: their launch is timed in the stage-1 slot
+        kt = {"pair": kt["stage1"], "stage3": kt["stage3"], "wall": kt["wall"]}
+    stages = [k for k in ("stage1", "stage2", "stage3", "pair") if k in kt]
     dom = max(stages, key=lambda k: kt[k][0] * kt[k][1])
     field.time = field.device_time()[0]
     return el, dom, kt[dom], kt, clock
@@ -149,7 +160,8 @@ def batch_bench(steps, nsims=256, seed=4):
         b.close()
         value = cells * steps / el
         out[name] = {"value": value, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * el / steps,
-                     "hbm_frac_algorithmic": value * sum(ALG_BYTES[name]) / HBM_PEAK,
+                     "hbm_frac_algorithmic": value * sum(ALG_BYTES[name][k] for k in
+                                                         ("stage1", "stage2", "stage3")) / HBM_PEAK,
                      "tiles": {"rows": rows, "workgroups": wg, "autotune_ms_per_step": tuned_ms},
                      "finite": bool(all(np.isfinite(x).all() for x in last)),
                      "mass_drift": [float((x[0, 2:-2, 2:-2].sum() - m0) / m0)
@@ -222,16 +234,40 @@ def variants_bench(n, steps):
     return out
 
 
-def read_traffic(name):
+def read_profile(name, key):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
+            return json.load(f).get(key)
     except Exception:
         return None
+
+
+def kernel_table(kt, prec, cells, n):
+    """per kernel kind: mean launch time, algorithmic HBM rate and its fraction
+    of 8 TB/s, counted HBM traffic (FETCH_SIZE + WRITE_SIZE profile) and VALU
+    issue rate against its peak (SQ_INSTS_VALU profile)"""
+    out = {}
+    for k, (ms, cnt) in kt.items():
+        if k == "wall" or cnt == 0:
+            continue
+        key = f"{'stage' + k[-1] if k.startswith('stage') else 'pair'}_{prec}_two_bubbles_{n}"
+        alg = ALG_BYTES["f64" if prec == "fp64" else "mixed"][k] * cells
+        e = {"kernel": KERNEL[k].format(m=0 if prec == "fp64" else 1,
+                                        p="FP64" if prec == "fp64" else "mixed"),
+             "mean_launch_ms": ms, "launches": cnt, "algorithmic_bytes_per_launch": alg,
+             "achieved_GBps": alg / (ms * 1e-3) / 1e9, "hbm_frac": alg / (ms * 1e-3) / HBM_PEAK,
+             "traffic": read_profile(key + "_traffic.json", "hbm_bytes_per_launch")}
+        pd = read_profile(key + "_issue.json", "per_dispatch")
+        if pd and pd.get("SQ_INSTS_VALU"):
+            v = pd["SQ_INSTS_VALU"]
+            e["issue"] = {"valu_instructions_per_launch": v, "achieved": v / (ms * 1e-3),
+                          "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
+                          "frac": v / (ms * 1e-3) / VALU_PEAK}
+        out[k] = e
+    return out
 
 
 def main():
@@ -251,6 +287,9 @@ def main():
                     help="steps of the open-boundary case (0: skip)")
     ap.add_argument("--variant-steps", type=int, default=200,
                     help="steps of the LSRK3 / Cockburn-Gau variants (0: skip)")
+    ap.add_argument("--error-steps", default="100,1000",
+                    help="steps after which the mixed run is compared with FP64 "
+                         "(comma-separated, increasing; empty: no comparison)")
     ap.add_argument("--clock-warmup-s", type=float, default=1.0,
                     help="seconds of untimed steps before the warmup steps")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -301,17 +340,17 @@ def main():
     ms_step = 1e3 * el / args.steps
     log(f"[bench] FP64 {value:.4e} updates/s, {ms_step:.4f} ms/step, dominant {dom} "
         f"{dom_kt}, profiled-window kernel means {kt}")
-    si = ("stage1", "stage2", "stage3").index(dom)
-    alg = ALG_BYTES["f64"][si] * cells
-    achieved = alg / (dom_kt[0] * 1e-3)
-    roof = {"bound": "hbm", "kernel": f"stage_kernel<0, {si + 1}> (FP64 stage {si + 1})",
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-            "traffic": (read_traffic(f"stage{si + 1}_fp64_two_bubbles_{n}_traffic.json")),
-            "algorithmic_bytes_per_launch": alg, "mean_launch_ms": dom_kt[0],
-            "profiled_launches": dom_kt[1],
+    kernels = kernel_table(kt, "fp64", cells, n)
+    dk = kernels[dom]
+    roof = {"bound": "hbm", "kernel": dk["kernel"], "achieved": dk["achieved_GBps"],
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": dk["hbm_frac"],
+            "traffic": dk["traffic"], "algorithmic_bytes_per_launch": dk["algorithmic_bytes_per_launch"],
+            "mean_launch_ms": dom_kt[0], "profiled_launches": dom_kt[1],
             "timing": "HIP events on the context's stream around every launch of a "
                       "window of eager steps run right after the timed (graph-replay) steps"}
+    if "issue" in dk:
+        roof["issue"] = dk["issue"]
+    step_bytes = sum(ALG_BYTES["f64"][k] for k in kt if k != "wall")
     f.close()
     del f
 
@@ -332,7 +371,7 @@ def main():
         # error against FP64 with a common fixed dt (CFL 0.1 of the
         # initial state) at steps 100 and 1000, whatever --steps is
         dt0 = ic.cfl_dt(p, nodes, 0.1)
-        marks = (100, 1000)
+        marks = tuple(int(x) for x in args.error_steps.split(",") if x)
         outs = {}
         for prec, pp in (("f64", p), ("mixed", pm)):
             a = d.Field(pp, nodes)
@@ -353,17 +392,20 @@ def main():
         del outs
         mixed = {"value": vm, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * elm / args.steps,
                  "speedup_vs_fp64": vm / value,
-                 "hbm_frac_algorithmic": vm * sum(ALG_BYTES["mixed"]) / HBM_PEAK,
                  "kernel_ms": {k: ktm[k][0] for k in ktm},
                  "clock_warmup": {"steps": clockm[0], "seconds": clockm[1]},
                  "dominant_kernel_frac": None,
                  "error_vs_fp64": errs,
                  "error_config": f"same initial state, fixed dt={dt0:.6e} (CFL 0.1 of the "
                                  f"initial state) in both precisions"}
-        sim = ("stage1", "stage2", "stage3").index(domm)
-        mixed["dominant_kernel"] = f"stage_kernel<1, {sim + 1}> (mixed stage {sim + 1})"
-        mixed["dominant_kernel_frac"] = (ALG_BYTES["mixed"][sim] * cells /
-                                         (domm_kt[0] * 1e-3)) / HBM_PEAK
+        mk = kernel_table(ktm, "mixed", cells, n)
+        mixed["kernels"] = mk
+        mixed["dominant_kernel"] = mk[domm]["kernel"]
+        mixed["dominant_kernel_frac"] = mk[domm]["hbm_frac"]
+        step_bytes_m = sum(ALG_BYTES["mixed"][k] for k in ktm if k != "wall")
+        mixed["algorithmic_bytes_per_update"] = step_bytes_m
+        mixed["hbm_frac_algorithmic"] = vm * step_bytes_m / HBM_PEAK
+        mixed["hbm_frac_unfused_bytes"] = vm * 192 / HBM_PEAK
 
     batch = batch_bench(args.batch_steps) if args.batch_steps > 0 else None
     open_bc = open_boundary_bench(args.open_n, args.open_steps) if args.open_steps > 0 else None
@@ -379,8 +421,11 @@ def main():
                                f"RK3-TVD, CFL 0.1 device-side dt, FP64",
                    "grid": [n + 4, n + 4], "physics": "Re=5 We=10 Pr=20 viscous_r=-1.5 cv_r=2.5",
                    "device": dev},
-        "hbm_frac_algorithmic": value * sum(ALG_BYTES["f64"]) / HBM_PEAK,
+        "algorithmic_bytes_per_update": step_bytes,
+        "hbm_frac_algorithmic": value * step_bytes / HBM_PEAK,
+        "hbm_frac_unfused_bytes": value * 256 / HBM_PEAK,
         "kernel_ms": {k: kt[k][0] for k in kt},
+        "kernels": kernels,
         "clock_warmup": {"steps": clock[0], "seconds": clock[1],
                          "note": "untimed steps before the warmup steps, until the GPU "
                                  "clocks reach their sustained level"},

@@ -1,7 +1,8 @@
 """Summaries of rocprofv3 CSV output for the stage kernels.
 
-Two reports, each written as a short Markdown file (and, for traffic, one JSON
-file per FP64 stage kernel that bench.py reads for the roofline's `traffic`):
+Three reports, each written as a short Markdown file (and, for traffic and
+issue counts, one JSON file per stage kernel that bench.py reads for the
+roofline's `traffic` and `issue` entries):
 
   stats:    per-kernel call count and durations from `--kernel-trace --stats`
             (<prefix>_kernel_stats.csv) and, with --trace, the same per kernel
@@ -10,10 +11,14 @@ file per FP64 stage kernel that bench.py reads for the roofline's `traffic`):
   traffic:  HBM bytes per dispatch from two `--pmc` runs (FETCH_SIZE, then
             WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
             reads for a calibration kernel of known size in the same run
+  issue:    instructions per dispatch (SQ_INSTS_VALU, SQ_INSTS_LDS, ...) and
+            wave count from one `--pmc` run, summed over the device
 
     python -m dim2d_finite_volume_amd.rocprof_summary stats STATS.csv OUT.md [--trace TRACE.csv] --command "..."
     python -m dim2d_finite_volume_amd.rocprof_summary traffic FETCH.csv WRITE.csv OUTDIR \
         --calib SUBSTRING:READ_BYTES:WRITE_BYTES --workload two_bubbles_4096 --command "..."
+    python -m dim2d_finite_volume_amd.rocprof_summary issue COUNTERS.csv OUTDIR \
+        --workload two_bubbles_4096 --command "..."
 """
 import argparse
 import csv
@@ -23,9 +28,25 @@ import re
 from collections import defaultdict
 
 STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
+PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)>")
+
+
+def kernel_key(name):
+    """file-name key of a stage kernel: stage<s>_<fp64|mixed> or pair_<fp64|mixed>"""
+    m = STAGE_RE.search(name)
+    if m:
+        return f"stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
+    m = PAIR_RE.search(name)
+    if m:
+        return f"pair_{'mixed' if m.group(1) == '1' else 'fp64'}"
+    return None
 
 
 def short_name(name):
+    m = PAIR_RE.search(name)
+    if m:
+        prec = "mixed" if m.group(1) == "1" else "FP64"
+        return f"stage_pair_kernel<{m.group(1)}> ({prec} stages 1+2)"
     m = STAGE_RE.search(name)
     if m:
         prec = "mixed" if m.group(1) == "1" else "FP64"
@@ -123,16 +144,44 @@ def traffic(args):
         w_kib, _ = write.get(name, (0.0, 0))
         total = f_kib * 1024.0 * (fr or 1.0) + w_kib * 1024.0 * (fw or 1.0)
         lines.append(f"| {short_name(name)} | {n} | {f_kib:.1f} | {w_kib:.1f} | {total:.4e} |")
-        m = STAGE_RE.search(name)
-        if m and m.group(1) == "0":
+        key = kernel_key(name)
+        if key:
             js = {"kernel": short_name(name), "workload": args.workload,
                   "fetch_kib_per_dispatch": f_kib, "write_kib_per_dispatch": w_kib,
                   "fetch_scale": fr, "write_scale": fw, "hbm_bytes_per_launch": total,
                   "command": args.command}
-            with open(os.path.join(args.outdir, f"stage{m.group(2)}_fp64_{args.workload}_traffic.json"),
-                      "w") as f:
+            with open(os.path.join(args.outdir, f"{key}_{args.workload}_traffic.json"), "w") as f:
                 json.dump(js, f, indent=1)
     with open(os.path.join(args.outdir, f"stage_kernels_{args.workload}_traffic.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def issue(args):
+    """instructions per dispatch of the stage kernels (device totals)"""
+    acc = defaultdict(lambda: defaultdict(list))
+    with open(args.csv) as f:
+        for r in csv.DictReader(f):
+            acc[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted(k for k in acc if kernel_key(k))
+    counters = sorted({c for k in names for c in acc[k]})
+    lines = ["# Instructions per dispatch (rocprofv3 --pmc, device totals)", "",
+             f"Command: `{args.command}`", "",
+             "| kernel | dispatches | " + " | ".join(counters) + " |",
+             "|---|---:|" + "---:|" * len(counters)]
+    os.makedirs(args.outdir, exist_ok=True)
+    for name in names:
+        d = acc[name]
+        n = max(len(v) for v in d.values())
+        mean = {c: sum(v) / len(v) for c, v in d.items()}
+        lines.append(f"| {short_name(name)} | {n} | " +
+                     " | ".join(f"{mean.get(c, 0):.4g}" for c in counters) + " |")
+        js = {"kernel": short_name(name), "workload": args.workload, "dispatches": n,
+              "per_dispatch": mean, "command": args.command}
+        with open(os.path.join(args.outdir, f"{kernel_key(name)}_{args.workload}_issue.json"),
+                  "w") as f:
+            json.dump(js, f, indent=1)
+    with open(os.path.join(args.outdir, f"stage_kernels_{args.workload}_issue.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
@@ -153,8 +202,13 @@ def main():
     b.add_argument("--calib", required=True)
     b.add_argument("--workload", required=True)
     b.add_argument("--command", required=True)
+    c = sp.add_parser("issue")
+    c.add_argument("csv")
+    c.add_argument("outdir")
+    c.add_argument("--workload", required=True)
+    c.add_argument("--command", required=True)
     args = ap.parse_args()
-    stats(args) if args.mode == "stats" else traffic(args)
+    {"stats": stats, "traffic": traffic, "issue": issue}[args.mode](args)
 
 
 if __name__ == "__main__":
