@@ -28,7 +28,7 @@ EXPORTS = (
     "dim2d_diagnostics", "dim2d_time", "dim2d_synchronize", "dim2d_set_tiles",
     "dim2d_set_stage_fusion", "dim2d_profile_enable", "dim2d_profile_read",
     "dim2d_get_params", "dim2d_openbc_undermined", "dim2d_extend", "dim2d_adapt_domain",
-    "dim2d_check_steady_state",
+    "dim2d_check_steady_state", "dim2d_time_dev_noalloc",
     "dim2d_batch_create", "dim2d_batch_destroy", "dim2d_batch_upload", "dim2d_batch_download",
     "dim2d_batch_integrate", "dim2d_batch_set_tiles", "dim2d_batch_synchronize",
     "dim2d_batch_autotune",
@@ -83,6 +83,7 @@ def load() -> ctypes.CDLL:
         "dim2d_upload_device": (ctypes.c_int, [vp, vp]),
         "dim2d_download_device": (ctypes.c_int, [vp, vp]),
         "dim2d_time_dev": (ctypes.c_int, [vp, d, dp]),
+        "dim2d_time_dev_noalloc": (ctypes.c_int, [ctypes.POINTER(Params), vp, vp, vp]),
         "dim2d_apply_bc": (ctypes.c_int, [vp, d]),
         "dim2d_rk3_step": (ctypes.c_int, [vp, d, d]),
         "dim2d_integrate": (ctypes.c_int, [vp, d, d, d, i32]),

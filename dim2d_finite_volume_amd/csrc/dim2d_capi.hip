@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -931,6 +932,88 @@ int dim2d_cfl_dt(dim2d_ctx* c, double cfl, double out[4]) {
   out[2] = rmin;
   out[3] = rmax;
   if (h.nonfinite[slot]) return fail(DIM2D_ENONFINITE, "%u non-finite cells", h.nonfinite[slot]);
+  return DIM2D_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// allocation-free time derivative
+// ------------------------------------------------------------------------
+// Device scratch of dim2d_time_dev_noalloc: the interleaved state and its
+// time derivative (two fields of pitch x ny cells, 32 B per cell, each
+// starting kShift64 doubles past a 128-byte line like the contexts' fields).
+// Every GPU the library runs on holds one copy for the life of the process.
+// In host code `scratch` names a host-side stand-in, never device memory: the
+// device address of each GPU's copy comes from hipGetSymbolAddress.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// the current device's copy of `scratch`, looked up on the first call made
+// on that device
+int scratch_for_current_device(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(DIM2D_EINVAL, "device %d beyond the %d devices the scratch table holds", dev,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(DIM2D_EDEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return DIM2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dim2d_time_dev_noalloc(const dim2d_params* params, const double* d_nodes, double* d_time_dev,
+                           void* stream) {
+  int st = validate(params);
+  if (st) return st;
+  if (!d_nodes || !d_time_dev) return fail(DIM2D_EINVAL, "NULL device array");
+  const DevParams p = make_dev_params(*params);
+  const size_t n = (size_t)p.ncell * 4;  // doubles per interleaved field
+  const size_t field = (n + kShift64 + 15) / 16 * 16;  // whole 128-byte lines
+  if (2 * field > kScratchDoubles)
+    return fail(DIM2D_EINVAL,
+                "grid %dx%d needs %zu bytes of scratch, more than the %zu bytes "
+                "dim2d_time_dev_noalloc holds (at most about %zu cells of pitch x ny)",
+                params->nx, params->ny, 2 * field * sizeof(double), kScratchBytes,
+                kScratchDoubles / 8);
+  double* base = nullptr;
+  if ((st = scratch_for_current_device(&base))) return st;
+  double* U = base + kShift64;
+  double* D = base + field + kShift64;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // every cell of U is written by the pack, every interior cell of D by the
+  // stage kernel (the frame too with open edges), every cell of the output
+  // by the unpack: nothing is read that this call did not write
+  HIP_TRY(launch_pack(p, d_nodes, U, s));
+  StageArgs a;
+  memset(&a, 0, sizeof a);
+  a.U = U;
+  a.S64 = U;
+  a.O64 = D;
+  const bool open = is_open(p);
+  if (params->operators == DIM2D_OPS_COCKBURN_GAU) HIP_TRY(launch_cg_stage(p, 0, a, s));
+  else HIP_TRY(launch_stage(p, 0, 0, a, default_block_x(p), default_rows(p), s));
+  if (open) HIP_TRY(launch_frame(p, 0, 0, a, s));
+  HIP_TRY(launch_unpack(p, D, d_time_dev, open ? 0 : 1, kDebugReal, s));
   return DIM2D_OK;
 }
 

@@ -109,6 +109,21 @@ int dim2d_download_device(dim2d_ctx* ctx, double* d_nodes);
  * including bc_operators_gen%apply_bc_on_fluxes for wall edges (:171-177). */
 int dim2d_time_dev(dim2d_ctx* ctx, double t, double* time_dev);
 
+/* dim2d_time_dev without a context and without allocating: the time
+ * derivative of the planar device array d_nodes(nx,ny,4) into the planar
+ * device array d_time_dev(nx,ny,4) (ghost cells set to debug_real unless the
+ * edges are open), FP64 whatever params->precision says, stream-ordered on
+ * `stream` (a hipStream_t; NULL for the default stream) with no host
+ * synchronisation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory, such as code under HIP graph capture and
+ * tight serving loops.  Its working memory is a 64 MiB device array each GPU
+ * holds for the life of the process, enough for about 1 million cells of
+ * pitch x ny (pitch = nx rounded up to 32), e.g. 1020 x 1020 points;
+ * larger grids return DIM2D_EINVAL.  Calls on one GPU share that array:
+ * calls that overlap on one GPU (other streams, other host threads) race. */
+int dim2d_time_dev_noalloc(const dim2d_params* params, const double* d_nodes, double* d_time_dev,
+                           void* stream);
+
 /* Fill the ghost cells of the device state (bc_operators_gen%apply_bc_on_nodes,
  * bc_operators_gen_class.f:238-381).  Reflection and periodic ghosts are also
  * resolved on the fly inside the stage kernels. */
