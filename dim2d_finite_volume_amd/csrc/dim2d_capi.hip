@@ -214,12 +214,14 @@ struct dim2d_ctx {
   // (bit k: kind k = stage 1, 2, 3, boundary kernels)
   bool profiling = false;
   int profile_mask = 0;
-  // dim2d_integrate replays two captured steps per launch when not profiling;
-  // one graph per step parity (which buffer holds u^n and which reduction slot
-  // a step reads alternate with it), valid for the dt/cfl/tiles it was made with
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  double g_dt[2] = {0.0, 0.0};
-  int g_cfl[2] = {0, 0};
+  // dim2d_integrate replays captured steps when not profiling: graphs of 8
+  // steps, then of 2 steps for the remainder ([0]: 2 steps, [1]: 8 steps);
+  // one graph per step parity (which buffer holds u^n and which reduction
+  // slot a step reads alternate with it), valid for the dt/cfl/tiles it was
+  // made with
+  hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  double g_dt[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  int g_cfl[2][2] = {{0, 0}, {0, 0}};
   struct Timed {
     hipEvent_t a, b;
     int kind;
@@ -255,11 +257,12 @@ hipError_t timed_launch(dim2d_ctx* c, int kind, F&& launch) {
 }
 
 void drop_graphs(dim2d_ctx* c) {
-  for (int k = 0; k < 2; ++k)
-    if (c->gexec[k]) {
-      (void)hipGraphExecDestroy(c->gexec[k]);
-      c->gexec[k] = nullptr;
-    }
+  for (int g = 0; g < 2; ++g)
+    for (int k = 0; k < 2; ++k)
+      if (c->gexec[g][k]) {
+        (void)hipGraphExecDestroy(c->gexec[g][k]);
+        c->gexec[g][k] = nullptr;
+      }
 }
 
 void release_timed(dim2d_ctx* c) {
@@ -742,28 +745,33 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
       c->red_valid = true;
     }
   }
-  // pairs of steps from a captured graph: no host launch gaps inside a pair.
-  // Not while profiling (event pairs around launches).  With open edges the
-  // frame kernels' fork and join events take the side stream into the
-  // capture, so the graph holds them as parallel branches.
+  // steps replayed from captured graphs, 8 at a time, then 2 at a time: no
+  // host launch gaps inside a graph, one (~8 us) between graphs.  Not while
+  // profiling (event pairs around launches).  With open edges the frame
+  // kernels' fork and join events take the side stream into the capture, so
+  // the graph holds them as parallel branches.
   int s = 0;
   const bool graphs = !c->profiling;
-  for (; graphs && s + 1 < nsteps; s += 2) {
+  constexpr int kGraphSteps[2] = {2, 8};
+  while (graphs) {
+    const int gi = nsteps - s >= kGraphSteps[1] ? 1 : nsteps - s >= kGraphSteps[0] ? 0 : -1;
+    if (gi < 0) break;
+    const int len = kGraphSteps[gi];
     const int k = (int)(c->steps & 1);
-    if (c->gexec[k] && (c->g_dt[k] != dt_fixed || c->g_cfl[k] != cfl_mode)) {
-      (void)hipGraphExecDestroy(c->gexec[k]);
-      c->gexec[k] = nullptr;
+    hipGraphExec_t& gx = c->gexec[gi][k];
+    if (gx && (c->g_dt[gi][k] != dt_fixed || c->g_cfl[gi][k] != cfl_mode)) {
+      (void)hipGraphExecDestroy(gx);
+      gx = nullptr;
     }
-    if (!c->gexec[k]) {
-      // capture two steps; run_step's bookkeeping (buffer swap, step count)
+    if (!gx) {
+      // capture `len` steps; run_step's bookkeeping (buffer swap, step count)
       // is undone afterwards, the graph launch below makes the steps happen
       double* u = c->U;
       double* u2 = c->U2;
       const long long steps0 = c->steps;
       const bool rv0 = c->red_valid;
       HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      st = run_step(c, dt_fixed, cfl_mode);
-      if (!st) st = run_step(c, dt_fixed, cfl_mode);
+      for (int q = 0; q < len && !st; ++q) st = run_step(c, dt_fixed, cfl_mode);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(c->stream, &g);
       c->U = u;
@@ -775,15 +783,16 @@ int dim2d_integrate(dim2d_ctx* c, double t0, double dt_fixed, double cfl, int32_
         if (st) return st;
         HIP_TRY(e);
       }
-      const hipError_t ei = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+      const hipError_t ei = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIP_TRY(ei);
-      c->g_dt[k] = dt_fixed;
-      c->g_cfl[k] = cfl_mode;
+      c->g_dt[gi][k] = dt_fixed;
+      c->g_cfl[gi][k] = cfl_mode;
     }
-    HIP_TRY(hipGraphLaunch(c->gexec[k], c->stream));
-    c->steps += 2;  // two buffer swaps: U and U2 are where they were
+    HIP_TRY(hipGraphLaunch(gx, c->stream));
+    c->steps += len;  // an even number of buffer swaps: U and U2 are where they were
     c->red_valid = true;
+    s += len;
   }
   for (; s < nsteps; ++s) {
     st = run_step(c, dt_fixed, cfl_mode);

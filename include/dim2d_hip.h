@@ -138,9 +138,10 @@ int dim2d_apply_bc(dim2d_ctx* ctx, double t);
 int dim2d_rk3_step(dim2d_ctx* ctx, double t, double dt);
 
 /* nsteps steps of the context's integrator without host synchronisation
- * (pairs of steps are replayed from a captured HIP graph, one per step parity,
- * unless profiling is on; with open edges the frame kernels are branches of
- * the graph beside the stage kernels).
+ * (steps are replayed from captured HIP graphs of 8 steps, then of 2 for the
+ * remainder, one graph per step parity, unless profiling is on; with open
+ * edges the frame kernels are branches of the graph beside the stage
+ * kernels).
  * cfl > 0: dt of each step from the CFL rule below, evaluated on the device
  *          (fused into the third stage of the previous step); dt_fixed ignored.
  *          In DIM2D_PREC_MIXED that evaluation takes c and |u| in FP32 after

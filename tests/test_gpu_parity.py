@@ -563,17 +563,18 @@ def test_batch_mixed_edges_matches_single(orc, prec):
 @pytest.mark.parametrize("case,prec", [("periodic_x_wall_y", 0), ("reflection", 1), ("open", 0),
                                        ("open", 1)])
 def test_graph_replay_matches_eager(orc, case, prec):
-    """dim2d_integrate replays pairs of steps from captured HIP graphs (one per
-    step parity); profiling makes it launch every step eagerly.  Several
-    replays with host work in between, odd step counts and a dt change give
-    bitwise the eager results, fixed-dt and CFL."""
+    """dim2d_integrate replays steps from captured HIP graphs of 8 and of 2
+    steps (one per step parity); profiling makes it launch every step
+    eagerly.  Several replays with host work in between, odd step counts,
+    counts that take 8-step graphs and a dt change give bitwise the eager
+    results, fixed-dt and CFL."""
     d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], precision=prec)
     dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
     g = d.Field(p, nodes.copy())
     e = d.Field(p, nodes.copy())
     e.profile(True, kinds=["stage1"])
     for n, kw in ((4, {"dt": dt}), (5, {"dt": dt}), (2, {"dt": 0.5 * dt}), (7, {"cfl": 0.1}),
-                  (3, {"cfl": 0.1})):
+                  (3, {"cfl": 0.1}), (19, {"cfl": 0.1}), (10, {"dt": dt}), (8, {"dt": dt})):
         g.run(n, **kw)
         a = g.download()
         e.run(n, **kw)
