@@ -1095,6 +1095,40 @@ static void obc_eigenqties_bc(const orc_model* m, const double v[4], double pe[5
   pe[4] = orc_speed_of_sound_local(m, v);
 }
 
+/* eigenvalues and left/right eigenvectors of the primitive variables
+ * [rho, u, v, P] in direction x (ydir 0) or y (ydir 1), ns_vdw2d_prim_module.f:
+ * compute_x_eigenvalues/left/righteigenvector_ns_vdw2d :504-722,
+ * compute_y_... :539-891; un is the velocity along the direction.  Fortran
+ * matrices A(row,col) are stored as A[row-1][col-1]. */
+static void ns_vdw2d_eigen(int ydir, double md, double un, double c, double lam[4],
+                           double L[4][4], double R[4][4]) {
+  const double q_c = 0.5 * md * c;
+  const double inv_c = 1.0 / (c * c);
+  const double inv_q_c = 1.0 / (md * c);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) L[a][b] = R[a][b] = 0.0;
+  lam[0] = un; lam[1] = un; lam[2] = un - c; lam[3] = un + c;
+  if (!ydir) {
+    L[2][0] = 1.0;
+    L[0][1] = 1.0; L[3][1] = -1.0 / (c * c);
+    L[1][2] = -q_c; L[3][2] = 0.5;
+    L[1][3] = q_c; L[3][3] = 0.5;
+    R[1][0] = 1.0; R[2][0] = inv_c; R[3][0] = inv_c;
+    R[2][1] = -inv_q_c; R[3][1] = inv_q_c;
+    R[0][2] = 1.0;
+    R[2][3] = 1.0; R[3][3] = 1.0;
+  } else {
+    L[1][0] = 1.0;
+    L[0][1] = 1.0; L[3][1] = -1.0 / (c * c);
+    L[2][2] = -q_c; L[3][2] = 0.5;
+    L[2][3] = q_c; L[3][3] = 0.5;
+    R[1][0] = 1.0; R[2][0] = inv_c; R[3][0] = inv_c;
+    R[0][1] = 1.0;
+    R[2][2] = -inv_q_c; R[3][2] = inv_q_c;
+    R[2][3] = 1.0; R[3][3] = 1.0;
+  }
+}
+
 /* compute_timedev_with_openbc (hedstrom_xy_module.f:445-558) with the
  * eigen-quantities of ns_vdw2d_prim_module.f:249-891 (Fortran matrices
  * A(row,col) stored as A[row-1][col-1]) */
@@ -1108,34 +1142,9 @@ void orc_openbc_timedev(const orc_model* m, const double* n, int i, int j, int y
     obc_eigenqties_bc(m, v, pe);
   const double md = pe[0], ux = pe[1], uy = pe[2], P = pe[3], c = pe[4];
   double lam[4], L[4][4], R[4][4], J[4][4];
-  const double q_c = 0.5 * md * c;
-  const double inv_c = 1.0 / (c * c);
-  const double inv_q_c = 1.0 / (md * c);
   for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b) L[a][b] = R[a][b] = J[a][b] = 0.0;
-  if (!ydir) {
-    /* compute_x_eigenvalues/left/righteigenvector_ns_vdw2d :504-722 */
-    lam[0] = ux; lam[1] = ux; lam[2] = ux - c; lam[3] = ux + c;
-    L[2][0] = 1.0;
-    L[0][1] = 1.0; L[3][1] = -1.0 / (c * c);
-    L[1][2] = -q_c; L[3][2] = 0.5;
-    L[1][3] = q_c; L[3][3] = 0.5;
-    R[1][0] = 1.0; R[2][0] = inv_c; R[3][0] = inv_c;
-    R[2][1] = -inv_q_c; R[3][1] = inv_q_c;
-    R[0][2] = 1.0;
-    R[2][3] = 1.0; R[3][3] = 1.0;
-  } else {
-    /* compute_y_eigenvalues/left/righteigenvector_ns_vdw2d :539-891 */
-    lam[0] = uy; lam[1] = uy; lam[2] = uy - c; lam[3] = uy + c;
-    L[1][0] = 1.0;
-    L[0][1] = 1.0; L[3][1] = -1.0 / (c * c);
-    L[2][2] = -q_c; L[3][2] = 0.5;
-    L[2][3] = q_c; L[3][3] = 0.5;
-    R[1][0] = 1.0; R[2][0] = inv_c; R[3][0] = inv_c;
-    R[0][1] = 1.0;
-    R[2][2] = -inv_q_c; R[3][2] = inv_q_c;
-    R[2][3] = 1.0; R[3][3] = 1.0;
-  }
+    for (int b = 0; b < 4; ++b) J[a][b] = 0.0;
+  ns_vdw2d_eigen(ydir, md, ydir ? uy : ux, c, lam, L, R);
   /* compute_jacobian_cons_to_prim_ns_vdw2d :249-320 */
   const double dEdrho = 0.5 * (ux * ux + uy * uy) - 6.0 * md -
                         1.0 / 3.0 * m->cv_r * (P + 9.0 * (md - 2.0) * md);
@@ -1706,4 +1715,545 @@ void orc_sums(const orc_model* m, const double* nodes, double out[4]) {
       for (int i = 1 + BC_SIZE; i <= m->nx - BC_SIZE; ++i) s += ND(nodes, i, j, k);
     out[k - 1] = s;
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* new grid points of the buffer layers: the characteristic            */
+/* extrapolation of bf_layer_newgrdpt/bf_newgrdpt_class.f              */
+/* ------------------------------------------------------------------ */
+
+#define GN(g, i, j, k) \
+  (g)->nodes[((size_t)((k) - 1) * (size_t)(g)->ny + (size_t)((j) - 1)) * (size_t)(g)->nx + \
+             (size_t)((i) - 1)]
+
+/* compute_prim_var_ns_vdw2d, ns_vdw2d_prim_module.f:51-64 */
+static void prim_var(const orc_model* m, const double v[4], double p[4]) {
+  p[0] = v[0];
+  p[1] = v[1] / v[0];
+  p[2] = v[2] / v[0];
+  p[3] = classical_pressure_local(m, v);
+}
+
+/* compute_cons_var_ns_vdw2d, ns_vdw2d_prim_module.f:83-108 */
+static void cons_var(const orc_model* m, const double p[4], double v[4]) {
+  const double T = (p[3] + 3.0 * sq(p[0])) * (3.0 - p[0]) / (8.0 * p[0]);
+  v[0] = p[0];
+  v[1] = p[0] * p[1];
+  v[2] = p[0] * p[2];
+  v[3] = 0.5 * p[0] * (sq(p[1]) + sq(p[2])) + p[0] * (8.0 / 3.0 * m->cv_r * T - 3.0 * p[0]);
+}
+
+/* compute_body_forces(..., prim=.true.), pmodel_eq_class.f:1840-1918 */
+static void body_forces_prim(const orc_model* m, double b[4]) {
+  b[0] = 0.0;
+  b[1] = 0.0;
+  b[2] = m->gravity_ac ? -m->gravity : 0.0;
+  b[3] = 0.0;
+}
+
+/* get_prim_obc_eigenqties, pmodel_eq_class.f:2142-2195: [rho, ux, uy, P, c]
+ * of the point (obc_eigenqties_bc) or of the far field (obc_eigenqties_lin) */
+static void obc_eigenqties(const orc_model* m, const double ff[4], int eigen_lin,
+                           const double v[4], double pe[5]) {
+  obc_eigenqties_bc(m, eigen_lin ? ff : v, pe);
+}
+
+/* n_coords_module.f: the diagonal coordinates.  get_x_coord and get_y_coord
+ * are restated as written there (sqrt(2)/4 (n1 +- n2)); they only place the
+ * far-field evaluation, which is uniform here. */
+static double n1_coord(double x, double y) { return sqrt(2.0) * 0.5 * (x - y); }
+static double n2_coord(double x, double y) { return sqrt(2.0) * 0.5 * (x + y); }
+
+/* compute_xy_to_n_var / compute_n_to_xy_var, pmodel_eq_class.f:3173-3231 */
+static void xy_to_n(const double a[4], double b[4]) {
+  const double a1 = a[1], a2 = a[2];
+  b[0] = a[0];
+  b[1] = 0.5 * sqrt(2.0) * (a1 - a2);
+  b[2] = 0.5 * sqrt(2.0) * (a1 + a2);
+  b[3] = a[3];
+}
+static void n_to_xy(const double a[4], double b[4]) {
+  const double a1 = a[1], a2 = a[2];
+  b[0] = a[0];
+  b[1] = 0.5 * sqrt(2.0) * (a1 + a2);
+  b[2] = 0.5 * sqrt(2.0) * (a2 - a1);
+  b[3] = a[3];
+}
+
+/* compute_x_transM_ns_vdw2d(md, uy, c) and compute_y_transM_ns_vdw2d(md, ux,
+ * c), ns_vdw2d_prim_module.f:345-482, applied as MATMUL(gradient, transM) */
+static void trans_term(int ydir, const double pe[5], const double g[4], double out[4]) {
+  const double md = pe[0], c = pe[4];
+  if (!ydir) {
+    const double uy = pe[2];
+    out[0] = g[0] * uy + g[2] * md;
+    out[1] = g[1] * uy;
+    out[2] = g[2] * uy + g[3] * (1.0 / md);
+    out[3] = g[2] * (c * c * md) + g[3] * uy;
+  } else {
+    const double ux = pe[1];
+    out[0] = g[0] * ux + g[1] * md;
+    out[1] = g[1] * ux + g[3] * (1.0 / md);
+    out[2] = g[2] * ux;
+    out[3] = g[1] * (c * c * md) + g[3] * ux;
+  }
+}
+
+/* the gradient procedures the new-point procedures pass
+ * (sd_operators_fd_module.f: gradient_{x,y}_interior and the one-sided
+ * gradient_x_x_oneside_{L0,R0}, gradient_y_y_oneside_{L0,R0}) */
+static grad_fn gradient_proc(int ydir, int type) {
+  if (!ydir)
+    return type == ORC_GRAD_I ? gradient_x_interior
+                              : (type == ORC_GRAD_L0 ? gradient_x_x_oneside_L0 : gradient_x_x_oneside_R0);
+  return type == ORC_GRAD_I ? gradient_y_interior
+                            : (type == ORC_GRAD_L0 ? gradient_y_y_oneside_L0 : gradient_y_y_oneside_R0);
+}
+
+/* compute_gradient_prim, pmodel_eq_class.f:3113-3153 */
+static void gradient_prim(const orc_model* m, const orc_bf_grid* g, int i, int j, grad_fn gr,
+                          double dn, double out[4]) {
+  orc_model mg = *m;
+  mg.nx = g->nx;
+  mg.ny = g->ny;
+  out[0] = gr(&mg, g->nodes, i, j, orc_mass_density, dn);
+  out[1] = gr(&mg, g->nodes, i, j, orc_velocity_x, dn);
+  out[2] = gr(&mg, g->nodes, i, j, orc_velocity_y, dn);
+  out[3] = gr(&mg, g->nodes, i, j, orc_classical_pressure, dn);
+}
+
+static void grid_point(const orc_bf_grid* g, int i, int j, double v[4]) {
+  for (int k = 1; k <= 4; ++k) v[k - 1] = GN(g, i, j, k);
+}
+
+/* get_interpolation_coeff_1D, bf_newgrdpt_class.f:2560-2590: per variable,
+ * the slope and intercept of the line through (x_map(1), nodes(1,k)) and
+ * (x_map(2), nodes(2,k)) */
+void orc_interp_coeff_1d(const double x_map[2], const double nodes[8], double coeff[8]) {
+  for (int k = 0; k < 4; ++k) {
+    coeff[k] = (nodes[4 + k] - nodes[k]) / (x_map[1] - x_map[0]);
+    coeff[4 + k] = nodes[k] - coeff[k] * x_map[0];
+  }
+}
+
+/* interpolate_1D, :2592-2612 */
+void orc_interpolate_1d(double x, const double coeff[8], double out[4]) {
+  for (int k = 0; k < 4; ++k) out[k] = x * coeff[k] + coeff[4 + k];
+}
+
+/* get_interpolation_coeff_2D with get_plane_normal_vector and cross_product,
+ * :2665-2858: the plane through the three points (x(l), y(l), nodes(l,k)) as
+ * z = c1 x + c2 y + c3 from the normal n = AB x AC */
+void orc_interp_coeff_2d(const double x_map[3], const double y_map[3], const double nodes[12],
+                         double coeff[12]) {
+  for (int k = 0; k < 4; ++k) {
+    const double A[3] = {x_map[0], y_map[0], nodes[k]};
+    const double B[3] = {x_map[1], y_map[1], nodes[4 + k]};
+    const double C[3] = {x_map[2], y_map[2], nodes[8 + k]};
+    double AB[3], AC[3];
+    for (int q = 0; q < 3; ++q) {
+      AB[q] = B[q] - A[q];
+      AC[q] = C[q] - A[q];
+    }
+    const double n0 = AB[1] * AC[2] - AB[2] * AC[1];
+    const double n1 = -AB[0] * AC[2] + AB[2] * AC[0];
+    const double n2 = AB[0] * AC[1] - AB[1] * AC[0];
+    coeff[k] = -n0 / n2;
+    coeff[4 + k] = -n1 / n2;
+    coeff[8 + k] = -A[0] * coeff[k] - A[1] * coeff[4 + k] + A[2];
+  }
+}
+
+/* interpolate_2D, :2767-2790 */
+void orc_interpolate_2d(double x, double y, const double coeff[12], double out[4]) {
+  for (int k = 0; k < 4; ++k) out[k] = x * coeff[k] + y * coeff[4 + k] + coeff[8 + k];
+}
+
+/* compute_NewtonCotes_integration, :2858-2875: trapezoidal rule over dt */
+void orc_newton_cotes(const double d0[4], const double d1[4], double dt, double out[4]) {
+  for (int k = 0; k < 4; ++k) out[k] = (d0[k] + 0.5 * (d1[k] - d0[k])) * dt;
+}
+
+/* the far-field state in the primitive variables and its body forces, for
+ * an incoming characteristic (the else branches of :1300-1320) */
+static void far_field_amp(const orc_model* m, const double ff[4], int rotate, double n_amp[4],
+                          double t_amp[4], double b_amp[4]) {
+  double p[4], b[4];
+  prim_var(m, ff, p);
+  body_forces_prim(m, b);
+  if (rotate) {
+    xy_to_n(p, n_amp);
+    xy_to_n(b, b_amp);
+  } else {
+    for (int k = 0; k < 4; ++k) {
+      n_amp[k] = p[k];
+      b_amp[k] = b[k];
+    }
+  }
+  for (int k = 0; k < 4; ++k) t_amp[k] = 0.0;
+}
+
+/* compute_newgrdpt_x (ydir 0), :1029-1361, and compute_newgrdpt_y (ydir 1),
+ * :1417-1746, in one body: the direction swaps the roles of the indices, the
+ * coordinate maps, the transverse gradient and the eigen-decomposition. */
+static void newgrdpt_dir(const orc_model* m, const double ff[4], int eigen_lin, double dt,
+                         const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1, int ydir,
+                         int side_right, int grad_type, double out[4]) {
+  const int a_n = ydir ? g1->align_y - g0->align_y : g1->align_x - g0->align_x;
+  const int a_t = ydir ? g1->align_x - g0->align_x : g1->align_y - g0->align_y;
+  const int p1 = ydir ? j1 : i1;  /* index along the direction */
+  const int q1 = ydir ? i1 : j1;  /* transverse index */
+  int p_eigen, p0a, p0b, p1a, p1b;
+  if (side_right) {
+    p_eigen = p1 - 1;
+    p0a = a_n + p_eigen - 1;
+    p0b = a_n + p_eigen;
+    p1a = p_eigen - 1;
+    p1b = p_eigen;
+  } else {
+    p_eigen = p1 + 1;
+    p0a = a_n + p_eigen;
+    p0b = a_n + p_eigen + 1;
+    p1a = p_eigen;
+    p1b = p_eigen + 1;
+  }
+  const int q0 = a_t + q1;
+  const double* map0 = ydir ? g0->y_map : g0->x_map;
+  const double* map1 = ydir ? g1->y_map : g1->x_map;
+  const double* tmap1 = ydir ? g1->x_map : g1->y_map;
+  const double dt_n = tmap1[1] - tmap1[0]; /* spacing of the transverse gradient */
+  const grad_fn gr = gradient_proc(!ydir, grad_type);
+  const double s1 = map1[p1 - 1];
+#define PT(g, pp, qq, v) (ydir ? grid_point(g, qq, pp, v) : grid_point(g, pp, qq, v))
+#define GRAD(g, pp, qq, o) \
+  (ydir ? gradient_prim(m, g, qq, pp, gr, dt_n, o) : gradient_prim(m, g, pp, qq, gr, dt_n, o))
+
+  double xm[2], pts[8], v[4], pe[5], gv[4];
+  double inter_nodes0[8], inter_trans0[8], inter_bf0[8], inter_trans1[8], inter_bf1[8];
+  /* time t - dt */
+  xm[0] = map0[p0a - 1];
+  xm[1] = map0[p0b - 1];
+  PT(g0, p0a, q0, v);
+  prim_var(m, v, pts);
+  PT(g0, p0b, q0, v);
+  prim_var(m, v, pts + 4);
+  orc_interp_coeff_1d(xm, pts, inter_nodes0);
+  for (int l = 0; l < 2; ++l) {
+    const int pp = l ? p0b : p0a;
+    PT(g0, pp, q0, v);
+    obc_eigenqties(m, ff, eigen_lin, v, pe);
+    GRAD(g0, pp, q0, gv);
+    trans_term(ydir, pe, gv, pts + 4 * l);
+  }
+  orc_interp_coeff_1d(xm, pts, inter_trans0);
+  body_forces_prim(m, pts);
+  body_forces_prim(m, pts + 4);
+  orc_interp_coeff_1d(xm, pts, inter_bf0);
+  /* time t */
+  xm[0] = map1[p1a - 1];
+  xm[1] = map1[p1b - 1];
+  for (int l = 0; l < 2; ++l) {
+    const int pp = l ? p1b : p1a;
+    PT(g1, pp, q1, v);
+    obc_eigenqties(m, ff, eigen_lin, v, pe);
+    GRAD(g1, pp, q1, gv);
+    trans_term(ydir, pe, gv, pts + 4 * l);
+  }
+  orc_interp_coeff_1d(xm, pts, inter_trans1);
+  body_forces_prim(m, pts);
+  body_forces_prim(m, pts + 4);
+  orc_interp_coeff_1d(xm, pts, inter_bf1);
+  double t_amp1[4], b_amp1[4];
+  orc_interpolate_1d(s1, inter_trans1, t_amp1);
+  orc_interpolate_1d(s1, inter_bf1, b_amp1);
+  /* eigen-decomposition at the neighbour (p_eigen, q1) at time t */
+  PT(g1, p_eigen, q1, v);
+  obc_eigenqties(m, ff, eigen_lin, v, pe);
+  double lam[4], L[4][4], R[4][4];
+  ns_vdw2d_eigen(ydir, pe[0], ydir ? pe[2] : pe[1], pe[4], lam, L, R);
+  double char_amp[4];
+  for (int k = 0; k < 4; ++k) {
+    const double s0 = s1 - lam[k] * dt;
+    const int inside = side_right ? (lam[k] >= 0) : !(lam[k] > 0);
+    double n_amp0[4], t_amp0[4], b_amp0[4];
+    if (inside) {
+      orc_interpolate_1d(s0, inter_nodes0, n_amp0);
+      orc_interpolate_1d(s0, inter_trans0, t_amp0);
+      orc_interpolate_1d(s0, inter_bf0, b_amp0);
+    } else {
+      far_field_amp(m, ff, 0, n_amp0, t_amp0, b_amp0);
+    }
+    double d0[4], d1[4], integ[4];
+    for (int l = 0; l < 4; ++l) {
+      d0[l] = t_amp0[l] - b_amp0[l];
+      d1[l] = t_amp1[l] - b_amp1[l];
+    }
+    orc_newton_cotes(d0, d1, dt, integ);
+    double acc = 0.0;
+    for (int l = 0; l < 4; ++l) acc += (n_amp0[l] - integ[l]) * L[l][k];
+    char_amp[k] = acc;
+  }
+  double prim[4];
+  for (int col = 0; col < 4; ++col) {
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k) acc += char_amp[k] * R[k][col];
+    prim[col] = acc;
+  }
+  cons_var(m, prim, out);
+#undef PT
+#undef GRAD
+}
+
+void orc_newgrdpt_x(const orc_model* m, const double ff[4], int eigen_lin, double dt,
+                    const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1, int side_right,
+                    int grad_y, double out[4]) {
+  newgrdpt_dir(m, ff, eigen_lin, dt, g0, g1, i1, j1, 0, side_right, grad_y, out);
+}
+
+void orc_newgrdpt_y(const orc_model* m, const double ff[4], int eigen_lin, double dt,
+                    const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1, int side_right,
+                    int grad_x, double out[4]) {
+  newgrdpt_dir(m, ff, eigen_lin, dt, g0, g1, i1, j1, 1, side_right, grad_x, out);
+}
+
+/* get_n_transverse_data_for_interpolation, :2402-2556: at each of the three
+ * points, the gradient across the direction n (the n2-derivative for n1,
+ * the n1-derivative for n2) of the primitive variables, its velocity turned
+ * into (n1, n2) components, times the transverse matrix of the point's
+ * eigen-quantities in (n1, n2) components */
+static void n_transverse(const orc_model* m, const double ff[4], int eigen_lin,
+                         const orc_bf_grid* g, const int idx[6], int n_dir, const int grads[6],
+                         double dx, double dy, double nodes_inter[12]) {
+  for (int k = 0; k < 3; ++k) {
+    const int ix = idx[2 * k], iy = idx[2 * k + 1];
+    double xg[4], yg[4], ngp[4], ng[4], v[4], pe[5], pn[4];
+    gradient_prim(m, g, ix, iy, gradient_proc(0, grads[2 * k]), dx, xg);
+    gradient_prim(m, g, ix, iy, gradient_proc(1, grads[2 * k + 1]), dy, yg);
+    for (int l = 0; l < 4; ++l)
+      ngp[l] = n_dir == ORC_N1 ? n2_coord(xg[l], yg[l]) : n1_coord(xg[l], yg[l]);
+    xy_to_n(ngp, ng);
+    grid_point(g, ix, iy, v);
+    obc_eigenqties(m, ff, eigen_lin, v, pe);
+    xy_to_n(pe, pn);
+    for (int l = 0; l < 4; ++l) pe[l] = pn[l];
+    trans_term(n_dir == ORC_N1 ? 0 : 1, pe, ng, nodes_inter + 4 * k);
+  }
+}
+
+/* compute_newgrdpt_xy, :1825-2400: characteristic extrapolation along the
+ * diagonal direction n_dir (ORC_N1: n1 = (x - y)/sqrt2, ORC_N2: n2 = (x +
+ * y)/sqrt2) from a plane through three points.  grads: the x and y gradient
+ * procedures (ORC_GRAD_*) of points 1, 2, 3; eigen_idx: (i, j) of the
+ * eigen-decomposition point in grid 1; inter_idx1: (i, j) of the three
+ * points in grid 1. */
+void orc_newgrdpt_xy(const orc_model* m, const double ff[4], int eigen_lin, double dt,
+                     const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1, int n_dir,
+                     int side_right, const int grads[6], const int eigen_idx[2],
+                     const int inter_idx1[6], double out[4]) {
+  const double x1 = g1->x_map[i1 - 1], y1 = g1->y_map[j1 - 1];
+  const double dx = g1->x_map[1] - g1->x_map[0];
+  const double dy = g1->y_map[1] - g1->y_map[0];
+  const double n1_1 = n1_coord(x1, y1), n2_1 = n2_coord(x1, y1);
+  int inter_idx0[6];
+  for (int k = 0; k < 3; ++k) {
+    inter_idx0[2 * k] = g1->align_x - g0->align_x + inter_idx1[2 * k];
+    inter_idx0[2 * k + 1] = g1->align_y - g0->align_y + inter_idx1[2 * k + 1];
+  }
+  double n1i[3], n2i[3], nodes_inter[12], v[4], p[4], b[4];
+  double inter_nodes0[12], inter_trans0[12], inter_bf0[12], inter_trans1[12], inter_bf1[12];
+  for (int k = 0; k < 3; ++k) {
+    const int ix = inter_idx0[2 * k], iy = inter_idx0[2 * k + 1];
+    const double x = g0->x_map[ix - 1], y = g0->y_map[iy - 1];
+    n1i[k] = n1_coord(x, y);
+    n2i[k] = n2_coord(x, y);
+    grid_point(g0, ix, iy, v);
+    prim_var(m, v, p);
+    xy_to_n(p, nodes_inter + 4 * k);
+  }
+  orc_interp_coeff_2d(n1i, n2i, nodes_inter, inter_nodes0);
+  n_transverse(m, ff, eigen_lin, g0, inter_idx0, n_dir, grads, dx, dy, nodes_inter);
+  orc_interp_coeff_2d(n1i, n2i, nodes_inter, inter_trans0);
+  for (int k = 0; k < 3; ++k) {
+    body_forces_prim(m, b);
+    xy_to_n(b, nodes_inter + 4 * k);
+  }
+  orc_interp_coeff_2d(n1i, n2i, nodes_inter, inter_bf0);
+  for (int k = 0; k < 3; ++k) {
+    const int ix = inter_idx1[2 * k], iy = inter_idx1[2 * k + 1];
+    const double x = g1->x_map[ix - 1], y = g1->y_map[iy - 1];
+    n1i[k] = n1_coord(x, y);
+    n2i[k] = n2_coord(x, y);
+    body_forces_prim(m, b);
+    xy_to_n(b, nodes_inter + 4 * k);
+  }
+  orc_interp_coeff_2d(n1i, n2i, nodes_inter, inter_bf1);
+  n_transverse(m, ff, eigen_lin, g1, inter_idx1, n_dir, grads, dx, dy, nodes_inter);
+  orc_interp_coeff_2d(n1i, n2i, nodes_inter, inter_trans1);
+  double t_amp1[4], b_amp1[4];
+  orc_interpolate_2d(n1_1, n2_1, inter_trans1, t_amp1);
+  orc_interpolate_2d(n1_1, n2_1, inter_bf1, b_amp1);
+  double pe[5], pn[4];
+  grid_point(g1, eigen_idx[0], eigen_idx[1], v);
+  obc_eigenqties(m, ff, eigen_lin, v, pe);
+  xy_to_n(pe, pn);
+  for (int l = 0; l < 4; ++l) pe[l] = pn[l];
+  double lam[4], L[4][4], R[4][4];
+  const int ydir = n_dir == ORC_N2;
+  ns_vdw2d_eigen(ydir, pe[0], ydir ? pe[2] : pe[1], pe[4], lam, L, R);
+  double char_amp[4];
+  for (int k = 0; k < 4; ++k) {
+    const double n1_0 = ydir ? n1_1 : n1_1 - lam[k] * dt;
+    const double n2_0 = ydir ? n2_1 - lam[k] * dt : n2_1;
+    const int inside = side_right ? (lam[k] >= 0) : !(lam[k] > 0);
+    double n_amp0[4], t_amp0[4], b_amp0[4];
+    if (inside) {
+      orc_interpolate_2d(n1_0, n2_0, inter_nodes0, n_amp0);
+      orc_interpolate_2d(n1_0, n2_0, inter_trans0, t_amp0);
+      orc_interpolate_2d(n1_0, n2_0, inter_bf0, b_amp0);
+    } else {
+      far_field_amp(m, ff, 1, n_amp0, t_amp0, b_amp0);
+    }
+    double d0[4], d1[4], integ[4];
+    for (int l = 0; l < 4; ++l) {
+      d0[l] = t_amp0[l] - b_amp0[l];
+      d1[l] = t_amp1[l] - b_amp1[l];
+    }
+    orc_newton_cotes(d0, d1, dt, integ);
+    double acc = 0.0;
+    for (int l = 0; l < 4; ++l) acc += (n_amp0[l] - integ[l]) * L[l][k];
+    char_amp[k] = acc;
+  }
+  double prim_n[4], prim[4];
+  for (int col = 0; col < 4; ++col) {
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k) acc += char_amp[k] * R[k][col];
+    prim_n[col] = acc;
+  }
+  n_to_xy(prim_n, prim);
+  cons_var(m, prim, out);
+}
+
+/* compute_newgrdpt_local, bf_newgrdpt_class.f:245-1000, for the corner
+ * procedures and the edge procedures (the diagonal *_edge_type procedures of
+ * irregular buffer layers are not restated) */
+void orc_newgrdpt_local(const orc_model* m, const double ff[4], int eigen_lin, double dt,
+                        const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1,
+                        int procedure, int grad_type, double out[4]) {
+  enum { I = ORC_GRAD_I, L0 = ORC_GRAD_L0, R0 = ORC_GRAD_R0 };
+  switch (procedure) {
+    case ORC_NG_SW_CORNER: {
+      const int gr[6] = {L0, L0, I, L0, L0, I};
+      const int e[2] = {i1 + 1, j1 + 1};
+      const int idx[6] = {i1 + 1, j1 + 1, i1 + 2, j1 + 1, i1 + 1, j1 + 2};
+      orc_newgrdpt_xy(m, ff, eigen_lin, dt, g0, g1, i1, j1, ORC_N2, 0, gr, e, idx, out);
+      return;
+    }
+    case ORC_NG_SE_CORNER: {
+      const int gr[6] = {I, L0, R0, L0, R0, I};
+      const int e[2] = {i1 - 1, j1 + 1};
+      const int idx[6] = {i1 - 2, j1 + 1, i1 - 1, j1 + 1, i1 - 1, j1 + 2};
+      orc_newgrdpt_xy(m, ff, eigen_lin, dt, g0, g1, i1, j1, ORC_N1, 1, gr, e, idx, out);
+      return;
+    }
+    case ORC_NG_NW_CORNER: {
+      const int gr[6] = {L0, I, L0, R0, I, R0};
+      const int e[2] = {i1 + 1, j1 - 1};
+      const int idx[6] = {i1 + 1, j1 - 2, i1 + 1, j1 - 1, i1 + 2, j1 - 1};
+      orc_newgrdpt_xy(m, ff, eigen_lin, dt, g0, g1, i1, j1, ORC_N1, 0, gr, e, idx, out);
+      return;
+    }
+    case ORC_NG_NE_CORNER: {
+      const int gr[6] = {R0, I, I, R0, R0, R0};
+      const int e[2] = {i1 - 1, j1 - 1};
+      const int idx[6] = {i1 - 1, j1 - 2, i1 - 2, j1 - 1, i1 - 1, j1 - 1};
+      orc_newgrdpt_xy(m, ff, eigen_lin, dt, g0, g1, i1, j1, ORC_N2, 1, gr, e, idx, out);
+      return;
+    }
+    case ORC_NG_S_EDGE: orc_newgrdpt_y(m, ff, eigen_lin, dt, g0, g1, i1, j1, 0, grad_type, out); return;
+    case ORC_NG_N_EDGE: orc_newgrdpt_y(m, ff, eigen_lin, dt, g0, g1, i1, j1, 1, grad_type, out); return;
+    case ORC_NG_W_EDGE: orc_newgrdpt_x(m, ff, eigen_lin, dt, g0, g1, i1, j1, 0, grad_type, out); return;
+    case ORC_NG_E_EDGE: orc_newgrdpt_x(m, ff, eigen_lin, dt, g0, g1, i1, j1, 1, grad_type, out); return;
+    default:
+      for (int k = 0; k < 4; ++k) out[k] = NAN;
+  }
+}
+
+/* get_config_id and get_config_procedures, bf_newgrdpt_procedure_module.f:
+ * the 3x3 neighbourhood of a new point, bit k-1 set when neighbour k is a
+ * boundary point of the grid (k = 1..8: SW, S, SE, W, E, NW, N, NE).  The
+ * configurations a rectangular grid meets when it grows by one layer, with
+ * the reference's procedure for each; -1 for the others. */
+int orc_newgrdpt_config_procedure(int config_id, int* procedure, int* grad_type) {
+  static const int table[][3] = {
+      {1, ORC_NG_NE_CORNER, ORC_GRAD_I},  {3, ORC_NG_N_EDGE, ORC_GRAD_R0},
+      {4, ORC_NG_NW_CORNER, ORC_GRAD_I},  {6, ORC_NG_N_EDGE, ORC_GRAD_L0},
+      {7, ORC_NG_N_EDGE, ORC_GRAD_I},     {9, ORC_NG_E_EDGE, ORC_GRAD_R0},
+      {20, ORC_NG_W_EDGE, ORC_GRAD_R0},   {32, ORC_NG_SE_CORNER, ORC_GRAD_I},
+      {40, ORC_NG_E_EDGE, ORC_GRAD_L0},   {41, ORC_NG_E_EDGE, ORC_GRAD_I},
+      {96, ORC_NG_S_EDGE, ORC_GRAD_R0},   {128, ORC_NG_SW_CORNER, ORC_GRAD_I},
+      {144, ORC_NG_W_EDGE, ORC_GRAD_L0},  {148, ORC_NG_W_EDGE, ORC_GRAD_I},
+      {192, ORC_NG_S_EDGE, ORC_GRAD_L0},  {224, ORC_NG_S_EDGE, ORC_GRAD_I}};
+  for (size_t q = 0; q < sizeof table / sizeof table[0]; ++q)
+    if (table[q][0] == config_id) {
+      *procedure = table[q][1];
+      *grad_type = table[q][2];
+      return 0;
+    }
+  return -1;
+}
+
+/* A rectangular grid grown by one layer at the edges with add[e] = 1 (W, E,
+ * S, N): the old state at time t (nodes1, old extents m->nx x m->ny) moves to
+ * its place in the new grid, and every new point comes from
+ * orc_newgrdpt_local with the procedure of its configuration of old points,
+ * the state at t - dt (nodes0, old extents) and the far field ff.  Every new
+ * point reads old points only, so the order of the points does not matter.
+ * out: (m->nx + add W + add E) x (m->ny + add S + add N) x 4.
+ * Returns 0, or -1 for an unexpected configuration. */
+int orc_extend_newgrdpt(const orc_model* m, const double ff[4], double dt, const double* nodes0,
+                        const double* nodes1, const double* x_map, const double* y_map,
+                        const int add[4], double* out) {
+  const int nxo = m->nx, nyo = m->ny;
+  const int aw = add[0], as = add[2];
+  const int nxn = nxo + add[0] + add[1], nyn = nyo + add[2] + add[3];
+  const double dx = x_map[1] - x_map[0], dy = y_map[1] - y_map[0];
+  double* xn = (double*)malloc(sizeof(double) * (size_t)nxn);
+  double* yn = (double*)malloc(sizeof(double) * (size_t)nyn);
+  for (int i = 1; i <= nxn; ++i) xn[i - 1] = x_map[0] + (i - 1 - aw) * dx;
+  for (int j = 1; j <= nyn; ++j) yn[j - 1] = y_map[0] + (j - 1 - as) * dy;
+  for (int i = 1; i <= nxo; ++i) xn[i - 1 + aw] = x_map[i - 1];
+  for (int j = 1; j <= nyo; ++j) yn[j - 1 + as] = y_map[j - 1];
+  for (size_t q = 0; q < (size_t)nxn * nyn * 4; ++q) out[q] = NAN;
+  for (int k = 1; k <= 4; ++k)
+    for (int j = 1; j <= nyo; ++j)
+      for (int i = 1; i <= nxo; ++i)
+        out[((size_t)(k - 1) * nyn + (j - 1 + as)) * nxn + (i - 1 + aw)] =
+            nodes1[((size_t)(k - 1) * nyo + (j - 1)) * nxo + (i - 1)];
+  const orc_bf_grid g0 = {nxo, nyo, aw, as, x_map, y_map, nodes0};
+  const orc_bf_grid g1 = {nxn, nyn, 0, 0, xn, yn, out};
+  int status = 0;
+  for (int j = 1; j <= nyn; ++j)
+    for (int i = 1; i <= nxn; ++i) {
+      const int io = i - aw, jo = j - as;
+      if (io >= 1 && io <= nxo && jo >= 1 && jo <= nyo) continue;
+      int config = 0, bit = 0;
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di) {
+          if (!di && !dj) continue;
+          const int a = io + di, b = jo + dj;
+          if (a >= 1 && a <= nxo && b >= 1 && b <= nyo) config |= 1 << bit;
+          ++bit;
+        }
+      int proc, gt;
+      double v[4];
+      if (orc_newgrdpt_config_procedure(config, &proc, &gt)) {
+        status = -1;
+        continue;
+      }
+      orc_newgrdpt_local(m, ff, 0, dt, &g0, &g1, i, j, proc, gt, v);
+      for (int k = 1; k <= 4; ++k) out[((size_t)(k - 1) * nyn + (j - 1)) * nxn + (i - 1)] = v[k - 1];
+    }
+  free(xn);
+  free(yn);
+  return status;
 }

@@ -117,6 +117,66 @@ int orc_run(const orc_model*, double t0, int nsteps, const double* dts,
 void orc_openbc_timedev(const orc_model*, const double* nodes, int i, int j, int ydir, int right,
                         int grad_type, double dn, const double pe[5], double out[4]);
 
+/* ---- new grid points of the buffer layers (bf_layer_newgrdpt/bf_newgrdpt_class.f) ----
+ * A grid as the reference passes it to the new-point procedures: extents,
+ * alignment bf_align(1,1), bf_align(2,1) (only differences between the two
+ * grids matter), coordinate maps and nodes(nx,ny,4).  Grid 0 holds the state
+ * at t - dt, grid 1 the state at t and the new point (i1, j1) (1-based).
+ * ff is the conservative far-field state (the initial condition's
+ * get_far_field, uniform here); eigen_lin selects obc_eigenqties_lin instead
+ * of the default obc_eigenqties_bc (parameters_input.f:649). */
+typedef struct orc_bf_grid {
+  int nx, ny;
+  int align_x, align_y;
+  const double* x_map;
+  const double* y_map;
+  const double* nodes;
+} orc_bf_grid;
+/* gradient procedures: interior (centered), one-sided L0, one-sided R0 */
+enum { ORC_GRAD_I = 0, ORC_GRAD_L0 = 1, ORC_GRAD_R0 = 2 };
+/* diagonal directions of compute_newgrdpt_xy */
+enum { ORC_N1 = 1, ORC_N2 = 2 };
+/* procedure types (parameters_constant.f *_edge_type, *_corner_type) */
+enum {
+  ORC_NG_N_EDGE = 0, ORC_NG_S_EDGE = 1, ORC_NG_E_EDGE = 2, ORC_NG_W_EDGE = 3,
+  ORC_NG_NE_CORNER = 4, ORC_NG_NW_CORNER = 5, ORC_NG_SE_CORNER = 6, ORC_NG_SW_CORNER = 7
+};
+/* helpers: nodes/coeff as [point][variable] (2 or 3 points x 4 variables) */
+void orc_interp_coeff_1d(const double x_map[2], const double nodes[8], double coeff[8]);
+void orc_interpolate_1d(double x, const double coeff[8], double out[4]);
+void orc_interp_coeff_2d(const double x_map[3], const double y_map[3], const double nodes[12],
+                         double coeff[12]);
+void orc_interpolate_2d(double x, double y, const double coeff[12], double out[4]);
+void orc_newton_cotes(const double d0[4], const double d1[4], double dt, double out[4]);
+/* compute_newgrdpt_x / _y: side_right 1 (right) or 0 (left); grad_* the
+ * transverse gradient procedure (ORC_GRAD_*) */
+void orc_newgrdpt_x(const orc_model*, const double ff[4], int eigen_lin, double dt,
+                    const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1, int side_right,
+                    int grad_y, double out[4]);
+void orc_newgrdpt_y(const orc_model*, const double ff[4], int eigen_lin, double dt,
+                    const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1, int side_right,
+                    int grad_x, double out[4]);
+/* compute_newgrdpt_xy: grads[6] = x and y gradient procedures of the three
+ * interpolation points; eigen_idx[2] and inter_idx1[6] are (i, j) pairs in grid 1 */
+void orc_newgrdpt_xy(const orc_model*, const double ff[4], int eigen_lin, double dt,
+                     const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1, int n_dir,
+                     int side_right, const int grads[6], const int eigen_idx[2],
+                     const int inter_idx1[6], double out[4]);
+/* compute_newgrdpt_local for a procedure ORC_NG_* and gradient ORC_GRAD_* */
+void orc_newgrdpt_local(const orc_model*, const double ff[4], int eigen_lin, double dt,
+                        const orc_bf_grid* g0, const orc_bf_grid* g1, int i1, int j1,
+                        int procedure, int grad_type, double out[4]);
+/* get_config_procedures for the configurations of a rectangle grown by one
+ * layer; returns 0, or -1 for a configuration outside that set */
+int orc_newgrdpt_config_procedure(int config_id, int* procedure, int* grad_type);
+/* a rectangular grid grown by one layer at the edges with add[e] = 1
+ * (W, E, S, N; each 0 or 1); the old state at t (nodes1) keeps its place,
+ * new points from the state at t - dt (nodes0) by the characteristic
+ * extrapolation.  out: the new grid's nodes.  Returns 0 or -1. */
+int orc_extend_newgrdpt(const orc_model*, const double ff[4], double dt, const double* nodes0,
+                        const double* nodes1, const double* x_map, const double* y_map,
+                        const int add[4], double* out);
+
 /* coordinates (field_abstract_class.f:286-315) */
 void orc_coordinates(int nx, int ny, double x_min, double x_max, double y_min,
                      double y_max, double* x_map, double* y_map);

@@ -36,6 +36,21 @@ class Model(ctypes.Structure):
     ]
 
 
+class BfGrid(ctypes.Structure):
+    """orc_bf_grid: a grid handed to the new-point procedures."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("align_x", ctypes.c_int), ("align_y", ctypes.c_int),
+        ("x_map", _dp), ("y_map", _dp), ("nodes", _dp),
+    ]
+
+
+GRAD_I, GRAD_L0, GRAD_R0 = 0, 1, 2
+N1_DIRECTION, N2_DIRECTION = 1, 2
+(NG_N_EDGE, NG_S_EDGE, NG_E_EDGE, NG_W_EDGE,
+ NG_NE_CORNER, NG_NW_CORNER, NG_SE_CORNER, NG_SW_CORNER) = range(8)
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
@@ -107,6 +122,22 @@ def lib():
         L.orc_ridders.restype = ctypes.c_double
         L.orc_ridders.argtypes = [ROOTFN, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+        G = ctypes.POINTER(BfGrid)
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_interp_coeff_1d.argtypes = [_dp, _dp, _dp]
+        L.orc_interpolate_1d.argtypes = [ctypes.c_double, _dp, _dp]
+        L.orc_interp_coeff_2d.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_interpolate_2d.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_newton_cotes.argtypes = [_dp, _dp, ctypes.c_double, _dp]
+        ng = [M, _dp, ctypes.c_int, ctypes.c_double, G, G, ctypes.c_int, ctypes.c_int]
+        L.orc_newgrdpt_x.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_newgrdpt_y.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_newgrdpt_xy.argtypes = ng + [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp]
+        L.orc_newgrdpt_local.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_newgrdpt_config_procedure.restype = ctypes.c_int
+        L.orc_newgrdpt_config_procedure.argtypes = [ctypes.c_int, _ip, _ip]
+        L.orc_extend_newgrdpt.restype = ctypes.c_int
+        L.orc_extend_newgrdpt.argtypes = [M, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _ip, _dp]
         _lib = L
     return _lib
 
@@ -218,4 +249,135 @@ def extend(nodes, w: int, e: int, s: int, n: int):
 def sums(m: Model, nodes):
     out = np.zeros(4)
     lib().orc_sums(ctypes.byref(m), _ptr(nodes), _ptr(out))
+    return out
+
+
+def _ints(v):
+    return (ctypes.c_int * len(v))(*[int(q) for q in v])
+
+
+class _Grid:
+    """Keeps the arrays of an orc_bf_grid alive.  nodes: (4, ny, nx)."""
+
+    def __init__(self, nodes, x_map, y_map, align=(0, 0)):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        self.x_map = np.ascontiguousarray(x_map, dtype=np.float64)
+        self.y_map = np.ascontiguousarray(y_map, dtype=np.float64)
+        ne, ny, nx = self.nodes.shape
+        assert ne == 4 and self.x_map.size == nx and self.y_map.size == ny
+        self.c = BfGrid(nx, ny, int(align[0]), int(align[1]), _ptr(self.x_map),
+                        _ptr(self.y_map), _ptr(self.nodes))
+
+
+def bf_grid(nodes, x_map, y_map, align=(0, 0)):
+    """A grid for the new-point functions: nodes (4, ny, nx), its maps and its
+    alignment (bf_align(1,1), bf_align(2,1))."""
+    return _Grid(nodes, x_map, y_map, align)
+
+
+def interp_coeff_1d(x_map, nodes):
+    """nodes (2 points, 4 variables) -> coefficients (2, 4): slope, intercept."""
+    out = np.zeros((2, 4))
+    lib().orc_interp_coeff_1d(_ptr(np.ascontiguousarray(x_map, dtype=np.float64)),
+                              _ptr(np.ascontiguousarray(nodes, dtype=np.float64)), _ptr(out))
+    return out
+
+
+def interpolate_1d(x, coeff):
+    out = np.zeros(4)
+    lib().orc_interpolate_1d(float(x), _ptr(np.ascontiguousarray(coeff, dtype=np.float64)),
+                             _ptr(out))
+    return out
+
+
+def interp_coeff_2d(x_map, y_map, nodes):
+    out = np.zeros((3, 4))
+    lib().orc_interp_coeff_2d(_ptr(np.ascontiguousarray(x_map, dtype=np.float64)),
+                              _ptr(np.ascontiguousarray(y_map, dtype=np.float64)),
+                              _ptr(np.ascontiguousarray(nodes, dtype=np.float64)), _ptr(out))
+    return out
+
+
+def interpolate_2d(x, y, coeff):
+    out = np.zeros(4)
+    lib().orc_interpolate_2d(float(x), float(y),
+                             _ptr(np.ascontiguousarray(coeff, dtype=np.float64)), _ptr(out))
+    return out
+
+
+def newton_cotes(d0, d1, dt):
+    out = np.zeros(4)
+    lib().orc_newton_cotes(_ptr(np.ascontiguousarray(d0, dtype=np.float64)),
+                           _ptr(np.ascontiguousarray(d1, dtype=np.float64)), float(dt), _ptr(out))
+    return out
+
+
+def _ff(far_field):
+    return np.ascontiguousarray(far_field, dtype=np.float64)
+
+
+def newgrdpt_x(m: Model, far_field, dt, g0, g1, i1, j1, right, grad, eigen_lin=False):
+    out = np.zeros(4)
+    ff = _ff(far_field)
+    lib().orc_newgrdpt_x(ctypes.byref(m), _ptr(ff), int(eigen_lin), float(dt),
+                         ctypes.byref(g0.c), ctypes.byref(g1.c), i1, j1, int(right), grad,
+                         _ptr(out))
+    return out
+
+
+def newgrdpt_y(m: Model, far_field, dt, g0, g1, i1, j1, right, grad, eigen_lin=False):
+    out = np.zeros(4)
+    ff = _ff(far_field)
+    lib().orc_newgrdpt_y(ctypes.byref(m), _ptr(ff), int(eigen_lin), float(dt),
+                         ctypes.byref(g0.c), ctypes.byref(g1.c), i1, j1, int(right), grad,
+                         _ptr(out))
+    return out
+
+
+def newgrdpt_xy(m: Model, far_field, dt, g0, g1, i1, j1, n_dir, right, grads, eigen_idx,
+                inter_idx1, eigen_lin=False):
+    """grads: 6 gradient procedures (x1, y1, x2, y2, x3, y3); eigen_idx (i, j);
+    inter_idx1: three (i, j) pairs, all 1-based in grid 1."""
+    out = np.zeros(4)
+    ff = _ff(far_field)
+    flat = [q for pair in inter_idx1 for q in pair]
+    lib().orc_newgrdpt_xy(ctypes.byref(m), _ptr(ff), int(eigen_lin), float(dt),
+                          ctypes.byref(g0.c), ctypes.byref(g1.c), i1, j1, n_dir, int(right),
+                          _ints(grads), _ints(eigen_idx), _ints(flat), _ptr(out))
+    return out
+
+
+def newgrdpt_local(m: Model, far_field, dt, g0, g1, i1, j1, procedure, grad=GRAD_I,
+                   eigen_lin=False):
+    out = np.zeros(4)
+    ff = _ff(far_field)
+    lib().orc_newgrdpt_local(ctypes.byref(m), _ptr(ff), int(eigen_lin), float(dt),
+                             ctypes.byref(g0.c), ctypes.byref(g1.c), i1, j1, procedure, grad,
+                             _ptr(out))
+    return out
+
+
+def newgrdpt_config_procedure(config_id):
+    """(procedure, gradient) of a configuration, or None."""
+    p = ctypes.c_int()
+    g = ctypes.c_int()
+    if lib().orc_newgrdpt_config_procedure(config_id, ctypes.byref(p), ctypes.byref(g)):
+        return None
+    return p.value, g.value
+
+
+def extend_newgrdpt(m: Model, far_field, dt, nodes0, nodes1, x_map, y_map, w, e, s, n):
+    """The grid grown by one layer at the edges with w/e/s/n = 1; new points
+    from the characteristic extrapolation (state nodes0 at t - dt, nodes1 at t)."""
+    nodes0 = np.ascontiguousarray(nodes0, dtype=np.float64)
+    nodes1 = np.ascontiguousarray(nodes1, dtype=np.float64)
+    x_map = np.ascontiguousarray(x_map, dtype=np.float64)
+    y_map = np.ascontiguousarray(y_map, dtype=np.float64)
+    out = np.zeros((4, m.ny + s + n, m.nx + w + e))
+    ff = _ff(far_field)
+    st = lib().orc_extend_newgrdpt(ctypes.byref(m), _ptr(ff), float(dt), _ptr(nodes0),
+                                   _ptr(nodes1), _ptr(x_map), _ptr(y_map), _ints([w, e, s, n]),
+                                   _ptr(out))
+    if st:
+        raise ValueError("unexpected new-point configuration")
     return out
