@@ -203,6 +203,71 @@ def open_boundary_bench(n, steps):
     return res
 
 
+def open_extension_bench(n, steps, every, gap, band):
+    """configs[4] with the domain extension: the bubble's interface starts
+    `gap` points from the east edge; every `every` steps the FP64 run calls
+    adapt_domain (density criterion eps = 0.3 over a band of `band` points,
+    one layer per edge and call, new points by the characteristic
+    extrapolation from the far field of the initial condition).  The mixed
+    run replays the same growth schedule, so both end on the same grid and
+    the precision error is taken point by point.  Throughput counts the
+    updates of the grid as it grows, over the whole run (checks, growths and
+    reallocations included)."""
+    import dataclasses
+
+    import numpy as np
+
+    import dim2d_finite_volume_amd as d
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    radius, flow = 0.1, (0.2, 0.05)
+    dx = 1.0 / (n - 1)
+    center = (0.5 - radius - gap * dx, 0.0)
+    p, nodes = ic.open_bubble(n=n, center=center, radius=radius, flow=flow)
+    dt = ic.cfl_dt(p, nodes, 0.1)
+    ff = ic.far_field(p, 0.95, flow)
+    res, finals, schedule = {}, {}, []
+    for prec, name in ((d.PREC_FP64, "f64"), (d.PREC_MIXED, "mixed")):
+        f = d.Field(dataclasses.replace(p, precision=prec), nodes)
+        f.set_far_field(ff)
+        f.synchronize()
+        updates, done, k = 0, 0, 0
+        t0 = time.perf_counter()
+        while done < steps:
+            m = min(every, steps - done)
+            f.run(m, dt=dt)
+            updates += (f.params.nx - 4) * (f.params.ny - 4) * m
+            done += m
+            if prec == d.PREC_FP64:
+                a = f.adapt_domain("WESN", distance=band, grow=1, md_threshold=0.3)
+                schedule.append(a.tolist())
+            else:
+                a = schedule[k]
+                if any(a):
+                    f.extend(*a)
+            k += 1
+        f.synchronize()
+        el = time.perf_counter() - t0
+        finals[name] = f.download()
+        grid = [f.params.nx, f.params.ny]
+        f.close()
+        grown = np.array(schedule).sum(axis=0).tolist()
+        res[name] = {"value": updates / el, "unit": "grid-pt updates/s",
+                     "ms_per_step": 1e3 * el / steps, "final_grid": grid,
+                     "finite": bool(np.isfinite(finals[name]).all())}
+        log(f"[bench] open extension {n}^2 {name}: {res[name]['value']:.4e} updates/s, "
+            f"grid {grid}, layers added {grown}")
+    del nodes
+    res["layers_added_wesn"] = grown
+    res["growth_calls"] = int(sum(1 for a in schedule if any(a)))
+    res["mixed"]["error_vs_fp64"] = rel_errors(finals["mixed"], finals["f64"])
+    res["config"] = (f"vapour bubble r=0.1 in liquid, T0=0.95, (u,v)=(0.2,0.05), {n}x{n} "
+                     f"interior at the start, interface {gap} points from the east edge, "
+                     f"Hedstrom open edges, adapt_domain every {every} steps (band {band}, "
+                     f"eps 0.3, one layer per call, characteristic new points), fixed "
+                     f"dt={dt:.4e}, {steps} steps")
+    return res
+
+
 def variants_bench(n, steps):
     """The other integrator and operator choices on the headline's two-bubble
     state at n^2, CFL 0.1 device-side dt, FP64: the low-storage RK3 (periodic
@@ -285,6 +350,11 @@ def main():
     ap.add_argument("--open-n", type=int, default=8192)
     ap.add_argument("--open-steps", type=int, default=2000,
                     help="steps of the open-boundary case (0: skip)")
+    ap.add_argument("--ext-steps", type=int, default=2000,
+                    help="steps of the open-boundary case with domain extension (0: skip)")
+    ap.add_argument("--ext-every", type=int, default=50)
+    ap.add_argument("--ext-gap", type=int, default=300)
+    ap.add_argument("--ext-band", type=int, default=340)
     ap.add_argument("--variant-steps", type=int, default=200,
                     help="steps of the LSRK3 / Cockburn-Gau variants (0: skip)")
     ap.add_argument("--error-steps", default="100,1000",
@@ -409,6 +479,8 @@ def main():
 
     batch = batch_bench(args.batch_steps) if args.batch_steps > 0 else None
     open_bc = open_boundary_bench(args.open_n, args.open_steps) if args.open_steps > 0 else None
+    open_ext = (open_extension_bench(args.open_n, args.ext_steps, args.ext_every, args.ext_gap,
+                                     args.ext_band) if args.ext_steps > 0 else None)
     variants = variants_bench(n, args.variant_steps) if args.variant_steps > 0 else None
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
@@ -435,6 +507,7 @@ def main():
         "mixed": mixed,
         "batch": batch,
         "open_boundary": open_bc,
+        "open_extension": open_ext,
         "variants": variants,
         "checks": {"mass_drift": float((diag1[0] - diag0[0]) / diag0[0]),
                    "nonfinite": int(diag1[6]), "t_end": t_end, "dt_last": dt_last},

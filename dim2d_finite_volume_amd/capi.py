@@ -27,7 +27,8 @@ EXPORTS = (
     "dim2d_time_dev", "dim2d_apply_bc", "dim2d_rk3_step", "dim2d_integrate", "dim2d_cfl_dt",
     "dim2d_diagnostics", "dim2d_time", "dim2d_synchronize", "dim2d_set_tiles",
     "dim2d_set_stage_fusion", "dim2d_profile_enable", "dim2d_profile_read",
-    "dim2d_get_params", "dim2d_openbc_undermined", "dim2d_extend", "dim2d_adapt_domain",
+    "dim2d_get_params", "dim2d_openbc_undermined", "dim2d_set_far_field", "dim2d_extend",
+    "dim2d_adapt_domain",
     "dim2d_check_steady_state", "dim2d_time_dev_noalloc",
     "dim2d_batch_create", "dim2d_batch_destroy", "dim2d_batch_upload", "dim2d_batch_download",
     "dim2d_batch_integrate", "dim2d_batch_set_tiles", "dim2d_batch_synchronize",
@@ -98,6 +99,7 @@ def load() -> ctypes.CDLL:
         "dim2d_get_params": (ctypes.c_int, [vp, ctypes.POINTER(Params)]),
         "dim2d_check_steady_state": (ctypes.c_int, [vp, d, dp, ctypes.POINTER(i32)]),
         "dim2d_openbc_undermined": (ctypes.c_int, [vp, i32, d, ctypes.POINTER(i32)]),
+        "dim2d_set_far_field": (ctypes.c_int, [vp, ctypes.c_void_p]),
         "dim2d_extend": (ctypes.c_int, [vp, i32, i32, i32, i32]),
         "dim2d_adapt_domain": (ctypes.c_int, [vp, i32, i32, d, i32, ctypes.POINTER(i32)]),
         "dim2d_batch_create": (ctypes.c_int, [ctypes.POINTER(Params), i32, ctypes.POINTER(vp)]),

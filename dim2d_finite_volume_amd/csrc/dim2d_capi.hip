@@ -197,6 +197,13 @@ struct dim2d_ctx {
   float* D2 = nullptr;
   double* T = nullptr;   // planar staging of uploads, downloads and time derivatives
   StepCtl* ctl = nullptr;
+  // open edges with a far field (dim2d_set_far_field): every step first
+  // copies the two outermost rows and columns of u^n into F0, the state at
+  // t - dt that the new points of a domain extension are extrapolated from
+  double ff[4] = {0.0, 0.0, 0.0, 0.0};
+  bool ff_set = false;
+  double* F0 = nullptr;
+  bool f0_valid = false;  // F0 holds the frame of the last step's u^n
   int block_x = 64;
   int rows = 16;
   // RK3-TVD stages 1 and 2 in one launch (stage pair) where the edges allow
@@ -383,6 +390,7 @@ int dim2d_ctx_destroy(dim2d_ctx* c) {
   field_free(c->D2);
   (void)hipFree(c->T);
   (void)hipFree(c->ctl);
+  (void)hipFree(c->F0);
   release_timed(c);
   drop_graphs(c);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -471,6 +479,7 @@ static int copy_in(dim2d_ctx* c, const double* src, hipMemcpyKind kind) {
   HIP_TRY(launch_pack(p, planar, c->U, c->stream));
   c->uploaded = true;
   c->red_valid = false;
+  c->f0_valid = false;
   return DIM2D_OK;
 }
 
@@ -616,6 +625,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     return DIM2D_OK;
   };
   int st = DIM2D_OK;
+  if (c->ff_set) HIP_TRY(launch_save_frame(p, c->U, c->F0, c->stream));
   if (c->cg) {
     // Cockburn-Gau stencils: FP64 RK3-TVD stages u^n -> A -> B -> U2, no walls
     a.U = c->U;
@@ -707,6 +717,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
   std::swap(c->U, c->U2);
   c->steps++;
   c->red_valid = true;
+  if (c->ff_set) c->f0_valid = true;
   return DIM2D_OK;
 }
 
@@ -824,18 +835,53 @@ int dim2d_openbc_undermined(dim2d_ctx* c, int32_t distance, double md_threshold,
   return DIM2D_OK;
 }
 
+int dim2d_set_far_field(dim2d_ctx* c, const double far_field[4]) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "the edges are not open (DIM2D_BC_OPEN)");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (!far_field) {
+    (void)hipFree(c->F0);
+    c->F0 = nullptr;
+    c->ff_set = false;
+    c->f0_valid = false;
+    drop_graphs(c);  // captured steps hold the frame snapshot
+    return DIM2D_OK;
+  }
+  for (int k = 0; k < 4; ++k)
+    if (!isfinite(far_field[k])) return fail(DIM2D_EINVAL, "far_field[%d] is not finite", k);
+  if (!(far_field[0] > 0.0 && far_field[0] < 3.0))
+    return fail(DIM2D_EINVAL, "far-field density %g outside (0, 3)", far_field[0]);
+  if (!c->F0) {
+    const size_t n = (size_t)frame_cells(c->dp.nx, c->dp.ny) * 4;
+    HIP_TRY(hipMalloc(&c->F0, n * sizeof(double)));
+    c->f0_valid = false;
+    drop_graphs(c);
+  }
+  for (int k = 0; k < 4; ++k) c->ff[k] = far_field[k];
+  c->ff_set = true;
+  return DIM2D_OK;
+}
+
 int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int32_t add_n) {
   if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
   if (!c->uploaded) return fail(DIM2D_ESTATE, "extend before upload");
   if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "only open (DIM2D_BC_OPEN) edges are extended");
   if (add_w < 0 || add_e < 0 || add_s < 0 || add_n < 0)
     return fail(DIM2D_EINVAL, "grid points to add must be >= 0");
+  const bool characteristic = c->ff_set;
+  if (characteristic && (add_w > 1 || add_e > 1 || add_s > 1 || add_n > 1))
+    return fail(DIM2D_EINVAL,
+                "with a far field the grid grows by at most one layer per edge and call");
   const long long nx = (long long)c->prm.nx + add_w + add_e;
   const long long ny = (long long)c->prm.ny + add_s + add_n;
   if (nx > kMaxExtent || ny > kMaxExtent)
     return fail(DIM2D_EINVAL, "extended grid %lldx%lld exceeds %d points per axis", nx, ny,
                 kMaxExtent);
   if (nx == c->prm.nx && ny == c->prm.ny) return DIM2D_OK;
+  if (characteristic && !c->f0_valid)
+    return fail(DIM2D_ESTATE,
+                "no step since the last upload or extension: the state at t - dt that new "
+                "points are extrapolated from is missing");
   // the new grid keeps dx and dy: x_map(bc+1) = x_min moves add_w points west
   // (field_abstract_class.f:286-315 spacing)
   dim2d_params q = c->prm;
@@ -856,10 +902,24 @@ int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int3
   const double h = std::min(n->dp.dx, n->dp.dy);
   if (e == hipSuccess)
     e = hipMemcpyAsync(&n->ctl->h, &h, sizeof h, hipMemcpyHostToDevice, n->stream);
+  // new points by the characteristic extrapolation from the last step's
+  // u^n frame (t - dt) and the state (t), over the copy rule's values
+  if (e == hipSuccess && characteristic) {
+    const double x0 = q.x_min - kBcSize * n->dp.dx, y0 = q.y_min - kBcSize * n->dp.dy;
+    e = launch_newgrdpt(n->dp, c->dp, add_w, add_s, x0, y0, c->ff, n->ctl, c->F0, n->U,
+                        n->stream);
+  }
+  if (e == hipSuccess && characteristic)
+    e = hipMalloc(&n->F0, (size_t)frame_cells(n->dp.nx, n->dp.ny) * 4 * sizeof(double));
   if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
   if (e != hipSuccess) {
     dim2d_ctx_destroy(n);
     return fail(DIM2D_EDEVICE, "domain extension failed: %s", hipGetErrorString(e));
+  }
+  if (characteristic) {
+    for (int k = 0; k < 4; ++k) n->ff[k] = c->ff[k];
+    n->ff_set = true;
+    n->f0_valid = false;
   }
   if (c->block_x_set) {  // else the new grid keeps the default of its own size
     n->block_x = c->block_x;
@@ -891,6 +951,9 @@ int dim2d_adapt_domain(dim2d_ctx* c, int32_t adapt_mask, int32_t distance, doubl
   if (adapt_mask < 0 || adapt_mask > 15)
     return fail(DIM2D_EINVAL, "adapt_mask out of range: %d", adapt_mask);
   if (grow < 1) return fail(DIM2D_EINVAL, "grow must be >= 1, got %d", grow);
+  if (c->ff_set && grow != 1)
+    return fail(DIM2D_EINVAL, "with a far field the grid grows by one layer per call, got %d",
+                grow);
   int32_t cnt[4];
   int st = dim2d_openbc_undermined(c, distance, md_threshold, cnt);
   if (st) return st;

@@ -143,6 +143,18 @@ hipError_t launch_undermined(const DevParams& p, int dist, double md_thr, const 
 hipError_t launch_extend(const DevParams& np, const DevParams& op, int add_w, int add_s,
                          const double* oldU, double* newU, hipStream_t s);
 
+// open grids grown by one layer (dim2d_newgrdpt.hip): cells of the frame
+// snapshot (the two outermost rows and columns) of an nx x ny grid; the
+// snapshot of U; the new points of grid `np`, grown from `op` by aw columns
+// west and as rows south (0 or 1; east and north implied by the extents),
+// whose old points are already in U; x0, y0 are the coordinates of np's
+// point (0, 0), ff the conservative far-field state, dt is read from ctl
+long long frame_cells(int nx, int ny);
+hipError_t launch_save_frame(const DevParams& p, const double* U, double* F0, hipStream_t s);
+hipError_t launch_newgrdpt(const DevParams& np, const DevParams& op, int aw, int as, double x0,
+                           double y0, const double ff[4], const StepCtl* ctl, const double* F0,
+                           double* U, hipStream_t s);
+
 // workgroups of the persistent batch kernels that fit the device at once
 int max_resident_workgroups(int mixed);
 

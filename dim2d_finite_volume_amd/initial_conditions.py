@@ -96,6 +96,18 @@ def open_bubble(n: int = 8192, precision: int = capi.PREC_MIXED, center=(0.0, 0.
     return p, nodes
 
 
+def far_field(p: ModelParams, T0: float = 0.95, flow=(0.0, 0.0), phase: str = "liquid"):
+    """The conservative far-field state of the bubble conditions: the
+    surrounding phase at rest temperature T0 carried by the mean flow
+    (get_far_field of dim2d_ic/bubble_transported/ic_class.f with the
+    saturation densities of this module)."""
+    rho_l, rho_v = saturated_densities(T0)
+    rho = rho_l if phase == "liquid" else rho_v
+    u0, v0 = flow
+    return np.array([rho, rho * u0, rho * v0,
+                     0.5 * rho * (u0 * u0 + v0 * v0) + rho * (8.0 / 3.0 * p.cv_r * T0 - 3.0 * rho)])
+
+
 def batch_configs(nsims: int = 256, seed: int = 4, precision: int = capi.PREC_FP64,
                   nmin: int = 64, nmax: int = 512):
     """Independent drop/bubble simulations: square grids n = round(exp(U[ln nmin, ln nmax])),

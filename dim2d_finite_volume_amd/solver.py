@@ -175,8 +175,23 @@ class Field:
                    "dim2d_openbc_undermined")
         return np.array(out[:], dtype=np.int32)
 
+    def set_far_field(self, far_field) -> None:
+        """The conservative far-field state of the open edges (the initial
+        condition's get_far_field), or None.  With it, extend() computes new
+        points by the reference's characteristic extrapolation
+        (bf_newgrdpt_class.f) from the last step, one layer per edge and call."""
+        if far_field is None:
+            capi.check(self._lib.dim2d_set_far_field(self._ctx, None), "dim2d_set_far_field")
+            return
+        ff = np.ascontiguousarray(far_field, dtype=np.float64)
+        if ff.shape != (4,):
+            raise ValueError("far_field must hold 4 conservative values")
+        capi.check(self._lib.dim2d_set_far_field(self._ctx, ff.ctypes.data),
+                   "dim2d_set_far_field")
+
     def extend(self, w: int = 0, e: int = 0, s: int = 0, n: int = 0) -> None:
-        """Grow the grid at open edges by whole columns/rows (dx, dy kept)."""
+        """Grow the grid at open edges by whole columns/rows (dx, dy kept);
+        with a far field set, by one layer with extrapolated new points."""
         capi.check(self._lib.dim2d_extend(self._ctx, w, e, s, n), "dim2d_extend")
         self._refresh_params()
 

@@ -243,25 +243,48 @@ int dim2d_get_params(dim2d_ctx* ctx, dim2d_params* out);
 int dim2d_openbc_undermined(dim2d_ctx* ctx, int32_t distance, double md_threshold,
                             int32_t out[4]);
 
+/* Open edges only (blocking).  The conservative far-field state [rho, rho u,
+ * rho v, E] that the new points of a domain extension take their incoming
+ * characteristics from: the initial condition's get_far_field
+ * (pmodel_eq_class.f:2097-2109), uniform here as for the reference's
+ * transported-bubble conditions.  Setting it switches dim2d_extend to the
+ * reference's new-point rule and makes every step first copy the two
+ * outermost rows and columns of u^n into a snapshot (the state at t - dt of
+ * that rule; 4 (nx + ny - 4) cells); NULL switches both off.  No counterpart
+ * in the reference's interface: there the far field comes from the initial
+ * condition module compiled into the solver. */
+int dim2d_set_far_field(dim2d_ctx* ctx, const double far_field[4]);
+
 /* Open edges only (blocking).  Grow the grid by add_w/add_e columns and
  * add_s/add_n rows, keeping dx and dy: x_min moves add_w*dx west, x_max
  * add_e*dx east, and so on.  The state moves to its place in the larger
- * grid; every new point takes the value of the nearest point of the old grid
- * (the old edge point repeated along the edge normal, the old corner point in
- * the new corners).  Time, dt and the step count carry over; the ctx handle
- * stays valid and its device buffers are reallocated.  The reference grows
- * buffer layers (bf_interface_icr%adapt_domain_extension,
- * bf_interface_icr_class.f:251-386) whose new points come from a
- * characteristic extrapolation in time (bf_layer_newgrdpt); here the whole
- * edge moves, and new points repeat the far-field state at the edge. */
+ * grid.  Time, dt and the step count carry over; the ctx handle stays valid
+ * and its device buffers are reallocated.
+ * New points, with a far field set (dim2d_set_far_field): each add_* is 0 or
+ * 1 and at least one step must have run since the last upload or extension
+ * (DIM2D_ESTATE otherwise).  Every new point is extrapolated in time along
+ * the characteristics from the last step's u^n (t - dt) and the current
+ * state (t) by the reference's procedure for its configuration of old
+ * points (bf_newgrdpt_class.f:183-2400 compute_newgrdpt_x/_y/_xy with
+ * bf_newgrdpt_procedure_module.f get_config_procedures): compute_newgrdpt_x
+ * on new columns, _y on new rows (centered transverse gradient, one-sided at
+ * their ends), _xy along the diagonal at new corners; outgoing
+ * characteristics from the grid, incoming ones from the far field.  The
+ * reference adds these points around each new interior point of its buffer
+ * layers (bf_interface_icr_class.f:251-386); here whole edges move, one layer
+ * per call.
+ * Without a far field every new point takes the value of the nearest point
+ * of the old grid (the old edge point repeated along the edge normal, the old
+ * corner point in the new corners), any number of layers at once (no
+ * counterpart in the reference). */
 int dim2d_extend(dim2d_ctx* ctx, int32_t add_w, int32_t add_e, int32_t add_s, int32_t add_n);
 
 /* field_extended%adapt_domain (field_extended_class.f:427-452), called by the
  * host between steps: for each edge whose bit is set in adapt_mask (1 W, 2 E,
  * 4 S, 8 N: the reference's adapt_*_choice, parameters_input.f:796-817)
  * where dim2d_openbc_undermined(distance, md_threshold) finds points, the
- * edge moves out by `grow` points (dim2d_extend).  out[0..3] = points added
- * per edge. */
+ * edge moves out by `grow` points (dim2d_extend; grow must be 1 with a far
+ * field).  out[0..3] = points added per edge. */
 int dim2d_adapt_domain(dim2d_ctx* ctx, int32_t adapt_mask, int32_t distance, double md_threshold,
                        int32_t grow, int32_t out[4]);
 

@@ -1,8 +1,12 @@
 """Domain extension at open edges through the C ABI: the undermined-point
 counts (are_openbc_undermined, pmodel_eq_class.f:1991-2070) against the CPU
-oracle, the extension's copy rule against numpy, and an extended context
-against a context created on the larger grid.  Counts and copies are exact;
-trajectories after an extension are bitwise equal to the fresh context's."""
+oracle, the extension's copy rule against numpy, the new points of the
+reference's characteristic extrapolation (bf_newgrdpt_class.f) against the
+oracle after one and after repeated growths, and an extended context against
+a context created on the larger grid.  Counts and copies are exact;
+trajectories after an extension are bitwise equal to the fresh context's;
+extrapolated new points agree with the oracle within 1e-12 (1e-11 after six
+growths, where the rounding of the steps in between accumulates)."""
 import numpy as np
 import pytest
 
@@ -158,3 +162,207 @@ def test_adapt_domain_keeps_a_bubble_run_alive(orc):
     assert err[0] < 2e-2 and err[3] < 5e-2, err
     f.close()
     g.close()
+
+
+# ------------------------------------------- new points by characteristic extrapolation
+NEWPT_EDGES = [(1, 1, 1, 1), (0, 1, 0, 0), (1, 0, 1, 0), (0, 1, 0, 1)]
+
+
+def _stepped_bubble(d, orc, precision, center=(0.3, 0.12), flow=(0.3, 0.1), n=60, steps=5,
+                    cfl=False):
+    """A bubble near the NE corner after `steps` steps, with a far field set;
+    returns the field, the oracle model, the state before and after the last
+    step, that step's dt and the far field."""
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = _edge_bubble(n=n, center=center, flow=flow, precision=precision)
+    m = _model(orc, p)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ff = ic.far_field(p, 0.95, flow)
+    f = d.Field(p, nodes)
+    f.set_far_field(ff)
+    if cfl:
+        f.run(steps - 1, cfl=0.1)
+        before = f.download()
+        f.run(1, cfl=0.1)
+        dt = f.device_time()[1]
+    else:
+        f.run(steps - 1, dt=dt)
+        before = f.download()
+        f.run(1, dt=dt)
+    return f, m, before, f.download(), dt, ff
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("edges", NEWPT_EDGES)
+def test_extend_new_points_match_oracle(orc, precision, edges):
+    """dim2d_extend with a far field: old points move unchanged, new points
+    equal the oracle's characteristic extrapolation (orc_extend_newgrdpt, the
+    procedures of bf_newgrdpt_class.f) from the same two states within 1e-12
+    (FP64 on both sides; the oracle evaluates without contraction)."""
+    d = _lib()
+    f, m, before, after, dt, ff = _stepped_bubble(d, orc, precision)
+    xm, ym = f.params.coordinates()
+    print("case", precision, edges, "dt", dt, flush=True)
+    f.extend(*edges)
+    got = f.download()
+    want = orc.extend_newgrdpt(m, ff, dt, before, after, xm, ym, *edges)
+    w, e, s, n = edges
+    ny, nx = after.shape[1:]
+    assert np.array_equal(got[:, s:s + ny, w:w + nx], after)
+    new = np.ones(got.shape[1:], bool)
+    new[s:s + ny, w:w + nx] = False
+    assert np.isfinite(got).all()
+    scale = np.abs(want[:, new]).max(axis=1)
+    err = (np.abs(got[:, new] - want[:, new]).max(axis=1) / scale)
+    print("new-point rel err", err)
+    assert err.max() < 1e-12, err
+    f.close()
+
+
+def test_extend_new_points_cfl_dt(orc):
+    """With the device CFL rule the extrapolation uses that step's dt."""
+    d = _lib()
+    f, m, before, after, dt, ff = _stepped_bubble(d, orc, 0, cfl=True, steps=9)
+    xm, ym = f.params.coordinates()
+    f.extend(1, 1, 1, 1)
+    got = f.download()
+    want = orc.extend_newgrdpt(m, ff, dt, before, after, xm, ym, 1, 1, 1, 1)
+    new = np.ones(got.shape[1:], bool)
+    new[1:-1, 1:-1] = False
+    err = np.abs(got[:, new] - want[:, new]).max() / np.abs(want[:, new]).max()
+    assert err < 1e-12, err
+    f.close()
+
+
+def test_characteristic_extension_rules():
+    d = _lib()
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = _edge_bubble()
+    f = d.Field(p, nodes)
+    f.set_far_field(ic.far_field(p, 0.95, (0.3, 0.0)))
+    with pytest.raises(d.capi.Dim2dError):  # no step yet: no state at t - dt
+        f.extend(0, 1, 0, 0)
+    f.run(2, dt=1e-4)
+    with pytest.raises(d.capi.Dim2dError):  # one layer per call
+        f.extend(0, 2, 0, 0)
+    with pytest.raises(d.capi.Dim2dError):
+        f.adapt_domain("E", distance=5, grow=4)
+    f.extend(0, 1, 0, 0)
+    with pytest.raises(d.capi.Dim2dError):  # the snapshot belongs to the old grid
+        f.extend(0, 1, 0, 0)
+    f.run(1, dt=1e-4)
+    f.extend(0, 1, 0, 0)
+    assert f.params.nx == p.nx + 2
+    f.set_far_field(None)  # copy rule again, any number of layers
+    f.extend(0, 3, 0, 0)
+    with pytest.raises(ValueError):
+        f.set_far_field([1.0, 0.0])
+    with pytest.raises(d.capi.Dim2dError):
+        f.set_far_field([float("nan"), 0.0, 0.0, 1.0])
+    f.close()
+    q = d.ModelParams(nx=40, ny=40)  # reflection edges
+    g = d.Field(q, np.ones((4, 40, 40)))
+    with pytest.raises(d.capi.Dim2dError):
+        g.set_far_field([1.4, 0.0, 0.0, 2.0])
+    g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_far_field_context_steps_like_a_plain_one(orc, precision):
+    """The frame snapshot is a side copy: with a far field set, steps (graph
+    replays included) and a characteristic extension leave the trajectory of
+    the state bitwise that of a context without one started from the same
+    state."""
+    d = _lib()
+    f, m, before, after, dt, ff = _stepped_bubble(d, orc, precision)
+    f.extend(0, 1, 0, 1)
+    g = d.Field(f.params, f.download())
+    f.run(19, dt=dt)
+    g.run(19, dt=dt)
+    assert np.array_equal(f.download(), g.download())
+    f.extend(0, 1, 0, 1)
+    g2 = d.Field(f.params, f.download())
+    f.run(10, cfl=0.1)
+    g2.run(10, cfl=0.1)
+    assert np.array_equal(f.download(), g2.download())
+    for h in (f, g, g2):
+        h.close()
+
+
+def test_characteristic_growth_ahead_of_a_bubble_matches_a_large_domain():
+    """512^2, a vapour bubble whose interface starts 100 points from the east
+    edge, carried by (u, v) = (0.2, 0.05); adapt_domain every 40 steps with
+    the reference's new-point rule (a band of 120 points, density criterion
+    eps = 0.3) grows the east edge one layer at a time.  The interface and
+    its acoustic transient stay clear of the edge over the 480 steps, so the
+    interior of the grown grid must agree with a grid that is large from the
+    start: new points extrapolated from the near-uniform state at the edge
+    reproduce it.  What reaches the edge is the stencils' numerical tail of
+    the bubble's field (a few points per step): measured 1.3e-10 relative
+    (momentum), bounded here at 1e-8."""
+    d = _lib()
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    n, gap, radius, flow = 512, 100, 0.1, (0.2, 0.05)
+    dx = 1.0 / (n - 1)
+    center = (0.5 - radius - gap * dx, 0.0)
+    p, nodes = _edge_bubble(n=n, center=center, radius=radius, flow=flow)
+    dt = ic.cfl_dt(p, nodes, 0.1)
+    f = d.Field(p, nodes)
+    f.set_far_field(ic.far_field(p, 0.95, flow))
+    added = np.zeros(4, dtype=np.int64)
+    for _ in range(12):
+        f.run(40, dt=dt)
+        added += f.adapt_domain("WESN", distance=120, grow=1, md_threshold=0.3)
+    print("added per edge", added, "final grid", f.params.nx, f.params.ny)
+    assert added[1] >= 4 and added[0] == 0 and added[2] == 0 and added[3] == 0
+    small = f.download()
+    q = d.ModelParams(nx=f.params.nx + 40, ny=f.params.ny, bc=OPEN, x_min=f.params.x_min,
+                      x_max=f.params.x_max + 40 * dx, y_min=f.params.y_min, y_max=f.params.y_max)
+    big0 = ic.circles_field(q, 0.95, [center], radius, "vapor", 2.0 * np.sqrt(q.we) * dx, flow=flow)
+    g = d.Field(q, big0)
+    g.run(480, dt=dt)
+    big = g.download()[:, :, :small.shape[2]]
+    # the interior (the open frames differ: the small grid's east frame is
+    # interior in the large one, and its NE / SE corners are corners there)
+    err = max_rel_err(small, big)
+    print("grown vs large domain rel err", err)
+    assert max(err) < 1e-8, err
+    f.close()
+    g.close()
+
+
+@pytest.mark.parametrize("edges", [(0, 1, 0, 1), (1, 0, 1, 0)])
+def test_repeated_growth_matches_oracle(orc, edges):
+    """Steps and one-layer growths in turn (8 steps, grow, ... 6 times) on a
+    smooth pulse in a uniform flow: the device field equals the oracle's
+    (orc_run then orc_extend_newgrdpt on the oracle's own states) within
+    1e-11 after every growth."""
+    d = _lib()
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    n, flow = 40, (0.3, 0.1) if edges[1] else (-0.3, -0.1)
+    p = d.ModelParams(nx=n + 4, ny=n + 4, bc=OPEN, precision=0)
+    ff = ic.far_field(p, 0.95, flow)
+    X, Y = np.meshgrid(*p.coordinates())
+    rho = ff[0] * (1.0 + 0.02 * np.exp(-((X - 0.3) ** 2 + (Y - 0.25) ** 2) / 0.02))
+    nodes = np.empty((4, p.ny, p.nx))
+    nodes[0], nodes[1], nodes[2] = rho, rho * flow[0], rho * flow[1]
+    nodes[3] = 0.5 * rho * (flow[0] ** 2 + flow[1] ** 2) + rho * (8 / 3 * p.cv_r * 0.95 - 3 * rho)
+    dt = ic.cfl_dt(p, nodes, 0.1)
+    f = d.Field(p, nodes)
+    f.set_far_field(ff)
+    ref = nodes.copy()
+    xm, ym = p.coordinates()
+    for g in range(6):
+        m = _model(orc, d.ModelParams(nx=ref.shape[2], ny=ref.shape[1], bc=OPEN))
+        for _ in range(8):
+            prev = ref.copy()
+            assert orc.run(m, ref, xm, ym, np.array([dt])) == 0
+        f.run(8, dt=dt)
+        ref = orc.extend_newgrdpt(m, ff, dt, prev, ref, xm, ym, *edges)
+        f.extend(*edges)
+        xm, ym = f.params.coordinates()
+        got = f.download()
+        err = max_rel_err(got, ref, region=(slice(None), slice(None)))
+        print("growth", g, "grid", got.shape, "rel err", err, flush=True)
+        assert max(err) < 1e-11, (g, err)
+    f.close()
