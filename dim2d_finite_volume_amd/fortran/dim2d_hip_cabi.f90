@@ -158,6 +158,13 @@ module dim2d_hip_cabi
       integer(c_int) :: st
     end function dim2d_openbc_undermined
 
+    function dim2d_set_far_field(ctx, far_field) bind(C, name="dim2d_set_far_field") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: far_field(4)
+      integer(c_int) :: st
+    end function dim2d_set_far_field
+
     function dim2d_extend(ctx, add_w, add_e, add_s, add_n) bind(C, name="dim2d_extend") result(st)
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
