@@ -121,20 +121,26 @@ class Field:
         capi.check(self._lib.dim2d_apply_bc(self._ctx, self.time), "dim2d_apply_bc")
 
     def write_data(self, path: str, dt: float | None = None, t_max: float | None = None,
-                   attrs: dict | None = None) -> None:
+                   attrs: dict | None = None, ic=None, detail_print: float | None = None,
+                   md_threshold: float | None = None) -> None:
         """field%write_data (field_abstract_class.f:597-610): the current state
-        as one output file in the reference's NetCDF layout (io_netcdf)."""
+        as one output file in the reference's NetCDF layout, with the header
+        of nf90_write_header (io_netcdf; ic: an io_netcdf.InitialCondition)."""
         from . import io_netcdf
         io_netcdf.write_data(path, self.download(), self.x_map, self.y_map, self.time,
-                             self.params, dt=dt, t_max=t_max, attrs=attrs)
+                             self.params, dt=dt, t_max=t_max, attrs=attrs, ic=ic,
+                             detail_print=detail_print, md_threshold=md_threshold)
 
     @classmethod
-    def restart(cls, params: ModelParams, path: str) -> "Field":
+    def restart(cls, params: ModelParams | None, path: str) -> "Field":
         """A field whose state and time come from an output file: the
         reference's restart (-r, io_operators_class.f:183-236).  The file's
-        grid must be the one params describe."""
+        grid must be the one params describe; with params None they come
+        from the file's header (io_netcdf.header_params)."""
         from . import io_netcdf
-        nodes, x_map, y_map, time, _ = io_netcdf.read_data(path)
+        nodes, x_map, y_map, time, attrs = io_netcdf.read_data(path)
+        if params is None:
+            params, _ = io_netcdf.header_params(attrs, nodes.shape[2], nodes.shape[1])
         if nodes.shape != (4, params.ny, params.nx):
             raise ValueError(f"{path}: grid {nodes.shape[2]}x{nodes.shape[1]} is not "
                              f"{params.nx}x{params.ny}")

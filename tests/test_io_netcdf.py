@@ -1,6 +1,8 @@
 """Output files in the reference's NetCDF layout (nf90_operators_module.f
 nf90_def_var_model / nf90_put_var_model): structure, the Fortran element
 order and exact round trips (CPU only)."""
+import dataclasses
+
 import numpy as np
 import pytest
 from scipy.io import netcdf_file
@@ -37,6 +39,8 @@ def test_layout_matches_the_reference_writer(tmp_path):
         assert f.title == b"DIM2D" and f.history.startswith(b"date ")
         assert (f.Re, f.We, f.Pr, f.viscous_r, f.cv_r, f.gravity) == (5.0, 10.0, 20.0, -1.5, 2.5, 0.0)
         assert (f.x_min, f.x_max, f.y_min, f.y_max, f.dt) == (-0.5, 0.5, -0.5, 0.5, 1e-3)
+        # real(rkind) attributes are doubles, as nf90_put_att writes them
+        assert all(isinstance(getattr(f, a), np.float64) for a in ("dt", "Re", "x_min"))
         assert np.array_equal(f.variables["x"][:], xm) and np.array_equal(f.variables["y"][:], ym)
     finally:
         f.close()
@@ -59,3 +63,100 @@ def test_bad_shapes_rejected(tmp_path):
     xm, ym = p.coordinates()
     with pytest.raises(ValueError):
         io_netcdf.write_data(str(tmp_path / "x.nc"), np.zeros((4, 10, 8)), xm, ym, 0.0, p)
+
+
+# The global attribute names nf90_write_header writes (nf90_operators_module.f:
+# 200-336, :352-597, :613-894, :913-952), in order, for three configurations
+# with the reference's default switches (parameters_input.f: perturbations of
+# the open boundary off, obc_eigenqties_bc, obc_edge_xy_flux,
+# obc_edge_flux_capillarity, obc_edge_overlap_ac on).
+HEAD = ["title", "history", "institution", "source", "prog_commit", "references", "convention",
+        "x_min", "x_max", "y_min", "y_max", "t_max", "dt", "detail_print"]
+SIM = ["viscous_r", "Re", "We", "Pr", "cv_r", "gravity"]
+REF_NAMES = {
+    # bubble_transported with hedstrom_xy and the density threshold of the
+    # domain extension on
+    "transported": HEAD + ["initial_conditions", "flow_velocity", "temperature",
+                           "boundary_conditions", "obc_eigenqties_strategy",
+                           "obc_edge_xy_strategy", "obc_edge_flux_strategy",
+                           "obc_edge_overlap_ac", "openbc_md_threshold_ac",
+                           "openbc_md_threshold"] + SIM,
+    # bubble_next_to_wall with a flow, a constant wall heat source and
+    # wall_S_reflection (the reference also writes wall_heat_source_stop when
+    # its stop switch is on: this solver has no stop)
+    "wall": HEAD + ["initial_conditions", "flow_velocity", "temperature", "flow_profile",
+                    "phase_at_center", "wall_surface_type", "wall_micro_contact_angle",
+                    "wall_heat_source", "wall_maximum_heat_source", "boundary_conditions"] + SIM,
+    # drop_retraction with the initial perturbation on, reflection_xy
+    "drop": HEAD + ["initial_conditions", "flow_velocity", "temperature", "ic_perturbation_ac",
+                    "ic_perturbation_amp", "boundary_conditions"] + SIM,
+}
+
+
+def _configs():
+    IC = io_netcdf.InitialCondition
+    return {
+        "transported": (ModelParams(nx=20, ny=18, bc=(3, 3, 3, 3)),
+                        IC("bubble_transported", 0.95, 0.1), 0.3),
+        "wall": (ModelParams(nx=20, ny=18, bc=(0, 0, 2, 0), wall_contact_angle=60.0,
+                             wall_heat_flux=0.02, gravity=0.5),
+                 IC("bubble_next_to_wall", 0.93, 0.05, "linear_profile", "vapor"), None),
+        "drop": (ModelParams(nx=20, ny=18, we=12.0),
+                 IC("drop_retraction", 0.97, 0.0, phase_at_center="liquid",
+                    perturbation_amp=1e-3), None),
+    }
+
+
+@pytest.mark.parametrize("name", ["transported", "wall", "drop"])
+def test_header_names_follow_the_reference(tmp_path, name):
+    p, ic, thr = _configs()[name]
+    xm, ym = p.coordinates()
+    path = str(tmp_path / "data0.nc")
+    io_netcdf.write_data(path, _state(20, 18), xm, ym, 0.0, p, dt=1e-4, t_max=2.0, ic=ic,
+                         detail_print=0.0003, md_threshold=thr)
+    f = netcdf_file(path, "r", mmap=False)
+    try:
+        assert list(f._attributes) == REF_NAMES[name]
+    finally:
+        f.close()
+
+
+@pytest.mark.parametrize("name", ["transported", "wall", "drop"])
+def test_header_round_trip(tmp_path, name):
+    p, ic, thr = _configs()[name]
+    xm, ym = p.coordinates()
+    path = str(tmp_path / "data1.nc")
+    io_netcdf.write_data(path, _state(20, 18), xm, ym, 0.5, p, dt=1e-4, t_max=2.0, ic=ic,
+                         md_threshold=thr)
+    _, _, _, _, attrs = io_netcdf.read_data(path)
+    q, ic2 = io_netcdf.header_params(attrs, p.nx, p.ny)
+    assert q == p
+    if ic.name not in io_netcdf.WALL_ICS:
+        # the header carries phase and flow profile for the wall conditions only
+        ic = dataclasses.replace(ic, phase_at_center="vapor", flow_profile="parabolic_profile")
+    assert ic2 == ic
+
+
+def test_reference_header_is_read():
+    """A header as the reference's writer fills it (codes trimmed or padded
+    as nf90_write_header does) gives the model, domain, boundary and initial
+    condition back."""
+    attrs = {"title": "DIM2D", "history": "date 2015/03/02 time 10:11:12",
+             "institution": "x", "source": "x", "prog_commit": "x", "references": "x",
+             "convention": "cf-1.6", "x_min": -0.25, "x_max": 0.75, "y_min": 0.0,
+             "y_max": 1.0, "t_max": 10.0, "dt": 1e-3, "detail_print": 0.01,
+             "initial_conditions": "bubble_next_to_wall", "flow_velocity": 0.2,
+             "temperature": 0.95, "flow_profile": "parabolic_profile",
+             "phase_at_center": "vapor ", "wall_surface_type": "uniform_surface     ",
+             "wall_micro_contact_angle": 45.0, "wall_heat_source": "constant_heat_source",
+             "wall_maximum_heat_source": 0.01, "boundary_conditions": "wall_S_reflection",
+             "viscous_r": -1.5, "Re": 5.0, "We": 10.0, "Pr": 20.0, "cv_r": 2.5,
+             "gravity": 0.0}
+    p, ic = io_netcdf.header_params(attrs, 30, 30)
+    assert p.bc == (0, 0, 2, 0)
+    assert (p.x_min, p.x_max, p.y_min, p.y_max) == (-0.25, 0.75, 0.0, 1.0)
+    assert (p.wall_contact_angle, p.wall_heat_flux) == (45.0, 0.01)
+    assert ic == io_netcdf.InitialCondition("bubble_next_to_wall", 0.95, 0.2,
+                                            "parabolic_profile", "vapor")
+    with pytest.raises(ValueError):
+        io_netcdf.header_params(dict(attrs, boundary_conditions="poinsot_xy"), 30, 30)
