@@ -751,21 +751,31 @@ def test_check_steady_state_uniform_and_nonfinite():
 @pytest.mark.parametrize("case", ["periodic_x_wall_y", "open"])
 def test_restart_from_output_file_continues_exactly(orc, tmp_path, case):
     """field%write_data then a restart from that file (-r) continues the run
-    bit for bit: the file holds the state, ghost cells included, and the time."""
-    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case])
+    bit for bit: the file holds the state, ghost cells included, and the time.
+    A restart that takes its parameters from the file's header (wall angle
+    and heat flux through the description of a wall initial condition) does
+    the same."""
+    from dim2d_finite_volume_amd import io_netcdf
+    phys = {"wall_contact_angle": 70.0, "wall_heat_flux": 0.01} if case != "open" else {}
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], **phys)
     dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
     f = d.Field(p, nodes)
     f.run(5, dt=dt)
     path = str(tmp_path / "data1.nc")
-    f.write_data(path, dt=dt)
+    ic = io_netcdf.InitialCondition("bubble_next_to_wall" if phys else "bubble_transported")
+    f.write_data(path, dt=dt, ic=ic)
     g = d.Field.restart(p, path)
-    assert g.time == f.time
+    h = d.Field.restart(None, path)
+    assert h.params == p
+    assert g.time == f.time == h.time
     f.run(7, dt=dt)
     g.run(7, dt=dt)
+    h.run(7, dt=dt)
     assert np.array_equal(f.download(), g.download())
+    assert np.array_equal(f.download(), h.download())
     assert np.array_equal(f.device_time(), g.device_time())
-    f.close()
-    g.close()
+    for x in (f, g, h):
+        x.close()
 
 
 # ---- Cockburn and Gau operators ----
