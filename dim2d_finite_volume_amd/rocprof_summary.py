@@ -13,12 +13,18 @@ roofline's `traffic` and `issue` entries):
             reads for a calibration kernel of known size in the same run
   issue:    instructions per dispatch (SQ_INSTS_VALU, SQ_INSTS_LDS, ...) and
             wave count from one `--pmc` run, summed over the device
+  gaps:     the time of a step split into kernels and the idle gaps between
+            them, from a kernel trace (<prefix>_kernel_trace.csv), for steps
+            [skip, skip + steps) of each precision (a step starts at its
+            stage-1 or stage-pair launch)
 
     python -m dim2d_finite_volume_amd.rocprof_summary stats STATS.csv OUT.md [--trace TRACE.csv] --command "..."
     python -m dim2d_finite_volume_amd.rocprof_summary traffic FETCH.csv WRITE.csv OUTDIR \
         --calib SUBSTRING:READ_BYTES:WRITE_BYTES --workload two_bubbles_4096 --command "..."
     python -m dim2d_finite_volume_amd.rocprof_summary issue COUNTERS.csv OUTDIR \
         --workload two_bubbles_4096 --command "..."
+    python -m dim2d_finite_volume_amd.rocprof_summary gaps TRACE.csv OUT.md --skip 2 \
+        --steps 20 --command "..."
 """
 import argparse
 import csv
@@ -186,6 +192,83 @@ def issue(args):
     print("\n".join(lines))
 
 
+def step_kind(name):
+    """(precision, role) of a kernel of the stepping, or None"""
+    m = PAIR_RE.search(name)
+    if m:
+        return int(m.group(1)), "pair"
+    m = STAGE_RE.search(name)
+    if m and m.group(2) in ("1", "2", "3"):
+        return int(m.group(1)), f"stage{m.group(2)}"
+    if "dim2d::wall_kernel" in name:
+        return None, "wall"
+    if "dim2d::frame_kernel" in name:
+        return None, "frame"
+    return None
+
+
+def gaps(args):
+    rows = []
+    with open(args.trace) as f:
+        for r in csv.DictReader(f):
+            k = step_kind(r["Kernel_Name"])
+            if k is not None:
+                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), k))
+    rows.sort()
+    lines = ["# Step time: kernels and idle gaps (rocprofv3 --kernel-trace)", "",
+             f"Command: `{args.command}`", "",
+             f"Steps {args.skip} to {args.skip + args.steps - 1} of each precision; a step runs "
+             "from its first launch (stage pair or stage 1) to the next step's.  Busy: the "
+             "kernels' own durations (overlapping launches counted once); idle: the rest.", ""]
+    for prec, label in ((0, "FP64"), (1, "mixed")):
+        # the wall and frame kernels carry no precision: they belong to the
+        # precision of the stage launch before them
+        seq, cur = [], None
+        for r in rows:
+            if r[2][0] is not None:
+                cur = r[2][0]
+            if cur == prec:
+                seq.append(r)
+        starts = [i for i, r in enumerate(seq) if r[2][1] in ("pair", "stage1")]
+        sel = starts[args.skip:args.skip + args.steps + 1]
+        if len(sel) < 2:
+            continue
+        periods, busy, by_role, trans = [], [], defaultdict(float), defaultdict(list)
+        for a, b in zip(sel, sel[1:]):
+            t0, t1 = seq[a][0], seq[b][0]
+            periods.append((t1 - t0) / 1e3)
+            spans = sorted((r[0], min(r[1], t1)) for r in seq[a:b])
+            tot, end = 0, t0
+            for s0, s1 in spans:
+                if s1 > end:
+                    tot += s1 - max(s0, end)
+                    end = s1
+            busy.append(tot / 1e3)
+            for r in seq[a:b]:
+                by_role[r[2][1]] += (r[1] - r[0]) / 1e3
+            for x, y in zip(seq[a:b + 1], seq[a + 1:b + 1]):
+                trans[(x[2][1], y[2][1])].append(max(0, y[0] - x[1]) / 1e3)
+        n = len(periods)
+        per = sorted(periods)[n // 2]
+        mean_p, mean_b = sum(periods) / n, sum(busy) / n
+        lines += [f"## {label}: {n} steps", "",
+                  f"- step period: median {per:.1f} us, mean {mean_p:.1f} us",
+                  f"- kernels busy: {mean_b:.1f} us per step ({100 * mean_b / mean_p:.1f} %); "
+                  f"idle {mean_p - mean_b:.1f} us ({100 * (1 - mean_b / mean_p):.1f} %)", "",
+                  "| kernel | us per step | % of step |", "|---|---:|---:|"]
+        for role, t in sorted(by_role.items(), key=lambda kv: -kv[1]):
+            lines.append(f"| {role} | {t / n:.1f} | {100 * t / n / mean_p:.1f} |")
+        lines += ["", "| gap after -> before | count per step | mean us | total us per step |",
+                  "|---|---:|---:|---:|"]
+        for (x, y), v in sorted(trans.items(), key=lambda kv: -sum(kv[1])):
+            lines.append(f"| {x} -> {y} | {len(v) / n:.1f} | {sum(v) / len(v):.2f} | "
+                         f"{sum(v) / n:.2f} |")
+        lines.append("")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sp = ap.add_subparsers(dest="mode", required=True)
@@ -207,8 +290,14 @@ def main():
     c.add_argument("outdir")
     c.add_argument("--workload", required=True)
     c.add_argument("--command", required=True)
+    g = sp.add_parser("gaps")
+    g.add_argument("trace")
+    g.add_argument("out")
+    g.add_argument("--skip", type=int, default=2)
+    g.add_argument("--steps", type=int, default=20)
+    g.add_argument("--command", required=True)
     args = ap.parse_args()
-    {"stats": stats, "traffic": traffic, "issue": issue}[args.mode](args)
+    {"stats": stats, "traffic": traffic, "issue": issue, "gaps": gaps}[args.mode](args)
 
 
 if __name__ == "__main__":
