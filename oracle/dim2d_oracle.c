@@ -14,6 +14,20 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Planted mistakes in the step composition, for the test that shows the
+ * composition pins fail on each of them (tests/test_oracle_composition.py
+ * compiles copies with -DORC_PLANT=1..9).  The default build plants none:
+ *   1 no ghost fill after stage 1          2 stage 2 blends the stage-1 state
+ *   3 stage-3 weight 1/4 for 1/3           4 periodic x / walls y: wall rows
+ *   5 periodic x / walls y: ghost rows       before the x-periodic fill
+ *     not x-periodic (stale corners)       6 reflection: corners before W/E
+ *   7 stages 2, 3 take the derivative      8 walls W/E, periodic y: ghost
+ *     of u^n                                 columns not y-periodic
+ *   9 periodic x and y: the y fill over the interior columns only */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 /* NF90_FILL_DOUBLE, the reference's debug_real (parameters_input.f:892) */
 #define DEBUG_REAL 9.9692099683868690e+36
 #define BC_SIZE 2
@@ -1436,8 +1450,19 @@ void orc_apply_bc_on_nodes(const orc_model* m, double t, const double* x_map, co
   const int allrefl = m->bc_w == ORC_BC_REFLECTION && m->bc_e == ORC_BC_REFLECTION &&
                       m->bc_s == ORC_BC_REFLECTION && m->bc_n == ORC_BC_REFLECTION;
   if (allrefl) {
+    if (ORC_PLANT == 6) { /* planted: corners from the W/E ghosts of the last fill */
+      reflect_y(m, nodes, 1, 1, 1, B);
+      reflect_y(m, nodes, 1, 1, nx - B + 1, nx);
+      reflect_y(m, nodes, 0, ny - B + 1, 1, B);
+      reflect_y(m, nodes, 0, ny - B + 1, nx - B + 1, nx);
+    }
     reflect_x(m, nodes, 1, 1, B + 1, ny - B);              /* W */
     reflect_x(m, nodes, 0, nx - B + 1, B + 1, ny - B);     /* E */
+    if (ORC_PLANT == 6) {
+      reflect_y(m, nodes, 1, 1, B + 1, nx - B);
+      reflect_y(m, nodes, 0, ny - B + 1, B + 1, nx - B);
+      return;
+    }
     reflect_y(m, nodes, 1, 1, 1, B);                       /* SW (reflection_y) */
     reflect_y(m, nodes, 1, 1, B + 1, nx - B);              /* S */
     reflect_y(m, nodes, 1, 1, nx - B + 1, nx);             /* SE */
@@ -1448,15 +1473,18 @@ void orc_apply_bc_on_nodes(const orc_model* m, double t, const double* x_map, co
   }
   if (m->bc_w == ORC_BC_PERIODIC && m->bc_s == ORC_BC_PERIODIC) {
     periodic_x(m, nodes, 1 + B, ny - B);
-    periodic_y(m, nodes, 1, nx);
+    if (ORC_PLANT == 9) periodic_y(m, nodes, 1 + B, nx - B); /* planted */
+    else periodic_y(m, nodes, 1, nx);
     return;
   }
   if (m->bc_w == ORC_BC_PERIODIC) { /* periodic x, walls (or reflection) in y */
-    periodic_x(m, nodes, 1 + B, ny - B);
+    if (ORC_PLANT != 4) periodic_x(m, nodes, 1 + B, ny - B);
     if (m->bc_s == ORC_BC_WALL) wall_edge(m, nodes, nodes_tmp, 0, dx, dy);
     else reflect_y(m, nodes, 1, 1, B + 1, nx - B);
     if (m->bc_n == ORC_BC_WALL) wall_edge(m, nodes, nodes_tmp, 1, dx, dy);
     else reflect_y(m, nodes, 0, ny - B + 1, B + 1, nx - B);
+    if (ORC_PLANT == 4) periodic_x(m, nodes, 1 + B, ny - B); /* planted: after the walls */
+    if (ORC_PLANT == 5) return;                              /* planted: no ghost-row fill */
     periodic_x(m, nodes, 1, B);
     periodic_x(m, nodes, ny - B + 1, ny);
     return;
@@ -1472,6 +1500,7 @@ void orc_apply_bc_on_nodes(const orc_model* m, double t, const double* x_map, co
       else reflect_x(m, nodes, 1, 1, B + 1, ny - B);
       if (m->bc_e == ORC_BC_WALL) wall_edge(m, nodes, nodes_tmp, 3, dx, dy);
       else reflect_x(m, nodes, 0, nx - B + 1, B + 1, ny - B);
+      if (ORC_PLANT == 8) return; /* planted: no ghost-column fill */
       periodic_y(m, nodes, 1, B);
       periodic_y(m, nodes, nx - B + 1, nx);
       return;
@@ -1512,7 +1541,10 @@ void orc_rk3_step(const orc_model* m, double t, double dt, const double* x_map,
   double* time_dev = work + plane * 4;
   const int B = BC_SIZE;
   const double b2 = 0.75;
-  const double b3 = 1.0 / 3.0;
+  const double b3 = ORC_PLANT == 3 ? 0.25 : 1.0 / 3.0;
+  /* the state stages 2 and 3 blend with (u^n) and differentiate */
+  const double* blend = ORC_PLANT == 2 ? nodes : nodes_tmp;
+  const double* dsrc = ORC_PLANT == 7 ? nodes_tmp : nodes;
 
   /* stage 1 (compute_1st_step :52-100) over the whole array */
   orc_compute_time_dev(m, t, nodes, x_map, y_map, time_dev);
@@ -1520,7 +1552,7 @@ void orc_rk3_step(const orc_model* m, double t, double dt, const double* x_map,
     nodes_tmp[q] = nodes[q];
     nodes[q] = nodes[q] + dt * time_dev[q];
   }
-  orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
+  if (ORC_PLANT != 1) orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
 
   /* x/y_borders (field_abstract_class.f:244-270): widened to the ghost cells
    * on edges whose time derivative is computed (open boundaries) */
@@ -1529,21 +1561,21 @@ void orc_rk3_step(const orc_model* m, double t, double dt, const double* x_map,
   const int j0 = op ? 1 : B + 1, j1 = op ? ny : ny - B;
 
   /* stage 2 (compute_2nd_step :205-269) over x/y_borders */
-  orc_compute_time_dev(m, t, nodes, x_map, y_map, time_dev);
+  orc_compute_time_dev(m, t, dsrc, x_map, y_map, time_dev);
   for (int k = 1; k <= 4; ++k)
     for (int j = j0; j <= j1; ++j)
       for (int i = i0; i <= i1; ++i)
         ND(nodes, i, j, k) =
-            b2 * ND(nodes_tmp, i, j, k) + (1.0 - b2) * (ND(nodes, i, j, k) + dt * ND(time_dev, i, j, k));
+            b2 * ND(blend, i, j, k) + (1.0 - b2) * (ND(nodes, i, j, k) + dt * ND(time_dev, i, j, k));
   orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
 
   /* stage 3 (compute_3rd_step :424-490) */
-  orc_compute_time_dev(m, t, nodes, x_map, y_map, time_dev);
+  orc_compute_time_dev(m, t, dsrc, x_map, y_map, time_dev);
   for (int k = 1; k <= 4; ++k)
     for (int j = j0; j <= j1; ++j)
       for (int i = i0; i <= i1; ++i)
         ND(nodes, i, j, k) =
-            b3 * ND(nodes_tmp, i, j, k) + (1.0 - b3) * (ND(nodes, i, j, k) + dt * ND(time_dev, i, j, k));
+            b3 * ND(blend, i, j, k) + (1.0 - b3) * (ND(nodes, i, j, k) + dt * ND(time_dev, i, j, k));
   orc_apply_bc_on_nodes(m, t, x_map, y_map, nodes_tmp, nodes);
 }
 

@@ -7,6 +7,7 @@ layout nodes(nx, ny, ne); in numpy they are held as C arrays of shape
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -64,82 +65,99 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        M = ctypes.POINTER(Model)
-        L.orc_flux_term.restype = ctypes.c_double
-        L.orc_flux_term.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                    ctypes.c_double, ctypes.c_double]
-        L.orc_mattsson_density_stencil.restype = ctypes.c_double
-        L.orc_mattsson_density_stencil.argtypes = L.orc_flux_term.argtypes
-        L.orc_cg_density_stencil.restype = ctypes.c_double
-        L.orc_cg_density_stencil.argtypes = L.orc_flux_term.argtypes
-        L.orc_compute_time_dev.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
-        L.orc_apply_bc_on_nodes.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
-        L.orc_rk3_step.argtypes = [M, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
-        L.orc_lsrk3_step.argtypes = L.orc_rk3_step.argtypes
-        L.orc_classical_pressure.restype = ctypes.c_double
-        L.orc_classical_pressure.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int]
-        L.orc_openbc_timedev.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                         ctypes.c_void_p, _dp]
-        L.orc_lsrk3_amplification.restype = ctypes.c_double
-        L.orc_lsrk3_amplification.argtypes = [ctypes.c_double]
-        L.orc_run.restype = ctypes.c_int
-        L.orc_run.argtypes = [M, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
-        L.orc_coordinates.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp, _dp]
-        L.orc_cfl_dt.argtypes = [M, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
-        L.orc_sums.argtypes = [M, _dp, _dp]
-        L.orc_openbc_undermined_point.restype = ctypes.c_int
-        L.orc_openbc_undermined_point.argtypes = [M, _dp]
-        L.orc_openbc_undermined_at.restype = ctypes.c_int
-        L.orc_openbc_undermined_at.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                               ctypes.c_double, ctypes.c_double]
-        L.orc_openbc_undermined.argtypes = [M, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                            ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
-        for name in ("orc_mass_density_liquid", "orc_mass_density_vapor"):
-            getattr(L, name).restype = ctypes.c_double
-            getattr(L, name).argtypes = [ctypes.c_double]
-        L.orc_surface_tension.restype = ctypes.c_double
-        L.orc_surface_tension.argtypes = [M, ctypes.c_double]
-        L.orc_speed_of_sound_local.restype = ctypes.c_double
-        L.orc_speed_of_sound_local.argtypes = [M, _dp]
-        L.orc_dwall_internal_energy_dmd.restype = ctypes.c_double
-        L.orc_dwall_internal_energy_dmd.argtypes = [M] + [ctypes.c_double] * 3
-        L.orc_wall_temperature.restype = ctypes.c_double
-        L.orc_wall_temperature.argtypes = [M] + [ctypes.c_double] * 5
-        L.orc_wall_root_f.restype = ctypes.c_double
-        L.orc_wall_root_f.argtypes = [M, ctypes.c_double, _dp, _dp] + [ctypes.c_double] * 5
-        L.orc_wall_md_ghost.restype = ctypes.c_double
-        L.orc_wall_md_ghost.argtypes = [M, ctypes.c_double, ctypes.c_double, _dp, _dp] + \
-            [ctypes.c_double] * 5
-        L.orc_wall_flux.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
-        ROOTFN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
-        L.ROOTFN = ROOTFN
-        L.orc_root_brackets.restype = ctypes.c_int
-        L.orc_root_brackets.argtypes = [ROOTFN, ctypes.c_void_p, ctypes.c_double,
-                                        ctypes.c_double, _dp]
-        L.orc_ridders.restype = ctypes.c_double
-        L.orc_ridders.argtypes = [ROOTFN, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
-                                  ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
-        G = ctypes.POINTER(BfGrid)
-        _ip = ctypes.POINTER(ctypes.c_int)
-        L.orc_interp_coeff_1d.argtypes = [_dp, _dp, _dp]
-        L.orc_interpolate_1d.argtypes = [ctypes.c_double, _dp, _dp]
-        L.orc_interp_coeff_2d.argtypes = [_dp, _dp, _dp, _dp]
-        L.orc_interpolate_2d.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
-        L.orc_newton_cotes.argtypes = [_dp, _dp, ctypes.c_double, _dp]
-        ng = [M, _dp, ctypes.c_int, ctypes.c_double, G, G, ctypes.c_int, ctypes.c_int]
-        L.orc_newgrdpt_x.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
-        L.orc_newgrdpt_y.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
-        L.orc_newgrdpt_xy.argtypes = ng + [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp]
-        L.orc_newgrdpt_local.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
-        L.orc_newgrdpt_config_procedure.restype = ctypes.c_int
-        L.orc_newgrdpt_config_procedure.argtypes = [ctypes.c_int, _ip, _ip]
-        L.orc_extend_newgrdpt.restype = ctypes.c_int
-        L.orc_extend_newgrdpt.argtypes = [M, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _ip, _dp]
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path: str):
+    """Route this module's calls to another build of the oracle at `path` (the
+    copies with a planted mistake that the composition tests compile)."""
+    global _lib
+    saved = lib()
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _load(path: str):
+    L = ctypes.CDLL(path)
+    M = ctypes.POINTER(Model)
+    L.orc_flux_term.restype = ctypes.c_double
+    L.orc_flux_term.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, ctypes.c_double]
+    L.orc_mattsson_density_stencil.restype = ctypes.c_double
+    L.orc_mattsson_density_stencil.argtypes = L.orc_flux_term.argtypes
+    L.orc_cg_density_stencil.restype = ctypes.c_double
+    L.orc_cg_density_stencil.argtypes = L.orc_flux_term.argtypes
+    L.orc_compute_time_dev.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
+    L.orc_apply_bc_on_nodes.argtypes = [M, ctypes.c_double, _dp, _dp, _dp, _dp]
+    L.orc_rk3_step.argtypes = [M, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+    L.orc_lsrk3_step.argtypes = L.orc_rk3_step.argtypes
+    L.orc_classical_pressure.restype = ctypes.c_double
+    L.orc_classical_pressure.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int]
+    L.orc_openbc_timedev.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_void_p, _dp]
+    L.orc_lsrk3_amplification.restype = ctypes.c_double
+    L.orc_lsrk3_amplification.argtypes = [ctypes.c_double]
+    L.orc_run.restype = ctypes.c_int
+    L.orc_run.argtypes = [M, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.orc_coordinates.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp, _dp]
+    L.orc_cfl_dt.argtypes = [M, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+    L.orc_sums.argtypes = [M, _dp, _dp]
+    L.orc_openbc_undermined_point.restype = ctypes.c_int
+    L.orc_openbc_undermined_point.argtypes = [M, _dp]
+    L.orc_openbc_undermined_at.restype = ctypes.c_int
+    L.orc_openbc_undermined_at.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double]
+    L.orc_openbc_undermined.argtypes = [M, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    for name in ("orc_mass_density_liquid", "orc_mass_density_vapor"):
+        getattr(L, name).restype = ctypes.c_double
+        getattr(L, name).argtypes = [ctypes.c_double]
+    L.orc_surface_tension.restype = ctypes.c_double
+    L.orc_surface_tension.argtypes = [M, ctypes.c_double]
+    L.orc_speed_of_sound_local.restype = ctypes.c_double
+    L.orc_speed_of_sound_local.argtypes = [M, _dp]
+    L.orc_dwall_internal_energy_dmd.restype = ctypes.c_double
+    L.orc_dwall_internal_energy_dmd.argtypes = [M] + [ctypes.c_double] * 3
+    L.orc_wall_temperature.restype = ctypes.c_double
+    L.orc_wall_temperature.argtypes = [M] + [ctypes.c_double] * 5
+    L.orc_wall_root_f.restype = ctypes.c_double
+    L.orc_wall_root_f.argtypes = [M, ctypes.c_double, _dp, _dp] + [ctypes.c_double] * 5
+    L.orc_wall_md_ghost.restype = ctypes.c_double
+    L.orc_wall_md_ghost.argtypes = [M, ctypes.c_double, ctypes.c_double, _dp, _dp] + \
+        [ctypes.c_double] * 5
+    L.orc_wall_flux.argtypes = [M, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+    ROOTFN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
+    L.ROOTFN = ROOTFN
+    L.orc_root_brackets.restype = ctypes.c_int
+    L.orc_root_brackets.argtypes = [ROOTFN, ctypes.c_void_p, ctypes.c_double,
+                                    ctypes.c_double, _dp]
+    L.orc_ridders.restype = ctypes.c_double
+    L.orc_ridders.argtypes = [ROOTFN, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    G = ctypes.POINTER(BfGrid)
+    _ip = ctypes.POINTER(ctypes.c_int)
+    L.orc_interp_coeff_1d.argtypes = [_dp, _dp, _dp]
+    L.orc_interpolate_1d.argtypes = [ctypes.c_double, _dp, _dp]
+    L.orc_interp_coeff_2d.argtypes = [_dp, _dp, _dp, _dp]
+    L.orc_interpolate_2d.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
+    L.orc_newton_cotes.argtypes = [_dp, _dp, ctypes.c_double, _dp]
+    ng = [M, _dp, ctypes.c_int, ctypes.c_double, G, G, ctypes.c_int, ctypes.c_int]
+    L.orc_newgrdpt_x.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_newgrdpt_y.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_newgrdpt_xy.argtypes = ng + [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp]
+    L.orc_newgrdpt_local.argtypes = ng + [ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_newgrdpt_config_procedure.restype = ctypes.c_int
+    L.orc_newgrdpt_config_procedure.argtypes = [ctypes.c_int, _ip, _ip]
+    L.orc_extend_newgrdpt.restype = ctypes.c_int
+    L.orc_extend_newgrdpt.argtypes = [M, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _ip, _dp]
+    return L
 
 
 def _ptr(a: np.ndarray):
