@@ -133,8 +133,9 @@ def cpu_baseline(seconds_hint=15.0):
 def batch_bench(steps, nsims=256, seed=4):
     """256 independent drop/bubble simulations, n^2 with n
     log-uniform in [64, 512], reflection edges, each with its own fixed dt
-    (CFL 0.1 of its initial state); one persistent kernel per RK stage under a
-    HIP graph with tiles chosen by dim2d_batch_autotune; FP64 and mixed."""
+    (CFL 0.1 of its initial state); persistent kernels under a HIP graph, RK
+    stages 1 and 2 fused in one (the stage pair) and stage 3 in another, with
+    tiles chosen by dim2d_batch_autotune; FP64 and mixed."""
     import dataclasses
 
     import numpy as np
@@ -152,6 +153,7 @@ def batch_bench(steps, nsims=256, seed=4):
             b.upload(i, nodes)
         mass0 = [b.download(i)[0, 2:-2, 2:-2].sum() for i in (0, nsims - 1)]
         rows, wg, tuned_ms = b.autotune(dts, 10)
+        tiles = b.tiles()
         b.integrate(dts, 2)
         t0 = time.perf_counter()
         b.integrate(dts, steps)
@@ -159,14 +161,18 @@ def batch_bench(steps, nsims=256, seed=4):
         last = [b.download(i) for i in (0, nsims - 1)]
         b.close()
         value = cells * steps / el
+        unfused = sum(ALG_BYTES[name][k] for k in ("stage1", "stage2", "stage3"))
+        fused = ALG_BYTES[name]["pair"] + ALG_BYTES[name]["stage3"]
         out[name] = {"value": value, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * el / steps,
-                     "hbm_frac_algorithmic": value * sum(ALG_BYTES[name][k] for k in
-                                                         ("stage1", "stage2", "stage3")) / HBM_PEAK,
-                     "tiles": {"rows": rows, "workgroups": wg, "autotune_ms_per_step": tuned_ms},
+                     "algorithmic_bytes_per_update": fused if tiles["stage_fusion"] else unfused,
+                     "hbm_frac_algorithmic": value * (fused if tiles["stage_fusion"] else unfused)
+                     / HBM_PEAK,
+                     "hbm_frac_unfused_bytes": value * unfused / HBM_PEAK,
+                     "tiles": dict(tiles, autotune_ms_per_step=tuned_ms),
                      "finite": bool(all(np.isfinite(x).all() for x in last)),
                      "mass_drift": [float((x[0, 2:-2, 2:-2].sum() - m0) / m0)
                                     for x, m0 in zip(last, mass0)]}
-        log(f"[bench] batch {name}: {value:.4e} updates/s, tiles rows={rows} wg={wg}")
+        log(f"[bench] batch {name}: {value:.4e} updates/s, tiles {tiles}")
     return out
 
 

@@ -32,7 +32,7 @@ EXPORTS = (
     "dim2d_check_steady_state", "dim2d_time_dev_noalloc",
     "dim2d_batch_create", "dim2d_batch_destroy", "dim2d_batch_upload", "dim2d_batch_download",
     "dim2d_batch_integrate", "dim2d_batch_set_tiles", "dim2d_batch_synchronize",
-    "dim2d_batch_autotune",
+    "dim2d_batch_autotune", "dim2d_batch_set_stage_fusion", "dim2d_batch_get_tiles",
 )
 
 
@@ -110,6 +110,8 @@ def load() -> ctypes.CDLL:
         "dim2d_batch_set_tiles": (ctypes.c_int, [vp, i32, i32, i32]),
         "dim2d_batch_synchronize": (ctypes.c_int, [vp]),
         "dim2d_batch_autotune": (ctypes.c_int, [vp, dp, ctypes.c_int32, dp]),
+        "dim2d_batch_set_stage_fusion": (ctypes.c_int, [vp, i32, i32, i32]),
+        "dim2d_batch_get_tiles": (ctypes.c_int, [vp, ctypes.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

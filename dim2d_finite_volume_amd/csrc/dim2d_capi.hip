@@ -1125,6 +1125,12 @@ struct dim2d_batch {
   size_t stage_n = 0;
   int block_x = 64, rows = 32, wg = 0;
   int wg_eff = 1;  // persistent workgroups actually launched
+  // stages 1 and 2 fused (stage pair), on when every simulation supports it:
+  // its own work list of 56-column strips and pair_rows-row tiles
+  bool fuse = false;
+  int pair_rows = 32, pair_wg = 0, pair_wg_eff = 1;
+  WorkItem* d_pitems = nullptr;
+  int npitems = 0;
   bool any_wall = false;
   hipStream_t stream = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1132,29 +1138,45 @@ struct dim2d_batch {
   long long steps = 0;
 };
 
-static int batch_rebuild_items(dim2d_batch* b) {
+// the tiles of every simulation (strips of `ow` output columns, `rows` rows),
+// largest simulations first so the persistent grid drains evenly
+static std::vector<WorkItem> batch_items(const dim2d_batch* b, int ow, int rows) {
   std::vector<WorkItem> items;
-  const int ow = b->block_x - 2 * kBcSize;
   for (int s = 0; s < b->n; ++s) {
     const DevParams& p = b->hp[s];
     const int nbx = (p.nx - 2 * kBcSize + ow - 1) / ow;
-    const int nby = (p.ny - 2 * kBcSize + b->rows - 1) / b->rows;
+    const int nby = (p.ny - 2 * kBcSize + rows - 1) / rows;
     for (int by = 0; by < nby; ++by)
       for (int bx = 0; bx < nbx; ++bx) items.push_back(WorkItem{s, bx, by, 0});
   }
-  // largest simulations first so the persistent grid drains evenly
   std::stable_sort(items.begin(), items.end(), [&](const WorkItem& x, const WorkItem& y) {
     return b->hp[x.sim].nx * (long long)b->hp[x.sim].ny > b->hp[y.sim].nx * (long long)b->hp[y.sim].ny;
   });
-  (void)hipFree(b->d_items);
-  b->d_items = nullptr;
-  b->nitems = (int)items.size();
-  HIP_TRY(hipMalloc(&b->d_items, items.size() * sizeof(WorkItem)));
-  HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem),
-                    hipMemcpyHostToDevice));
+  return items;
+}
+
+static int upload_items(const std::vector<WorkItem>& items, WorkItem** d, int* n) {
+  (void)hipFree(*d);
+  *d = nullptr;
+  *n = (int)items.size();
+  HIP_TRY(hipMalloc(d, items.size() * sizeof(WorkItem)));
+  HIP_TRY(hipMemcpy(*d, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  return DIM2D_OK;
+}
+
+static int batch_rebuild_items(dim2d_batch* b) {
+  int st = upload_items(batch_items(b, b->block_x - 2 * kBcSize, b->rows), &b->d_items, &b->nitems);
+  if (st) return st;
   b->wg_eff = b->wg > 0 ? std::min(b->wg, b->nitems)
                        : std::min(b->nitems, max_resident_workgroups(b->mixed));
   if (b->wg_eff < 1) b->wg_eff = 1;
+  if (b->fuse) {
+    st = upload_items(batch_items(b, kPairOutCols, b->pair_rows), &b->d_pitems, &b->npitems);
+    if (st) return st;
+    b->pair_wg_eff = b->pair_wg > 0 ? std::min(b->pair_wg, b->npitems)
+                                    : std::min(b->npitems, max_resident_pair_workgroups(b->mixed));
+    if (b->pair_wg_eff < 1) b->pair_wg_eff = 1;
+  }
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
@@ -1173,6 +1195,32 @@ static int batch_step_launches(dim2d_batch* b, double* U, double* U2) {
   a.dts = b->d_dts;
   const int wg = b->wg_eff;
   hipStream_t s = b->stream;
+  if (b->fuse) {
+    // stages 1 and 2 in one launch (the wall ghost cells of the stage-1 state
+    // inside it), then the stage-2 state's wall ghosts, stage 3 and its walls,
+    // as a single context's fused step
+    a.U = U;
+    if (!b->mixed) a.O64 = b->B;
+    else a.O32 = b->D2;
+    HIP_TRY(launch_stage_pair_batch(b->dp, b->d_pitems, b->npitems, b->mixed, a, b->pair_rows,
+                                    b->pair_wg_eff, s));
+    if (!b->mixed) {
+      if (b->any_wall)
+        HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, b->B, nullptr, b->B, nullptr, U, s));
+      a.S64 = b->B;
+      a.O64 = U2;
+    } else {
+      if (b->any_wall)
+        HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 1, U, nullptr, b->D2, nullptr, b->D2, U, s));
+      a.S32 = b->D2;
+      a.O32 = nullptr;
+      a.O64 = U2;
+    }
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, b->mixed, 3, a, b->block_x, b->rows, wg, s));
+    if (b->any_wall)
+      HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, U2, nullptr, U2, nullptr, U, s));
+    return DIM2D_OK;
+  }
   if (!b->mixed) {
     a.U = U;
     a.S64 = U;
@@ -1243,6 +1291,8 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
     b->any_wall |= has_wall(p);
     b->hp.push_back(p);
   }
+  b->fuse = true;
+  for (const DevParams& p : b->hp) b->fuse = b->fuse && pair_supported(p);
   b->total = off;
   b->h_dts.assign(n, 0.0);
   hipError_t e = hipGetDevice(&b->device);
@@ -1252,7 +1302,7 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
     e = hipMemcpy(b->dp, b->hp.data(), n * sizeof(DevParams), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = field_alloc(&b->U, off);
   if (e == hipSuccess) e = field_alloc(&b->U2, off);
-  if (e == hipSuccess && !b->mixed) e = field_alloc(&b->A, off);
+  if (e == hipSuccess && !b->mixed && !b->fuse) e = field_alloc(&b->A, off);
   if (e == hipSuccess && !b->mixed) e = field_alloc(&b->B, off);
   if (e == hipSuccess && b->mixed) e = field_alloc(&b->D1, off);
   if (e == hipSuccess && b->mixed) e = field_alloc(&b->D2, off);
@@ -1293,6 +1343,7 @@ int dim2d_batch_destroy(dim2d_batch* b) {
   (void)hipFree(b->d_dts);
   (void)hipFree(b->stage);
   (void)hipFree(b->d_items);
+  (void)hipFree(b->d_pitems);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return DIM2D_OK;
@@ -1308,6 +1359,43 @@ int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows, int32_t
   if (rows) b->rows = rows;
   b->wg = workgroups;
   return batch_rebuild_items(b);
+}
+
+int dim2d_batch_set_stage_fusion(dim2d_batch* b, int32_t fuse, int32_t rows_per_block,
+                                 int32_t workgroups) {
+  if (!b) return fail(DIM2D_EINVAL, "batch is NULL");
+  if (rows_per_block < 0 || rows_per_block > 4096 || workgroups < 0)
+    return fail(DIM2D_EINVAL, "invalid tile arguments");
+  if (fuse) {
+    for (int i = 0; i < b->n; ++i)
+      if (!pair_supported(b->hp[i]))
+        return fail(DIM2D_EINVAL,
+                    "simulation %d: stage fusion needs reflection, periodic or S/N wall edges "
+                    "and at least 8 interior points per axis", i);
+  } else if (!b->mixed && !b->A) {
+    // the separate FP64 stages need the stage-1 buffer, which a fused batch
+    // does not allocate
+    hipError_t e = field_alloc(&b->A, b->total);
+    if (e == hipSuccess) e = hipMemset(b->A, 0, b->total * sizeof(double));
+    if (e != hipSuccess)
+      return fail(e == hipErrorOutOfMemory ? DIM2D_ENOMEM : DIM2D_EDEVICE, "batch allocation: %s",
+                  hipGetErrorString(e));
+  }
+  b->fuse = fuse != 0;
+  if (rows_per_block) b->pair_rows = rows_per_block;
+  b->pair_wg = workgroups;
+  return batch_rebuild_items(b);
+}
+
+int dim2d_batch_get_tiles(dim2d_batch* b, int32_t out[6]) {
+  if (!b || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  out[0] = b->fuse ? 1 : 0;
+  out[1] = b->pair_rows;
+  out[2] = b->fuse ? b->pair_wg_eff : 0;
+  out[3] = b->block_x;
+  out[4] = b->rows;
+  out[5] = b->wg_eff;
+  return DIM2D_OK;
 }
 
 int dim2d_batch_upload(dim2d_batch* b, int32_t sim, const double* nodes) {
@@ -1377,6 +1465,9 @@ int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, doub
   const int resident = max_resident_workgroups(b->mixed);
   const int rows_c[] = {8, 12, 16, 24, 32};
   const int wg_c[] = {resident, 2 * resident};
+  const int president = max_resident_pair_workgroups(b->mixed);
+  const int prows_c[] = {16, 24, 32, 48, 60};
+  const int pwg_c[] = {president, 2 * president};
   const int steps = nsteps & ~1;  // whole graph replays
   // keep the state: candidates run real steps
   double* keep = nullptr;
@@ -1384,31 +1475,58 @@ int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, doub
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int st = DIM2D_OK;
   double best = INFINITY;
-  int best_rows = b->rows, best_wg = b->wg;
   bool kept = false;
+  // ms per step of the current tiles
+  auto time_tiles = [&](double* ms_out) -> int {
+    int s2 = batch_rebuild_items(b);
+    if (s2) return s2;
+    s2 = dim2d_batch_integrate(b, dts, 2);  // warm-up, graph capture
+    if (s2) return s2;
+    HIP_TRY(hipEventRecord(e0, b->stream));
+    s2 = dim2d_batch_integrate(b, dts, steps);
+    if (s2) return s2;
+    HIP_TRY(hipEventRecord(e1, b->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *ms_out = ms / steps;
+    return DIM2D_OK;
+  };
   auto run = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(keep, b->U, b->total * sizeof(double), hipMemcpyDeviceToDevice,
                            b->stream));
     kept = true;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
+    double ms = 0.0;
+    if (b->fuse) {
+      // the stage pair's tiles first, with the stage-3 tiles as they are
+      int best_r = b->pair_rows, best_w = b->pair_wg;
+      double pbest = INFINITY;
+      for (int r : prows_c)
+        for (int w : pwg_c) {
+          b->pair_rows = r;
+          b->pair_wg = w;
+          int s2 = time_tiles(&ms);
+          if (s2) return s2;
+          if (ms < pbest) {
+            pbest = ms;
+            best_r = r;
+            best_w = w;
+          }
+        }
+      b->pair_rows = best_r;
+      b->pair_wg = best_w;
+    }
+    int best_rows = b->rows, best_wg = b->wg;
     for (int r : rows_c)
       for (int w : wg_c) {
         b->rows = r;
         b->wg = w;
-        int s2 = batch_rebuild_items(b);
+        int s2 = time_tiles(&ms);
         if (s2) return s2;
-        s2 = dim2d_batch_integrate(b, dts, 2);  // warm-up, graph capture
-        if (s2) return s2;
-        HIP_TRY(hipEventRecord(e0, b->stream));
-        s2 = dim2d_batch_integrate(b, dts, steps);
-        if (s2) return s2;
-        HIP_TRY(hipEventRecord(e1, b->stream));
-        HIP_TRY(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms / steps < best) {
-          best = ms / steps;
+        if (ms < best) {
+          best = ms;
           best_rows = r;
           best_wg = w;
         }
@@ -1437,8 +1555,8 @@ int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, doub
   (void)hipStreamSynchronize(b->stream);
   (void)hipFree(keep);
   if (st) return st;
-  out[0] = best_rows;
-  out[1] = best_wg;
+  out[0] = b->rows;
+  out[1] = b->wg;
   out[2] = best;
   return DIM2D_OK;
 }

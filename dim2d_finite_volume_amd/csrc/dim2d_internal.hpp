@@ -96,6 +96,13 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
 bool pair_supported(const DevParams& p);
 hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, int rows,
                              hipStream_t s);
+// the stage pair over a batch work list of pair tiles (56-column strips,
+// `rows` rows), persistent grid of `workgroups` two-wave workgroups, per-
+// simulation dt from a.dts; every simulation must satisfy pair_supported
+hipError_t launch_stage_pair_batch(const DevParams* d_params, const WorkItem* d_items,
+                                   int nitems, int mixed, const StageArgs& a, int rows,
+                                   int workgroups, hipStream_t s);
+constexpr int kPairOutCols = 56;  // output columns per stage-pair strip
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
                               int mixed, int stage, const StageArgs& a, int block_x, int rows,
                               int workgroups, hipStream_t s);
@@ -157,5 +164,7 @@ hipError_t launch_newgrdpt(const DevParams& np, const DevParams& op, int aw, int
 
 // workgroups of the persistent batch kernels that fit the device at once
 int max_resident_workgroups(int mixed);
+// the same for the batch stage-pair kernel (two-wave workgroups)
+int max_resident_pair_workgroups(int mixed);
 
 }  // namespace dim2d

@@ -1199,7 +1199,8 @@ __global__ void wall_batch_kernel(const DevParams* P, int nsim, int mixed_in, co
 // Walls on W/E edges and open edges take the separate stage kernels.
 // Every output cell sees the same operands in the same order as with the
 // separate kernels and the wall kernel between them.
-constexpr int kPairOut = kLanes - 8;  // output columns per strip (lanes 4..59)
+constexpr int kPairOut = kPairOutCols;  // output columns per strip (lanes 4..59)
+static_assert(kPairOut == kLanes - 8, "pair strips: 4 halo lanes per side");
 constexpr int kPairLag = 6;           // rows the consumer runs behind the producer
 constexpr int kPairSlots = 8;         // LDS ring rows (a power of two)
 
@@ -1454,31 +1455,67 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
   }
 }
 
+// One tile of the stage pair: strip bx, output rows r0 .. r1-1 of simulation
+// p.  Both waves pass exactly kPairLag + (r1 - r0 + 2) barriers, and the
+// consumer's last ring read precedes its last barrier, so a workgroup may run
+// tiles back to back on one ring.
+template <int MIXED>
+__device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a, double dt,
+                                          int bx, int r0, int r1, bool swap,
+                                          PairRing<MIXED>& ring) {
+  using RD = double;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  // the consumer's rows r0-2 .. r1-1 after kPairLag idle barriers
+  const int steps = kPairLag + (r1 - r0 + 2);
+  if ((threadIdx.x < kLanes) != swap) pair_producer<RD, R, MIXED>(p, a, dt, bx, r0, r1, steps, ring);
+  else pair_consumer<RD, R, MIXED>(p, a, dt, bx, r0, r1, ring);
+}
+
 // minimum waves per SIMD: 2 (up to 256 VGPRs), as the stage kernels
 template <int MIXED>
 __global__ void __launch_bounds__(2 * kLanes) __attribute__((amdgpu_waves_per_eu(2, 8)))
 stage_pair_kernel(const DevParams p, const StageArgs a, int rows) {
   __shared__ PairRing<MIXED> ring;
-  using RD = double;
-  using R = typename std::conditional<MIXED, float, double>::type;
   double dt = a.dt_fixed;
   if (a.cfl_mode) dt = cfl_dt_from(a.ctl, a.parity);
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
     a.ctl->dt = dt;
     clear_slot(a.ctl, a.parity ^ 1);
   }
-  const int bx = blockIdx.x;
   const int r0 = kBcSize + blockIdx.y * rows;
   const int r1 = min(r0 + rows, p.ny - kBcSize);
-  // the consumer's rows r0-2 .. r1-1 after kPairLag idle barriers
-  const int steps = kPairLag + (r1 - r0 + 2);
   // FP64 (2 waves per SIMD): the roles of the two waves alternate between
   // neighbouring workgroups, so that each SIMD of a CU holds producers and
   // consumers alike rather than two of one kind (measured 4 % faster at
   // 4096^2; in mixed precision, 3 waves per SIMD, 9 % slower)
   const bool swap = !MIXED && (((blockIdx.x + blockIdx.y) & 1) != 0);
-  if ((threadIdx.x < kLanes) != swap) pair_producer<RD, R, MIXED>(p, a, dt, bx, r0, r1, steps, ring);
-  else pair_consumer<RD, R, MIXED>(p, a, dt, bx, r0, r1, ring);
+  pair_tile<MIXED>(p, a, dt, blockIdx.x, r0, r1, swap, ring);
+}
+
+// persistent batch form of the stage pair: one grid walks the pair tiles
+// (56-column strips) of all simulations, each with its fixed dt; a.U and
+// a.O64/a.O32 are batch bases offset by DevParams::offset
+#ifndef DIM2D_PAIR_BATCH_WAVES_MIXED
+#define DIM2D_PAIR_BATCH_WAVES_MIXED 3
+#endif
+template <int MIXED>
+__global__ void __launch_bounds__(2 * kLanes)
+__attribute__((amdgpu_waves_per_eu(MIXED ? DIM2D_PAIR_BATCH_WAVES_MIXED : 2, 8)))
+stage_pair_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restrict__ items,
+                        int nitems, const StageArgs a, const double* __restrict__ dts, int rows) {
+  __shared__ PairRing<MIXED> ring;
+  const bool swap = !MIXED && (blockIdx.x & 1) != 0;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const DevParams p = P[w.sim];
+    StageArgs b = a;
+    b.U = a.U + p.offset;
+    b.O64 = a.O64 ? a.O64 + p.offset : nullptr;
+    b.O32 = a.O32 ? a.O32 + p.offset : nullptr;
+    const int r0 = kBcSize + w.by * rows;
+    const int r1 = min(r0 + rows, p.ny - kBcSize);
+    pair_tile<MIXED>(p, b, dts[w.sim], w.bx, r0, r1, swap, ring);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -2500,6 +2537,18 @@ hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, 
   return hipGetLastError();
 }
 
+hipError_t launch_stage_pair_batch(const DevParams* d_params, const WorkItem* d_items,
+                                   int nitems, int mixed, const StageArgs& a, int rows,
+                                   int workgroups, hipStream_t s) {
+  if (mixed)
+    hipLaunchKernelGGL(stage_pair_batch_kernel<1>, dim3(workgroups), dim3(2 * kLanes), 0, s,
+                       d_params, d_items, nitems, a, a.dts, rows);
+  else
+    hipLaunchKernelGGL(stage_pair_batch_kernel<0>, dim3(workgroups), dim3(2 * kLanes), 0, s,
+                       d_params, d_items, nitems, a, a.dts, rows);
+  return hipGetLastError();
+}
+
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
                               int mixed, int stage, const StageArgs& a, int block_x, int rows,
                               int workgroups, hipStream_t s) {
@@ -2641,6 +2690,21 @@ hipError_t launch_extend(const DevParams& np, const DevParams& op, int add_w, in
   hipLaunchKernelGGL(extend_kernel, dim3((unsigned)blocks), dim3(256), 0, s, np, op, add_w, add_s,
                      oldU, newU);
   return hipGetLastError();
+}
+
+int max_resident_pair_workgroups(int mixed) {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  int per = 0;
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, stage_pair_batch_kernel<1>, 2 * kLanes, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, stage_pair_batch_kernel<0>, 2 * kLanes, 0);
+  if (e != hipSuccess || per <= 0) per = 4;
+  return ncu * per;
 }
 
 int max_resident_workgroups(int mixed) {

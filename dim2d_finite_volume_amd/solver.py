@@ -322,10 +322,26 @@ class Batch:
         capi.check(self._lib.dim2d_batch_set_tiles(self._b, block_x, rows, workgroups),
                    "dim2d_batch_set_tiles")
 
+    def set_stage_fusion(self, fuse: bool, rows: int = 0, workgroups: int = 0) -> None:
+        """RK3-TVD stages 1 and 2 in one persistent launch (on by default when
+        every simulation's edges allow it); rows and workgroups of its tiles."""
+        capi.check(self._lib.dim2d_batch_set_stage_fusion(self._b, int(bool(fuse)), rows,
+                                                          workgroups),
+                   "dim2d_batch_set_stage_fusion")
+
+    def tiles(self) -> dict:
+        """The batch's launch shapes (dim2d_batch_get_tiles)."""
+        out = (ctypes.c_int32 * 6)()
+        capi.check(self._lib.dim2d_batch_get_tiles(self._b, out), "dim2d_batch_get_tiles")
+        return {"stage_fusion": bool(out[0]), "pair_rows": out[1], "pair_workgroups": out[2],
+                "block_x": out[3], "rows": out[4], "workgroups": out[5]}
+
     def autotune(self, dts, nsteps: int = 10):
-        """Pick the tile height and persistent workgroup count by timing
-        (dim2d_batch_autotune); the state is left as it was.  Returns
-        (rows, workgroups, ms_per_step)."""
+        """Pick the tile heights and persistent workgroup counts by timing
+        (dim2d_batch_autotune): the stage pair's first when stages 1 and 2 are
+        fused, then stage 3's (or all three stages' otherwise); the state is left
+        as it was.  Returns (rows, workgroups, ms_per_step) of the last search;
+        tiles() gives the rest."""
         d = np.ascontiguousarray(dts, dtype=np.float64)
         out = np.zeros(3)
         capi.check(self._lib.dim2d_batch_autotune(self._b, _dptr(d), nsteps, _dptr(out)),

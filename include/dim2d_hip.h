@@ -312,11 +312,29 @@ int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows_per_bloc
                           int32_t workgroups);
 int dim2d_batch_synchronize(dim2d_batch* b);
 
+/* RK3-TVD stages 1 and 2 of every simulation in one persistent launch (the
+ * stage pair of dim2d_set_stage_fusion: the stage-1 state stays on chip),
+ * then stage 3 as before.  On by default when every simulation has
+ * reflection, periodic or S/N wall edges and at least 8 interior points per
+ * axis; fuse = 1 on any other batch returns DIM2D_EINVAL.  rows_per_block
+ * and workgroups shape the pair's tiles (56-column strips) and persistent
+ * grid (0: 32 rows; one workgroup per tile, capped by residency).  No
+ * counterpart in the reference. */
+int dim2d_batch_set_stage_fusion(dim2d_batch* b, int32_t fuse, int32_t rows_per_block,
+                                 int32_t workgroups);
+/* The launch shapes in use: out = {stage fusion on, pair rows, pair
+ * workgroups launched, block_x, rows, workgroups launched}. */
+int dim2d_batch_get_tiles(dim2d_batch* b, int32_t out[6]);
+
 /* Tile autotuner of a batch (no counterpart in the reference): times nsteps
- * (>= 2) steps with the given dts for each candidate tile height (8, 12, 16,
- * 24, 32 rows) and persistent workgroup count (1x and 2x the workgroups that
- * fit the device at once), keeps the fastest, and restores the state it
- * started from.  out[0] = rows, out[1] = workgroups, out[2] = ms per step. */
+ * (>= 2) steps with the given dts for each candidate, keeps the fastest, and
+ * restores the state it started from.  With stage fusion it first tries the
+ * pair's tile heights (16, 24, 32, 48, 60 rows) and persistent workgroup
+ * counts (1x and 2x the pair workgroups that fit the device at once), then
+ * the other stage kernels' tile heights (8, 12, 16, 24, 32 rows) and
+ * workgroup counts.  out[0] = rows, out[1] = workgroups of the latter,
+ * out[2] = ms per step of the fastest candidate; dim2d_batch_get_tiles gives
+ * the pair's. */
 int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, double out[3]);
 
 #ifdef __cplusplus

@@ -480,39 +480,52 @@ def test_cfl_and_diagnostics(orc):
     assert diag[4] == it.min() and diag[5] == it.max() and diag[6] == 0
 
 
-def test_batch_matches_single(orc):
-    """Persistent batch kernels under a HIP graph give the single-simulation result."""
+@pytest.mark.parametrize("fuse", [False, True])
+def test_batch_matches_single(orc, fuse):
+    """Persistent batch kernels under a HIP graph give the single-simulation
+    result, with stages 1 and 2 fused (against single runs that fuse them)
+    and apart."""
     d = _lib()
     from dim2d_finite_volume_amd import initial_conditions as ic
     sims = ic.batch_configs(nsims=5, seed=11, nmin=20, nmax=90)
     dts = [ic.cfl_dt(p, n, 0.1) for p, n in sims]
     b = d.Batch([p for p, _ in sims])
+    assert b.tiles()["stage_fusion"]  # every edge here allows it
     b.set_tiles(64, 16)
+    if not fuse:
+        b.set_stage_fusion(False)
     for i, (p, n) in enumerate(sims):
         b.upload(i, n)
     # the autotuner runs real steps on candidate tilings, then restores the state
     rows, wg, ms = b.autotune(dts, 4)
     assert rows in (8, 12, 16, 24, 32) and wg > 0 and ms > 0
+    t = b.tiles()
+    assert t["stage_fusion"] == fuse and t["rows"] == rows
+    if fuse:
+        assert t["pair_rows"] in (16, 24, 32, 48, 60) and t["pair_workgroups"] > 0
     b.integrate(dts, 6)
     for i, (p, n) in enumerate(sims):
         f = d.Field(p, n)
         f.set_tiles(64, 16)
-        f.set_stage_fusion(False)  # the batch kernels launch the stages apart
+        f.set_stage_fusion(fuse)
         f.run(6, dt=dts[i])
         single = f.download()
         got = b.download(i)
         assert np.array_equal(got[:, 2:-2, 2:-2], single[:, 2:-2, 2:-2]), i
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_batch_mixed_edges_matches_single(orc, prec):
+@pytest.mark.parametrize("prec,fuse", [(0, False), (1, False), (0, True), (1, True)])
+def test_batch_mixed_edges_matches_single(orc, prec, fuse):
     """A batch whose simulations have different edges (reflection, periodic,
     periodic x with walls y, a south wall with a 120-degree contact angle and
-    heat flux, gravity) and sizes: 7 steps of the persistent batch kernels and
-    the batched wall kernel equal single-simulation runs bitwise on every
-    interior cell and on the inner wall ghost row (density and momenta).
-    (A 60-degree angle on the 66x70 drop goes non-finite within 2 steps in the
-    oracle too, so that case is not used.)"""
+    heat flux, gravity, walls on W/E) and sizes: 7 steps of the persistent
+    batch kernels and the batched wall kernel equal single-simulation runs
+    bitwise on every interior cell and on the inner wall ghost row (density
+    and momenta), and the oracle to 1e-10 (mixed: 1e-6).  With stages 1 and 2
+    fused (the W/E-wall simulation left out: the stage pair takes S/N walls
+    only, and a batch holding it runs the stages apart) against single runs
+    that fuse them.  (A 60-degree angle on the 66x70 drop goes non-finite
+    within 2 steps in the oracle too, so that case is not used.)"""
     d = _lib()
     specs = [
         dict(bc=BC_CASES["reflection"], nx=40, ny=33),
@@ -523,6 +536,8 @@ def test_batch_mixed_edges_matches_single(orc, prec):
         dict(bc=BC_CASES["periodic_x_wall_y"], nx=48, ny=90, gravity=0.5),
         dict(bc=BC_CASES["wall_x_reflection_y"], nx=57, ny=61, wall_contact_angle=70.0),
     ]
+    if fuse:
+        specs = specs[:-1]
     sims = []
     for sp in specs:
         sp = dict(sp)
@@ -530,13 +545,14 @@ def test_batch_mixed_edges_matches_single(orc, prec):
         _, p, m, xm, ym, nodes = setup_case(orc, bc, nx=nx, ny=ny, precision=prec, **sp)
         sims.append((p, nodes, orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0], m, xm, ym))
     b = d.Batch([s[0] for s in sims])
+    assert b.tiles()["stage_fusion"] == fuse
     for i, s in enumerate(sims):
         b.upload(i, s[1])
     dts = [s[2] for s in sims]
     b.integrate(dts, 7)
     for i, (p, n, dt, m, xm, ym) in enumerate(sims):
         f = d.Field(p, n.copy())
-        f.set_stage_fusion(False)  # the batch kernels launch the stages apart
+        f.set_stage_fusion(fuse)
         f.run(7, dt=dt)
         single = f.download()
         f.close()
@@ -686,6 +702,14 @@ def test_stage_fusion_rejects_unsupported_edges():
             f.set_stage_fusion(True)
         f.set_stage_fusion(False)
         f.close()
+    # a batch holding one such simulation runs the stages apart
+    ps = [d.ModelParams(nx=40, ny=36, bc=BC_CASES["reflection"]),
+          d.ModelParams(nx=40, ny=36, bc=BC_CASES["wall_x_reflection_y"])]
+    b = d.Batch(ps)
+    assert not b.tiles()["stage_fusion"]
+    with pytest.raises(RuntimeError):
+        b.set_stage_fusion(True)
+    b.close()
 
 
 def test_set_tiles_rejects_other_widths():
