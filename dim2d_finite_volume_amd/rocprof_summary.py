@@ -35,10 +35,19 @@ from collections import defaultdict
 
 STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
 PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)>")
+BSTAGE_RE = re.compile(r"dim2d::stage_batch_kernel<(\d+),\s*(\d+)>")
+BPAIR_RE = re.compile(r"dim2d::stage_pair_batch_kernel<(\d+)>")
 
 
 def kernel_key(name):
-    """file-name key of a stage kernel: stage<s>_<fp64|mixed> or pair_<fp64|mixed>"""
+    """file-name key of a stage kernel: [batch_]stage<s>_<fp64|mixed> or
+    [batch_]pair_<fp64|mixed>"""
+    m = BSTAGE_RE.search(name)
+    if m:
+        return f"batch_stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
+    m = BPAIR_RE.search(name)
+    if m:
+        return f"batch_pair_{'mixed' if m.group(1) == '1' else 'fp64'}"
     m = STAGE_RE.search(name)
     if m:
         return f"stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
@@ -49,6 +58,14 @@ def kernel_key(name):
 
 
 def short_name(name):
+    m = BPAIR_RE.search(name)
+    if m:
+        prec = "mixed" if m.group(1) == "1" else "FP64"
+        return f"stage_pair_batch_kernel<{m.group(1)}> (batch, {prec} stages 1+2)"
+    m = BSTAGE_RE.search(name)
+    if m:
+        prec = "mixed" if m.group(1) == "1" else "FP64"
+        return f"stage_batch_kernel<{m.group(1)}, {m.group(2)}> (batch, {prec} stage {m.group(2)})"
     m = PAIR_RE.search(name)
     if m:
         prec = "mixed" if m.group(1) == "1" else "FP64"
