@@ -430,7 +430,8 @@ int dim2d_set_stage_fusion(dim2d_ctx* c, int32_t fuse, int32_t rows_per_block) {
   if (fuse && (c->lsrk || c->cg || !pair_supported(c->dp)))
     return fail(DIM2D_EINVAL,
                 "stage fusion needs the RK3-TVD integrator with the Mattsson operators, "
-                "reflection, periodic or S/N wall edges and at least 8 interior points per axis");
+                "reflection, periodic, S/N wall or open edges and at least 8 interior points "
+                "per axis");
   c->fuse = fuse;
   if (rows_per_block) {
     c->pair_rows = rows_per_block;
@@ -654,23 +655,43 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     // three swaps: the new state is in U2, as for RK3-TVD
   } else if (c->fuse) {
     // stages 1 and 2 in one launch: u^n -> stage-2 state (B, or its increment
-    // D2 in mixed precision), the stage-1 state and its wall ghosts inside
+    // D2 in mixed precision), the stage-1 state and its wall ghosts inside.
+    // Open edges: the stage-1 frame (into A, or the increments D1) before it,
+    // since the pair reads it, and the stage-2 frame after it, since that
+    // reads the band of s1 beside the frame the pair stores there.
     a.U = c->U;
-    if (!c->mixed) a.O64 = c->B;
-    else a.O32 = c->D2;
+    if (open) {
+      a.S64 = c->U;
+      if (!c->mixed) a.O64 = c->A;
+      else a.O32 = c->D1;
+      HIP_TRY(timed_launch(c, 3, [&] { return launch_frame(p, c->mixed, 1, a, c->stream); }));
+      a.F64 = c->mixed ? nullptr : c->A;
+      a.F32 = c->mixed ? c->D1 : nullptr;
+    }
+    a.O64 = c->mixed ? nullptr : c->B;
+    a.O32 = c->mixed ? c->D2 : nullptr;
     HIP_TRY(timed_launch(c, 0, [&] { return launch_stage_pair(p, c->mixed, a, c->pair_rows, c->stream); }));
+    if (open) {
+      a.S64 = c->mixed ? nullptr : c->A;
+      a.S32 = c->mixed ? c->D1 : nullptr;
+      HIP_TRY(timed_launch(c, 3, [&] { return launch_frame(p, c->mixed, 2, a, c->stream); }));
+    }
     if (!c->mixed) {
       if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->B, nullptr, c->B, nullptr, c->U, c->stream); }));
       a.O64 = c->U2;
       a.S64 = c->B;
+      if ((st = fork_frame(0, 3))) return st;
       HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 0, 3, a, c->block_x, c->rows, c->stream); }));
+      if ((st = frame(0, 3))) return st;
       if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
     } else {
       if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 1, c->U, nullptr, c->D2, nullptr, c->D2, c->U, c->stream); }));
       a.S32 = c->D2;
       a.O32 = nullptr;
       a.O64 = c->U2;
+      if ((st = fork_frame(1, 3))) return st;
       HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 1, 3, a, c->block_x, c->rows, c->stream); }));
+      if ((st = frame(1, 3))) return st;
       if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
     }
   } else if (!c->mixed) {

@@ -81,6 +81,8 @@ struct StageArgs {
   float* O32;         // FP32 output (mixed stages 1, 2)
   double* Q;          // low-storage RK register q (read and written in place)
   StepCtl* ctl;
+  double* F64;        // stage pair with open edges: s1 of the frame and the band
+  float* F32;         // beside it (FP64), or its increment (mixed)
   const double* dts;  // per-simulation dt (batch)
   double dt_fixed;
   int cfl_mode;       // 1: stage 1 takes dt from the reductions
@@ -91,8 +93,10 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
                         int rows, hipStream_t s);
 // RK3-TVD stages 1 and 2 in one launch (stage pair): u^n (a.U) -> stage-2
 // state in a.O64 (FP64) or its increment in a.O32 (mixed); dt as stage 1.
-// pair_supported: reflection, periodic and S/N wall edges, grids of at least
-// 8 interior points per axis.
+// pair_supported: reflection, periodic, S/N wall and open edges, grids of at
+// least 8 interior points per axis.  With open edges the stage-1 frame kernel
+// (output a.F64 / a.F32) runs before it and the stage-2 frame kernel (input
+// the same buffer) after it.
 bool pair_supported(const DevParams& p);
 hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, int rows,
                              hipStream_t s);

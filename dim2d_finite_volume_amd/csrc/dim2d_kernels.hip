@@ -1241,10 +1241,29 @@ __device__ __forceinline__ int pair_map_row(const DevParams& p, int r, bool& fli
 
 // s1 of (row r, lane l) of strip bx from the ring: ghost cells of reflection
 // edges come from the mirror lane/row with the normal momentum negated
-template <int MIXED>
+template <int MIXED, bool OPEN = false>
 __device__ __forceinline__ void pair_gather(const DevParams& p, const PairRing<MIXED>& ring,
-                                            int bx, int r, int l, double v[4], float inc[4]) {
+                                            const StageArgs& a, int bx, int r, int l, double v[4],
+                                            float inc[4]) {
   const int cc = kBcSize + bx * kPairOut - 4 + l;
+  if (OPEN && (cc < kBcSize || cc >= p.nx - kBcSize || r < kBcSize || r >= p.ny - kBcSize)) {
+    // a frame cell of open edges: s1 from the stage-1 frame kernel (a.F64,
+    // or u^n + a.F32), which ran before this launch; lanes and rows outside
+    // the array read the nearest cell (their values are never used)
+    const long long o =
+        ((long long)min(max(r, 0), p.ny - 1) * p.pitch + min(max(cc, 0), p.nx - 1)) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (MIXED) {
+        inc[k] = a.F32[o + k];
+        v[k] = a.U[o + k] + (double)inc[k];
+      } else {
+        v[k] = a.F64[o + k];
+        inc[k] = 0.f;
+      }
+    }
+    return;
+  }
   int ll = l;
   bool fx = false, fy = false;
   if (cc < kBcSize && p.bc_w == kBcReflection) {
@@ -1290,10 +1309,10 @@ __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<
   if (c >= 1 && c <= p.nx - 2) {
     double v1[4], v2[4], y0[4], y1[4];
     float t[4];
-    pair_gather<MIXED>(p, ring, bx, r1, lane, v1, t);
-    pair_gather<MIXED>(p, ring, bx, r2, lane, v2, t);
-    pair_gather<MIXED>(p, ring, bx, r1, lane - 1, y0, t);
-    pair_gather<MIXED>(p, ring, bx, r1, lane + 1, y1, t);
+    pair_gather<MIXED>(p, ring, a, bx, r1, lane, v1, t);
+    pair_gather<MIXED>(p, ring, a, bx, r2, lane, v2, t);
+    pair_gather<MIXED>(p, ring, a, bx, r1, lane - 1, y0, t);
+    pair_gather<MIXED>(p, ring, a, bx, r1, lane + 1, y1, t);
     bool fx;
     const int cm = map_col(p, c, fx);
     const double T_guess = p.c38cv * temperature_eff_at(p, a.U, c, r1);
@@ -1321,7 +1340,7 @@ __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<
 
 // the producer wave: stage 1 of rows r0-2 .. rlast into the ring, then the
 // north wall rows when the tile holds them, then idle barriers up to `steps`
-template <typename RD, typename R, int MIXED>
+template <typename RD, typename R, int MIXED, bool OPEN>
 __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArgs& a, double dt,
                                               int bx, int r0, int r1, int steps,
                                               PairRing<MIXED>& ring) {
@@ -1329,6 +1348,11 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
   const int c = kBcSize + bx * kPairOut - 4 + lane;
   bool flx;
   const int cm = pair_map_col(p, c, flx);
+  // open edges: the stage-2 frame kernel reads s1 of the first interior
+  // row/column beside each edge, which this wave stores for its own output
+  // cells (a.F64, or the increment in a.F32)
+  const bool band_col = OPEN && lane >= 4 && lane < kLanes - 4 && c <= p.nx - kBcSize - 1;
+  const bool edge_col = c == kBcSize || c == p.nx - kBcSize - 1;
   // The consumer reads s1 up to row r1 + 1, which is a ghost row when
   // r1 + 1 >= ny - 2 (the top tile, and a tile ending one row below it).  The
   // ghost rows of a reflecting or wall edge are the consumer's mirrors and the
@@ -1361,16 +1385,21 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
     if (r >= r0 - 2) {
       const int sl = r & (kPairSlots - 1);
       const double* S = sr.raw[K];
+      const bool band = OPEN && band_col && r >= r0 && r < r1 &&
+                        (edge_col || r == kBcSize || r == p.ny - kBcSize - 1);
+      const long long o = ((long long)r * p.pitch + c) * 4;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double dl = dt * (double)L[k];
         if (MIXED) {
-          const float o = (float)dl;
-          ring.d[MIXED ? sl : 0][k][lane] = o;
-          ring.s[sl][k][lane] = S[k] + (double)o;
+          const float o32 = (float)dl;
+          ring.d[MIXED ? sl : 0][k][lane] = o32;
+          ring.s[sl][k][lane] = S[k] + (double)o32;
+          if (band) a.F32[o + k] = o32;
         } else {
           ring.s[sl][k][lane] = S[k] + dl;
           ring.u[MIXED ? 0 : sl][k][lane] = S[k];
+          if (band) a.F64[o + k] = S[k] + dl;
         }
       }
     }
@@ -1394,7 +1423,7 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
 
 // the consumer wave: kPairLag idle barriers, the south wall rows when the
 // tile holds them, then stage 2 of rows r0 .. r1-1 from the ring
-template <typename RD, typename R, int MIXED>
+template <typename RD, typename R, int MIXED, bool OPEN>
 __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArgs& a, double dt,
                                               int bx, int r0, int r1, PairRing<MIXED>& ring) {
   const int lane = threadIdx.x & (kLanes - 1);
@@ -1409,8 +1438,8 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
   }
   const int q0 = r0 - 2;
   StripRings<RD, R> sr;
-  pair_gather<MIXED>(p, ring, bx, q0, lane, sr.raw[0], sr.inc[0]);
-  pair_gather<MIXED>(p, ring, bx, q0 + 1, lane, sr.raw[1], sr.inc[1]);
+  pair_gather<MIXED, OPEN>(p, ring, a, bx, q0, lane, sr.raw[0], sr.inc[0]);
+  pair_gather<MIXED, OPEN>(p, ring, a, bx, q0 + 1, lane, sr.raw[1], sr.inc[1]);
   strip_start(p, sr);
   int r = q0;
   auto row = [&](auto slot) __attribute__((always_inline)) {
@@ -1418,7 +1447,7 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
     constexpr int K1 = (K + 1) % kRing;
     constexpr int K2 = (K + 2) % kRing;
     // s1 of row r+2 from the ring (written at least one barrier ago)
-    pair_gather<MIXED>(p, ring, bx, r + 2, lane, sr.raw[K2], sr.inc[K2]);
+    pair_gather<MIXED, OPEN>(p, ring, a, bx, r + 2, lane, sr.raw[K2], sr.inc[K2]);
     R L[4];
     strip_time_dev<RD, R, K>(p, sr, r, false, false, L);
     if (out_col && r >= r0) {
@@ -1459,7 +1488,7 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
 // p.  Both waves pass exactly kPairLag + (r1 - r0 + 2) barriers, and the
 // consumer's last ring read precedes its last barrier, so a workgroup may run
 // tiles back to back on one ring.
-template <int MIXED>
+template <int MIXED, bool OPEN = false>
 __device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a, double dt,
                                           int bx, int r0, int r1, bool swap,
                                           PairRing<MIXED>& ring) {
@@ -1467,13 +1496,20 @@ __device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a
   using R = typename std::conditional<MIXED, float, double>::type;
   // the consumer's rows r0-2 .. r1-1 after kPairLag idle barriers
   const int steps = kPairLag + (r1 - r0 + 2);
-  if ((threadIdx.x < kLanes) != swap) pair_producer<RD, R, MIXED>(p, a, dt, bx, r0, r1, steps, ring);
-  else pair_consumer<RD, R, MIXED>(p, a, dt, bx, r0, r1, ring);
+  if ((threadIdx.x < kLanes) != swap)
+    pair_producer<RD, R, MIXED, OPEN>(p, a, dt, bx, r0, r1, steps, ring);
+  else
+    pair_consumer<RD, R, MIXED, OPEN>(p, a, dt, bx, r0, r1, ring);
 }
 
-// minimum waves per SIMD: 2 (up to 256 VGPRs), as the stage kernels
-template <int MIXED>
-__global__ void __launch_bounds__(2 * kLanes) __attribute__((amdgpu_waves_per_eu(2, 8)))
+// minimum waves per SIMD: 2 in FP64 (up to 256 VGPRs), as the stage kernels;
+// 3 in mixed precision (168 VGPRs; its LDS ring fits 6 workgroups per CU).  OPEN:
+// all four edges open (hedstrom_xy); the frame cells of s1 come from the
+// stage-1 frame kernel, launched before this kernel, and the stage-2 frame
+// kernel, launched after it, reads the band of s1 this kernel stores.
+template <int MIXED, bool OPEN>
+__global__ void __launch_bounds__(2 * kLanes)
+__attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2, 8)))
 stage_pair_kernel(const DevParams p, const StageArgs a, int rows) {
   __shared__ PairRing<MIXED> ring;
   double dt = a.dt_fixed;
@@ -1489,7 +1525,7 @@ stage_pair_kernel(const DevParams p, const StageArgs a, int rows) {
   // consumers alike rather than two of one kind (measured 4 % faster at
   // 4096^2; in mixed precision, 3 waves per SIMD, 9 % slower)
   const bool swap = !MIXED && (((blockIdx.x + blockIdx.y) & 1) != 0);
-  pair_tile<MIXED>(p, a, dt, blockIdx.x, r0, r1, swap, ring);
+  pair_tile<MIXED, OPEN>(p, a, dt, blockIdx.x, r0, r1, swap, ring);
 }
 
 // persistent batch form of the stage pair: one grid walks the pair tiles
@@ -2517,9 +2553,6 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
 }
 
 bool pair_supported(const DevParams& p) {
-  const int b[4] = {p.bc_w, p.bc_e, p.bc_s, p.bc_n};
-  for (int k = 0; k < 4; ++k)
-    if (b[k] == kBcOpen) return false;
   if (p.bc_w == kBcWall || p.bc_e == kBcWall) return false;
   // the producer's maps reach 4 rows/columns past an edge: the images must be
   // interior cells
@@ -2530,10 +2563,16 @@ hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, 
                              hipStream_t s) {
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
   dim3 grid((ni + kPairOut - 1) / kPairOut, (nj + rows - 1) / rows);
-  if (mixed)
-    hipLaunchKernelGGL(stage_pair_kernel<1>, grid, dim3(2 * kLanes), 0, s, p, a, rows);
-  else
-    hipLaunchKernelGGL(stage_pair_kernel<0>, grid, dim3(2 * kLanes), 0, s, p, a, rows);
+  const bool open = p.bc_n == kBcOpen;
+#define DIM2D_PK(M, O) hipLaunchKernelGGL((stage_pair_kernel<M, O>), grid, dim3(2 * kLanes), 0, s, p, a, rows)
+  if (mixed) {
+    if (open) DIM2D_PK(1, true);
+    else DIM2D_PK(1, false);
+  } else {
+    if (open) DIM2D_PK(0, true);
+    else DIM2D_PK(0, false);
+  }
+#undef DIM2D_PK
   return hipGetLastError();
 }
 

@@ -197,14 +197,19 @@ int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
  * per cell and step in FP64, 32 in mixed precision); fuse = 0 launches the
  * stages apart.  rows_per_block: tile height of the fused launch (0 keeps the
  * current value, default from the grid size).  On by default where it
- * applies: RK3-TVD with the Mattsson operators, reflection, periodic or S/N
- * wall edges, at least 8 interior points per axis; DIM2D_EINVAL otherwise.
- * Results equal those of the separate launches. */
+ * applies: RK3-TVD with the Mattsson operators, reflection, periodic, S/N
+ * wall or open edges, at least 8 interior points per axis; DIM2D_EINVAL
+ * otherwise.  With open edges the ghost frame's stage 1 runs before the
+ * fused launch and its stage 2 after it.  Results equal those of the separate
+ * launches to rounding (the launch shapes contract products into FMAs
+ * differently). */
 int dim2d_set_stage_fusion(dim2d_ctx* ctx, int32_t fuse, int32_t rows_per_block);
 
 /* Kernel timing with HIP events recorded on the context's stream around the
  * kernel launches of the following steps whose kind is in the bit mask
- * `enable` (1 stage 1, 2 stage 2, 4 stage 3, 8 wall ghost kernels; 15 all),
+ * `enable` (1 stage 1, 2 stage 2, 4 stage 3, 8 boundary kernels: the wall
+ * ghost kernel, and with open edges and stage fusion the frame kernels of
+ * stages 1 and 2; 15 all),
  * or off (enable = 0, which also discards what was recorded).  Every event
  * pair costs a few microseconds of step time, and while profiling,
  * dim2d_integrate launches the steps one by one instead of replaying them
@@ -215,7 +220,7 @@ int dim2d_profile_enable(dim2d_ctx* ctx, int32_t enable);
 /* Mean duration (ms) and launch count per kernel since enabling (blocking):
  * out[0],out[1] stage 1, or the fused stages 1 and 2 with stage fusion on
  * (out[2], out[3] then stay 0); out[2],out[3] stage 2; out[4],out[5] stage 3;
- * out[6],out[7] wall ghost kernel. */
+ * out[6],out[7] boundary kernels (as in dim2d_profile_enable). */
 int dim2d_profile_read(dim2d_ctx* ctx, double out[8]);
 
 /* ------------------------------------------------------------------ */
