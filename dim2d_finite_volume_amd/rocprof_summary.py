@@ -34,14 +34,14 @@ import re
 from collections import defaultdict
 
 STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
-PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)>")
+PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)(?:,\s*(true|false))?>")
 BSTAGE_RE = re.compile(r"dim2d::stage_batch_kernel<(\d+),\s*(\d+)>")
 BPAIR_RE = re.compile(r"dim2d::stage_pair_batch_kernel<(\d+)>")
 
 
 def kernel_key(name):
     """file-name key of a stage kernel: [batch_]stage<s>_<fp64|mixed> or
-    [batch_]pair_<fp64|mixed>"""
+    [batch_]pair_<fp64|mixed>[_open]"""
     m = BSTAGE_RE.search(name)
     if m:
         return f"batch_stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
@@ -53,7 +53,7 @@ def kernel_key(name):
         return f"stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
     m = PAIR_RE.search(name)
     if m:
-        return f"pair_{'mixed' if m.group(1) == '1' else 'fp64'}"
+        return f"pair_{'mixed' if m.group(1) == '1' else 'fp64'}{'_open' if m.group(2) == 'true' else ''}"
     return None
 
 
@@ -69,6 +69,8 @@ def short_name(name):
     m = PAIR_RE.search(name)
     if m:
         prec = "mixed" if m.group(1) == "1" else "FP64"
+        if m.group(2) == "true":
+            return f"stage_pair_kernel<{m.group(1)}, true> ({prec} stages 1+2, open edges)"
         return f"stage_pair_kernel<{m.group(1)}> ({prec} stages 1+2)"
     m = STAGE_RE.search(name)
     if m:
