@@ -1159,16 +1159,21 @@ struct dim2d_batch {
   long long steps = 0;
 };
 
-// the tiles of every simulation (strips of `ow` output columns, `rows` rows),
-// largest simulations first so the persistent grid drains evenly
+// the tiles of every simulation (strips of `ow` output columns, at most
+// `rows` rows), largest simulations first so the persistent grid drains
+// evenly.  A simulation's tiles share one height, the smallest that keeps
+// their number (a 64-row grid in 60-row tiles takes two 32-row tiles, not
+// 60 + 4 rows: every tile pays the pipeline's warm-up and halo rows).
 static std::vector<WorkItem> batch_items(const dim2d_batch* b, int ow, int rows) {
   std::vector<WorkItem> items;
   for (int s = 0; s < b->n; ++s) {
     const DevParams& p = b->hp[s];
+    const int nj = p.ny - 2 * kBcSize;
     const int nbx = (p.nx - 2 * kBcSize + ow - 1) / ow;
-    const int nby = (p.ny - 2 * kBcSize + rows - 1) / rows;
+    const int nby = (nj + rows - 1) / rows;
+    const int h = (nj + nby - 1) / nby;
     for (int by = 0; by < nby; ++by)
-      for (int bx = 0; bx < nbx; ++bx) items.push_back(WorkItem{s, bx, by, 0});
+      for (int bx = 0; bx < nbx; ++bx) items.push_back(WorkItem{s, bx, by, h});
   }
   std::stable_sort(items.begin(), items.end(), [&](const WorkItem& x, const WorkItem& y) {
     return b->hp[x.sim].nx * (long long)b->hp[x.sim].ny > b->hp[y.sim].nx * (long long)b->hp[y.sim].ny;
@@ -1223,8 +1228,8 @@ static int batch_step_launches(dim2d_batch* b, double* U, double* U2) {
     a.U = U;
     if (!b->mixed) a.O64 = b->B;
     else a.O32 = b->D2;
-    HIP_TRY(launch_stage_pair_batch(b->dp, b->d_pitems, b->npitems, b->mixed, a, b->pair_rows,
-                                    b->pair_wg_eff, s));
+    HIP_TRY(launch_stage_pair_batch(b->dp, b->d_pitems, b->npitems, b->mixed, a, b->pair_wg_eff,
+                                    s));
     if (!b->mixed) {
       if (b->any_wall)
         HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, b->B, nullptr, b->B, nullptr, U, s));
@@ -1237,7 +1242,7 @@ static int batch_step_launches(dim2d_batch* b, double* U, double* U2) {
       a.O32 = nullptr;
       a.O64 = U2;
     }
-    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, b->mixed, 3, a, b->block_x, b->rows, wg, s));
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, b->mixed, 3, a, wg, s));
     if (b->any_wall)
       HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, U2, nullptr, U2, nullptr, U, s));
     return DIM2D_OK;
@@ -1246,34 +1251,34 @@ static int batch_step_launches(dim2d_batch* b, double* U, double* U2) {
     a.U = U;
     a.S64 = U;
     a.O64 = b->A;
-    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 1, a, b->block_x, b->rows, wg, s));
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 1, a, wg, s));
     if (b->any_wall)
       HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, b->A, nullptr, b->A, nullptr, U, s));
     a.S64 = b->A;
     a.O64 = b->B;
-    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 2, a, b->block_x, b->rows, wg, s));
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 2, a, wg, s));
     if (b->any_wall)
       HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, b->B, nullptr, b->B, nullptr, U, s));
     a.S64 = b->B;
     a.O64 = U2;
-    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 3, a, b->block_x, b->rows, wg, s));
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 0, 3, a, wg, s));
     if (b->any_wall)
       HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, U2, nullptr, U2, nullptr, U, s));
   } else {
     a.U = U;
     a.O32 = b->D1;
-    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 1, a, b->block_x, b->rows, wg, s));
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 1, a, wg, s));
     if (b->any_wall)
       HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 1, U, nullptr, b->D1, nullptr, b->D1, U, s));
     a.S32 = b->D1;
     a.O32 = b->D2;
-    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 2, a, b->block_x, b->rows, wg, s));
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 2, a, wg, s));
     if (b->any_wall)
       HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 1, U, nullptr, b->D2, nullptr, b->D2, U, s));
     a.S32 = b->D2;
     a.O32 = nullptr;
     a.O64 = U2;
-    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 3, a, b->block_x, b->rows, wg, s));
+    HIP_TRY(launch_stage_batch(b->dp, b->d_items, b->nitems, 1, 3, a, wg, s));
     if (b->any_wall)
       HIP_TRY(launch_wall_batch(b->hp.data(), b->dp, b->n, 0, nullptr, U2, nullptr, U2, nullptr, U, s));
   }
@@ -1487,7 +1492,7 @@ int dim2d_batch_autotune(dim2d_batch* b, const double* dts, int32_t nsteps, doub
   const int rows_c[] = {8, 12, 16, 24, 32};
   const int wg_c[] = {resident, 2 * resident};
   const int president = max_resident_pair_workgroups(b->mixed);
-  const int prows_c[] = {16, 24, 32, 48, 60};
+  const int prows_c[] = {16, 24, 32, 48, 60, 96, 128};
   const int pwg_c[] = {president, 2 * president};
   const int steps = nsteps & ~1;  // whole graph replays
   // keep the state: candidates run real steps

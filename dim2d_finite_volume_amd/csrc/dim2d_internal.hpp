@@ -67,9 +67,10 @@ struct StepCtl {
   unsigned int pad[2];
 };
 
-// one tile of the batch work list
+// one tile of the batch work list: strip bx, rows 2 + by*rows .. of
+// simulation sim (rows: that simulation's tile height)
 struct WorkItem {
-  int sim, bx, by, pad;
+  int sim, bx, by, rows;
 };
 
 // kernel launch helpers (dim2d_kernels.hip)
@@ -100,16 +101,18 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
 bool pair_supported(const DevParams& p);
 hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, int rows,
                              hipStream_t s);
-// the stage pair over a batch work list of pair tiles (56-column strips,
-// `rows` rows), persistent grid of `workgroups` two-wave workgroups, per-
-// simulation dt from a.dts; every simulation must satisfy pair_supported
+// the stage pair over a batch work list of pair tiles (56-column strips),
+// persistent grid of `workgroups` two-wave workgroups, per-simulation dt from
+// a.dts; every simulation must satisfy pair_supported
 hipError_t launch_stage_pair_batch(const DevParams* d_params, const WorkItem* d_items,
-                                   int nitems, int mixed, const StageArgs& a, int rows,
-                                   int workgroups, hipStream_t s);
+                                   int nitems, int mixed, const StageArgs& a, int workgroups,
+                                   hipStream_t s);
 constexpr int kPairOutCols = 56;  // output columns per stage-pair strip
+// one RK stage over a batch work list of 60-column strips, persistent grid
+// of `workgroups` one-wave workgroups
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
-                              int mixed, int stage, const StageArgs& a, int block_x, int rows,
-                              int workgroups, hipStream_t s);
+                              int mixed, int stage, const StageArgs& a, int workgroups,
+                              hipStream_t s);
 // wall ghost cells of the state `st` (= U + S32 in mixed mode) written into
 // O64 or O32; tmp is u^n for the root bracket guess.
 hipError_t launch_wall(const DevParams& p, int mixed_in, const double* U, const double* S64,

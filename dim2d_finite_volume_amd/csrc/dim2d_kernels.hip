@@ -883,7 +883,7 @@ template <int MIXED, int STAGE>
 __global__ void __launch_bounds__(kLanes)
 __attribute__((amdgpu_waves_per_eu(MIXED ? DIM2D_WAVES_MIXED : DIM2D_WAVES_FP64, 8)))
 stage_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restrict__ items, int nitems,
-                   const StageArgs a, const double* __restrict__ dts, int rows) {
+                   const StageArgs a, const double* __restrict__ dts) {
   // density and its differences stay FP64 in both modes: the capillary
   // second differences of O(1) densities are scaled by 1/dx^2, so FP32
   // densities (absolute rounding ~1e-7) would leave errors of order 1e-7/dx^2
@@ -899,7 +899,7 @@ stage_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restrict__
     b.S32 = a.S32 ? a.S32 + p.offset : nullptr;
     b.O64 = a.O64 ? a.O64 + p.offset : nullptr;
     b.O32 = a.O32 ? a.O32 + p.offset : nullptr;
-    stage_tile<RD, R, MIXED, STAGE, false>(p, b, dts[w.sim], w.bx, w.by, rows, red);
+    stage_tile<RD, R, MIXED, STAGE, false>(p, b, dts[w.sim], w.bx, w.by, w.rows, red);
   }
 }
 
@@ -1538,7 +1538,7 @@ template <int MIXED>
 __global__ void __launch_bounds__(2 * kLanes)
 __attribute__((amdgpu_waves_per_eu(MIXED ? DIM2D_PAIR_BATCH_WAVES_MIXED : 2, 8)))
 stage_pair_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restrict__ items,
-                        int nitems, const StageArgs a, const double* __restrict__ dts, int rows) {
+                        int nitems, const StageArgs a, const double* __restrict__ dts) {
   __shared__ PairRing<MIXED> ring;
   const bool swap = !MIXED && (blockIdx.x & 1) != 0;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -1548,8 +1548,8 @@ stage_pair_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restr
     b.U = a.U + p.offset;
     b.O64 = a.O64 ? a.O64 + p.offset : nullptr;
     b.O32 = a.O32 ? a.O32 + p.offset : nullptr;
-    const int r0 = kBcSize + w.by * rows;
-    const int r1 = min(r0 + rows, p.ny - kBcSize);
+    const int r0 = kBcSize + w.by * w.rows;
+    const int r1 = min(r0 + w.rows, p.ny - kBcSize);
     pair_tile<MIXED>(p, b, dts[w.sim], w.bx, r0, r1, swap, ring);
   }
 }
@@ -2577,25 +2577,24 @@ hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, 
 }
 
 hipError_t launch_stage_pair_batch(const DevParams* d_params, const WorkItem* d_items,
-                                   int nitems, int mixed, const StageArgs& a, int rows,
-                                   int workgroups, hipStream_t s) {
+                                   int nitems, int mixed, const StageArgs& a, int workgroups,
+                                   hipStream_t s) {
   if (mixed)
     hipLaunchKernelGGL(stage_pair_batch_kernel<1>, dim3(workgroups), dim3(2 * kLanes), 0, s,
-                       d_params, d_items, nitems, a, a.dts, rows);
+                       d_params, d_items, nitems, a, a.dts);
   else
     hipLaunchKernelGGL(stage_pair_batch_kernel<0>, dim3(workgroups), dim3(2 * kLanes), 0, s,
-                       d_params, d_items, nitems, a, a.dts, rows);
+                       d_params, d_items, nitems, a, a.dts);
   return hipGetLastError();
 }
 
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
-                              int mixed, int stage, const StageArgs& a, int block_x, int rows,
-                              int workgroups, hipStream_t s) {
-  (void)block_x;
+                              int mixed, int stage, const StageArgs& a, int workgroups,
+                              hipStream_t s) {
   const int wg = workgroups;
 #define DIM2D_BL(M, S)                                                                     \
   hipLaunchKernelGGL((stage_batch_kernel<M, S>), dim3(wg), dim3(kLanes), 0, s, d_params,  \
-                     d_items, nitems, a, a.dts, rows)
+                     d_items, nitems, a, a.dts)
   if (!mixed) {
     if (stage == 1) DIM2D_BL(0, 1);
     else if (stage == 2) DIM2D_BL(0, 2);

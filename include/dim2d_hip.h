@@ -309,10 +309,12 @@ int dim2d_batch_upload(dim2d_batch* b, int32_t sim, const double* nodes);
 int dim2d_batch_download(dim2d_batch* b, int32_t sim, double* nodes);
 /* nsteps RK3 steps of every simulation, each with its own fixed dt (dts[n]). */
 int dim2d_batch_integrate(dim2d_batch* b, const double* dts, int32_t nsteps);
-/* Tile selection for the batch: block_x 64 (one strip per workgroup; the
- * persistent kernel does not pair strips) or 0, rows per tile as in
- * dim2d_set_tiles, and the number of persistent workgroups (0: one per work
- * item, capped by residency). */
+/* Tile selection for the batch's separate stage kernels (stage 3, or all
+ * three without stage fusion): block_x 64 (one strip per workgroup; the
+ * persistent kernel does not pair strips) or 0, rows per tile at most (each
+ * simulation's tiles share the smallest height that keeps their number), and
+ * the number of persistent workgroups (0: one per work item, capped by
+ * residency). */
 int dim2d_batch_set_tiles(dim2d_batch* b, int32_t block_x, int32_t rows_per_block,
                           int32_t workgroups);
 int dim2d_batch_synchronize(dim2d_batch* b);
@@ -322,9 +324,10 @@ int dim2d_batch_synchronize(dim2d_batch* b);
  * then stage 3 as before.  On by default when every simulation has
  * reflection, periodic or S/N wall edges and at least 8 interior points per
  * axis; fuse = 1 on any other batch returns DIM2D_EINVAL.  rows_per_block
- * and workgroups shape the pair's tiles (56-column strips) and persistent
- * grid (0: 32 rows; one workgroup per tile, capped by residency).  No
- * counterpart in the reference. */
+ * (at most, balanced per simulation as in dim2d_batch_set_tiles) and
+ * workgroups shape the pair's tiles (56-column strips) and persistent grid
+ * (0: 32 rows; one workgroup per tile, capped by residency).  No counterpart
+ * in the reference. */
 int dim2d_batch_set_stage_fusion(dim2d_batch* b, int32_t fuse, int32_t rows_per_block,
                                  int32_t workgroups);
 /* The launch shapes in use: out = {stage fusion on, pair rows, pair
@@ -334,7 +337,9 @@ int dim2d_batch_get_tiles(dim2d_batch* b, int32_t out[6]);
 /* Tile autotuner of a batch (no counterpart in the reference): times nsteps
  * (>= 2) steps with the given dts for each candidate, keeps the fastest, and
  * restores the state it started from.  With stage fusion it first tries the
- * pair's tile heights (16, 24, 32, 48, 60 rows) and persistent workgroup
+ * pair's tile heights (16, 24, 32, 48, 60, 96, 128 rows at most: a
+ * simulation's tiles share the smallest height that keeps their number) and
+ * persistent workgroup
  * counts (1x and 2x the pair workgroups that fit the device at once), then
  * the other stage kernels' tile heights (8, 12, 16, 24, 32 rows) and
  * workgroup counts.  out[0] = rows, out[1] = workgroups of the latter,

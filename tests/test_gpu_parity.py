@@ -502,7 +502,7 @@ def test_batch_matches_single(orc, fuse):
     t = b.tiles()
     assert t["stage_fusion"] == fuse and t["rows"] == rows
     if fuse:
-        assert t["pair_rows"] in (16, 24, 32, 48, 60) and t["pair_workgroups"] > 0
+        assert t["pair_rows"] in (16, 24, 32, 48, 60, 96, 128) and t["pair_workgroups"] > 0
     b.integrate(dts, 6)
     for i, (p, n) in enumerate(sims):
         f = d.Field(p, n)
