@@ -320,6 +320,12 @@ static int default_pair_rows(const DevParams& p) {
   return wg32 < 500 ? 8 : wg32 < 2000 ? 16 : 60;
 }
 
+// stage fusion (the stage pair) applies to the Mattsson operators on the
+// edges pair_supported takes; the low-storage RK3 (FP64) not with open edges
+static bool fusion_applies(const dim2d_ctx* c) {
+  return !c->cg && pair_supported(c->dp) && !(c->lsrk && is_open(c->dp));
+}
+
 int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (!out) return fail(DIM2D_EINVAL, "out is NULL");
   *out = nullptr;
@@ -334,7 +340,7 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   c->cg = params->operators == DIM2D_OPS_COCKBURN_GAU;
   c->rows = default_rows(c->dp);
   c->block_x = default_block_x(c->dp);
-  c->fuse = !c->lsrk && !c->cg && pair_supported(c->dp);
+  c->fuse = fusion_applies(c);
   c->pair_rows = default_pair_rows(c->dp);
   const size_t n = (size_t)c->dp.ncell * 4;
   hipError_t e = hipGetDevice(&c->device);
@@ -427,11 +433,10 @@ int dim2d_set_stage_fusion(dim2d_ctx* c, int32_t fuse, int32_t rows_per_block) {
   if (fuse != 0 && fuse != 1) return fail(DIM2D_EINVAL, "fuse must be 0 or 1, got %d", fuse);
   if (rows_per_block != 0 && (rows_per_block < 4 || rows_per_block > 4096))
     return fail(DIM2D_EINVAL, "rows_per_block must be 0 or in [4, 4096], got %d", rows_per_block);
-  if (fuse && (c->lsrk || c->cg || !pair_supported(c->dp)))
+  if (fuse && !fusion_applies(c))
     return fail(DIM2D_EINVAL,
-                "stage fusion needs the RK3-TVD integrator with the Mattsson operators, "
-                "reflection, periodic, S/N wall or open edges and at least 8 interior points "
-                "per axis");
+                "stage fusion needs the Mattsson operators, reflection, periodic, S/N wall or "
+                "(RK3-TVD only) open edges and at least 8 interior points per axis");
   c->fuse = fuse;
   if (rows_per_block) {
     c->pair_rows = rows_per_block;
@@ -626,6 +631,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     return DIM2D_OK;
   };
   int st = DIM2D_OK;
+  bool result_in_u = false;  // the new state is in U (else in U2, swapped below)
   if (c->ff_set) HIP_TRY(launch_save_frame(p, c->U, c->F0, c->stream));
   if (c->cg) {
     // Cockburn-Gau stencils: FP64 RK3-TVD stages u^n -> A -> B -> U2, no walls
@@ -637,6 +643,21 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
       a.O64 = out[s];
       HIP_TRY(timed_launch(c, s, [&] { return launch_cg_stage(p, 1 + s, a, c->stream); }));
     }
+  } else if (c->lsrk && c->fuse) {
+    // low-storage stages 1 and 2 in one launch, u1 and q1 on chip: u0 (U) ->
+    // u2 (U2) and q2 (Q); stage 3 (U2, Q) -> U, so the new state is in U and
+    // the buffers are not swapped below.  The wall root guess is each stage's
+    // input state, except for the ghosts of u2 (u1 is not stored): u0
+    a.U = c->U;
+    a.Q = c->Q;
+    a.O64 = c->U2;
+    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage_pair(p, 0, 1, a, c->pair_rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
+    a.S64 = c->U2;
+    a.O64 = c->U;
+    HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 0, 6, a, c->block_x, c->rows, c->stream); }));
+    if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U, nullptr, c->U, nullptr, c->U2, c->stream); }));
+    result_in_u = true;
   } else if (c->lsrk) {
     // low-storage RK3: the state ping-pongs U -> U2 -> U -> U2, q stays in Q;
     // the wall root guess is each stage's input state
@@ -670,7 +691,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     }
     a.O64 = c->mixed ? nullptr : c->B;
     a.O32 = c->mixed ? c->D2 : nullptr;
-    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage_pair(p, c->mixed, a, c->pair_rows, c->stream); }));
+    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage_pair(p, c->mixed, 0, a, c->pair_rows, c->stream); }));
     if (open) {
       a.S64 = c->mixed ? nullptr : c->A;
       a.S32 = c->mixed ? c->D1 : nullptr;
@@ -735,7 +756,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     if ((st = frame(1, 3))) return st;
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
   }
-  std::swap(c->U, c->U2);
+  if (!result_in_u) std::swap(c->U, c->U2);
   c->steps++;
   c->red_valid = true;
   if (c->ff_set) c->f0_valid = true;
@@ -954,7 +975,7 @@ int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int3
     n->pair_rows = c->pair_rows;
     n->pair_rows_set = true;
   }
-  n->fuse = c->fuse && pair_supported(n->dp);
+  n->fuse = c->fuse && fusion_applies(n);
   n->uploaded = true;
   n->steps = c->steps;
   n->profiling = c->profiling;

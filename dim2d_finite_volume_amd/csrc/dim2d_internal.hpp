@@ -99,7 +99,9 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
 // (output a.F64 / a.F32) runs before it and the stage-2 frame kernel (input
 // the same buffer) after it.
 bool pair_supported(const DevParams& p);
-hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, int rows,
+// lsrk: the low-storage RK3's stages 1 and 2 instead (FP64 without open
+// edges): u0 (a.U) -> u2 in a.O64 and q2 in a.Q
+hipError_t launch_stage_pair(const DevParams& p, int mixed, int lsrk, const StageArgs& a, int rows,
                              hipStream_t s);
 // the stage pair over a batch work list of pair tiles (56-column strips),
 // persistent grid of `workgroups` two-wave workgroups, per-simulation dt from

@@ -1340,7 +1340,8 @@ __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<
 
 // the producer wave: stage 1 of rows r0-2 .. rlast into the ring, then the
 // north wall rows when the tile holds them, then idle barriers up to `steps`
-template <typename RD, typename R, int MIXED, bool OPEN>
+// LS: the low-storage RK3's stages 1 and 2 (FP64; the ring's u rows hold q1)
+template <typename RD, typename R, int MIXED, bool OPEN, bool LS>
 __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArgs& a, double dt,
                                               int bx, int r0, int r1, int steps,
                                               PairRing<MIXED>& ring) {
@@ -1396,6 +1397,11 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
           ring.d[MIXED ? sl : 0][k][lane] = o32;
           ring.s[sl][k][lane] = S[k] + (double)o32;
           if (band) a.F32[o + k] = o32;
+        } else if (LS) {
+          // low-storage stage 1: q1 = dt L, u1 = u0 + B1 q1
+          const double q = 0.0 + dl;
+          ring.s[sl][k][lane] = S[k] + (1.0 / 3.0) * q;
+          ring.u[MIXED ? 0 : sl][k][lane] = q;
         } else {
           ring.s[sl][k][lane] = S[k] + dl;
           ring.u[MIXED ? 0 : sl][k][lane] = S[k];
@@ -1423,7 +1429,7 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
 
 // the consumer wave: kPairLag idle barriers, the south wall rows when the
 // tile holds them, then stage 2 of rows r0 .. r1-1 from the ring
-template <typename RD, typename R, int MIXED, bool OPEN>
+template <typename RD, typename R, int MIXED, bool OPEN, bool LS>
 __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArgs& a, double dt,
                                               int bx, int r0, int r1, PairRing<MIXED>& ring) {
   const int lane = threadIdx.x & (kLanes - 1);
@@ -1453,7 +1459,18 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
     if (out_col && r >= r0) {
       const long long row = (long long)r * p.pitch * 4;
       const double* S = sr.raw[K];
-      if (!MIXED) {
+      if (!MIXED && LS) {
+        // low-storage stage 2: q2 = A2 q1 + dt L, u2 = u1 + B2 q2
+        double vout[4], q[4];
+        const int sl = r & (kPairSlots - 1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          q[k] = (-5.0 / 9.0) * ring.u[MIXED ? 0 : sl][k][lane] + dt * (double)L[k];
+          vout[k] = S[k] + (15.0 / 16.0) * q[k];
+        }
+        st_cell(a.O64 + row, uc, vout);
+        st_cell(a.Q + row, uc, q);
+      } else if (!MIXED) {
         double vout[4];
         const int sl = r & (kPairSlots - 1);
 #pragma unroll
@@ -1488,18 +1505,19 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
 // p.  Both waves pass exactly kPairLag + (r1 - r0 + 2) barriers, and the
 // consumer's last ring read precedes its last barrier, so a workgroup may run
 // tiles back to back on one ring.
-template <int MIXED, bool OPEN = false>
+template <int MIXED, bool OPEN = false, bool LS = false>
 __device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a, double dt,
                                           int bx, int r0, int r1, bool swap,
                                           PairRing<MIXED>& ring) {
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
+  static_assert(!(LS && (MIXED || OPEN)), "the low-storage pair is FP64 without open edges");
   // the consumer's rows r0-2 .. r1-1 after kPairLag idle barriers
   const int steps = kPairLag + (r1 - r0 + 2);
   if ((threadIdx.x < kLanes) != swap)
-    pair_producer<RD, R, MIXED, OPEN>(p, a, dt, bx, r0, r1, steps, ring);
+    pair_producer<RD, R, MIXED, OPEN, LS>(p, a, dt, bx, r0, r1, steps, ring);
   else
-    pair_consumer<RD, R, MIXED, OPEN>(p, a, dt, bx, r0, r1, ring);
+    pair_consumer<RD, R, MIXED, OPEN, LS>(p, a, dt, bx, r0, r1, ring);
 }
 
 // minimum waves per SIMD: 2 in FP64 (up to 256 VGPRs), as the stage kernels;
@@ -1507,7 +1525,7 @@ __device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a
 // all four edges open (hedstrom_xy); the frame cells of s1 come from the
 // stage-1 frame kernel, launched before this kernel, and the stage-2 frame
 // kernel, launched after it, reads the band of s1 this kernel stores.
-template <int MIXED, bool OPEN>
+template <int MIXED, bool OPEN, bool LS>
 __global__ void __launch_bounds__(2 * kLanes)
 __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2, 8)))
 stage_pair_kernel(const DevParams p, const StageArgs a, int rows) {
@@ -1525,7 +1543,7 @@ stage_pair_kernel(const DevParams p, const StageArgs a, int rows) {
   // consumers alike rather than two of one kind (measured 4 % faster at
   // 4096^2; in mixed precision, 3 waves per SIMD, 9 % slower)
   const bool swap = !MIXED && (((blockIdx.x + blockIdx.y) & 1) != 0);
-  pair_tile<MIXED, OPEN>(p, a, dt, blockIdx.x, r0, r1, swap, ring);
+  pair_tile<MIXED, OPEN, LS>(p, a, dt, blockIdx.x, r0, r1, swap, ring);
 }
 
 // persistent batch form of the stage pair: one grid walks the pair tiles
@@ -2559,18 +2577,22 @@ bool pair_supported(const DevParams& p) {
   return p.nx >= 2 * kBcSize + 8 && p.ny >= 2 * kBcSize + 8;
 }
 
-hipError_t launch_stage_pair(const DevParams& p, int mixed, const StageArgs& a, int rows,
+hipError_t launch_stage_pair(const DevParams& p, int mixed, int lsrk, const StageArgs& a, int rows,
                              hipStream_t s) {
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
   dim3 grid((ni + kPairOut - 1) / kPairOut, (nj + rows - 1) / rows);
   const bool open = p.bc_n == kBcOpen;
-#define DIM2D_PK(M, O) hipLaunchKernelGGL((stage_pair_kernel<M, O>), grid, dim3(2 * kLanes), 0, s, p, a, rows)
-  if (mixed) {
-    if (open) DIM2D_PK(1, true);
-    else DIM2D_PK(1, false);
+#define DIM2D_PK(M, O, L) \
+  hipLaunchKernelGGL((stage_pair_kernel<M, O, L>), grid, dim3(2 * kLanes), 0, s, p, a, rows)
+  if (lsrk) {
+    if (mixed || open) return hipErrorInvalidValue;
+    DIM2D_PK(0, false, true);
+  } else if (mixed) {
+    if (open) DIM2D_PK(1, true, false);
+    else DIM2D_PK(1, false, false);
   } else {
-    if (open) DIM2D_PK(0, true);
-    else DIM2D_PK(0, false);
+    if (open) DIM2D_PK(0, true, false);
+    else DIM2D_PK(0, false, false);
   }
 #undef DIM2D_PK
   return hipGetLastError();

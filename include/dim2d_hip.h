@@ -191,18 +191,20 @@ int dim2d_synchronize(dim2d_ctx* ctx);
  * the tiling. */
 int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
 
-/* RK3-TVD stage fusion (no counterpart in the reference, whose stages each
- * write a full field, rk3tvd_steps_module.f:52-490): fuse = 1 computes stages
- * 1 and 2 in one launch whose stage-1 state stays on chip (64 fewer HBM bytes
- * per cell and step in FP64, 32 in mixed precision); fuse = 0 launches the
- * stages apart.  rows_per_block: tile height of the fused launch (0 keeps the
- * current value, default from the grid size).  On by default where it
- * applies: RK3-TVD with the Mattsson operators, reflection, periodic, S/N
- * wall or open edges, at least 8 interior points per axis; DIM2D_EINVAL
- * otherwise.  With open edges the ghost frame's stage 1 runs before the
- * fused launch and its stage 2 after it.  Results equal those of the separate
- * launches to rounding (the launch shapes contract products into FMAs
- * differently). */
+/* Stage fusion (no counterpart in the reference, whose stages each write a
+ * full field, rk3tvd_steps_module.f:52-490): fuse = 1 computes stages 1 and 2
+ * in one launch whose stage-1 state stays on chip (RK3-TVD: 64 fewer HBM
+ * bytes per cell and step in FP64, 32 in mixed precision; low-storage RK3:
+ * 128 fewer, u1 and q1 on chip); fuse = 0 launches the stages apart.
+ * rows_per_block: tile height of the fused launch (0 keeps the current
+ * value, default from the grid size).  On by default where it applies: the
+ * Mattsson operators, reflection, periodic, S/N wall or (RK3-TVD only) open
+ * edges, at least 8 interior points per axis; DIM2D_EINVAL otherwise.  With
+ * open edges the ghost frame's stage 1 runs before the fused launch and its
+ * stage 2 after it.  Results equal those of the separate launches to
+ * rounding (the launch shapes contract products into FMAs differently; the
+ * low-storage RK3's wall root for the ghosts of u2 starts from u0, u1 not
+ * being stored). */
 int dim2d_set_stage_fusion(dim2d_ctx* ctx, int32_t fuse, int32_t rows_per_block);
 
 /* Kernel timing with HIP events recorded on the context's stream around the
