@@ -28,7 +28,7 @@ EXPORTS = (
     "dim2d_diagnostics", "dim2d_time", "dim2d_synchronize", "dim2d_set_tiles",
     "dim2d_set_stage_fusion", "dim2d_profile_enable", "dim2d_profile_read",
     "dim2d_get_params", "dim2d_openbc_undermined", "dim2d_set_far_field", "dim2d_extend",
-    "dim2d_adapt_domain",
+    "dim2d_adapt_domain", "dim2d_get_growth", "dim2d_shrink", "dim2d_openbc_undermined_decrease",
     "dim2d_check_steady_state", "dim2d_time_dev_noalloc",
     "dim2d_batch_create", "dim2d_batch_destroy", "dim2d_batch_upload", "dim2d_batch_download",
     "dim2d_batch_integrate", "dim2d_batch_set_tiles", "dim2d_batch_synchronize",
@@ -102,6 +102,9 @@ def load() -> ctypes.CDLL:
         "dim2d_set_far_field": (ctypes.c_int, [vp, ctypes.c_void_p]),
         "dim2d_extend": (ctypes.c_int, [vp, i32, i32, i32, i32]),
         "dim2d_adapt_domain": (ctypes.c_int, [vp, i32, i32, d, i32, ctypes.POINTER(i32)]),
+        "dim2d_get_growth": (ctypes.c_int, [vp, ctypes.POINTER(i32)]),
+        "dim2d_shrink": (ctypes.c_int, [vp, i32, i32, i32, i32]),
+        "dim2d_openbc_undermined_decrease": (ctypes.c_int, [vp, i32, d, ctypes.POINTER(i32)]),
         "dim2d_batch_create": (ctypes.c_int, [ctypes.POINTER(Params), i32, ctypes.POINTER(vp)]),
         "dim2d_batch_destroy": (ctypes.c_int, [vp]),
         "dim2d_batch_upload": (ctypes.c_int, [vp, i32, dp]),

@@ -204,6 +204,10 @@ struct dim2d_ctx {
   bool ff_set = false;
   double* F0 = nullptr;
   bool f0_valid = false;  // F0 holds the frame of the last step's u^n
+  // columns/rows added per edge (W, E, S, N) since creation: the grid the
+  // context was created on is the reference's interior domain, the rest its
+  // buffer layers
+  int grown[4] = {0, 0, 0, 0};
   int block_x = 64;
   int rows = 16;
   // RK3-TVD stages 1 and 2 in one launch (stage pair) where the edges allow
@@ -904,26 +908,18 @@ int dim2d_set_far_field(dim2d_ctx* c, const double far_field[4]) {
   return DIM2D_OK;
 }
 
-int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int32_t add_n) {
-  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
-  if (!c->uploaded) return fail(DIM2D_ESTATE, "extend before upload");
-  if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "only open (DIM2D_BC_OPEN) edges are extended");
-  if (add_w < 0 || add_e < 0 || add_s < 0 || add_n < 0)
-    return fail(DIM2D_EINVAL, "grid points to add must be >= 0");
-  const bool characteristic = c->ff_set;
-  if (characteristic && (add_w > 1 || add_e > 1 || add_s > 1 || add_n > 1))
-    return fail(DIM2D_EINVAL,
-                "with a far field the grid grows by at most one layer per edge and call");
+// The context on the grid grown by add_w .. add_n points per edge (negative:
+// cropped), the state moved to its place: the caller's handle keeps pointing
+// at `c`.  New points (add > 0) by the characteristic rule with a far field
+// set, else the copy rule; a crop only copies.
+static int resize_domain(dim2d_ctx* c, int add_w, int add_e, int add_s, int add_n) {
+  const bool characteristic = c->ff_set && (add_w > 0 || add_e > 0 || add_s > 0 || add_n > 0);
   const long long nx = (long long)c->prm.nx + add_w + add_e;
   const long long ny = (long long)c->prm.ny + add_s + add_n;
   if (nx > kMaxExtent || ny > kMaxExtent)
     return fail(DIM2D_EINVAL, "extended grid %lldx%lld exceeds %d points per axis", nx, ny,
                 kMaxExtent);
   if (nx == c->prm.nx && ny == c->prm.ny) return DIM2D_OK;
-  if (characteristic && !c->f0_valid)
-    return fail(DIM2D_ESTATE,
-                "no step since the last upload or extension: the state at t - dt that new "
-                "points are extrapolated from is missing");
   // the new grid keeps dx and dy: x_map(bc+1) = x_min moves add_w points west
   // (field_abstract_class.f:286-315 spacing)
   dim2d_params q = c->prm;
@@ -951,18 +947,22 @@ int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int3
     e = launch_newgrdpt(n->dp, c->dp, add_w, add_s, x0, y0, c->ff, n->ctl, c->F0, n->U,
                         n->stream);
   }
-  if (e == hipSuccess && characteristic)
+  if (e == hipSuccess && c->ff_set)
     e = hipMalloc(&n->F0, (size_t)frame_cells(n->dp.nx, n->dp.ny) * 4 * sizeof(double));
   if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
   if (e != hipSuccess) {
     dim2d_ctx_destroy(n);
     return fail(DIM2D_EDEVICE, "domain extension failed: %s", hipGetErrorString(e));
   }
-  if (characteristic) {
+  if (c->ff_set) {
     for (int k = 0; k < 4; ++k) n->ff[k] = c->ff[k];
     n->ff_set = true;
     n->f0_valid = false;
   }
+  n->grown[0] = c->grown[0] + add_w;
+  n->grown[1] = c->grown[1] + add_e;
+  n->grown[2] = c->grown[2] + add_s;
+  n->grown[3] = c->grown[3] + add_n;
   if (c->block_x_set) {  // else the new grid keeps the default of its own size
     n->block_x = c->block_x;
     n->block_x_set = true;
@@ -987,20 +987,126 @@ int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int3
   return DIM2D_OK;
 }
 
+int dim2d_extend(dim2d_ctx* c, int32_t add_w, int32_t add_e, int32_t add_s, int32_t add_n) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "extend before upload");
+  if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "only open (DIM2D_BC_OPEN) edges are extended");
+  if (add_w < 0 || add_e < 0 || add_s < 0 || add_n < 0)
+    return fail(DIM2D_EINVAL, "grid points to add must be >= 0");
+  if (c->ff_set && (add_w > 1 || add_e > 1 || add_s > 1 || add_n > 1))
+    return fail(DIM2D_EINVAL,
+                "with a far field the grid grows by at most one layer per edge and call");
+  if (add_w + add_e + add_s + add_n == 0) return DIM2D_OK;
+  if (c->ff_set && !c->f0_valid)
+    return fail(DIM2D_ESTATE,
+                "no step since the last upload or extension: the state at t - dt that new "
+                "points are extrapolated from is missing");
+  return resize_domain(c, add_w, add_e, add_s, add_n);
+}
+
+int dim2d_get_growth(dim2d_ctx* c, int32_t out[4]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  for (int k = 0; k < 4; ++k) out[k] = c->grown[k];
+  return DIM2D_OK;
+}
+
+int dim2d_shrink(dim2d_ctx* c, int32_t del_w, int32_t del_e, int32_t del_s, int32_t del_n) {
+  if (!c) return fail(DIM2D_EINVAL, "ctx is NULL");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "shrink before upload");
+  if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "only open (DIM2D_BC_OPEN) edges are shrunk");
+  const int32_t del[4] = {del_w, del_e, del_s, del_n};
+  for (int k = 0; k < 4; ++k)
+    if (del[k] < 0 || del[k] > c->grown[k])
+      return fail(DIM2D_EINVAL,
+                  "edge %d: %d points to remove, but %d were added since the context was "
+                  "created", k, del[k], c->grown[k]);
+  if (del_w + del_e + del_s + del_n == 0) return DIM2D_OK;
+  return resize_domain(c, -del_w, -del_e, -del_s, -del_n);
+}
+
+// the check bands of the decrease: for each edge grown since creation, the
+// `distance` rows (S, N) or columns (W, E) on each side of the edge's place
+// at creation, along the created grid's extent and `distance` - 1 points
+// beyond it (the ends of the S/N bands; W/E bands one more point south),
+// clipped to the points whose 3x3 neighbourhood lies in the grid
+// (get_check_line_param and check_layer_interior / check_layer_bf,
+// bf_decrease_module.f:252-330, 448-600); an edge not grown gets an empty band
+static Rects4 decrease_bands(const dim2d_ctx* c, int d) {
+  const int nx = c->dp.nx, ny = c->dp.ny;
+  const int gw = c->grown[0], ge = c->grown[1], gs = c->grown[2], gn = c->grown[3];
+  // 0-based columns/rows of the created grid's W/S ghost cells next to the
+  // interior (align_W, align_S) and of its E/N ones (align_E, align_N)
+  const int aw = 1 + gw, ae = nx - 2 - ge, as = 1 + gs, an = ny - 2 - gn;
+  Rects4 r;
+  auto set = [&](int k, int i0, int i1, int j0, int j1) {  // inclusive bounds
+    i0 = std::max(i0, 1);
+    i1 = std::min(i1, nx - 2);
+    j0 = std::max(j0, 1);
+    j1 = std::min(j1, ny - 2);
+    const bool on = c->grown[k] > 0 && i0 <= i1 && j0 <= j1;
+    r.i0[k] = on ? i0 : 0;
+    r.i1[k] = on ? i1 + 1 : 0;
+    r.j0[k] = on ? j0 : 0;
+    r.j1[k] = on ? j1 + 1 : 0;
+  };
+  set(0, aw - d + 1, aw + d, as - d, an + d - 1);  // W
+  set(1, ae - d, ae + d - 1, as - d, an + d - 1);  // E
+  set(2, aw - d + 1, ae + d - 1, as - d + 1, as + d);  // S
+  set(3, aw - d + 1, ae + d - 1, an - d, an + d - 1);  // N
+  return r;
+}
+
+int dim2d_openbc_undermined_decrease(dim2d_ctx* c, int32_t distance, double md_threshold,
+                                     int32_t out[4]) {
+  if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(DIM2D_ESTATE, "openbc_undermined_decrease before upload");
+  if (!is_open(c->dp)) return fail(DIM2D_EINVAL, "the edges are not open (DIM2D_BC_OPEN)");
+  if (distance < 1) return fail(DIM2D_EINVAL, "distance must be >= 1, got %d", distance);
+  if (!(md_threshold < 0.0) && !(md_threshold <= 1.0))
+    return fail(DIM2D_EINVAL, "md_threshold must be < 0 (off) or in [0, 1], got %g", md_threshold);
+  int* counts = reinterpret_cast<int*>(c->T);
+  HIP_TRY(hipMemsetAsync(counts, 0, 4 * sizeof(int), c->stream));
+  HIP_TRY(launch_undermined_rects(c->dp, decrease_bands(c, distance), md_threshold, c->U, counts,
+                                  c->stream));
+  HIP_TRY(hipMemcpyAsync(out, counts, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DIM2D_OK;
+}
+
 int dim2d_adapt_domain(dim2d_ctx* c, int32_t adapt_mask, int32_t distance, double md_threshold,
                        int32_t grow, int32_t out[4]) {
   if (!c || !out) return fail(DIM2D_EINVAL, "NULL argument");
-  if (adapt_mask < 0 || adapt_mask > 15)
+  if (adapt_mask < 0 || adapt_mask > 255)
     return fail(DIM2D_EINVAL, "adapt_mask out of range: %d", adapt_mask);
   if (grow < 1) return fail(DIM2D_EINVAL, "grow must be >= 1, got %d", grow);
   if (c->ff_set && grow != 1)
     return fail(DIM2D_EINVAL, "with a far field the grid grows by one layer per call, got %d",
                 grow);
-  int32_t cnt[4];
-  int st = dim2d_openbc_undermined(c, distance, md_threshold, cnt);
+  int32_t cnt[4], del[4] = {0, 0, 0, 0};
+  int st;
+  if (adapt_mask >> 4) {
+    // bf_interface_dcr%adapt_domain_extension: the decrease first
+    // (remove_inactivated_bf_layers, bf_interface_dcr_class.f:101-216)
+    st = dim2d_openbc_undermined_decrease(c, distance, md_threshold, cnt);
+    if (st) return st;
+    for (int k = 0; k < 4; ++k)
+      del[k] = ((adapt_mask >> (4 + k)) & 1) && c->grown[k] > 0 && cnt[k] == 0 ? c->grown[k] : 0;
+    st = dim2d_shrink(c, del[0], del[1], del[2], del[3]);
+    if (st) return st;
+  }
+  // then the increase, on the grid as it now is
+  st = dim2d_openbc_undermined(c, distance, md_threshold, cnt);
   if (st) return st;
-  for (int k = 0; k < 4; ++k) out[k] = ((adapt_mask >> k) & 1) && cnt[k] > 0 ? grow : 0;
-  return dim2d_extend(c, out[0], out[1], out[2], out[3]);
+  int32_t add[4];
+  for (int k = 0; k < 4; ++k) add[k] = (adapt_mask >> k) & 1 && cnt[k] > 0 ? grow : 0;
+  if (c->ff_set && !c->f0_valid) {
+    // a crop in this call: the extrapolation's state at t - dt is gone, the
+    // increase waits for the next call
+    for (int k = 0; k < 4; ++k) add[k] = 0;
+  }
+  st = dim2d_extend(c, add[0], add[1], add[2], add[3]);
+  for (int k = 0; k < 4; ++k) out[k] = add[k] - del[k];
+  return st;
 }
 
 int dim2d_synchronize(dim2d_ctx* c) {

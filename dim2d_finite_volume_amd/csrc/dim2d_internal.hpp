@@ -151,6 +151,14 @@ hipError_t launch_absmax(const DevParams& p, int frame, const double* F, unsigne
 // open edges: number of undermined points (imaginary sound speed; with
 // md_thr >= 0 also a density inside the two-phase band) in the `dist`
 // interior rows/columns nearest each edge, added into counts[0..3] (W, E, S, N)
+// rectangles [i0, i1) x [j0, j1) of grid points (0-based)
+struct Rects4 {
+  int i0[4], i1[4], j0[4], j1[4];
+};
+// undermined points per rectangle into counts[0..3] (counts cleared by the
+// caller); every point's 3x3 neighbourhood must lie in the array
+hipError_t launch_undermined_rects(const DevParams& p, const Rects4& r, double md_thr,
+                                   const double* U, int* counts, hipStream_t s);
 hipError_t launch_undermined(const DevParams& p, int dist, double md_thr, const double* U,
                              int* counts, hipStream_t s);
 // state of grid `op` copied into grid `np`, grown by add_w columns west and

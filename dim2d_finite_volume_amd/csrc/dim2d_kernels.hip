@@ -2515,11 +2515,39 @@ __global__ void undermined_kernel(const DevParams p, int dist, double md_thr,
   }
 }
 
+// undermined points in up to four rectangles [i0, i1) x [j0, j1) (0-based,
+// every point with its 3x3 neighbourhood in the array), counted per rectangle
+__global__ void undermined_rects_kernel(const DevParams p, const Rects4 r, double md_thr,
+                                        const double* __restrict__ U, int* __restrict__ counts) {
+  long long area[4], total = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    area[e] = (long long)max(r.i1[e] - r.i0[e], 0) * max(r.j1[e] - r.j0[e], 0);
+    total += area[e];
+  }
+  int local[4] = {0, 0, 0, 0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    int e = 0;
+    long long q = t;
+    while (q >= area[e]) q -= area[e++];
+    const int w = r.i1[e] - r.i0[e];
+    const int i = r.i0[e] + (int)(q % w), j = r.j0[e] + (int)(q / w);
+    if (openbc_undermined_at(p, U, i, j, md_thr)) local[e] += 1;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const unsigned c = wave_sum_u((unsigned)local[e]);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&counts[e], (int)c);
+  }
+}
+
 // copy of the state into a grid grown by add_w columns on the west and add_s
 // rows on the south (and any number on the east and north): every point of
 // the new grid takes the value of the nearest point of the old grid (the old
 // points keep theirs, new points repeat the old edge point along the normal,
-// new corner points the old corner point)
+// new corner points the old corner point).  Negative counts crop: the new
+// grid is a sub-rectangle of the old one.
 __global__ void extend_kernel(const DevParams np, const DevParams op, int add_w, int add_s,
                               const double* __restrict__ oldU, double* __restrict__ newU) {
   const long long n = (long long)np.nx * np.ny;
@@ -2739,6 +2767,13 @@ hipError_t launch_cg_stage(const DevParams& p, int stage, const StageArgs& a, hi
 hipError_t launch_undermined(const DevParams& p, int dist, double md_thr, const double* U,
                              int* counts, hipStream_t s) {
   hipLaunchKernelGGL(undermined_kernel, dim3(1024), dim3(256), 0, s, p, dist, md_thr, U, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_undermined_rects(const DevParams& p, const Rects4& r, double md_thr,
+                                   const double* U, int* counts, hipStream_t s) {
+  hipLaunchKernelGGL(undermined_rects_kernel, dim3(1024), dim3(256), 0, s, p, r, md_thr, U,
+                     counts);
   return hipGetLastError();
 }
 

@@ -172,6 +172,20 @@ module dim2d_hip_cabi
       integer(c_int) :: st
     end function dim2d_extend
 
+    function dim2d_shrink(ctx, del_w, del_e, del_s, del_n) bind(C, name="dim2d_shrink") result(st)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: del_w, del_e, del_s, del_n
+      integer(c_int) :: st
+    end function dim2d_shrink
+
+    function dim2d_get_growth(ctx, out) bind(C, name="dim2d_get_growth") result(st)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: out(4)
+      integer(c_int) :: st
+    end function dim2d_get_growth
+
     function dim2d_adapt_domain(ctx, adapt_mask, distance, md_threshold, grow, out) &
         bind(C, name="dim2d_adapt_domain") result(st)
       import :: c_int, c_ptr, c_double, c_int32_t

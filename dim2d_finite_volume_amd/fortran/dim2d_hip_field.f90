@@ -162,10 +162,13 @@ contains
     steady = st /= 0
   end function check_steady_state
 
-  !> field_extended%adapt_domain: each open edge in adapt_mask (1 W, 2 E, 4 S,
-  !> 8 N) whose band of `distance` points is undermined moves out by `grow`
-  !> points (md_threshold < 0: imaginary sound speed only); the host grid,
-  !> maps and mirror follow, and the mirror is refreshed when the grid grew
+  !> field_extended%adapt_domain: each open edge in adapt_mask bits 4..7 (16 W,
+  !> 32 E, 64 S, 128 N) that has grown and whose decrease band holds no
+  !> undermined point goes back to its place at creation; then each open edge
+  !> in bits 0..3 (1 W, 2 E, 4 S, 8 N) whose band of `distance` points is
+  !> undermined moves out by `grow` points (md_threshold < 0: imaginary sound
+  !> speed only); the host grid, maps and mirror follow, and the mirror is
+  !> refreshed when the grid changed
   function adapt_domain(this, adapt_mask, distance, md_threshold, grow) result(added)
     class(hip_field), intent(inout) :: this
     integer, intent(in) :: adapt_mask, distance, grow
@@ -175,7 +178,7 @@ contains
     call dim2d_check(dim2d_adapt_domain(this%ctx, int(adapt_mask, c_int32_t), &
                      int(distance, c_int32_t), md_threshold, int(grow, c_int32_t), added), &
                      'dim2d_adapt_domain')
-    if (any(added > 0)) then
+    if (any(added /= 0)) then
       call dim2d_check(dim2d_get_params(this%ctx, q), 'dim2d_get_params')
       call set_grid(this, q)
       call this%sync_nodes()

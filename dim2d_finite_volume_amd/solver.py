@@ -201,19 +201,43 @@ class Field:
         capi.check(self._lib.dim2d_extend(self._ctx, w, e, s, n), "dim2d_extend")
         self._refresh_params()
 
+    def growth(self) -> np.ndarray:
+        """Columns/rows added per edge (W, E, S, N) since creation."""
+        out = (ctypes.c_int32 * 4)()
+        capi.check(self._lib.dim2d_get_growth(self._ctx, out), "dim2d_get_growth")
+        return np.array(out[:], dtype=np.int32)
+
+    def shrink(self, w: int = 0, e: int = 0, s: int = 0, n: int = 0) -> None:
+        """Remove columns/rows added earlier (at most growth() per edge)."""
+        capi.check(self._lib.dim2d_shrink(self._ctx, w, e, s, n), "dim2d_shrink")
+        self._refresh_params()
+
+    def openbc_undermined_decrease(self, distance: int = 5,
+                                   md_threshold: float | None = None) -> np.ndarray:
+        """Undermined points in each grown edge's decrease check band
+        (dim2d_openbc_undermined_decrease)."""
+        out = (ctypes.c_int32 * 4)()
+        thr = -1.0 if md_threshold is None else float(md_threshold)
+        capi.check(self._lib.dim2d_openbc_undermined_decrease(self._ctx, distance, thr, out),
+                   "dim2d_openbc_undermined_decrease")
+        return np.array(out[:], dtype=np.int32)
+
     def adapt_domain(self, edges: str = "WESN", distance: int = 5, grow: int = 32,
-                     md_threshold: float | None = None) -> np.ndarray:
+                     md_threshold: float | None = None, shrink: str = "") -> np.ndarray:
         """field_extended%adapt_domain (field_extended_class.f:427-452): each
-        listed edge whose band of `distance` points is undermined moves out by
-        `grow` points.  Returns the points added per edge (W, E, S, N)."""
+        edge listed in `shrink` that has grown and whose decrease band holds
+        no undermined point goes back to its place at creation; then each edge
+        listed in `edges` whose band of `distance` points is undermined moves
+        out by `grow` points.  Returns the points added (negative: removed)
+        per edge (W, E, S, N)."""
         mask = sum(self.EDGES[c] for c in set(edges.upper()))
+        mask += 16 * sum(self.EDGES[c] for c in set(shrink.upper()))
         out = (ctypes.c_int32 * 4)()
         thr = -1.0 if md_threshold is None else float(md_threshold)
         capi.check(self._lib.dim2d_adapt_domain(self._ctx, mask, distance, thr, grow, out),
                    "dim2d_adapt_domain")
         added = np.array(out[:], dtype=np.int32)
-        if added.any():
-            self._refresh_params()
+        self._refresh_params()
         return added
 
     def integrate(self, dt: float) -> None:

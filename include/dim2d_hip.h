@@ -286,12 +286,45 @@ int dim2d_set_far_field(dim2d_ctx* ctx, const double far_field[4]);
  * counterpart in the reference). */
 int dim2d_extend(dim2d_ctx* ctx, int32_t add_w, int32_t add_e, int32_t add_s, int32_t add_n);
 
-/* field_extended%adapt_domain (field_extended_class.f:427-452), called by the
- * host between steps: for each edge whose bit is set in adapt_mask (1 W, 2 E,
- * 4 S, 8 N: the reference's adapt_*_choice, parameters_input.f:796-817)
- * where dim2d_openbc_undermined(distance, md_threshold) finds points, the
- * edge moves out by `grow` points (dim2d_extend; grow must be 1 with a far
- * field).  out[0..3] = points added per edge. */
+/* Open edges only.  Columns/rows added per edge (W, E, S, N) by dim2d_extend
+ * since the context was created, less those removed by dim2d_shrink: the
+ * grid the context was created on plays the reference's interior domain, the
+ * added points its buffer layers. */
+int dim2d_get_growth(dim2d_ctx* ctx, int32_t out[4]);
+
+/* Open edges only (blocking).  Remove del_w/del_e columns and del_s/del_n
+ * rows added earlier (0 <= del_k <= dim2d_get_growth's out[k]): the grid is
+ * cropped, dx and dy kept, every remaining point keeps its state (the new
+ * ghost frame takes the state at its points, which open edges integrate as
+ * state).  Time, dt and the step count carry over; the handle stays valid.
+ * The reference removes whole buffer layers (bf_interface_dcr, the
+ * remove_inactivated_bf_layers of bf_interface_dcr_class.f:101-216). */
+int dim2d_shrink(dim2d_ctx* ctx, int32_t del_w, int32_t del_e, int32_t del_s, int32_t del_n);
+
+/* Open edges only (blocking).  The decrease check of the reference
+ * (check_if_bf_layer_remains, bf_decrease_module.f:90-600): out[0..3] =
+ * number of undermined points (criterion as in dim2d_openbc_undermined) in
+ * the check band of each edge that has grown since creation: `distance` rows
+ * (S, N) or columns (W, E) on each side of the edge's place at creation (the
+ * reference's search_dcr = obc_dct_distance), along the created grid's
+ * extent and distance - 1 points beyond it (the W/E bands one more point to
+ * the south, as get_check_line_param), clipped to the points whose 3x3
+ * neighbourhood lies in the grid; 0 for an edge that has not grown. */
+int dim2d_openbc_undermined_decrease(dim2d_ctx* ctx, int32_t distance, double md_threshold,
+                                     int32_t out[4]);
+
+/* field_extended%adapt_domain (field_extended_class.f:427-452,
+ * bf_interface_dcr_class.f:243-294), called by the host between steps.
+ * Bits 4..7 of adapt_mask (16 W, 32 E, 64 S, 128 N): the decrease first,
+ * every such edge that has grown and whose check band
+ * (dim2d_openbc_undermined_decrease) has no undermined point goes back to its
+ * place at creation (dim2d_shrink).  Bits 0..3 (1 W, 2 E, 4 S, 8 N: the
+ * reference's adapt_*_choice, parameters_input.f:796-817): then the increase,
+ * each such edge where dim2d_openbc_undermined(distance, md_threshold) finds
+ * points moves out by `grow` points (dim2d_extend; grow must be 1 with a far
+ * field, and with a far field an edge cropped in this call lets no edge grow
+ * before the next step).  out[0..3] = points added (negative: removed) per
+ * edge. */
 int dim2d_adapt_domain(dim2d_ctx* ctx, int32_t adapt_mask, int32_t distance, double md_threshold,
                        int32_t grow, int32_t out[4]);
 

@@ -258,6 +258,43 @@ def openbc_undermined(m: Model, nodes, dist: int, dx=1.0, dy=1.0, md_threshold=-
     return np.array(out[:], dtype=np.int32)
 
 
+def decrease_bands(nx: int, ny: int, growth, dist: int):
+    """The check band of each edge that has grown since creation (growth = the
+    W, E, S, N points added) in the decrease of the domain extension: the
+    reference's get_check_line_param (bf_decrease_module.f:252-330) around the
+    edge's place at creation (align_W/E/S/N, parameters_bf_layer.f:45-49),
+    the interior and buffer-layer parts (check_layer_interior :448-519,
+    check_layer_bf :522-600) taken together and clipped to the points whose
+    3x3 neighbourhood lies in the grid.  0-based inclusive (i0, i1, j0, j1)
+    per edge, or None."""
+    gw, ge, gs, gn = (int(g) for g in growth)
+    aw, ae, as_, an = 1 + gw, nx - 2 - ge, 1 + gs, ny - 2 - gn
+    d = int(dist)
+    raw = [(aw - d + 1, aw + d, as_ - d, an + d - 1),       # W
+           (ae - d, ae + d - 1, as_ - d, an + d - 1),       # E
+           (aw - d + 1, ae + d - 1, as_ - d + 1, as_ + d),  # S
+           (aw - d + 1, ae + d - 1, an - d, an + d - 1)]    # N
+    out = []
+    for k, (i0, i1, j0, j1) in enumerate(raw):
+        i0, i1, j0, j1 = max(i0, 1), min(i1, nx - 2), max(j0, 1), min(j1, ny - 2)
+        out.append((i0, i1, j0, j1) if int(growth[k]) > 0 and i0 <= i1 and j0 <= j1 else None)
+    return out
+
+
+def openbc_undermined_decrease(m: Model, nodes, growth, dist: int, dx=1.0, dy=1.0,
+                               md_threshold=-1.0):
+    """Undermined points (orc_openbc_undermined_at) in each decrease band."""
+    counts = np.zeros(4, dtype=np.int32)
+    for k, band in enumerate(decrease_bands(m.nx, m.ny, growth, dist)):
+        if band is None:
+            continue
+        i0, i1, j0, j1 = band
+        for j in range(j0, j1 + 1):
+            for i in range(i0, i1 + 1):
+                counts[k] += openbc_undermined_at(m, nodes, i + 1, j + 1, dx, dy, md_threshold)
+    return counts
+
+
 def extend(nodes, w: int, e: int, s: int, n: int):
     """The grid grown by w/e columns and s/n rows, every new point repeating
     the nearest old point (the domain extension's copy rule)."""

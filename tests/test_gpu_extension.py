@@ -366,3 +366,119 @@ def test_repeated_growth_matches_oracle(orc, edges):
         print("growth", g, "grid", got.shape, "rel err", err, flush=True)
         assert max(err) < 1e-11, (g, err)
     f.close()
+
+
+# ------------------------------------------- decrease: removing grown layers
+def test_shrink_crops_and_tracks_growth(orc):
+    """dim2d_extend counts the layers added per edge; dim2d_shrink removes
+    them again: the grid is the sub-rectangle, every point keeps its state,
+    time and dt carry over, and removing more than was added is an error."""
+    d = _lib()
+    p, nodes = _edge_bubble()
+    dt = orc.cfl_dt(_model(orc, p), nodes, *p.spacing(), 0.1)[0]
+    f = d.Field(p, nodes)
+    f.run(3, dt=dt)
+    f.extend(3, 7, 0, 5)
+    assert np.array_equal(f.growth(), [3, 7, 0, 5])
+    f.run(2, dt=dt)
+    before = f.download()
+    t0 = f.device_time()
+    with pytest.raises(d.capi.Dim2dError):
+        f.shrink(4, 0, 0, 0)
+    with pytest.raises(d.capi.Dim2dError):
+        f.shrink(0, 0, 1, 0)
+    with pytest.raises(d.capi.Dim2dError):
+        f.shrink(-1, 0, 0, 0)
+    f.shrink(2, 7, 0, 1)
+    assert np.array_equal(f.growth(), [1, 0, 0, 4])
+    assert (f.params.nx, f.params.ny) == (p.nx + 1, p.ny + 4)
+    assert np.array_equal(f.download(), before[:, 0:p.ny + 4, 2:2 + p.nx + 1])
+    x1, _ = f.params.coordinates()
+    x0, _ = p.coordinates()
+    assert np.allclose(x1[1:], x0, rtol=0, atol=1e-14)
+    assert np.array_equal(f.device_time(), t0)
+    f.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_shrunk_context_steps_like_a_fresh_one(orc, precision):
+    """After dim2d_shrink the context integrates exactly as a context created
+    on the cropped grid from the cropped state (fixed dt and device CFL), and
+    it may grow again (with a far field: after a step)."""
+    d = _lib()
+    from dim2d_finite_volume_amd import initial_conditions as ic
+    p, nodes = _edge_bubble(precision=precision)
+    dt = orc.cfl_dt(_model(orc, p), nodes, *p.spacing(), 0.1)[0]
+    f = d.Field(p, nodes)
+    f.set_far_field(ic.far_field(p, 0.95, (0.3, 0.0)))
+    f.run(2, dt=dt)
+    f.extend(1, 1, 0, 1)
+    f.run(2, dt=dt)
+    f.extend(1, 0, 1, 0)
+    f.run(2, dt=dt)
+    f.shrink(2, 0, 1, 0)
+    g = d.Field(f.params, f.download())
+    g.set_far_field(ic.far_field(p, 0.95, (0.3, 0.0)))
+    f.run(5, dt=dt)
+    g.run(5, dt=dt)
+    assert np.array_equal(f.download(), g.download())
+    f.run(4, cfl=0.1)
+    g.run(4, cfl=0.1)
+    assert np.array_equal(f.download(), g.download())
+    f.extend(0, 1, 0, 0)
+    assert np.array_equal(f.growth(), [0, 2, 0, 1])
+    f.close()
+    g.close()
+
+
+def test_openbc_undermined_decrease_matches_oracle(orc):
+    """The decrease bands' undermined counts (dim2d_openbc_undermined_decrease)
+    against the oracle's (the reference's check_line_neighbors over
+    get_check_line_param's bands, bf_decrease_module.f) on a bubble state
+    grown at three edges, for three distances and thresholds."""
+    d = _lib()
+    p, nodes = _edge_bubble(center=(0.2, -0.2), radius=0.1)
+    f = d.Field(p, nodes)
+    f.run(2, dt=1e-6)
+    f.extend(4, 6, 0, 3)
+    grown = f.download()
+    m = orc.make_model(f.params.nx, f.params.ny, re=p.re, we=p.we, pr=p.pr,
+                       viscous_r=p.viscous_r, cv_r=p.cv_r, bc=OPEN)
+    dx, dy = p.spacing()
+    for dist in (2, 5, 12):
+        for thr in (-1.0, 0.0, 0.3):
+            got = f.openbc_undermined_decrease(dist, None if thr < 0 else thr)
+            want = orc.openbc_undermined_decrease(m, grown, (4, 6, 0, 3), dist, dx, dy, thr)
+            print("decrease", dist, thr, got, want)
+            assert np.array_equal(got, want), (dist, thr, got, want)
+    assert f.openbc_undermined_decrease(5, 0.3)[2] == 0  # S has not grown
+    f.close()
+
+
+def test_adapt_domain_shrinks_clean_edges_back(orc):
+    """adapt_domain with the decrease bits: an edge grown earlier goes back to
+    its place at creation once its decrease band holds no undermined point,
+    and stays while it does; an edge that has not grown is never cut, and
+    growth in the same call uses the cropped grid."""
+    d = _lib()
+    p, nodes = _edge_bubble(center=(0.0, 0.0), radius=0.1, flow=(0.0, 0.0))
+    f = d.Field(p, nodes)
+    f.run(2, dt=1e-6)
+    f.extend(0, 8, 0, 0)
+    state = f.download()
+    # a two-phase point (density 1 at rest, T = 0.95) planted in the east
+    # decrease band, three points west of the created grid's east ghost column
+    # (p.nx - 2), keeps the east layer
+    planted = state.copy()
+    planted[:, p.ny // 2, p.nx - 5] = [1.0, 0.0, 0.0, 8.0 * p.cv_r * 0.95 / 3.0 - 3.0]
+    f.upload(planted)
+    assert f.openbc_undermined_decrease(5, 0.3)[1] > 0
+    out = f.adapt_domain("", distance=5, md_threshold=0.3, shrink="WESN")
+    assert np.array_equal(out, [0, 0, 0, 0]) and np.array_equal(f.growth(), [0, 8, 0, 0])
+    f.upload(state)
+    assert f.openbc_undermined_decrease(5, 0.3)[1] == 0
+    out = f.adapt_domain("", distance=5, md_threshold=0.3, shrink="WESN")
+    assert np.array_equal(out, [0, -8, 0, 0]) and np.array_equal(f.growth(), [0, 0, 0, 0])
+    assert (f.params.nx, f.params.ny) == (p.nx, p.ny)
+    assert np.array_equal(f.download(), state[:, :, :p.nx])
+    f.close()

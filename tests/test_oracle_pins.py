@@ -527,6 +527,74 @@ def test_openbc_undermined_band_counts(orc):
         assert np.array_equal(orc.openbc_undermined(m, nodes, dist), want), dist
 
 
+def test_decrease_bands_against_the_reference_check_lines():
+    """The decrease bands against the reference's check-line parameters worked
+    out by hand (get_check_line_param, bf_decrease_module.f:252-330) for a
+    20x16 interior grown by W 3, E 0, S 2, N 4 points, search distance 5: the
+    interior part and the buffer-layer part of each layer's check line,
+    converted to 0-based indices of the grown grid (x + 3 - 1, y + 2 - 1) and
+    joined, the ends clipped to the points whose 3x3 neighbourhood is in the
+    grid.  Edge E has not grown: no band."""
+    from oracle import oracle as orc_mod
+    nx0, ny0 = 24, 20                 # created grid, ghosts included
+    g = (3, 0, 2, 4)
+    nx, ny = nx0 + g[0] + g[1], ny0 + g[2] + g[3]
+    d = 5
+    align_w, align_e, align_s, align_n = 2, nx0 - 1, 2, ny0 - 1   # 1-based, created grid
+
+    def cur(i, j):  # 1-based created-grid coordinates -> 0-based grown grid
+        return i + g[0] - 1, j + g[2] - 1
+
+    # W layer: x from align_W - d + 1 (layer) to align_W + d (interior),
+    # y from align_S - d to align_N + d - 1
+    w_i0, w_j0 = cur(align_w - d + 1, align_s - d)
+    w_i1, w_j1 = cur(align_w + d, align_n + d - 1)
+    # S layer: x from align_W - d + 1 to align_E + d - 1, y from align_S - d + 1 to align_S + d
+    s_i0, s_j0 = cur(align_w - d + 1, align_s - d + 1)
+    s_i1, s_j1 = cur(align_e + d - 1, align_s + d)
+    # N layer: y from align_N - d to align_N + d - 1
+    n_i0, n_j0 = cur(align_w - d + 1, align_n - d)
+    n_i1, n_j1 = cur(align_e + d - 1, align_n + d - 1)
+
+    def clip(i0, i1, j0, j1):
+        return (max(i0, 1), min(i1, nx - 2), max(j0, 1), min(j1, ny - 2))
+
+    bands = orc_mod.decrease_bands(nx, ny, g, d)
+    assert bands[0] == clip(w_i0, w_i1, w_j0, w_j1) == (1, 9, 1, 24)
+    assert bands[1] is None
+    assert bands[2] == clip(s_i0, s_i1, s_j0, s_j1) == (1, 25, 1, 8)
+    assert bands[3] == clip(n_i0, n_i1, n_j0, n_j1) == (1, 25, 15, 24)
+
+
+def test_openbc_undermined_decrease_counts(orc):
+    """Counts of the decrease bands on a uniform liquid with planted two-phase
+    points, against membership by hand, for several growths and distances."""
+    n = 24
+    m = orc.make_model(n + 4, n + 4, bc=(orc.BC_OPEN,) * 4)
+    liquid = _rest_state(m.cv_r, 1.9, 0.95, 0.1, 0.0)
+    bad = _rest_state(m.cv_r, 1.0, 0.9)
+    nodes = np.ascontiguousarray(np.broadcast_to(liquid[:, None, None], (4, n + 4, n + 4)))
+    planted = [(1, 1), (3, 13), (8, 20), (14, 26), (20, 4), (26, 12), (12, 12), (2, 25)]
+    for i, j in planted:
+        nodes[:, j, i] = bad
+    for growth in ((3, 0, 2, 4), (0, 5, 0, 1), (6, 6, 6, 6)):
+        gw, ge, gs, gn = growth
+        aw, ae, as_, an = 1 + gw, n + 2 - ge, 1 + gs, n + 2 - gn
+        for dist in (1, 3, 5):
+            got = orc.openbc_undermined_decrease(m, nodes, growth, dist)
+            want = np.zeros(4, dtype=np.int32)
+            for i, j in planted:
+                if not (1 <= i <= n + 2 and 1 <= j <= n + 2):
+                    continue
+                ys = as_ - dist <= j <= an + dist - 1
+                xs = aw - dist + 1 <= i <= ae + dist - 1
+                want += [gw > 0 and aw - dist + 1 <= i <= aw + dist and ys,
+                         ge > 0 and ae - dist <= i <= ae + dist - 1 and ys,
+                         gs > 0 and xs and as_ - dist + 1 <= j <= as_ + dist,
+                         gn > 0 and xs and an - dist <= j <= an + dist - 1]
+            assert np.array_equal(got, want), (growth, dist, got, want)
+
+
 def test_extend_copy_rule():
     """New points repeat the nearest old point; old points keep their values."""
     from oracle import oracle as orc_mod
