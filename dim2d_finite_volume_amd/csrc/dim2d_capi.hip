@@ -1099,7 +1099,7 @@ int dim2d_adapt_domain(dim2d_ctx* c, int32_t adapt_mask, int32_t distance, doubl
   if (st) return st;
   int32_t add[4];
   for (int k = 0; k < 4; ++k) add[k] = (adapt_mask >> k) & 1 && cnt[k] > 0 ? grow : 0;
-  if (c->ff_set && !c->f0_valid) {
+  if (c->ff_set && del[0] + del[1] + del[2] + del[3] > 0) {
     // a crop in this call: the extrapolation's state at t - dt is gone, the
     // increase waits for the next call
     for (int k = 0; k < 4; ++k) add[k] = 0;
