@@ -1195,8 +1195,12 @@ __global__ void wall_batch_kernel(const DevParams* P, int nsim, int mixed_in, co
 //  * S/N walls: the wall root of each ghost cell (wall_ghost_values, as
 //    wall_kernel) from s1 of the first and far interior rows, in the tile that
 //    holds the wall: the consumer before its first row at the south wall, the
-//    producer after its last row at the north wall.
-// Walls on W/E edges and open edges take the separate stage kernels.
+//    producer after its last row at the north wall;
+//  * open edges (the 2-cell frame is state): the stage-1 frame kernel's
+//    output, read from HBM (pair_gather<MIXED, true>).
+// Walls on W/E edges take the separate stage kernels.  The same pair runs the
+// low-storage RK3's stages 1 and 2 (LS: q1 rides in the ring's u rows) and,
+// persistent over a work list, a batch's (stage_pair_batch_kernel).
 // Every output cell sees the same operands in the same order as with the
 // separate kernels and the wall kernel between them.
 constexpr int kPairOut = kPairOutCols;  // output columns per strip (lanes 4..59)
@@ -1524,7 +1528,8 @@ __device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a
 // 3 in mixed precision (168 VGPRs; its LDS ring fits 6 workgroups per CU).  OPEN:
 // all four edges open (hedstrom_xy); the frame cells of s1 come from the
 // stage-1 frame kernel, launched before this kernel, and the stage-2 frame
-// kernel, launched after it, reads the band of s1 this kernel stores.
+// kernel, launched after it, reads the band of s1 this kernel stores.  LS:
+// the low-storage RK3 (FP64, no open edges): u0 (a.U) -> u2 (a.O64), q2 (a.Q).
 template <int MIXED, bool OPEN, bool LS>
 __global__ void __launch_bounds__(2 * kLanes)
 __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2, 8)))
