@@ -145,6 +145,12 @@ DevParams make_dev_params(const dim2d_params& q) {
   p.wall_cos_angle = cos(q.wall_contact_angle * acos(-1.0) / 180.0);
   p.wall_q = q.wall_heat_flux;
   p.sigma_pref = 2.0 * sqrt(2.0) / (3.0 * sqrt(q.we));
+  p.f = DevParams::F32{(float)p.idx,        (float)p.idy,        (float)p.half_idx,
+                       (float)p.half_idy,   (float)p.q_idx,      (float)p.q_idy,
+                       (float)p.vr,         (float)p.two_vr,     (float)p.inv_re,
+                       (float)p.inv_we,     (float)p.two_inv_re, (float)p.two_inv_we,
+                       (float)p.c15cv,      (float)p.k_heat_x,   (float)p.k_heat_y,
+                       (float)p.k_cap_x,    (float)p.k_cap_y};
   return p;
 }
 

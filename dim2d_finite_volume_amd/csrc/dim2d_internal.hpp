@@ -50,6 +50,14 @@ struct DevParams {
   double wall_cos_angle;   // cos(contact angle)
   double wall_q;           // constant wall heat flux
   double sigma_pref;       // 2 sqrt(2)/(3 sqrt(We)) (surface tension prefactor)
+
+  // the constants FP32 fluxes take, rounded once (mixed precision): read as
+  // scalar operands instead of being converted in every row
+  struct F32 {
+    float idx, idy, half_idx, half_idy, q_idx, q_idy;
+    float vr, two_vr, inv_re, inv_we, two_inv_re, two_inv_we, c15cv;
+    float k_heat_x, k_heat_y, k_cap_x, k_cap_y;
+  } f;
 };
 
 // Step control block in device memory: dt and time of the running step and

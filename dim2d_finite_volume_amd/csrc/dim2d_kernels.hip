@@ -132,6 +132,10 @@ __device__ __forceinline__ double fast_rcp(double x) {
   return fma(r, e, r);
 }
 
+// constant p.<name> in the flux precision R: the double itself, or its FP32
+// copy in p.f (the same value as (float)p.<name>)
+#define PC(R, name) (std::is_same<R, float>::value ? (R)p.f.name : (R)p.name)
+
 // Own-column values a lane keeps in registers (what the y-faces need)
 template <typename RD, typename R>
 struct Cell {
@@ -251,10 +255,10 @@ __device__ __forceinline__ void x_face(const DevParams& p, const Nb<RD, R>& L, c
   const R ryy = (R)(((Ldn.rho + Rdn.rho) + (Lup.rho + Rup.rho) - (RD)2 * srho) * (RD)p.h2y);
   const R su = L.u + Rc.u;
   const R sv = L.v + Rc.v;
-  const R dudx = (Rc.u - L.u) * (R)p.idx;
-  const R dvdx = (Rc.v - L.v) * (R)p.idx;
-  const R dudy = ((Lup.u + Rup.u) - (Ldn.u + Rdn.u)) * (R)p.q_idy;
-  const R dvdy = ((Lup.v + Rup.v) - (Ldn.v + Rdn.v)) * (R)p.q_idy;
+  const R dudx = (Rc.u - L.u) * PC(R, idx);
+  const R dvdx = (Rc.v - L.v) * PC(R, idx);
+  const R dudy = ((Lup.u + Rup.u) - (Ldn.u + Rdn.u)) * PC(R, q_idy);
+  const R dvdy = ((Lup.v + Rup.v) - (Ldn.v + Rdn.v)) * PC(R, q_idy);
   const R rxy = rx * ry;
   const R grad2 = rx * rx + ry * ry;
   const R lap = rxx + ryy;
@@ -274,16 +278,16 @@ __device__ __forceinline__ void x_face(const DevParams& p, const Nb<RD, R>& L, c
     F2[3] = (R)(2.0 * (wall_w ? p.wall_q : -p.wall_q));
     return;
   }
-  const R visc = (R)p.two_vr * dudx + (R)p.vr * dvdy;
+  const R visc = PC(R, two_vr) * dudx + PC(R, vr) * dvdy;
   const R shear = dudy + dvdx;
   F2[0] = Lx.qx + Rx.qx;
-  F2[1] = (Lx.Pxx + Rx.Pxx) - (R)p.two_inv_re * visc -
-          (R)p.inv_we * ((R)p.c15cv * (Lc.w + Rc.w) * grad2 + srhoR * lap + rx2d);
-  F2[2] = (Lc.qxv + Rc.qxv) - (R)p.two_inv_re * shear + (R)p.two_inv_we * rxy;
-  const R visc_e2 = su * visc + sv * shear + (R)p.k_heat_x * (Rc.Tcl - Lc.Tcl);
-  const R cap_e2 = (R)p.c15cv * (Lx.uw + Rx.uw) * grad2 + F2[0] * lap + (R)0.5 * su * rx2d -
-                   rxy * sv - srhoR * rx * (dudx + dvdy) - (R)p.k_cap_x * (Rc.Tcap - Lc.Tcap);
-  F2[3] = (Lx.Hu + Rx.Hu) - (R)p.inv_re * visc_e2 - (R)p.inv_we * cap_e2;
+  F2[1] = (Lx.Pxx + Rx.Pxx) - PC(R, two_inv_re) * visc -
+          PC(R, inv_we) * (PC(R, c15cv) * (Lc.w + Rc.w) * grad2 + srhoR * lap + rx2d);
+  F2[2] = (Lc.qxv + Rc.qxv) - PC(R, two_inv_re) * shear + PC(R, two_inv_we) * rxy;
+  const R visc_e2 = su * visc + sv * shear + PC(R, k_heat_x) * (Rc.Tcl - Lc.Tcl);
+  const R cap_e2 = PC(R, c15cv) * (Lx.uw + Rx.uw) * grad2 + F2[0] * lap + (R)0.5 * su * rx2d -
+                   rxy * sv - srhoR * rx * (dudx + dvdy) - PC(R, k_cap_x) * (Rc.Tcap - Lc.Tcap);
+  F2[3] = (Lx.Hu + Rx.Hu) - PC(R, inv_re) * visc_e2 - PC(R, inv_we) * cap_e2;
 }
 
 // y-face flux at (i, j-1/2): D = (i, j-1), U = (i, j)  (g/dgdx/dgdy/d2gdx2/d2gdy2/dgdy_nl),
@@ -301,10 +305,10 @@ __device__ __forceinline__ void y_face(const DevParams& p, const Cell<RD, R>& D,
   const R rxx = (R)(((WD.rho + WU.rho) + (ED.rho + EU.rho) - (RD)2 * srho) * (RD)p.h2x);
   const R su = D.u + Uc.u;
   const R sv = D.v + Uc.v;
-  const R dudy = (Uc.u - D.u) * (R)p.idy;
-  const R dvdy = (Uc.v - D.v) * (R)p.idy;
-  const R dudx = ((ED.u + EU.u) - (WD.u + WU.u)) * (R)p.q_idx;
-  const R dvdx = ((ED.v + EU.v) - (WD.v + WU.v)) * (R)p.q_idx;
+  const R dudy = (Uc.u - D.u) * PC(R, idy);
+  const R dvdy = (Uc.v - D.v) * PC(R, idy);
+  const R dudx = ((ED.u + EU.u) - (WD.u + WU.u)) * PC(R, q_idx);
+  const R dvdx = ((ED.v + EU.v) - (WD.v + WU.v)) * PC(R, q_idx);
   const R rxy = rx * ry;
   const R grad2 = rx * rx + ry * ry;
   const R lap = rxx + ryy;
@@ -325,15 +329,15 @@ __device__ __forceinline__ void y_face(const DevParams& p, const Cell<RD, R>& D,
     return;
   }
   const R shear = dudy + dvdx;
-  const R visc = (R)p.vr * dudx + (R)p.two_vr * dvdy;
+  const R visc = PC(R, vr) * dudx + PC(R, two_vr) * dvdy;
   G2[0] = D.qy + Uc.qy;
-  G2[1] = (D.qxv + Uc.qxv) - (R)p.two_inv_re * shear + (R)p.two_inv_we * rxy;
-  G2[2] = (D.Pyy + Uc.Pyy) - (R)p.two_inv_re * visc -
-          (R)p.inv_we * ((R)p.c15cv * (D.w + Uc.w) * grad2 + srhoR * lap + ry2d);
-  const R visc_e2 = su * shear + sv * visc + (R)p.k_heat_y * (Uc.Tcl - D.Tcl);
-  const R cap_e2 = (R)p.c15cv * (D.vw + Uc.vw) * grad2 + G2[0] * lap + (R)0.5 * sv * ry2d -
-                   rxy * su - srhoR * ry * (dudx + dvdy) - (R)p.k_cap_y * (Uc.Tcap - D.Tcap);
-  G2[3] = (D.Hv + Uc.Hv) - (R)p.inv_re * visc_e2 - (R)p.inv_we * cap_e2;
+  G2[1] = (D.qxv + Uc.qxv) - PC(R, two_inv_re) * shear + PC(R, two_inv_we) * rxy;
+  G2[2] = (D.Pyy + Uc.Pyy) - PC(R, two_inv_re) * visc -
+          PC(R, inv_we) * (PC(R, c15cv) * (D.w + Uc.w) * grad2 + srhoR * lap + ry2d);
+  const R visc_e2 = su * shear + sv * visc + PC(R, k_heat_y) * (Uc.Tcl - D.Tcl);
+  const R cap_e2 = PC(R, c15cv) * (D.vw + Uc.vw) * grad2 + G2[0] * lap + (R)0.5 * sv * ry2d -
+                   rxy * su - srhoR * ry * (dudx + dvdy) - PC(R, k_cap_y) * (Uc.Tcap - D.Tcap);
+  G2[3] = (D.Hv + Uc.Hv) - PC(R, inv_re) * visc_e2 - PC(R, inv_we) * cap_e2;
 }
 
 // speed_of_sound (dim2d_prim_module.f:1179-1215), algebraically reduced to
@@ -655,7 +659,7 @@ __device__ __forceinline__ void strip_time_dev(const DevParams& p, StripRings<RD
   constexpr int K1 = (K + 1) % kRing;          // row r+1
   constexpr int K2 = (K + 2) % kRing;          // row r+2
   // fluxes are doubled (face sums), hence the 1/2
-  const R hidx = (R)p.half_idx, hidy = (R)p.half_idy;
+  const R hidx = PC(R, half_idx), hidy = PC(R, half_idy);
   Cell<RD, R>& cm1 = s.cell[KM];
   Cell<RD, R>& c0 = s.cell[K];
   Cell<RD, R>& c1 = s.cell[K1];
