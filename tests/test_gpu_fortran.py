@@ -41,17 +41,19 @@ def test_fortran_driver_matches_oracle(orc, tmp_path, bc, nsync):
     assert max(err) < 1e-10, err
 
 
-def test_fortran_adapt_domain_matches_python(tmp_path):
+@pytest.mark.parametrize("mask,shrink", [(2, ""), (2 + 16 * 15, "WESN")])
+def test_fortran_adapt_domain_matches_python(tmp_path, mask, shrink):
     """The Fortran field's adapt_domain loop (chunks of steps, then
     field_extended%adapt_domain) gives the same grid and the same bits as the
-    same calls made from Python."""
+    same calls made from Python, with the growth of the east edge alone and
+    with the decrease checks of every edge as well."""
     if not os.path.exists(DRIVER):
         pytest.fail("Fortran driver not built (make -C dim2d_finite_volume_amd/fortran)")
     import dim2d_finite_volume_amd as d
     from dim2d_finite_volume_amd import initial_conditions as ic
     p, nodes = ic.open_bubble(n=60, precision=0, center=(0.25, 0.0), radius=0.1, flow=(0.4, 0.0))
     dt = ic.cfl_dt(p, nodes, 0.1)
-    nt, every, mask, grow, thr = 600, 50, 2, 16, 0.3
+    nt, every, grow, thr = 600, 50, 16, 0.3
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     nodes.tofile(fin)
     r = subprocess.run([DRIVER, str(fin), str(fout), str(p.nx), str(p.ny), "3", "3", "3", "3",
@@ -64,7 +66,7 @@ def test_fortran_adapt_domain_matches_python(tmp_path):
     f = d.Field(p, nodes)
     for _ in range(nt // every):
         f.run(every, dt=dt)
-        f.adapt_domain("E", distance=5, grow=grow, md_threshold=thr)
+        f.adapt_domain("E", distance=5, grow=grow, md_threshold=thr, shrink=shrink)
     assert (nx, ny) == (f.params.nx, f.params.ny) and nx > p.nx
     got = np.fromfile(fout, dtype=np.float64).reshape(4, ny, nx)
     assert np.array_equal(got, f.download())
