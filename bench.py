@@ -131,11 +131,13 @@ def cpu_baseline(seconds_hint=15.0):
 
 
 def batch_bench(steps, nsims=256, seed=4):
-    """256 independent drop/bubble simulations, n^2 with n
+    """The batch workload: 256 independent drop/bubble simulations, n^2 with n
     log-uniform in [64, 512], reflection edges, each with its own fixed dt
     (CFL 0.1 of its initial state); persistent kernels under a HIP graph, RK
     stages 1 and 2 fused in one (the stage pair) and stage 3 in another, with
-    tiles chosen by dim2d_batch_autotune; FP64 and mixed."""
+    tiles chosen by dim2d_batch_autotune; FP64 and mixed, each in its default
+    mode (FP64 pipelined: two groups half a step apart, each launch running
+    one group's stage pairs beside the other's stage 3)."""
     import dataclasses
 
     import numpy as np
@@ -177,7 +179,7 @@ def batch_bench(steps, nsims=256, seed=4):
 
 
 def open_boundary_bench(n, steps):
-    """configs[4] on a fixed domain: a vapour bubble (r = 0.1) in a mean flow
+    """The 8192^2 open-edge workload on a fixed domain: a vapour bubble (r = 0.1) in a mean flow
     (0.2, 0.05) with Hedstrom open edges, mixed precision against FP64 with a
     common fixed dt (CFL 0.1 of the initial state)."""
     import dataclasses
@@ -210,7 +212,7 @@ def open_boundary_bench(n, steps):
 
 
 def open_extension_bench(n, steps, every, gap, band):
-    """configs[4] with the domain extension: the bubble's interface starts
+    """The 8192^2 open-edge workload with the domain extension: the bubble's interface starts
     `gap` points from the east edge; every `every` steps the FP64 run calls
     adapt_domain (density criterion eps = 0.3 over a band of `band` points,
     one layer per edge and call, new points by the characteristic
@@ -444,7 +446,7 @@ def main():
         vm = cells * args.steps / elm
         fm.close()
         del fm
-        # error against FP64 with a common fixed dt (CFL 0.1 of the
+        # the mixed-precision criterion: error against FP64 with a common fixed dt (CFL 0.1 of the
         # initial state) at steps 100 and 1000, whatever --steps is
         dt0 = ic.cfl_dt(p, nodes, 0.1)
         marks = tuple(int(x) for x in args.error_steps.split(",") if x)

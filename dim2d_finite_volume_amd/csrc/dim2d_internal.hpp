@@ -76,9 +76,14 @@ struct StepCtl {
 };
 
 // one tile of the batch work list: strip bx, rows 2 + by*rows .. of
-// simulation sim (rows: that simulation's tile height)
+// simulation sim (rows: that simulation's tile height); kind: what the tile
+// runs in the mixed launch of a pipelined batch (kItemPair: the stage pair on
+// a 56-column strip; kItemStage3: stage 3 on 60-column strips 2 bx and
+// 2 bx + 1, one per wave)
+constexpr int kItemPair = 0, kItemStage3 = 1;
 struct WorkItem {
   int sim, bx, by, rows;
+  int kind, pad0, pad1, pad2;
 };
 
 // kernel launch helpers (dim2d_kernels.hip)
@@ -118,6 +123,24 @@ hipError_t launch_stage_pair_batch(const DevParams* d_params, const WorkItem* d_
                                    int nitems, int mixed, const StageArgs& a, int workgroups,
                                    hipStream_t s);
 constexpr int kPairOutCols = 56;  // output columns per stage-pair strip
+// a pipelined batch's mixed launch: the items of kind kItemPair run the stage
+// pair (u^n in pU -> the stage-2 state in pO64, or its increment in pO32),
+// those of kind kItemStage3 run stage 3 (u^n in sU and the stage-2 state in
+// sS64, or its increment in sS32 -> u^n+1 in sO64), in one persistent grid of
+// two-wave workgroups; batch base pointers, per-simulation dt from dts
+struct MixArgs {
+  const double* pU;
+  double* pO64;
+  float* pO32;
+  const double* sU;
+  const double* sS64;
+  const float* sS32;
+  double* sO64;
+};
+hipError_t launch_stage_mix_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
+                                  int mixed, const MixArgs& m, const double* dts, int workgroups,
+                                  hipStream_t s);
+int max_resident_mix_workgroups(int mixed);
 // one RK stage over a batch work list of 60-column strips, persistent grid
 // of `workgroups` one-wave workgroups
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,

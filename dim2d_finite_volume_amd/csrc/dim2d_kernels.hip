@@ -1581,6 +1581,50 @@ stage_pair_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restr
   }
 }
 
+// The mixed launch of a pipelined batch: the simulations form two groups half
+// a step apart, so one launch runs the stage pairs of one group (bound by
+// instruction issue) beside the stage 3 of the other (bound by HBM), and the
+// two kinds of work share the CUs.  Both waves of a workgroup pass the same
+// barriers per item (stage-3 items have none), so a wave that finishes a
+// stage-3 strip early simply waits at the next pair item's first barrier.
+#ifndef DIM2D_MIX_WAVES_MIXED
+#define DIM2D_MIX_WAVES_MIXED 2
+#endif
+template <int MIXED>
+__global__ void __launch_bounds__(2 * kLanes)
+__attribute__((amdgpu_waves_per_eu(MIXED ? DIM2D_MIX_WAVES_MIXED : 2, 8)))
+stage_mix_batch_kernel(const DevParams* __restrict__ P, const WorkItem* __restrict__ items,
+                       int nitems, const MixArgs m, const double* __restrict__ dts) {
+  __shared__ PairRing<MIXED> ring;
+  using RD = double;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const bool swap = !MIXED && (blockIdx.x & 1) != 0;
+  RedAcc<false> red;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const DevParams p = P[w.sim];
+    const double dt = dts[w.sim];
+    StageArgs b;
+    memset(&b, 0, sizeof b);
+    if (w.kind == kItemPair) {
+      b.U = m.pU + p.offset;
+      if (MIXED) b.O32 = m.pO32 + p.offset;
+      else b.O64 = m.pO64 + p.offset;
+      const int r0 = kBcSize + w.by * w.rows;
+      const int r1 = min(r0 + w.rows, p.ny - kBcSize);
+      pair_tile<MIXED>(p, b, dt, w.bx, r0, r1, swap, ring);
+    } else {
+      const int strip = 2 * w.bx + (int)(threadIdx.x / kLanes);
+      if (strip * kOutCols >= p.nx - 2 * kBcSize) continue;
+      b.U = m.sU + p.offset;
+      if (MIXED) b.S32 = m.sS32 + p.offset;
+      else b.S64 = m.sS64 + p.offset;
+      b.O64 = m.sO64 + p.offset;
+      stage_tile<RD, R, MIXED, 3, false>(p, b, dt, strip, w.by, w.rows, red);
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // open boundaries (hedstrom_xy, bc_operators_hedstrom_xy_class.f:182-496)
 // ------------------------------------------------------------------------
@@ -2647,6 +2691,18 @@ hipError_t launch_stage_pair_batch(const DevParams* d_params, const WorkItem* d_
   return hipGetLastError();
 }
 
+hipError_t launch_stage_mix_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
+                                  int mixed, const MixArgs& m, const double* dts, int workgroups,
+                                  hipStream_t s) {
+  if (mixed)
+    hipLaunchKernelGGL(stage_mix_batch_kernel<1>, dim3(workgroups), dim3(2 * kLanes), 0, s,
+                       d_params, d_items, nitems, m, dts);
+  else
+    hipLaunchKernelGGL(stage_mix_batch_kernel<0>, dim3(workgroups), dim3(2 * kLanes), 0, s,
+                       d_params, d_items, nitems, m, dts);
+  return hipGetLastError();
+}
+
 hipError_t launch_stage_batch(const DevParams* d_params, const WorkItem* d_items, int nitems,
                               int mixed, int stage, const StageArgs& a, int workgroups,
                               hipStream_t s) {
@@ -2807,6 +2863,21 @@ int max_resident_pair_workgroups(int mixed) {
                              &per, stage_pair_batch_kernel<1>, 2 * kLanes, 0)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
                              &per, stage_pair_batch_kernel<0>, 2 * kLanes, 0);
+  if (e != hipSuccess || per <= 0) per = 4;
+  return ncu * per;
+}
+
+int max_resident_mix_workgroups(int mixed) {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  int per = 0;
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, stage_mix_batch_kernel<1>, 2 * kLanes, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, stage_mix_batch_kernel<0>, 2 * kLanes, 0);
   if (e != hipSuccess || per <= 0) per = 4;
   return ncu * per;
 }

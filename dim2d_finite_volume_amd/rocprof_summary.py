@@ -37,11 +37,15 @@ STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
 PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)(?:,\s*(true|false))?>")
 BSTAGE_RE = re.compile(r"dim2d::stage_batch_kernel<(\d+),\s*(\d+)>")
 BPAIR_RE = re.compile(r"dim2d::stage_pair_batch_kernel<(\d+)>")
+BMIX_RE = re.compile(r"dim2d::stage_mix_batch_kernel<(\d+)>")
 
 
 def kernel_key(name):
     """file-name key of a stage kernel: [batch_]stage<s>_<fp64|mixed> or
-    [batch_]pair_<fp64|mixed>[_open]"""
+    [batch_]pair_<fp64|mixed>[_open] or batch_mix_<fp64|mixed>"""
+    m = BMIX_RE.search(name)
+    if m:
+        return f"batch_mix_{'mixed' if m.group(1) == '1' else 'fp64'}"
     m = BSTAGE_RE.search(name)
     if m:
         return f"batch_stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
@@ -58,6 +62,11 @@ def kernel_key(name):
 
 
 def short_name(name):
+    m = BMIX_RE.search(name)
+    if m:
+        prec = "mixed" if m.group(1) == "1" else "FP64"
+        return (f"stage_mix_batch_kernel<{m.group(1)}> (pipelined batch, {prec} stages 1+2 "
+                f"of one group beside stage 3 of the other)")
     m = BPAIR_RE.search(name)
     if m:
         prec = "mixed" if m.group(1) == "1" else "FP64"

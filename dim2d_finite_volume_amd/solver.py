@@ -346,18 +346,23 @@ class Batch:
         capi.check(self._lib.dim2d_batch_set_tiles(self._b, block_x, rows, workgroups),
                    "dim2d_batch_set_tiles")
 
-    def set_stage_fusion(self, fuse: bool, rows: int = 0, workgroups: int = 0) -> None:
+    def set_stage_fusion(self, fuse: bool, rows: int = 0, workgroups: int = 0,
+                         pipeline: bool = False) -> None:
         """RK3-TVD stages 1 and 2 in one persistent launch (on by default when
-        every simulation's edges allow it); rows and workgroups of its tiles."""
-        capi.check(self._lib.dim2d_batch_set_stage_fusion(self._b, int(bool(fuse)), rows,
-                                                          workgroups),
+        every simulation's edges allow it); rows and workgroups of its tiles.
+        pipeline (with fuse): two groups of simulations half a step apart, one
+        group's stage pair running beside the other's stage 3 in each launch
+        (workgroups then sizes those launches); the same results bit for bit."""
+        mode = 2 if (fuse and pipeline) else int(bool(fuse))
+        capi.check(self._lib.dim2d_batch_set_stage_fusion(self._b, mode, rows, workgroups),
                    "dim2d_batch_set_stage_fusion")
 
     def tiles(self) -> dict:
         """The batch's launch shapes (dim2d_batch_get_tiles)."""
         out = (ctypes.c_int32 * 6)()
         capi.check(self._lib.dim2d_batch_get_tiles(self._b, out), "dim2d_batch_get_tiles")
-        return {"stage_fusion": bool(out[0]), "pair_rows": out[1], "pair_workgroups": out[2],
+        return {"stage_fusion": bool(out[0]), "pipelined": out[0] == 2,
+                "pair_rows": out[1], "pair_workgroups": out[2],
                 "block_x": out[3], "rows": out[4], "workgroups": out[5]}
 
     def autotune(self, dts, nsteps: int = 10):

@@ -361,12 +361,21 @@ int dim2d_batch_synchronize(dim2d_batch* b);
  * axis; fuse = 1 on any other batch returns DIM2D_EINVAL.  rows_per_block
  * (at most, balanced per simulation as in dim2d_batch_set_tiles) and
  * workgroups shape the pair's tiles (56-column strips) and persistent grid
- * (0: 32 rows; one workgroup per tile, capped by residency).  No counterpart
- * in the reference. */
+ * (0: 32 rows; one workgroup per tile, capped by residency).
+ * fuse = 2 pipelines the batch: the simulations form two groups of about
+ * equal cell counts, the second half a step behind the first, and each step
+ * is two persistent launches that run one group's stage pair beside the
+ * other group's stage 3 (instruction-bound and HBM-bound work side by side);
+ * workgroups then sizes those launches.  Results are those of fuse = 1, bit
+ * for bit.  The default is 2 for FP64 batches of two or more simulations
+ * that allow fusion, 1 for mixed-precision ones (whose stage pair needs more
+ * waves per SIMD than a mixed launch can hold).  No counterpart in the
+ * reference. */
 int dim2d_batch_set_stage_fusion(dim2d_batch* b, int32_t fuse, int32_t rows_per_block,
                                  int32_t workgroups);
-/* The launch shapes in use: out = {stage fusion on, pair rows, pair
- * workgroups launched, block_x, rows, workgroups launched}. */
+/* The launch shapes in use: out = {stage fusion (0, 1 or 2 as set), pair
+ * rows, pair (pipelined: mixed-launch) workgroups launched, block_x, rows,
+ * workgroups launched}. */
 int dim2d_batch_get_tiles(dim2d_batch* b, int32_t out[6]);
 
 /* Tile autotuner of a batch (no counterpart in the reference): times nsteps

@@ -198,22 +198,23 @@ def test_one_step_8192_open_windows(orc):
         assert max(err) < 1e-10, (j0, i0, err)
 
 
-@pytest.mark.parametrize("prec,fuse", [(0, True), (1, True), (0, False), (1, False)])
+@pytest.mark.parametrize("prec,fuse", [(0, "pipe"), (1, "pipe"), (0, True), (1, True),
+                                       (0, False), (1, False)])
 def test_batch_config_matches_single(prec, fuse):
     """The batch workload at its full size (256 simulations, n from 64 to 512,
     reflection edges, each with its own CFL dt): 3 steps of the persistent
-    batch kernels, stages 1 and 2 fused (the default) or apart; the smallest,
-    a middle and the largest simulation equal single-simulation runs of the
-    same mode bitwise; every simulation stays finite and keeps its mass to
-    1e-12."""
+    batch kernels, stages 1 and 2 fused (the default in mixed precision),
+    pipelined (two groups half a step apart; the default in FP64) or apart;
+    the smallest, a middle and the largest simulation equal
+    single-simulation runs of the same mode bitwise; every simulation stays
+    finite and keeps its mass to 1e-12."""
     import dim2d_finite_volume_amd as d
     from dim2d_finite_volume_amd import initial_conditions as ic
     sims = ic.batch_configs(precision=prec)
     dts = [ic.cfl_dt(p, n, 0.1) for p, n in sims]
     b = d.Batch([p for p, _ in sims])
-    assert b.tiles()["stage_fusion"]
-    if not fuse:
-        b.set_stage_fusion(False)
+    assert b.tiles()["stage_fusion"] and b.tiles()["pipelined"] == (prec == 0)
+    b.set_stage_fusion(bool(fuse), pipeline=fuse == "pipe")
     for i, (p, n) in enumerate(sims):
         b.upload(i, n)
     b.integrate(dts, 3)
@@ -222,7 +223,7 @@ def test_batch_config_matches_single(prec, fuse):
     for i in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
         p, n = sims[i]
         f = d.Field(p, n)
-        f.set_stage_fusion(fuse)
+        f.set_stage_fusion(bool(fuse))
         f.run(3, dt=dts[i])
         single = f.download()
         f.close()

@@ -495,41 +495,50 @@ def test_cfl_and_diagnostics(orc):
     assert diag[4] == it.min() and diag[5] == it.max() and diag[6] == 0
 
 
-@pytest.mark.parametrize("fuse", [False, True])
+@pytest.mark.parametrize("fuse", [False, True, "pipe"])
 def test_batch_matches_single(orc, fuse):
     """Persistent batch kernels under a HIP graph give the single-simulation
-    result, with stages 1 and 2 fused (against single runs that fuse them)
-    and apart."""
+    result, with stages 1 and 2 fused (against single runs that fuse them),
+    apart, and pipelined (two groups half a step apart, a stage pair beside a
+    stage 3 in each launch; the steps come in calls of 1, 4 and 1 so that the
+    first-pair-alone, graph replay, leftover step and odd-count buffer swap
+    all run)."""
     d = _lib()
     from dim2d_finite_volume_amd import initial_conditions as ic
     sims = ic.batch_configs(nsims=5, seed=11, nmin=20, nmax=90)
     dts = [ic.cfl_dt(p, n, 0.1) for p, n in sims]
     b = d.Batch([p for p, _ in sims])
-    assert b.tiles()["stage_fusion"]  # every edge here allows it
+    # every edge here allows fusion; FP64 batches are pipelined by default
+    assert b.tiles()["stage_fusion"] and b.tiles()["pipelined"]
     b.set_tiles(64, 16)
-    if not fuse:
-        b.set_stage_fusion(False)
+    b.set_stage_fusion(bool(fuse), pipeline=fuse == "pipe")
     for i, (p, n) in enumerate(sims):
         b.upload(i, n)
     # the autotuner runs real steps on candidate tilings, then restores the state
     rows, wg, ms = b.autotune(dts, 4)
     assert rows in (8, 12, 16, 24, 32) and wg > 0 and ms > 0
     t = b.tiles()
-    assert t["stage_fusion"] == fuse and t["rows"] == rows
+    assert t["stage_fusion"] == bool(fuse) and t["rows"] == rows
+    assert t["pipelined"] == (fuse == "pipe")
     if fuse:
         assert t["pair_rows"] in (16, 24, 32, 48, 60, 96, 128) and t["pair_workgroups"] > 0
-    b.integrate(dts, 6)
+    if fuse == "pipe":
+        for k in (1, 4, 1):
+            b.integrate(dts, k)
+    else:
+        b.integrate(dts, 6)
     for i, (p, n) in enumerate(sims):
         f = d.Field(p, n)
         f.set_tiles(64, 16)
-        f.set_stage_fusion(fuse)
+        f.set_stage_fusion(bool(fuse))
         f.run(6, dt=dts[i])
         single = f.download()
         got = b.download(i)
         assert np.array_equal(got[:, 2:-2, 2:-2], single[:, 2:-2, 2:-2]), i
 
 
-@pytest.mark.parametrize("prec,fuse", [(0, False), (1, False), (0, True), (1, True)])
+@pytest.mark.parametrize("prec,fuse", [(0, False), (1, False), (0, True), (1, True),
+                                       (0, "pipe"), (1, "pipe")])
 def test_batch_mixed_edges_matches_single(orc, prec, fuse):
     """A batch whose simulations have different edges (reflection, periodic,
     periodic x with walls y, a south wall with a 120-degree contact angle and
@@ -560,14 +569,17 @@ def test_batch_mixed_edges_matches_single(orc, prec, fuse):
         _, p, m, xm, ym, nodes = setup_case(orc, bc, nx=nx, ny=ny, precision=prec, **sp)
         sims.append((p, nodes, orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0], m, xm, ym))
     b = d.Batch([s[0] for s in sims])
-    assert b.tiles()["stage_fusion"] == fuse
+    assert b.tiles()["stage_fusion"] == bool(fuse)
+    assert b.tiles()["pipelined"] == (bool(fuse) and prec == 0)
+    if fuse:
+        b.set_stage_fusion(True, pipeline=fuse == "pipe")
     for i, s in enumerate(sims):
         b.upload(i, s[1])
     dts = [s[2] for s in sims]
     b.integrate(dts, 7)
     for i, (p, n, dt, m, xm, ym) in enumerate(sims):
         f = d.Field(p, n.copy())
-        f.set_stage_fusion(fuse)
+        f.set_stage_fusion(bool(fuse))
         f.run(7, dt=dt)
         single = f.download()
         f.close()
