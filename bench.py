@@ -279,9 +279,10 @@ def open_extension_bench(n, steps, every, gap, band):
 def variants_bench(n, steps):
     """The other integrator and operator choices on the headline's two-bubble
     state at n^2, CFL 0.1 device-side dt, FP64: the low-storage RK3 (periodic
-    x / wall y, as the headline) and the Cockburn-Gau stencils (periodic
-    edges: they take reflection and periodic edges only), with the Mattsson
-    RK3-TVD on the same periodic case beside them."""
+    x / wall y, as the headline; also in mixed precision) and the
+    Cockburn-Gau stencils (periodic edges: they take reflection and periodic
+    edges only), with the Mattsson RK3-TVD on the same periodic case beside
+    them."""
     import dataclasses
 
     import dim2d_finite_volume_amd as d
@@ -289,6 +290,7 @@ def variants_bench(n, steps):
     p, nodes = ic.two_bubbles(n=n)
     per = (d.BC_PERIODIC,) * 4
     cases = (("lsrk3", dict(integrator=d.INT_LSRK3)),
+             ("lsrk3_mixed", dict(integrator=d.INT_LSRK3, precision=d.PREC_MIXED)),
              ("rk3tvd_periodic", dict(bc=per)),
              ("cockburn_gau_periodic", dict(bc=per, operators=d.OPS_COCKBURN_GAU)))
     out = {}
@@ -303,7 +305,8 @@ def variants_bench(n, steps):
         out[name] = {"value": n * n * steps / el, "unit": "grid-pt updates/s",
                      "ms_per_step": 1e3 * el / steps}
         log(f"[bench] {name} {n}^2: {out[name]['value']:.4e} updates/s")
-    out["config"] = f"two-bubble state, {n}x{n} interior, FP64, CFL 0.1, {steps} timed steps"
+    out["config"] = (f"two-bubble state, {n}x{n} interior, FP64 (lsrk3_mixed: mixed precision), "
+                     f"CFL 0.1, {steps} timed steps")
     return out
 
 

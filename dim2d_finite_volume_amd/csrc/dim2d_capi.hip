@@ -76,8 +76,6 @@ int validate(const dim2d_params* p) {
     return fail(DIM2D_EINVAL, "unknown precision mode %d", p->precision);
   if (p->integrator != DIM2D_INT_RK3TVD && p->integrator != DIM2D_INT_LSRK3)
     return fail(DIM2D_EINVAL, "unknown integrator %d", p->integrator);
-  if (p->integrator == DIM2D_INT_LSRK3 && p->precision != DIM2D_PREC_FP64)
-    return fail(DIM2D_EINVAL, "the low-storage RK3 runs in FP64 only");
   if (p->operators != DIM2D_OPS_MATTSSON && p->operators != DIM2D_OPS_COCKBURN_GAU)
     return fail(DIM2D_EINVAL, "unknown operator set %d", p->operators);
   if (p->operators == DIM2D_OPS_COCKBURN_GAU) {
@@ -199,6 +197,7 @@ struct dim2d_ctx {
   double* A = nullptr;   // FP64 stage 1 output
   double* B = nullptr;   // FP64 stage 2 output
   double* Q = nullptr;   // low-storage RK register q
+  float* Q32 = nullptr;  // the same in mixed precision
   float* D1 = nullptr;   // mixed stage increments
   float* D2 = nullptr;
   double* T = nullptr;   // planar staging of uploads, downloads and time derivatives
@@ -359,9 +358,11 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (e == hipSuccess) e = field_alloc(&c->U2, n);
   if (e == hipSuccess && !c->mixed && !c->lsrk) e = field_alloc(&c->A, n);
   if (e == hipSuccess && !c->mixed && !c->lsrk) e = field_alloc(&c->B, n);
-  if (e == hipSuccess && c->lsrk) e = field_alloc(&c->Q, n);
-  if (e == hipSuccess && c->mixed) e = field_alloc(&c->D1, n);
-  if (e == hipSuccess && c->mixed) e = field_alloc(&c->D2, n);
+  if (e == hipSuccess && c->lsrk && !c->mixed) e = field_alloc(&c->Q, n);
+  if (e == hipSuccess && c->lsrk && c->mixed) e = field_alloc(&c->Q32, n);
+  // the low-storage RK3 keeps its stage states in FP64 in both modes
+  if (e == hipSuccess && c->mixed && !c->lsrk) e = field_alloc(&c->D1, n);
+  if (e == hipSuccess && c->mixed && !c->lsrk) e = field_alloc(&c->D2, n);
   if (e == hipSuccess) e = hipMalloc(&c->T, n * sizeof(double));
   if (e == hipSuccess && params->bc_n == DIM2D_BC_OPEN) {
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
@@ -374,6 +375,7 @@ int dim2d_ctx_create(const dim2d_params* params, dim2d_ctx** out) {
   if (e == hipSuccess && c->A) e = hipMemsetAsync(c->A, 0, n * sizeof(double), c->stream);
   if (e == hipSuccess && c->B) e = hipMemsetAsync(c->B, 0, n * sizeof(double), c->stream);
   if (e == hipSuccess && c->Q) e = hipMemsetAsync(c->Q, 0, n * sizeof(double), c->stream);
+  if (e == hipSuccess && c->Q32) e = hipMemsetAsync(c->Q32, 0, n * sizeof(float), c->stream);
   if (e == hipSuccess && c->D1) e = hipMemsetAsync(c->D1, 0, n * sizeof(float), c->stream);
   if (e == hipSuccess && c->D2) e = hipMemsetAsync(c->D2, 0, n * sizeof(float), c->stream);
   if (e == hipSuccess) {
@@ -402,6 +404,7 @@ int dim2d_ctx_destroy(dim2d_ctx* c) {
   field_free(c->A);
   field_free(c->B);
   field_free(c->Q);
+  field_free(c->Q32);
   field_free(c->D1);
   field_free(c->D2);
   (void)hipFree(c->T);
@@ -655,17 +658,20 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     }
   } else if (c->lsrk && c->fuse) {
     // low-storage stages 1 and 2 in one launch, u1 and q1 on chip: u0 (U) ->
-    // u2 (U2) and q2 (Q); stage 3 (U2, Q) -> U, so the new state is in U and
-    // the buffers are not swapped below.  The wall root guess is each stage's
-    // input state, except for the ghosts of u2 (u1 is not stored): u0
+    // u2 (U2) and q2 (Q, or Q32 in mixed precision); stage 3 (U2, Q) -> U, so
+    // the new state is in U and the buffers are not swapped below.  The wall
+    // root guess is each stage's input state, except for the ghosts of u2
+    // (u1 is not stored): u0.  The states are FP64 in both modes, so are the
+    // wall launches.
     a.U = c->U;
     a.Q = c->Q;
+    a.Q32 = c->Q32;
     a.O64 = c->U2;
-    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage_pair(p, 0, 1, a, c->pair_rows, c->stream); }));
+    HIP_TRY(timed_launch(c, 0, [&] { return launch_stage_pair(p, c->mixed, 1, a, c->pair_rows, c->stream); }));
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U2, nullptr, c->U2, nullptr, c->U, c->stream); }));
     a.S64 = c->U2;
     a.O64 = c->U;
-    HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, 0, 6, a, c->block_x, c->rows, c->stream); }));
+    HIP_TRY(timed_launch(c, 2, [&] { return launch_stage(p, c->mixed, 6, a, c->block_x, c->rows, c->stream); }));
     if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, c->U, nullptr, c->U, nullptr, c->U2, c->stream); }));
     result_in_u = true;
   } else if (c->lsrk) {
@@ -674,12 +680,13 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     double* in = c->U;
     double* out = c->U2;
     a.Q = c->Q;
+    a.Q32 = c->Q32;
     for (int s = 0; s < 3; ++s) {
       a.S64 = in;
       a.O64 = out;
-      if ((st = fork_frame(0, 4 + s))) return st;
-      HIP_TRY(timed_launch(c, s, [&] { return launch_stage(p, 0, 4 + s, a, c->block_x, c->rows, c->stream); }));
-      if ((st = frame(0, 4 + s))) return st;
+      if ((st = fork_frame(c->mixed, 4 + s))) return st;
+      HIP_TRY(timed_launch(c, s, [&] { return launch_stage(p, c->mixed, 4 + s, a, c->block_x, c->rows, c->stream); }));
+      if ((st = frame(c->mixed, 4 + s))) return st;
       if (wall) HIP_TRY(timed_launch(c, 3, [&] { return launch_wall(p, 0, nullptr, out, nullptr, out, nullptr, in, c->stream); }));
       std::swap(in, out);
     }
@@ -1586,7 +1593,7 @@ int dim2d_batch_create(const dim2d_params* params, int32_t n, dim2d_batch** out)
   b->fuse = true;
   for (const DevParams& p : b->hp) b->fuse = b->fuse && pair_supported(p);
   // pipelined by default in FP64: its mixed launches keep the FP64 pair's 2
-  // waves per SIMD (3.5 % faster a step over 256 simulations), while in mixed
+  // waves per SIMD (2-3.5 % faster a step over 256 simulations), while in mixed
   // precision they would drop the pair from 3 waves to 2 (9 % slower)
   b->pipe = b->fuse && !b->mixed && n >= 2;
   b->total = off;

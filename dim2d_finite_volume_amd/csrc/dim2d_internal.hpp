@@ -94,6 +94,7 @@ struct StageArgs {
   double* O64;        // FP64 output (FP64 stages, mixed stage 3, time_dev)
   float* O32;         // FP32 output (mixed stages 1, 2)
   double* Q;          // low-storage RK register q (read and written in place)
+  float* Q32;         // the same in mixed precision (FP32)
   StepCtl* ctl;
   double* F64;        // stage pair with open edges: s1 of the frame and the band
   float* F32;         // beside it (FP64), or its increment (mixed)

@@ -709,12 +709,13 @@ __device__ __forceinline__ void strip_time_dev(const DevParams& p, StripRings<RD
 template <typename RD, typename R, int MIXED, int STAGE, bool REDUCE>
 __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& a, double dt,
                                            int bx, int by, int rows, RedAcc<false>& red) {
-  // mixed stages 2 and 3 read u^n + increment; mixed stage 0/1 read u^n alone
-  constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
-  // u^n (RK3-TVD FP64 stages 2, 3) or q (low-storage stages 2, 3) is loaded
-  // apart from the stencil rows; every other RK input is the stencil row
-  // itself (S64, or U + S32 with S32 kept)
-  constexpr bool LOAD_U = !MIXED && (STAGE == 2 || STAGE == 3 || STAGE == 5 || STAGE == 6);
+  // mixed RK3-TVD stages 2 and 3 read u^n + increment, mixed stages 0/1 u^n
+  // alone; the low-storage stages read their FP64 state (S64) in both modes
+  constexpr int SRC = (MIXED && STAGE < 4) ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
+  // u^n (RK3-TVD FP64 stages 2, 3) or q (low-storage stages 2, 3; FP32 in
+  // mixed precision) is loaded apart from the stencil rows; every other RK
+  // input is the stencil row itself (S64, or U + S32 with S32 kept)
+  constexpr bool LOAD_U = (!MIXED && (STAGE == 2 || STAGE == 3)) || STAGE == 5 || STAGE == 6;
   const double* __restrict__ rk_src = STAGE >= 5 ? a.Q : a.U;
   const int lane = threadIdx.x & (kLanes - 1);
   const int c = bx * kOutCols + lane;  // column of this lane (0-based)
@@ -742,7 +743,10 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   const unsigned ucu = (unsigned)min(max(c, bx * kOutCols + kBcSize),
                                      min(bx * kOutCols + kLanes - kBcSize - 1,
                                          p.nx - kBcSize - 1));
-  auto load_u = [&](int r, double u[4]) { ld_cell(rk_src + (long long)r * p.pitch * 4, ucu, u); };
+  auto load_u = [&](int r, double u[4]) {
+    if (MIXED && STAGE >= 5) ld_cell(a.Q32 + (long long)r * p.pitch * 4, ucu, u);
+    else ld_cell(rk_src + (long long)r * p.pitch * 4, ucu, u);
+  };
 
   // rings: slot (q - (r0 - 2)) % kRing holds row q
   StripRings<RD, R> sr;
@@ -777,18 +781,25 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
         for (int k = 0; k < 4; ++k) vout[k] = (double)L[k];
         st_cell(a.O64 + row, uc, vout);
       } else if (STAGE >= 4) {
-        // low-storage RK3 stage s: q = A_s q + dt L, u = u + B_s q
+        // low-storage RK3 stage s: q = A_s q + dt L, u = u + B_s q (the state
+        // FP64 in both modes; mixed precision keeps q in FP32, the state
+        // update taking q before its rounding)
         constexpr int s = STAGE - 4;
         constexpr double LA = s == 0 ? 0.0 : (s == 1 ? -5.0 / 9.0 : -153.0 / 128.0);
         constexpr double LB = s == 0 ? 1.0 / 3.0 : (s == 1 ? 15.0 / 16.0 : 8.0 / 15.0);
         double q[4];
+        float q32[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           q[k] = (s == 0 ? 0.0 : LA * rkU[K][k]) + dt * (double)L[k];
           vout[k] = S[k] + LB * q[k];
+          q32[k] = (float)q[k];
         }
         st_cell(a.O64 + row, uc, vout);
-        if (s < 2) st_cell(a.Q + row, uc, q);
+        if (s < 2) {
+          if (MIXED) st_cell(a.Q32 + row, uc, q32);
+          else st_cell(a.Q + row, uc, q);
+        }
       } else if (!MIXED) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -1305,8 +1316,9 @@ __device__ __forceinline__ void pair_gather(const DevParams& p, const PairRing<M
 // consumer's outputs read, and those a reflecting W/E edge mirrors).  The
 // values are those wall_kernel stores between the separate stage kernels: in
 // mixed precision the ghost state goes through its FP32 increment against
-// u^n, as the stage-2 kernel would read it.
-template <int MIXED>
+// u^n, as the stage-2 kernel would read it (the low-storage RK3 keeps its
+// stage states in FP64 in both modes).
+template <int MIXED, bool LS = false>
 __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<MIXED>& ring,
                                int bx, bool north) {
   const int lane = threadIdx.x & (kLanes - 1);
@@ -1326,7 +1338,7 @@ __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<
     const double T_guess = p.c38cv * temperature_eff_at(p, a.U, c, r1);
     const double md_guess = a.U[((long long)r * p.pitch + cm) * 4];
     wall_ghost_values(p, north ? 1 : 0, v1, v2, y0[0], y1[0], T_guess, md_guess, g, go);
-    if (MIXED) {
+    if (MIXED && !LS) {
       const double* ug = a.U + ((long long)r * p.pitch + cm) * 4;
       const double* uo = a.U + ((long long)ro * p.pitch + cm) * 4;
 #pragma unroll
@@ -1348,7 +1360,8 @@ __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<
 
 // the producer wave: stage 1 of rows r0-2 .. rlast into the ring, then the
 // north wall rows when the tile holds them, then idle barriers up to `steps`
-// LS: the low-storage RK3's stages 1 and 2 (FP64; the ring's u rows hold q1)
+// LS: the low-storage RK3's stages 1 and 2 (the ring's s rows hold u1, its u
+// rows q1 in FP64, its d rows q1 in FP32 in mixed precision)
 template <typename RD, typename R, int MIXED, bool OPEN, bool LS>
 __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArgs& a, double dt,
                                               int bx, int r0, int r1, int steps,
@@ -1400,16 +1413,17 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double dl = dt * (double)L[k];
-        if (MIXED) {
+        if (LS) {
+          // low-storage stage 1: q1 = dt L, u1 = u0 + B1 q1
+          const double q = 0.0 + dl;
+          ring.s[sl][k][lane] = S[k] + (1.0 / 3.0) * q;
+          if (MIXED) ring.d[MIXED ? sl : 0][k][lane] = (float)q;
+          else ring.u[MIXED ? 0 : sl][k][lane] = q;
+        } else if (MIXED) {
           const float o32 = (float)dl;
           ring.d[MIXED ? sl : 0][k][lane] = o32;
           ring.s[sl][k][lane] = S[k] + (double)o32;
           if (band) a.F32[o + k] = o32;
-        } else if (LS) {
-          // low-storage stage 1: q1 = dt L, u1 = u0 + B1 q1
-          const double q = 0.0 + dl;
-          ring.s[sl][k][lane] = S[k] + (1.0 / 3.0) * q;
-          ring.u[MIXED ? 0 : sl][k][lane] = q;
         } else {
           ring.s[sl][k][lane] = S[k] + dl;
           ring.u[MIXED ? 0 : sl][k][lane] = S[k];
@@ -1428,7 +1442,7 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
     if (!step(Slot<4>{})) break;
   }
   if (top && p.bc_n == kBcWall) {
-    pair_wall_rows<MIXED>(p, a, ring, bx, true);
+    pair_wall_rows<MIXED, LS>(p, a, ring, bx, true);
     __syncthreads();
     ++g;
   }
@@ -1447,7 +1461,7 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
   const unsigned uc = (unsigned)min(max(c, 0), p.nx - 1);
   for (int g = 0; g < kPairLag; ++g) __syncthreads();
   if (r0 == kBcSize && p.bc_s == kBcWall) {
-    pair_wall_rows<MIXED>(p, a, ring, bx, false);
+    pair_wall_rows<MIXED, LS>(p, a, ring, bx, false);
     __threadfence_block();
   }
   const int q0 = r0 - 2;
@@ -1467,17 +1481,22 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
     if (out_col && r >= r0) {
       const long long row = (long long)r * p.pitch * 4;
       const double* S = sr.raw[K];
-      if (!MIXED && LS) {
+      if (LS) {
         // low-storage stage 2: q2 = A2 q1 + dt L, u2 = u1 + B2 q2
         double vout[4], q[4];
+        float q32[4];
         const int sl = r & (kPairSlots - 1);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          q[k] = (-5.0 / 9.0) * ring.u[MIXED ? 0 : sl][k][lane] + dt * (double)L[k];
+          const double q1 = MIXED ? (double)ring.d[MIXED ? sl : 0][k][lane]
+                                  : ring.u[MIXED ? 0 : sl][k][lane];
+          q[k] = (-5.0 / 9.0) * q1 + dt * (double)L[k];
           vout[k] = S[k] + (15.0 / 16.0) * q[k];
+          q32[k] = (float)q[k];
         }
         st_cell(a.O64 + row, uc, vout);
-        st_cell(a.Q + row, uc, q);
+        if (MIXED) st_cell(a.Q32 + row, uc, q32);
+        else st_cell(a.Q + row, uc, q);
       } else if (!MIXED) {
         double vout[4];
         const int sl = r & (kPairSlots - 1);
@@ -1519,7 +1538,7 @@ __device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a
                                           PairRing<MIXED>& ring) {
   using RD = double;
   using R = typename std::conditional<MIXED, float, double>::type;
-  static_assert(!(LS && (MIXED || OPEN)), "the low-storage pair is FP64 without open edges");
+  static_assert(!(LS && OPEN), "the low-storage pair runs without open edges");
   // the consumer's rows r0-2 .. r1-1 after kPairLag idle barriers
   const int steps = kPairLag + (r1 - r0 + 2);
   if ((threadIdx.x < kLanes) != swap)
@@ -1533,7 +1552,8 @@ __device__ __forceinline__ void pair_tile(const DevParams& p, const StageArgs& a
 // all four edges open (hedstrom_xy); the frame cells of s1 come from the
 // stage-1 frame kernel, launched before this kernel, and the stage-2 frame
 // kernel, launched after it, reads the band of s1 this kernel stores.  LS:
-// the low-storage RK3 (FP64, no open edges): u0 (a.U) -> u2 (a.O64), q2 (a.Q).
+// the low-storage RK3 (no open edges): u0 (a.U) -> u2 (a.O64), q2 (a.Q, or
+// a.Q32 in mixed precision).
 template <int MIXED, bool OPEN, bool LS>
 __global__ void __launch_bounds__(2 * kLanes)
 __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2, 8)))
@@ -1994,7 +2014,7 @@ template <int MIXED, int STAGE>
 __global__ void __launch_bounds__(2 * kFrameSeg) frame_kernel(const DevParams p, const StageArgs a) {
   __shared__ double win[kFrameWin * 4];
   __shared__ double tab[frame::kNumPrim * kFrameWin];
-  constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
+  constexpr int SRC = (MIXED && STAGE < 4) ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
   const int nxi = p.nx - 2 * kBcSize, nyi = p.ny - 2 * kBcSize;
   const int segx = (nxi + kFrameSeg - 1) / kFrameSeg, segy = (nyi + kFrameSeg - 1) / kFrameSeg;
   const int tid = threadIdx.x, line = tid / kFrameSeg, lt = tid % kFrameSeg;
@@ -2076,9 +2096,13 @@ __global__ void __launch_bounds__(2 * kFrameSeg) frame_kernel(const DevParams p,
     constexpr double LA = s == 0 ? 0.0 : (s == 1 ? -5.0 / 9.0 : -153.0 / 128.0);
     constexpr double LB = s == 0 ? 1.0 / 3.0 : (s == 1 ? 15.0 / 16.0 : 8.0 / 15.0);
     for (int k = 0; k < 4; ++k) {
-      const double q = (s == 0 ? 0.0 : LA * a.Q[o + k]) + dt * L[k];
+      const double qin = s == 0 ? 0.0 : (MIXED ? (double)a.Q32[o + k] : a.Q[o + k]);
+      const double q = (s == 0 ? 0.0 : LA * qin) + dt * L[k];
       a.O64[o + k] = S[k] + LB * q;
-      if (s < 2) a.Q[o + k] = q;
+      if (s < 2) {
+        if (MIXED) a.Q32[o + k] = (float)q;
+        else a.Q[o + k] = q;
+      }
     }
   } else if (!MIXED) {
     for (int k = 0; k < 4; ++k) {
@@ -2647,7 +2671,10 @@ hipError_t launch_stage(const DevParams& p, int mixed, int stage, const StageArg
     case 0: return launch_one<1, 0>(p, a, block_x, rows, s);
     case 1: return launch_one<1, 1>(p, a, block_x, rows, s);
     case 2: return launch_one<1, 2>(p, a, block_x, rows, s);
-    default: return launch_one<1, 3>(p, a, block_x, rows, s);
+    case 3: return launch_one<1, 3>(p, a, block_x, rows, s);
+    case 4: return launch_one<1, 4>(p, a, block_x, rows, s);
+    case 5: return launch_one<1, 5>(p, a, block_x, rows, s);
+    default: return launch_one<1, 6>(p, a, block_x, rows, s);
   }
 }
 
@@ -2666,8 +2693,9 @@ hipError_t launch_stage_pair(const DevParams& p, int mixed, int lsrk, const Stag
 #define DIM2D_PK(M, O, L) \
   hipLaunchKernelGGL((stage_pair_kernel<M, O, L>), grid, dim3(2 * kLanes), 0, s, p, a, rows)
   if (lsrk) {
-    if (mixed || open) return hipErrorInvalidValue;
-    DIM2D_PK(0, false, true);
+    if (open) return hipErrorInvalidValue;
+    if (mixed) DIM2D_PK(1, false, true);
+    else DIM2D_PK(0, false, true);
   } else if (mixed) {
     if (open) DIM2D_PK(1, true, false);
     else DIM2D_PK(1, false, false);
@@ -2777,7 +2805,10 @@ hipError_t launch_frame(const DevParams& p, int mixed, int stage, const StageArg
       case 0: DIM2D_FR(1, 0); break;
       case 1: DIM2D_FR(1, 1); break;
       case 2: DIM2D_FR(1, 2); break;
-      default: DIM2D_FR(1, 3); break;
+      case 3: DIM2D_FR(1, 3); break;
+      case 4: DIM2D_FR(1, 4); break;
+      case 5: DIM2D_FR(1, 5); break;
+      default: DIM2D_FR(1, 6); break;
     }
   }
 #undef DIM2D_FR

@@ -34,7 +34,7 @@ import re
 from collections import defaultdict
 
 STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
-PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)(?:,\s*(true|false))?>")
+PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)(?:,\s*(true|false))?(?:,\s*(true|false))?>")
 BSTAGE_RE = re.compile(r"dim2d::stage_batch_kernel<(\d+),\s*(\d+)>")
 BPAIR_RE = re.compile(r"dim2d::stage_pair_batch_kernel<(\d+)>")
 BMIX_RE = re.compile(r"dim2d::stage_mix_batch_kernel<(\d+)>")
@@ -57,7 +57,8 @@ def kernel_key(name):
         return f"stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
     m = PAIR_RE.search(name)
     if m:
-        return f"pair_{'mixed' if m.group(1) == '1' else 'fp64'}{'_open' if m.group(2) == 'true' else ''}"
+        return (f"pair_{'mixed' if m.group(1) == '1' else 'fp64'}"
+                f"{'_open' if m.group(2) == 'true' else ''}{'_lsrk3' if m.group(3) == 'true' else ''}")
     return None
 
 
@@ -80,6 +81,9 @@ def short_name(name):
         prec = "mixed" if m.group(1) == "1" else "FP64"
         if m.group(2) == "true":
             return f"stage_pair_kernel<{m.group(1)}, true> ({prec} stages 1+2, open edges)"
+        if m.group(3) == "true":
+            return (f"stage_pair_kernel<{m.group(1)}, false, true> "
+                    f"({prec} low-storage RK3 stages 1+2)")
         return f"stage_pair_kernel<{m.group(1)}> ({prec} stages 1+2)"
     m = STAGE_RE.search(name)
     if m:

@@ -46,8 +46,10 @@ extern "C" {
 /* time integrators */
 #define DIM2D_INT_RK3TVD 0 /* the reference's RK3-TVD (rk3tvd_steps_module.f) */
 #define DIM2D_INT_LSRK3 1  /* low-storage RK3, Williamson (1980) 2N-storage; no counterpart
-                              in the reference (its only integrator is RK3-TVD); FP64 only,
-                              single contexts only; three state-sized buffers instead of four */
+                              in the reference (its only integrator is RK3-TVD); single
+                              contexts only; FP64: three FP64 state-sized buffers instead of
+                              four; mixed precision: two FP64 states (stage states stay
+                              FP64) and the register q in FP32 */
 
 /* interior face stencils (the reference's compile-time sd_choice,
  * config/code_config/makefile_sd.mk) */
@@ -195,7 +197,8 @@ int dim2d_set_tiles(dim2d_ctx* ctx, int32_t block_x, int32_t rows_per_block);
  * full field, rk3tvd_steps_module.f:52-490): fuse = 1 computes stages 1 and 2
  * in one launch whose stage-1 state stays on chip (RK3-TVD: 64 fewer HBM
  * bytes per cell and step in FP64, 32 in mixed precision; low-storage RK3:
- * 128 fewer, u1 and q1 on chip); fuse = 0 launches the stages apart.
+ * 128 fewer in FP64, 96 in mixed precision, u1 and q1 on chip); fuse = 0
+ * launches the stages apart.
  * rows_per_block: tile height of the fused launch (0 keeps the current
  * value, default from the grid size).  On by default where it applies: the
  * Mattsson operators, reflection, periodic, S/N wall or (RK3-TVD only) open

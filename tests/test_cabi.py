@@ -74,13 +74,15 @@ def test_unsupported_boundary_combinations():
 
 
 def test_low_storage_rk_limits():
-    """The low-storage RK3 runs in FP64 and in single contexts only."""
+    """The low-storage RK3 runs in single contexts only, with the Mattsson
+    operators."""
     import dim2d_finite_volume_amd as d
     L = d.capi.load()
     ctx = ctypes.c_void_p()
-    p = d.ModelParams(nx=40, ny=40, precision=d.PREC_MIXED, integrator=d.INT_LSRK3).to_c()
+    p = d.ModelParams(nx=40, ny=40, operators=d.OPS_COCKBURN_GAU,
+                      integrator=d.INT_LSRK3).to_c()
     assert L.dim2d_ctx_create(ctypes.byref(p), ctypes.byref(ctx)) == d.capi.EINVAL
-    assert b"FP64 only" in L.dim2d_last_error()
+    assert b"RK3-TVD only" in L.dim2d_last_error()
     ps = (d.capi.Params * 2)(d.ModelParams(nx=40, ny=40).to_c(),
                              d.ModelParams(nx=40, ny=40, integrator=d.INT_LSRK3).to_c())
     b = ctypes.c_void_p()
