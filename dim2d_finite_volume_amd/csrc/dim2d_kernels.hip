@@ -152,8 +152,14 @@ struct XVals {
 
 // velocities, classical pressure (:425-453), classical temperature (:515-544),
 // capillarity pressure 1/(3-rho) (:1030-1062) and the transport products
-// (:830-1150), computed in FP64 from the FP64 state.
-template <typename RD, typename R>
+// (:830-1150), computed in FP64 from the FP64 state.  F32P (FP32 fluxes
+// only): the transport products from the FP32 operands instead, 8
+// conversions per cell instead of 13, each product ~1.5 FP32 ulp instead of
+// 0.5.  Every mixed stage but the last of a step takes it (stages 1 and 2
+// alone or as the stage pair, which is bound by instruction issue); the last
+// stage does not, since its fewer registers would let it run 3 waves per
+// SIMD, which is slower for that HBM-bound kernel.
+template <typename RD, typename R, bool F32P = false>
 __device__ __forceinline__ void make_cell(const DevParams& p, const double v[4], Cell<RD, R>& c,
                                           XVals<R>& x) {
   const double rho = v[0];
@@ -173,14 +179,25 @@ __device__ __forceinline__ void make_cell(const DevParams& p, const double v[4],
   c.Tcl = (R)(Tn * ir);
   c.Tcap = (R)0;
   c.qy = (R)v[2];
-  c.qxv = (R)(v[1] * vv);
-  c.Pyy = (R)(v[2] * vv + P);
-  c.Hv = (R)(H * vv);
-  c.vw = (R)(vv * w);
   x.qx = (R)v[1];
-  x.Pxx = (R)(v[1] * u + P);
-  x.Hu = (R)(H * u);
-  x.uw = (R)(u * w);
+  if (F32P && std::is_same<R, float>::value) {
+    const R Pr = (R)P, Hr = (R)H;
+    c.qxv = x.qx * c.v;
+    c.Pyy = c.qy * c.v + Pr;
+    c.Hv = Hr * c.v;
+    c.vw = c.v * c.w;
+    x.Pxx = x.qx * c.u + Pr;
+    x.Hu = Hr * c.u;
+    x.uw = c.u * c.w;
+  } else {
+    c.qxv = (R)(v[1] * vv);
+    c.Pyy = (R)(v[2] * vv + P);
+    c.Hv = (R)(H * vv);
+    c.vw = (R)(vv * w);
+    x.Pxx = (R)(v[1] * u + P);
+    x.Hu = (R)(H * u);
+    x.uw = (R)(u * w);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -632,14 +649,14 @@ struct StripRings {
 
 // the first cell of a strip's pipeline (row q0, slot 0), with no capillary
 // temperature: the warm-up rows that use it produce no output
-template <typename RD, typename R>
+template <typename RD, typename R, bool F32P = false>
 __device__ __forceinline__ void strip_start(const DevParams& p, StripRings<RD, R>& s) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     s.fy[kRing - 1][k] = (R)0;
     s.rkU[0][k] = 0.0;
   }
-  make_cell<RD, R>(p, s.raw[0], s.cell[0], s.xv[0]);
+  make_cell<RD, R, F32P>(p, s.raw[0], s.cell[0], s.xv[0]);
   s.cell[0].Tcap = (R)0;
   s.cell[kRing - 1] = s.cell[0];
   s.wst[0] = nb_shr(s.cell[0]);
@@ -652,7 +669,7 @@ __device__ __forceinline__ void strip_start(const DevParams& p, StripRings<RD, R
 // raw[K2]), exchanges the neighbouring columns by DPP, and forms the x-face
 // flux at (i-1/2, r), the y-face flux at (i, r+1/2) and the flux-difference
 // divergence plus body force (td_operators_class.f:181-194).
-template <typename RD, typename R, int K>
+template <typename RD, typename R, int K, bool F32P = false>
 __device__ __forceinline__ void strip_time_dev(const DevParams& p, StripRings<RD, R>& s, int r,
                                                bool wall_w, bool wall_e, R L[4]) {
   constexpr int KM = (K + kRing - 1) % kRing;  // row r-1
@@ -665,7 +682,7 @@ __device__ __forceinline__ void strip_time_dev(const DevParams& p, StripRings<RD
   Cell<RD, R>& c1 = s.cell[K1];
   const XVals<R>& x0 = s.xv[K];
   // cell of row r+1 and its neighbours
-  make_cell<RD, R>(p, s.raw[K1], c1, s.xv[K1]);
+  make_cell<RD, R, F32P>(p, s.raw[K1], c1, s.xv[K1]);
   s.wst[K1] = nb_shr(c1);
   s.est[K1] = nb_shl(c1);
   const Nb<RD, R>& c1W = s.wst[K1];
@@ -712,6 +729,8 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   // mixed RK3-TVD stages 2 and 3 read u^n + increment, mixed stages 0/1 u^n
   // alone; the low-storage stages read their FP64 state (S64) in both modes
   constexpr int SRC = (MIXED && STAGE < 4) ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
+  // FP32 transport products in every mixed stage but a step's last
+  constexpr bool F32P = MIXED && STAGE != 3 && STAGE != 6;
   // u^n (RK3-TVD FP64 stages 2, 3) or q (low-storage stages 2, 3; FP32 in
   // mixed precision) is loaded apart from the stencil rows; every other RK
   // input is the stencil row itself (S64, or U + S32 with S32 kept)
@@ -758,7 +777,7 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
   load_raw(r0 - 1, raw[1], inc[1]);
   load_raw(r0, raw[2], inc[2]);
   load_raw(r0 + 1, raw[3], inc[3]);
-  strip_start(p, sr);
+  strip_start<RD, R, F32P>(p, sr);
 
   auto row = [&](auto slot, int r) __attribute__((always_inline)) {
     constexpr int K = decltype(slot)::k;
@@ -770,7 +789,7 @@ __device__ __forceinline__ void stage_tile(const DevParams& p, const StageArgs& 
     load_raw(min(r + 4, r1 + 1), raw[K4], inc[K4]);
     if (LOAD_U) load_u(min(r + 1, r1 - 1), rkU[K1]);
     R L[4];
-    strip_time_dev<RD, R, K>(p, sr, r, wall_w, wall_e, L);
+    strip_time_dev<RD, R, K, F32P>(p, sr, r, wall_w, wall_e, L);
 
     const long long row = (long long)r * p.pitch * 4;  // uniform row offset
     if (out_col && r >= r0) {
@@ -1394,7 +1413,7 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
   load_raw(q0 + 1, sr.raw[1]);
   load_raw(q0 + 2, sr.raw[2]);
   load_raw(q0 + 3, sr.raw[3]);
-  strip_start(p, sr);
+  strip_start<RD, R, MIXED != 0>(p, sr);
   const int nrows = rlast - q0 + 1;
   int g = 0;
   auto step = [&](auto slot) __attribute__((always_inline)) {
@@ -1403,7 +1422,7 @@ __device__ __forceinline__ void pair_producer(const DevParams& p, const StageArg
     const int r = q0 + g;
     load_raw(min(r + 4, rlast + 2), sr.raw[K4]);
     R L[4];
-    strip_time_dev<RD, R, K>(p, sr, r, false, false, L);
+    strip_time_dev<RD, R, K, MIXED != 0>(p, sr, r, false, false, L);
     if (r >= r0 - 2) {
       const int sl = r & (kPairSlots - 1);
       const double* S = sr.raw[K];
@@ -1468,7 +1487,7 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
   StripRings<RD, R> sr;
   pair_gather<MIXED, OPEN>(p, ring, a, bx, q0, lane, sr.raw[0], sr.inc[0]);
   pair_gather<MIXED, OPEN>(p, ring, a, bx, q0 + 1, lane, sr.raw[1], sr.inc[1]);
-  strip_start(p, sr);
+  strip_start<RD, R, MIXED != 0>(p, sr);
   int r = q0;
   auto row = [&](auto slot) __attribute__((always_inline)) {
     constexpr int K = decltype(slot)::k;
@@ -1477,7 +1496,7 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
     // s1 of row r+2 from the ring (written at least one barrier ago)
     pair_gather<MIXED, OPEN>(p, ring, a, bx, r + 2, lane, sr.raw[K2], sr.inc[K2]);
     R L[4];
-    strip_time_dev<RD, R, K>(p, sr, r, false, false, L);
+    strip_time_dev<RD, R, K, MIXED != 0>(p, sr, r, false, false, L);
     if (out_col && r >= r0) {
       const long long row = (long long)r * p.pitch * 4;
       const double* S = sr.raw[K];
