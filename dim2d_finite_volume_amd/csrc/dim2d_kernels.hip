@@ -1336,10 +1336,11 @@ __device__ __forceinline__ void pair_gather(const DevParams& p, const PairRing<M
 // values are those wall_kernel stores between the separate stage kernels: in
 // mixed precision the ghost state goes through its FP32 increment against
 // u^n, as the stage-2 kernel would read it (the low-storage RK3 keeps its
-// stage states in FP64 in both modes).
+// stage states in FP64 in both modes).  outer: the outer ghost row too (a
+// tile whose first gathered row is the inner ghost row needs only that one).
 template <int MIXED, bool LS = false>
 __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<MIXED>& ring,
-                               int bx, bool north) {
+                               int bx, bool north, bool outer = true) {
   const int lane = threadIdx.x & (kLanes - 1);
   const int c = kBcSize + bx * kPairOut - 4 + lane;
   const int r = north ? p.ny - kBcSize : kBcSize - 1;  // inner ghost row
@@ -1366,13 +1367,13 @@ __device__ void pair_wall_rows(const DevParams& p, const StageArgs& a, PairRing<
         g[k] = ug[k] + (double)dg;
         go[k] = uo[k] + (double)dgo;
         ring.d[MIXED ? (r & (kPairSlots - 1)) : 0][k][lane] = dg;
-        ring.d[MIXED ? (ro & (kPairSlots - 1)) : 0][k][lane] = dgo;
+        if (outer) ring.d[MIXED ? (ro & (kPairSlots - 1)) : 0][k][lane] = dgo;
       }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       ring.s[r & (kPairSlots - 1)][k][lane] = g[k];
-      ring.s[ro & (kPairSlots - 1)][k][lane] = go[k];
+      if (outer) ring.s[ro & (kPairSlots - 1)][k][lane] = go[k];
     }
   }
 }
@@ -1479,8 +1480,10 @@ __device__ __forceinline__ void pair_consumer(const DevParams& p, const StageArg
   const bool out_col = lane >= 4 && lane < kLanes - 4 && c <= p.nx - kBcSize - 1;
   const unsigned uc = (unsigned)min(max(c, 0), p.nx - 1);
   for (int g = 0; g < kPairLag; ++g) __syncthreads();
-  if (r0 == kBcSize && p.bc_s == kBcWall) {
-    pair_wall_rows<MIXED, LS>(p, a, ring, bx, false);
+  if (r0 - 2 < kBcSize && p.bc_s == kBcWall) {
+    // the wall ghost rows this tile gathers: rows 0 and 1, or row 1 alone
+    // for a tile starting at row 3 (batch tiles of one row)
+    pair_wall_rows<MIXED, LS>(p, a, ring, bx, false, r0 == kBcSize);
     __threadfence_block();
   }
   const int q0 = r0 - 2;

@@ -581,6 +581,47 @@ def test_batch_matches_single(orc, fuse):
         assert np.array_equal(got[:, 2:-2, 2:-2], single[:, 2:-2, 2:-2]), i
 
 
+@pytest.mark.parametrize("prec,mode", [(0, True), (1, True), (0, "pipe")])
+def test_batch_pair_tiles_of_one_to_three_rows(orc, prec, mode):
+    """Batch stage-pair tiles of 1, 2 and 3 rows (the single-context API takes
+    4 and up): the second tile of a 1-row tiling starts at row 3 and gathers
+    the inner wall ghost row, which its own consumer then computes; every
+    tiling equals single fused runs bitwise, wall ghost rows included."""
+    d = _lib()
+    specs = [
+        dict(bc=BC_CASES["wall_s_reflection"], nx=66, ny=30, wall_contact_angle=120.0,
+             wall_heat_flux=0.01),
+        dict(bc=BC_CASES["periodic_x_wall_y"], nx=48, ny=25),
+        dict(bc=BC_CASES["reflection"], nx=40, ny=21),
+    ]
+    sims = []
+    for sp in specs:
+        sp = dict(sp)
+        bc, nx, ny = sp.pop("bc"), sp.pop("nx"), sp.pop("ny")
+        _, p, m, xm, ym, nodes = setup_case(orc, bc, nx=nx, ny=ny, precision=prec, **sp)
+        sims.append((p, nodes, orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]))
+    singles = []
+    for p, n, dt in sims:
+        f = d.Field(p, n.copy())
+        f.run(5, dt=dt)
+        singles.append(f.download())
+        f.close()
+    for rows in (1, 2, 3):
+        b = d.Batch([s[0] for s in sims])
+        b.set_stage_fusion(True, rows=rows, pipeline=mode == "pipe")
+        assert b.tiles()["pair_rows"] == rows
+        for i, s in enumerate(sims):
+            b.upload(i, s[1])
+        b.integrate([s[2] for s in sims], 5)
+        for i, (p, n, dt) in enumerate(sims):
+            got = b.download(i)
+            assert np.isfinite(got[:, 2:-2, 2:-2]).all(), (rows, i)
+            assert np.array_equal(got[:, 2:-2, 2:-2], singles[i][:, 2:-2, 2:-2]), (rows, i)
+            if p.bc[2] == W:
+                assert np.array_equal(got[:3, 1, 2:-2], singles[i][:3, 1, 2:-2]), (rows, i)
+        b.close()
+
+
 @pytest.mark.parametrize("prec,fuse", [(0, False), (1, False), (0, True), (1, True),
                                        (0, "pipe"), (1, "pipe")])
 def test_batch_mixed_edges_matches_single(orc, prec, fuse):
