@@ -883,10 +883,13 @@ static void wall_x_ghost(const orc_model* m, double* nodes, int i, int j, int ea
   ND(nodes, i, j, 3) = WALL_SIGN_TRANS * ND(nodes, i1, j, 3) / ND(nodes, i1, j, 1) * ND(nodes, i, j, 1);
 }
 
-/* wall face fluxes, wall_xy_equilibrium_module.f:249-445, 445-946 (interior
- * stencils, constant heat flux, no extra heat source). side_right=1 for N/E. */
+/* wall face fluxes, wall_xy_equilibrium_module.f:249-445, 445-946 (constant
+ * heat flux, no extra heat source). side_right=1 for N/E.  The stencils are
+ * the interior operator set's (the module takes `s`, the compiled
+ * sd_operators, :263,356: Mattsson or Cockburn-Gau); the temperature's
+ * gradients are the central ones of sd_operators_fd_module (:97-99). */
 static void wall_flux_y(const orc_model* m, const double* n, int i, int j, double dx, double dy,
-                        int side_right, double out[4]) {
+                        int side_right, const sd_ops* S, double out[4]) {
   prim_fn r = orc_mass_density;
   const double q = wall_heat_flux(m);
   double Tl;
@@ -900,11 +903,11 @@ static void wall_flux_y(const orc_model* m, const double* n, int i, int j, doubl
   const double T_half = 0.5 * (Tl + Th);
   const double md_half = 0.5 * (ND(n, i, j - 1, 1) + ND(n, i, j, 1));
   const double inv_mom_y = pressure_T_md(T_half, md_half);
-  const double visc_mom_x = s_dgdy(m, n, i, j, orc_velocity_x, dy);
-  const double cap_mom_x = -s_dgdx(m, n, i, j, r, dx) * s_dgdy(m, n, i, j, r, dy);
+  const double visc_mom_x = S->dgdy(m, n, i, j, orc_velocity_x, dy);
+  const double cap_mom_x = -S->dgdx(m, n, i, j, r, dx) * S->dgdy(m, n, i, j, r, dy);
   const double cap_mom_y =
-      s_g(m, n, i, j, r) * (s_d2gdx2(m, n, i, j, r, dx) + s_d2gdy2(m, n, i, j, r, dy)) +
-      0.5 * (sq(s_dgdx(m, n, i, j, r, dx)) - sq(s_dgdy(m, n, i, j, r, dy)));
+      S->g(m, n, i, j, r) * (S->d2gdx2(m, n, i, j, r, dx) + S->d2gdy2(m, n, i, j, r, dy)) +
+      0.5 * (sq(S->dgdx(m, n, i, j, r, dx)) - sq(S->dgdy(m, n, i, j, r, dy)));
   out[0] = 0.0;
   out[1] = -1.0 / m->re * visc_mom_x - 1.0 / m->we * cap_mom_x;
   out[2] = inv_mom_y - 1.0 / m->we * cap_mom_y;
@@ -912,7 +915,7 @@ static void wall_flux_y(const orc_model* m, const double* n, int i, int j, doubl
 }
 
 static void wall_flux_x(const orc_model* m, const double* n, int i, int j, double dx, double dy,
-                        int side_right, double out[4]) {
+                        int side_right, const sd_ops* S, double out[4]) {
   prim_fn r = orc_mass_density;
   const double q = wall_heat_flux(m);
   double Tl;
@@ -927,10 +930,10 @@ static void wall_flux_x(const orc_model* m, const double* n, int i, int j, doubl
   const double md_half = 0.5 * (ND(n, i - 1, j, 1) + ND(n, i, j, 1));
   const double inv_mom_x = pressure_T_md(T_half, md_half);
   const double cap_mom_x =
-      s_f(m, n, i, j, r) * (s_d2fdx2(m, n, i, j, r, dx) + s_d2fdy2(m, n, i, j, r, dy)) +
-      0.5 * (-sq(s_dfdx(m, n, i, j, r, dx)) + sq(s_dfdy(m, n, i, j, r, dy)));
-  const double visc_mom_y = s_dfdx(m, n, i, j, orc_velocity_y, dx);
-  const double cap_mom_y = -s_dfdx(m, n, i, j, r, dx) * s_dfdy(m, n, i, j, r, dy);
+      S->f(m, n, i, j, r) * (S->d2fdx2(m, n, i, j, r, dx) + S->d2fdy2(m, n, i, j, r, dy)) +
+      0.5 * (-sq(S->dfdx(m, n, i, j, r, dx)) + sq(S->dfdy(m, n, i, j, r, dy)));
+  const double visc_mom_y = S->dfdx(m, n, i, j, orc_velocity_y, dx);
+  const double cap_mom_y = -S->dfdx(m, n, i, j, r, dx) * S->dfdy(m, n, i, j, r, dy);
   out[0] = 0.0;
   out[1] = inv_mom_x - 1.0 / m->we * cap_mom_x;
   out[2] = -1.0 / m->re * visc_mom_y - 1.0 / m->we * cap_mom_y;
@@ -940,9 +943,9 @@ static void wall_flux_x(const orc_model* m, const double* n, int i, int j, doubl
 void orc_wall_flux(const orc_model* m, const double* n, int dir_y, int i, int j, double dx,
                    double dy, int side_right, double out[4]) {
   if (dir_y)
-    wall_flux_y(m, n, i, j, dx, dy, side_right, out);
+    wall_flux_y(m, n, i, j, dx, dy, side_right, interior_ops(m), out);
   else
-    wall_flux_x(m, n, i, j, dx, dy, side_right, out);
+    wall_flux_x(m, n, i, j, dx, dy, side_right, interior_ops(m), out);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1243,22 +1246,22 @@ void orc_compute_time_dev(const orc_model* m, double t, const double* nodes, con
   double w[4];
   if (m->bc_s == ORC_BC_WALL)
     for (int i = 1 + BC_SIZE; i <= nx - BC_SIZE; ++i) {
-      wall_flux_y(m, nodes, i, 1 + BC_SIZE, dx, dy, 0, w);
+      wall_flux_y(m, nodes, i, 1 + BC_SIZE, dx, dy, 0, ops, w);
       for (int k = 1; k <= 4; ++k) FY(i, 1 + BC_SIZE, k) = w[k - 1];
     }
   if (m->bc_n == ORC_BC_WALL)
     for (int i = 1 + BC_SIZE; i <= nx - BC_SIZE; ++i) {
-      wall_flux_y(m, nodes, i, ny - BC_SIZE + 1, dx, dy, 1, w);
+      wall_flux_y(m, nodes, i, ny - BC_SIZE + 1, dx, dy, 1, ops, w);
       for (int k = 1; k <= 4; ++k) FY(i, ny - BC_SIZE + 1, k) = w[k - 1];
     }
   if (m->bc_w == ORC_BC_WALL)
     for (int j = 1 + BC_SIZE; j <= ny - BC_SIZE; ++j) {
-      wall_flux_x(m, nodes, 1 + BC_SIZE, j, dx, dy, 0, w);
+      wall_flux_x(m, nodes, 1 + BC_SIZE, j, dx, dy, 0, ops, w);
       for (int k = 1; k <= 4; ++k) FX(1 + BC_SIZE, j, k) = w[k - 1];
     }
   if (m->bc_e == ORC_BC_WALL)
     for (int j = 1 + BC_SIZE; j <= ny - BC_SIZE; ++j) {
-      wall_flux_x(m, nodes, nx - BC_SIZE + 1, j, dx, dy, 1, w);
+      wall_flux_x(m, nodes, nx - BC_SIZE + 1, j, dx, dy, 1, ops, w);
       for (int k = 1; k <= 4; ++k) FX(nx - BC_SIZE + 1, j, k) = w[k - 1];
     }
 
