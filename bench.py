@@ -281,8 +281,8 @@ def variants_bench(n, steps):
     state at n^2, CFL 0.1 device-side dt, FP64: the low-storage RK3 (periodic
     x / wall y, as the headline; also in mixed precision) and the
     Cockburn-Gau stencils (periodic edges: they take reflection and periodic
-    edges only), with the Mattsson RK3-TVD on the same periodic case beside
-    them."""
+    edges only; FP64 and mixed precision), with the Mattsson RK3-TVD on the
+    same periodic case beside them."""
     import dataclasses
 
     import dim2d_finite_volume_amd as d
@@ -292,7 +292,9 @@ def variants_bench(n, steps):
     cases = (("lsrk3", dict(integrator=d.INT_LSRK3)),
              ("lsrk3_mixed", dict(integrator=d.INT_LSRK3, precision=d.PREC_MIXED)),
              ("rk3tvd_periodic", dict(bc=per)),
-             ("cockburn_gau_periodic", dict(bc=per, operators=d.OPS_COCKBURN_GAU)))
+             ("cockburn_gau_periodic", dict(bc=per, operators=d.OPS_COCKBURN_GAU)),
+             ("cockburn_gau_periodic_mixed", dict(bc=per, operators=d.OPS_COCKBURN_GAU,
+                                                  precision=d.PREC_MIXED)))
     out = {}
     for name, kw in cases:
         f = d.Field(dataclasses.replace(p, **kw), nodes)
@@ -305,7 +307,7 @@ def variants_bench(n, steps):
         out[name] = {"value": n * n * steps / el, "unit": "grid-pt updates/s",
                      "ms_per_step": 1e3 * el / steps}
         log(f"[bench] {name} {n}^2: {out[name]['value']:.4e} updates/s")
-    out["config"] = (f"two-bubble state, {n}x{n} interior, FP64 (lsrk3_mixed: mixed precision), "
+    out["config"] = (f"two-bubble state, {n}x{n} interior, FP64 (*_mixed: mixed precision), "
                      f"CFL 0.1, {steps} timed steps")
     return out
 

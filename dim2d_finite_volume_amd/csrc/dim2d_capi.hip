@@ -79,8 +79,8 @@ int validate(const dim2d_params* p) {
   if (p->operators != DIM2D_OPS_MATTSSON && p->operators != DIM2D_OPS_COCKBURN_GAU)
     return fail(DIM2D_EINVAL, "unknown operator set %d", p->operators);
   if (p->operators == DIM2D_OPS_COCKBURN_GAU) {
-    if (p->precision != DIM2D_PREC_FP64 || p->integrator != DIM2D_INT_RK3TVD)
-      return fail(DIM2D_EINVAL, "the Cockburn-Gau operators run in FP64 with RK3-TVD only");
+    if (p->integrator != DIM2D_INT_RK3TVD)
+      return fail(DIM2D_EINVAL, "the Cockburn-Gau operators run with RK3-TVD only");
     for (int k = 0; k < 4; ++k)
       if (b[k] == DIM2D_BC_WALL || b[k] == DIM2D_BC_OPEN)
         return fail(DIM2D_EINVAL,
@@ -148,7 +148,8 @@ DevParams make_dev_params(const dim2d_params& q) {
                        (float)p.vr,         (float)p.two_vr,     (float)p.inv_re,
                        (float)p.inv_we,     (float)p.two_inv_re, (float)p.two_inv_we,
                        (float)p.c15cv,      (float)p.k_heat_x,   (float)p.k_heat_y,
-                       (float)p.k_cap_x,    (float)p.k_cap_y};
+                       (float)p.k_cap_x,    (float)p.k_cap_y,    (float)p.inv_pr,
+                       (float)p.inv_repr};
   return p;
 }
 
@@ -570,7 +571,7 @@ static hipError_t launch_time_dev_fp64(dim2d_ctx* c) {
   a.U = c->U;
   a.S64 = c->U;
   a.O64 = c->U2;
-  hipError_t e = c->cg ? launch_cg_stage(p, 0, a, c->stream)
+  hipError_t e = c->cg ? launch_cg_stage(p, 0, 0, a, c->stream)
                        : launch_stage(p, 0, 0, a, c->block_x, c->rows, c->stream);
   // open edges: the ghost frame has a time derivative of its own
   if (e == hipSuccess && is_open(p)) e = launch_frame(p, 0, 0, a, c->stream);
@@ -646,7 +647,7 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
   int st = DIM2D_OK;
   bool result_in_u = false;  // the new state is in U (else in U2, swapped below)
   if (c->ff_set) HIP_TRY(launch_save_frame(p, c->U, c->F0, c->stream));
-  if (c->cg) {
+  if (c->cg && !c->mixed) {
     // Cockburn-Gau stencils: FP64 RK3-TVD stages u^n -> A -> B -> U2, no walls
     a.U = c->U;
     double* in[3] = {c->U, c->A, c->B};
@@ -654,8 +655,20 @@ static int run_step(dim2d_ctx* c, double dt_fixed, int cfl_mode) {
     for (int s = 0; s < 3; ++s) {
       a.S64 = in[s];
       a.O64 = out[s];
-      HIP_TRY(timed_launch(c, s, [&] { return launch_cg_stage(p, 1 + s, a, c->stream); }));
+      HIP_TRY(timed_launch(c, s, [&] { return launch_cg_stage(p, 0, 1 + s, a, c->stream); }));
     }
+  } else if (c->cg) {
+    // in mixed precision: u^n -> D1 -> D2 (FP32 increments) -> U2
+    a.U = c->U;
+    a.O32 = c->D1;
+    HIP_TRY(timed_launch(c, 0, [&] { return launch_cg_stage(p, 1, 1, a, c->stream); }));
+    a.S32 = c->D1;
+    a.O32 = c->D2;
+    HIP_TRY(timed_launch(c, 1, [&] { return launch_cg_stage(p, 1, 2, a, c->stream); }));
+    a.S32 = c->D2;
+    a.O32 = nullptr;
+    a.O64 = c->U2;
+    HIP_TRY(timed_launch(c, 2, [&] { return launch_cg_stage(p, 1, 3, a, c->stream); }));
   } else if (c->lsrk && c->fuse) {
     // low-storage stages 1 and 2 in one launch, u1 and q1 on chip: u0 (U) ->
     // u2 (U2) and q2 (Q, or Q32 in mixed precision); stage 3 (U2, Q) -> U, so
@@ -1243,7 +1256,7 @@ int dim2d_time_dev_noalloc(const dim2d_params* params, const double* d_nodes, do
   a.S64 = U;
   a.O64 = D;
   const bool open = is_open(p);
-  if (params->operators == DIM2D_OPS_COCKBURN_GAU) HIP_TRY(launch_cg_stage(p, 0, a, s));
+  if (params->operators == DIM2D_OPS_COCKBURN_GAU) HIP_TRY(launch_cg_stage(p, 0, 0, a, s));
   else HIP_TRY(launch_stage(p, 0, 0, a, default_block_x(p), default_rows(p), s));
   if (open) HIP_TRY(launch_frame(p, 0, 0, a, s));
   HIP_TRY(launch_unpack(p, D, d_time_dev, open ? 0 : 1, kDebugReal, s));

@@ -57,6 +57,7 @@ struct DevParams {
     float idx, idy, half_idx, half_idy, q_idx, q_idy;
     float vr, two_vr, inv_re, inv_we, two_inv_re, two_inv_we, c15cv;
     float k_heat_x, k_heat_y, k_cap_x, k_cap_y;
+    float inv_pr, inv_repr;
   } f;
 };
 
@@ -174,7 +175,8 @@ hipError_t launch_unpack(const DevParams& p, const double* F, double* planar, in
 // Cockburn and Gau operators, FP64: time derivative (stage 0) or RK3-TVD
 // stage 1-3 from a.S64 (and a.U), into a.O64; stage 3 also reduces for the CFL
 // step, as launch_stage does
-hipError_t launch_cg_stage(const DevParams& p, int stage, const StageArgs& a, hipStream_t s);
+hipError_t launch_cg_stage(const DevParams& p, int mixed, int stage, const StageArgs& a,
+                           hipStream_t s);
 // max |F| per variable over the interior (and the ghost frame when `frame`)
 // into out[0..3] as bits of non-negative doubles (atomicMax; clear out first);
 // non-finite values count as +inf

@@ -2258,114 +2258,145 @@ constexpr int kThreads = kTX * (kTY + 1);   // one y-face per thread
 constexpr int kH = 2;              // halo (bc_size)
 constexpr int kWX = kTX + 2 * kH;  // staged cells along x (20) and y (19)
 constexpr int kWY = kTY + 2 * kH;
-constexpr int kQ = 10;             // staged quantities per cell
-enum { Q_R = 0, Q_QX, Q_QY, Q_E, Q_U, Q_V, Q_P, Q_TCL, Q_TCAP, Q_W };
+constexpr int kQ = 9;              // staged quantities besides the density
+enum { Q_QX = 0, Q_QY, Q_E, Q_U, Q_V, Q_P, Q_TCL, Q_TCAP, Q_W };
 
 // one face buffer, for the x-faces (kTX+1 per row, kTY rows) and then the
-// y-faces (kTX per row, kTY+1 rows): 38.6 KB per tile, four tiles per CU
+// y-faces (kTX per row, kTY+1 rows).  The density stays FP64 in both modes
+// (its differences feed the capillary second differences); the other
+// quantities and the faces are in the flux precision R: 38.6 KB per tile in
+// FP64 (four tiles per CU), 21.4 KB with FP32 fluxes
 constexpr int kFaces = (kTX + 1) * kTY > kTX * (kTY + 1) ? (kTX + 1) * kTY : kTX * (kTY + 1);
+template <typename R>
 struct Tile {
-  double q[kQ][kWX * kWY];
-  double f[4][kFaces];
+  double rho[kWX * kWY];
+  R q[kQ][kWX * kWY];
+  R f[4][kFaces];
 };
 
-// staged value of quantity Q at tile-local cell (x, y) (halo included)
+// staged value of quantity Q (or the density) at tile-local cell (x, y)
 #define CG_AT(Q, x, y) (t.q[(Q)][(y) * kWX + (x)])
+#define CG_RHO(x, y) (t.rho[(y) * kWX + (x)])
 
 // the four-point face average of a product A*B (B = 1 when absent)
-__device__ __forceinline__ double fx4(const Tile& t, int qa, int qb, int x, int y) {
+template <typename R>
+__device__ __forceinline__ R fx4(const Tile<R>& t, int qa, int qb, int x, int y) {
   auto v = [&](int xx) { return qb < 0 ? CG_AT(qa, xx, y) : CG_AT(qa, xx, y) * CG_AT(qb, xx, y); };
-  return 1.0 / 12.0 * (-v(x - 2) + 7.0 * v(x - 1) + 7.0 * v(x) - v(x + 1));
+  return (R)(1.0 / 12.0) * (-v(x - 2) + (R)7 * v(x - 1) + (R)7 * v(x) - v(x + 1));
 }
-__device__ __forceinline__ double gy4(const Tile& t, int qa, int qb, int x, int y) {
+template <typename R>
+__device__ __forceinline__ R gy4(const Tile<R>& t, int qa, int qb, int x, int y) {
   auto v = [&](int yy) { return qb < 0 ? CG_AT(qa, x, yy) : CG_AT(qa, x, yy) * CG_AT(qb, x, yy); };
-  return 1.0 / 12.0 * (-v(y - 2) + 7.0 * v(y - 1) + 7.0 * v(y) - v(y + 1));
+  return (R)(1.0 / 12.0) * (-v(y - 2) + (R)7 * v(y - 1) + (R)7 * v(y) - v(y + 1));
 }
 
-// x-face (i-1/2, j) at tile-local (x, y) = cell i's position
-__device__ void x_face(const DevParams& p, const Tile& t, int x, int y, double F[4]) {
+// x-face (i-1/2, j) at tile-local (x, y) = cell i's position; the density's
+// face value and derivatives in FP64, rounded to R once
+template <typename R>
+__device__ void x_face(const DevParams& p, const Tile<R>& t, int x, int y, R F[4]) {
   const double idx = p.idx, idy = p.idy;
-  auto dfdx = [&](int Q) { return idx * (-CG_AT(Q, x - 1, y) + CG_AT(Q, x, y)); };
+  const R ridx = PC(R, idx), ridy = PC(R, idy);
+  auto dfdx = [&](int Q) { return ridx * (-CG_AT(Q, x - 1, y) + CG_AT(Q, x, y)); };
   auto dfdy = [&](int Q) {
-    return 1.0 / 24.0 * idy *
-           (+CG_AT(Q, x - 2, y - 1) - 7.0 * CG_AT(Q, x - 1, y - 1) - 7.0 * CG_AT(Q, x, y - 1) +
-            CG_AT(Q, x + 1, y - 1) - CG_AT(Q, x - 2, y + 1) + 7.0 * CG_AT(Q, x - 1, y + 1) +
-            7.0 * CG_AT(Q, x, y + 1) - CG_AT(Q, x + 1, y + 1));
+    return (R)(1.0 / 24.0) * ridy *
+           (+CG_AT(Q, x - 2, y - 1) - (R)7 * CG_AT(Q, x - 1, y - 1) - (R)7 * CG_AT(Q, x, y - 1) +
+            CG_AT(Q, x + 1, y - 1) - CG_AT(Q, x - 2, y + 1) + (R)7 * CG_AT(Q, x - 1, y + 1) +
+            (R)7 * CG_AT(Q, x, y + 1) - CG_AT(Q, x + 1, y + 1));
   };
-  auto row4 = [&](int Q, int yy) {
-    return -CG_AT(Q, x - 2, yy) + 7.0 * CG_AT(Q, x - 1, yy) + 7.0 * CG_AT(Q, x, yy) -
-           CG_AT(Q, x + 1, yy);
+  auto row4 = [&](int yy) {
+    return -CG_RHO(x - 2, yy) + 7.0 * CG_RHO(x - 1, yy) + 7.0 * CG_RHO(x, yy) - CG_RHO(x + 1, yy);
   };
   const double d2x = 0.5 * idx * idx *
-                     (CG_AT(Q_R, x - 2, y) - CG_AT(Q_R, x - 1, y) - CG_AT(Q_R, x, y) + CG_AT(Q_R, x + 1, y));
-  const double d2y = 1.0 / 12.0 * idy * idy * (row4(Q_R, y - 1) - 2.0 * row4(Q_R, y) + row4(Q_R, y + 1));
-  const double rx = dfdx(Q_R), ry = dfdy(Q_R);
-  const double dux = dfdx(Q_U), duy = dfdy(Q_U), dvx = dfdx(Q_V), dvy = dfdy(Q_V);
-  const double fu = fx4(t, Q_U, -1, x, y), fv = fx4(t, Q_V, -1, x, y);
-  const double fr = fx4(t, Q_R, -1, x, y), fqx = fx4(t, Q_QX, -1, x, y);
-  const double lap = d2x + d2y;
-  const double visc = p.two_vr * dux + p.vr * dvy;
-  const double shear = duy + dvx;
+                     (CG_RHO(x - 2, y) - CG_RHO(x - 1, y) - CG_RHO(x, y) + CG_RHO(x + 1, y));
+  const double d2y = 1.0 / 12.0 * idy * idy * (row4(y - 1) - 2.0 * row4(y) + row4(y + 1));
+  const double rxd = idx * (-CG_RHO(x - 1, y) + CG_RHO(x, y));
+  const double ryd = 1.0 / 24.0 * idy *
+                     (+CG_RHO(x - 2, y - 1) - 7.0 * CG_RHO(x - 1, y - 1) - 7.0 * CG_RHO(x, y - 1) +
+                      CG_RHO(x + 1, y - 1) - CG_RHO(x - 2, y + 1) + 7.0 * CG_RHO(x - 1, y + 1) +
+                      7.0 * CG_RHO(x, y + 1) - CG_RHO(x + 1, y + 1));
+  const double frd = 1.0 / 12.0 * (-CG_RHO(x - 2, y) + 7.0 * CG_RHO(x - 1, y) +
+                                   7.0 * CG_RHO(x, y) - CG_RHO(x + 1, y));
+  const R rx = (R)rxd, ry = (R)ryd, lap = (R)(d2x + d2y), fr = (R)frd;
+  const R dux = dfdx(Q_U), duy = dfdy(Q_U), dvx = dfdx(Q_V), dvy = dfdy(Q_V);
+  const R fu = fx4(t, Q_U, -1, x, y), fv = fx4(t, Q_V, -1, x, y);
+  const R fqx = fx4(t, Q_QX, -1, x, y);
+  const R visc = PC(R, two_vr) * dux + PC(R, vr) * dvy;
+  const R shear = duy + dvx;
   F[0] = fqx;
-  F[1] = fx4(t, Q_QX, Q_U, x, y) + fx4(t, Q_P, -1, x, y) - p.inv_re * visc -
-         p.inv_we * (p.c15cv * fx4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + fr * lap +
-                     0.5 * (-rx * rx + ry * ry));
-  F[2] = fx4(t, Q_QY, Q_U, x, y) - p.inv_re * shear - p.inv_we * (-rx * ry);
-  const double visc_e = fu * visc + fv * shear + p.inv_pr * dfdx(Q_TCL);
-  const double cap_e = p.c15cv * fx4(t, Q_W, Q_U, x, y) * (rx * rx + ry * ry) + fqx * lap +
-                       0.5 * fu * (-rx * rx + ry * ry) - rx * ry * fv - fr * rx * (dux + dvy) -
-                       p.inv_repr * dfdx(Q_TCAP);
-  F[3] = fx4(t, Q_E, Q_U, x, y) + fx4(t, Q_P, Q_U, x, y) - p.inv_re * visc_e - p.inv_we * cap_e;
+  F[1] = fx4(t, Q_QX, Q_U, x, y) + fx4(t, Q_P, -1, x, y) - PC(R, inv_re) * visc -
+         PC(R, inv_we) * (PC(R, c15cv) * fx4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + fr * lap +
+                          (R)0.5 * (-rx * rx + ry * ry));
+  F[2] = fx4(t, Q_QY, Q_U, x, y) - PC(R, inv_re) * shear - PC(R, inv_we) * (-rx * ry);
+  const R visc_e = fu * visc + fv * shear + PC(R, inv_pr) * dfdx(Q_TCL);
+  const R cap_e = PC(R, c15cv) * fx4(t, Q_W, Q_U, x, y) * (rx * rx + ry * ry) + fqx * lap +
+                  (R)0.5 * fu * (-rx * rx + ry * ry) - rx * ry * fv - fr * rx * (dux + dvy) -
+                  PC(R, inv_repr) * dfdx(Q_TCAP);
+  F[3] = fx4(t, Q_E, Q_U, x, y) + fx4(t, Q_P, Q_U, x, y) - PC(R, inv_re) * visc_e -
+         PC(R, inv_we) * cap_e;
 }
 
 // y-face (i, j-1/2) at tile-local (x, y) = cell j's position
-__device__ void y_face(const DevParams& p, const Tile& t, int x, int y, double G[4]) {
+template <typename R>
+__device__ void y_face(const DevParams& p, const Tile<R>& t, int x, int y, R G[4]) {
   const double idx = p.idx, idy = p.idy;
-  auto dgdy = [&](int Q) { return idy * (-CG_AT(Q, x, y - 1) + CG_AT(Q, x, y)); };
+  const R ridx = PC(R, idx), ridy = PC(R, idy);
+  auto dgdy = [&](int Q) { return ridy * (-CG_AT(Q, x, y - 1) + CG_AT(Q, x, y)); };
   auto dgdx = [&](int Q) {
-    return 1.0 / 24.0 * idx *
-           (+CG_AT(Q, x - 1, y - 2) - 7.0 * CG_AT(Q, x - 1, y - 1) - 7.0 * CG_AT(Q, x - 1, y) +
-            CG_AT(Q, x - 1, y + 1) - CG_AT(Q, x + 1, y - 2) + 7.0 * CG_AT(Q, x + 1, y - 1) +
-            7.0 * CG_AT(Q, x + 1, y) - CG_AT(Q, x + 1, y + 1));
+    return (R)(1.0 / 24.0) * ridx *
+           (+CG_AT(Q, x - 1, y - 2) - (R)7 * CG_AT(Q, x - 1, y - 1) - (R)7 * CG_AT(Q, x - 1, y) +
+            CG_AT(Q, x - 1, y + 1) - CG_AT(Q, x + 1, y - 2) + (R)7 * CG_AT(Q, x + 1, y - 1) +
+            (R)7 * CG_AT(Q, x + 1, y) - CG_AT(Q, x + 1, y + 1));
   };
-  auto col4 = [&](int Q, int xx) {
-    return -CG_AT(Q, xx, y - 2) + 7.0 * CG_AT(Q, xx, y - 1) + 7.0 * CG_AT(Q, xx, y) -
-           CG_AT(Q, xx, y + 1);
+  auto col4 = [&](int xx) {
+    return -CG_RHO(xx, y - 2) + 7.0 * CG_RHO(xx, y - 1) + 7.0 * CG_RHO(xx, y) - CG_RHO(xx, y + 1);
   };
-  const double d2x = 1.0 / 12.0 * idx * idx * (col4(Q_R, x - 1) - 2.0 * col4(Q_R, x) + col4(Q_R, x + 1));
+  const double d2x = 1.0 / 12.0 * idx * idx * (col4(x - 1) - 2.0 * col4(x) + col4(x + 1));
   const double d2y = 0.5 * idy * idy *
-                     (CG_AT(Q_R, x, y - 2) - CG_AT(Q_R, x, y - 1) - CG_AT(Q_R, x, y) + CG_AT(Q_R, x, y + 1));
-  const double rx = dgdx(Q_R), ry = dgdy(Q_R);
-  const double dux = dgdx(Q_U), duy = dgdy(Q_U), dvx = dgdx(Q_V), dvy = dgdy(Q_V);
-  const double gu = gy4(t, Q_U, -1, x, y), gv = gy4(t, Q_V, -1, x, y);
-  const double gr = gy4(t, Q_R, -1, x, y), gqy = gy4(t, Q_QY, -1, x, y);
-  const double lap = d2x + d2y;
-  const double shear = duy + dvx;
-  const double visc = p.vr * dux + p.two_vr * dvy;
+                     (CG_RHO(x, y - 2) - CG_RHO(x, y - 1) - CG_RHO(x, y) + CG_RHO(x, y + 1));
+  const double rxd = 1.0 / 24.0 * idx *
+                     (+CG_RHO(x - 1, y - 2) - 7.0 * CG_RHO(x - 1, y - 1) - 7.0 * CG_RHO(x - 1, y) +
+                      CG_RHO(x - 1, y + 1) - CG_RHO(x + 1, y - 2) + 7.0 * CG_RHO(x + 1, y - 1) +
+                      7.0 * CG_RHO(x + 1, y) - CG_RHO(x + 1, y + 1));
+  const double ryd = idy * (-CG_RHO(x, y - 1) + CG_RHO(x, y));
+  const double grd = 1.0 / 12.0 * (-CG_RHO(x, y - 2) + 7.0 * CG_RHO(x, y - 1) +
+                                   7.0 * CG_RHO(x, y) - CG_RHO(x, y + 1));
+  const R rx = (R)rxd, ry = (R)ryd, lap = (R)(d2x + d2y), gr = (R)grd;
+  const R dux = dgdx(Q_U), duy = dgdy(Q_U), dvx = dgdx(Q_V), dvy = dgdy(Q_V);
+  const R gu = gy4(t, Q_U, -1, x, y), gv = gy4(t, Q_V, -1, x, y);
+  const R gqy = gy4(t, Q_QY, -1, x, y);
+  const R shear = duy + dvx;
+  const R visc = PC(R, vr) * dux + PC(R, two_vr) * dvy;
   G[0] = gqy;
-  G[1] = gy4(t, Q_QX, Q_V, x, y) - p.inv_re * shear - p.inv_we * (-rx * ry);
-  G[2] = gy4(t, Q_QY, Q_V, x, y) + gy4(t, Q_P, -1, x, y) - p.inv_re * visc -
-         p.inv_we * (p.c15cv * gy4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + gr * lap +
-                     0.5 * (rx * rx - ry * ry));
-  const double visc_e = gu * shear + gv * visc + p.inv_pr * dgdy(Q_TCL);
-  const double cap_e = p.c15cv * gy4(t, Q_W, Q_V, x, y) * (rx * rx + ry * ry) + gqy * lap +
-                       0.5 * gv * (rx * rx - ry * ry) - rx * ry * gu - gr * ry * (dux + dvy) -
-                       p.inv_repr * dgdy(Q_TCAP);
-  G[3] = gy4(t, Q_E, Q_V, x, y) + gy4(t, Q_P, Q_V, x, y) - p.inv_re * visc_e - p.inv_we * cap_e;
+  G[1] = gy4(t, Q_QX, Q_V, x, y) - PC(R, inv_re) * shear - PC(R, inv_we) * (-rx * ry);
+  G[2] = gy4(t, Q_QY, Q_V, x, y) + gy4(t, Q_P, -1, x, y) - PC(R, inv_re) * visc -
+         PC(R, inv_we) * (PC(R, c15cv) * gy4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + gr * lap +
+                          (R)0.5 * (rx * rx - ry * ry));
+  const R visc_e = gu * shear + gv * visc + PC(R, inv_pr) * dgdy(Q_TCL);
+  const R cap_e = PC(R, c15cv) * gy4(t, Q_W, Q_V, x, y) * (rx * rx + ry * ry) + gqy * lap +
+                  (R)0.5 * gv * (rx * rx - ry * ry) - rx * ry * gu - gr * ry * (dux + dvy) -
+                  PC(R, inv_repr) * dgdy(Q_TCAP);
+  G[3] = gy4(t, Q_E, Q_V, x, y) + gy4(t, Q_P, Q_V, x, y) - PC(R, inv_re) * visc_e -
+         PC(R, inv_we) * cap_e;
 }
 
 #undef CG_AT
+#undef CG_RHO
 
 }  // namespace cg
 
-// STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages from S64 (stage
-// input) and U (u^n), as stage_kernel's FP64 stages, stage 3 with the CFL
-// reduction.  Grid: one workgroup of kThreads threads per tile.
-template <int STAGE>
+// STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages, as stage_kernel's:
+// FP64 (MIXED = 0) from S64 (stage input) and U (u^n), or mixed precision
+// (FP32 fluxes and stage increments: stage 1 U -> O32, stage 2 U + S32 ->
+// O32, stage 3 U + S32 -> O64), stage 3 with the CFL reduction.  Grid: one
+// workgroup of kThreads threads per tile.
+template <int MIXED, int STAGE>
 __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams p, const StageArgs a) {
   using namespace cg;
-  __shared__ Tile t;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  __shared__ Tile<R> t;
   constexpr bool FIRST = STAGE == 1, LAST = STAGE == 3;
+  // the stage input: S64, U alone (mixed stage 0/1) or U + S32 (mixed 2/3)
+  constexpr int SRC = MIXED ? ((STAGE == 2 || STAGE == 3) ? 2 : 1) : 0;
   double dt = a.dt_fixed;
   if (FIRST) {
     if (a.cfl_mode) dt = cfl_dt_from(a.ctl, a.parity);
@@ -2386,37 +2417,40 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
     pk[1] = peek(&a.ctl->rmin[a.parity ^ 1]);
     pk[2] = peek(&a.ctl->rmax[a.parity ^ 1]);
   }
-  // 1. state of the tile and its halo (virtual ghosts through the index maps)
+  // 1. state of the tile and its halo (virtual ghosts through the index
+  // maps), the per-cell quantities in FP64, stored in the flux precision
   for (int c = tid; c < kWX * kWY; c += kThreads) {
     const int x = c % kWX, y = c / kWX;
     const int i = min(max(i0 - kH + x, 0), p.nx - 1), j = min(max(j0 - kH + y, 0), p.ny - 1);
     bool fx, fy;
     const int cm = map_col(p, i, fx), rm = map_row(p, j, fy);
-    const double* s = a.S64 + ((long long)rm * p.pitch + cm) * 4;
-    const double r = s[0], qx = fx ? -s[1] : s[1], qy = fy ? -s[2] : s[2], E = s[3];
+    double s[4];
+    float inc[4];
+    load_state<SRC>(p, a.U, a.S64, a.S32, rm, (unsigned)cm, fx, fy, s, inc);
+    const double r = s[0], qx = s[1], qy = s[2], E = s[3];
     // reciprocals as the Mattsson stage kernels take them (hardware estimate
     // and two Newton steps, ~1 ulp) instead of five divisions
     const double ir = fast_rcp(r), w = fast_rcp(3.0 - r);
     const double u = qx * ir, v = qy * ir;
     const double Tn = E - 0.5 * r * (u * u + v * v) + 3.0 * r * r;
-    t.q[Q_R][c] = r;
-    t.q[Q_QX][c] = qx;
-    t.q[Q_QY][c] = qy;
-    t.q[Q_E][c] = E;
-    t.q[Q_U][c] = u;
-    t.q[Q_V][c] = v;
-    t.q[Q_P][c] = p.c3cv * w * Tn - 3.0 * r * r;  // classical_pressure_local
-    t.q[Q_TCL][c] = Tn * ir;                      // classical_temperature_eff
-    t.q[Q_W][c] = w;                              // capillarity_pressure
+    t.rho[c] = r;
+    t.q[Q_QX][c] = (R)qx;
+    t.q[Q_QY][c] = (R)qy;
+    t.q[Q_E][c] = (R)E;
+    t.q[Q_U][c] = (R)u;
+    t.q[Q_V][c] = (R)v;
+    t.q[Q_P][c] = (R)(p.c3cv * w * Tn - 3.0 * r * r);  // classical_pressure_local
+    t.q[Q_TCL][c] = (R)(Tn * ir);                      // classical_temperature_eff
+    t.q[Q_W][c] = (R)w;                                // capillarity_pressure
   }
   __syncthreads();
   // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell halo
   for (int c = tid; c < (kWX - 2) * (kWY - 2); c += kThreads) {
     const int x = 1 + c % (kWX - 2), y = 1 + c / (kWX - 2);
     const int k = y * kWX + x;
-    const double gx = 0.5 * p.idx * (-t.q[Q_R][k - 1] + t.q[Q_R][k + 1]);
-    const double gy = 0.5 * p.idy * (-t.q[Q_R][k - kWX] + t.q[Q_R][k + kWX]);
-    t.q[Q_TCAP][k] = 0.5 * fast_rcp(t.q[Q_R][k]) * (gx * gx + gy * gy);
+    const double gx = 0.5 * p.idx * (-t.rho[k - 1] + t.rho[k + 1]);
+    const double gy = 0.5 * p.idy * (-t.rho[k - kWX] + t.rho[k + kWX]);
+    t.q[Q_TCAP][k] = (R)(0.5 * fast_rcp(t.rho[k]) * (gx * gx + gy * gy));
   }
   __syncthreads();
   // 3. the faces of the tile, one of each kind per thread, and their
@@ -2426,24 +2460,25 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
   const int i = i0 + ti, j = j0 + tj;
   const bool out_cell = tj < kTY && i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1;
   if (tid < (kTX + 1) * kTY) {
-    double F[4];
+    R F[4];
     const int fi = tid % (kTX + 1), fj = tid / (kTX + 1);
-    cg::x_face(p, t, kH + fi, kH + fj, F);
+    cg::x_face<R>(p, t, kH + fi, kH + fj, F);
 #pragma unroll
     for (int k = 0; k < 4; ++k) t.f[k][tid] = F[k];
   }
   __syncthreads();
-  double L[4];
+  R L[4];
+  const R hidx = PC(R, idx), hidy = PC(R, idy);
   if (out_cell) {
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      L[k] = p.idx * (t.f[k][tj * (kTX + 1) + ti] - t.f[k][tj * (kTX + 1) + ti + 1]);
+      L[k] = hidx * (t.f[k][tj * (kTX + 1) + ti] - t.f[k][tj * (kTX + 1) + ti + 1]);
   }
   __syncthreads();
   {
-    double G[4];
+    R G[4];
     const int fi = tid % kTX, fj = tid / kTX;
-    cg::y_face(p, t, kH + fi, kH + fj, G);
+    cg::y_face<R>(p, t, kH + fi, kH + fj, G);
 #pragma unroll
     for (int k = 0; k < 4; ++k) t.f[k][tid] = G[k];
   }
@@ -2453,32 +2488,51 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
   red.init();
   if (out_cell) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) L[k] += p.idy * (t.f[k][tj * kTX + ti] - t.f[k][(tj + 1) * kTX + ti]);
+    for (int k = 0; k < 4; ++k) L[k] += hidy * (t.f[k][tj * kTX + ti] - t.f[k][(tj + 1) * kTX + ti]);
     const int c = (kH + tj) * kWX + kH + ti;
     if (p.gravity_ac) {
       // compute_body_forces (pmodel_eq_class.f:1840-1918)
-      L[2] -= t.q[Q_R][c] * p.gravity;
-      L[3] -= t.q[Q_QY][c] * p.gravity;
+      L[2] -= (R)(t.rho[c] * p.gravity);
+      L[3] -= (R)((double)t.q[Q_QY][c] * p.gravity);
     }
     const long long o = ((long long)j * p.pitch + i) * 4;
     double out[4];
     if (STAGE == 0) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) out[k] = L[k];
-    } else {
+      for (int k = 0; k < 4; ++k) out[k] = (double)L[k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
+    } else if (!MIXED) {
       // the stage input at an output cell is interior: no map or sign applies
-      const double S[4] = {t.q[Q_R][c], t.q[Q_QX][c], t.q[Q_QY][c], t.q[Q_E][c]};
+      const double S[4] = {t.rho[c], (double)t.q[Q_QX][c], (double)t.q[Q_QY][c],
+                           (double)t.q[Q_E][c]};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double inc = S[k] + dt * L[k];
+        const double inc = S[k] + dt * (double)L[k];
         if (STAGE == 1) out[k] = inc;
         else if (STAGE == 2) out[k] = 0.75 * a.U[o + k] + (1.0 - 0.75) * inc;
         else out[k] = (1.0 / 3.0) * a.U[o + k] + (1.0 - 1.0 / 3.0) * inc;
       }
-    }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
-    if (LAST) red.add(p, out);
+      for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
+    } else if (STAGE == 1 || STAGE == 2) {
+      // FP32 increments against u^n, as the Mattsson mixed stages
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double dl = dt * (double)L[k];
+        a.O32[o + k] = STAGE == 1 ? (float)dl : (float)(0.25 * ((double)a.S32[o + k] + dl));
+      }
+    } else {
+      // u^n + 2/3 (S32 + dt L) = (u^n + S32) + (2/3 dt L - 1/3 S32)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double s32 = (double)a.S32[o + k];
+        out[k] = (a.U[o + k] + s32) + ((2.0 / 3.0) * (dt * (double)L[k]) - (1.0 / 3.0) * s32);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
+    }
+    if (LAST) red.template add<MIXED != 0>(p, out);
   }
   if (LAST) {
     // the four waves combine in LDS; one thread commits the workgroup's values
@@ -2870,15 +2924,27 @@ hipError_t launch_absmax(const DevParams& p, int frame, const double* F, unsigne
   return hipGetLastError();
 }
 
-hipError_t launch_cg_stage(const DevParams& p, int stage, const StageArgs& a, hipStream_t s) {
+hipError_t launch_cg_stage(const DevParams& p, int mixed, int stage, const StageArgs& a,
+                           hipStream_t s) {
+  if (mixed && stage == 0) return hipErrorInvalidValue;  // time derivatives are FP64
   const int ni = p.nx - 2 * kBcSize, nj = p.ny - 2 * kBcSize;
   const dim3 grid((ni + cg::kTX - 1) / cg::kTX, (nj + cg::kTY - 1) / cg::kTY);
-  switch (stage) {
-    case 0: hipLaunchKernelGGL(cg_stage_kernel<0>, grid, dim3(cg::kThreads), 0, s, p, a); break;
-    case 1: hipLaunchKernelGGL(cg_stage_kernel<1>, grid, dim3(cg::kThreads), 0, s, p, a); break;
-    case 2: hipLaunchKernelGGL(cg_stage_kernel<2>, grid, dim3(cg::kThreads), 0, s, p, a); break;
-    default: hipLaunchKernelGGL(cg_stage_kernel<3>, grid, dim3(cg::kThreads), 0, s, p, a); break;
+#define DIM2D_CG(M, S) hipLaunchKernelGGL((cg_stage_kernel<M, S>), grid, dim3(cg::kThreads), 0, s, p, a)
+  if (!mixed) {
+    switch (stage) {
+      case 0: DIM2D_CG(0, 0); break;
+      case 1: DIM2D_CG(0, 1); break;
+      case 2: DIM2D_CG(0, 2); break;
+      default: DIM2D_CG(0, 3); break;
+    }
+  } else {
+    switch (stage) {
+      case 1: DIM2D_CG(1, 1); break;
+      case 2: DIM2D_CG(1, 2); break;
+      default: DIM2D_CG(1, 3); break;
+    }
   }
+#undef DIM2D_CG
   return hipGetLastError();
 }
 

@@ -54,8 +54,9 @@ extern "C" {
 /* interior face stencils (the reference's compile-time sd_choice,
  * config/code_config/makefile_sd.mk) */
 #define DIM2D_OPS_MATTSSON 0      /* mattsson_operators (sd_operators_class.f:185-1039) */
-#define DIM2D_OPS_COCKBURN_GAU 1  /* cg_operators (cg_operators/sd_operators_class.f): FP64,
-                                     RK3-TVD, reflection and periodic edges, single contexts */
+#define DIM2D_OPS_COCKBURN_GAU 1  /* cg_operators (cg_operators/sd_operators_class.f): FP64
+                                     or mixed precision, RK3-TVD, reflection and periodic
+                                     edges, single contexts */
 
 /* arithmetic of the stage kernels */
 #define DIM2D_PREC_FP64 0  /* everything in FP64 */

@@ -113,15 +113,15 @@ def test_bench_refuses_several_gpus():
 
 
 def test_cockburn_gau_limits():
-    """The Cockburn-Gau operators run in FP64 RK3-TVD single contexts with
-    reflection or periodic edges; anything else is rejected before the device."""
+    """The Cockburn-Gau operators run in RK3-TVD single contexts (FP64 or
+    mixed precision) with reflection or periodic edges; anything else is
+    rejected before the device."""
     import dim2d_finite_volume_amd as d
     L = d.capi.load()
     ctx = ctypes.c_void_p()
     cases = [
         (dict(operators=7), b"unknown operator set"),
-        (dict(operators=1, precision=1), b"FP64 with RK3-TVD"),
-        (dict(operators=1, integrator=1), b"FP64 with RK3-TVD"),
+        (dict(operators=1, integrator=1), b"with RK3-TVD only"),
         (dict(operators=1, bc=(1, 1, 2, 2)), b"reflection and periodic"),
         (dict(operators=1, bc=(3, 3, 3, 3)), b"reflection and periodic"),
     ]

@@ -963,3 +963,43 @@ def test_cockburn_gau_steps(orc, case):
     print(case, "CG CFL rel err", err)
     assert max(err) < 1e-10, err
     f.close()
+
+
+@pytest.mark.parametrize("case", CG_CASES)
+@pytest.mark.parametrize("shape", [(74, 41), (37, 52)])
+def test_cockburn_gau_steps_mixed(orc, case, shape):
+    """The Cockburn-Gau operators in mixed precision (FP32 fluxes and stage
+    increments, FP64 densities and state): 30 fixed-dt steps, then 10 with
+    the device CFL rule, against the FP64 oracle with the same operators,
+    density and energy to 1e-6, and against the FP64 GPU run of the same
+    operators to 1e-6; a mixed context's time derivative stays FP64 (1e-11
+    against the oracle)."""
+    import dataclasses
+    d, p, m, xm, ym, nodes = setup_case(orc, BC_CASES[case], nx=shape[0], ny=shape[1],
+                                        operators=1, precision=1)
+    got = d.Field(p, nodes.copy()).compute_time_dev()
+    assert max(max_rel_err(got, orc.time_dev(m, nodes, xm, ym))) < 1e-11
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    ref = nodes.copy()
+    assert orc.run(m, ref, xm, ym, np.full(30, dt)) == 0
+    f = d.Field(p, nodes.copy())
+    f.run(30, dt=dt)
+    got = f.download()
+    err = max_rel_err(got, ref)
+    print(case, shape, "mixed CG 30-step rel err", err)
+    assert np.isfinite(got).all()
+    assert err[0] < 1e-6 and err[3] < 1e-6, err
+    g64 = d.Field(dataclasses.replace(p, precision=0), nodes.copy())
+    g64.run(30, dt=dt)
+    e = max_rel_err(got, g64.download())
+    g64.close()
+    print(case, shape, "mixed vs FP64 CG", e)
+    assert e[0] < 1e-6 and e[3] < 1e-6, e
+    for _ in range(10):
+        dtc = orc.cfl_dt(m, ref, *p.spacing(), 0.1)[0]
+        assert orc.run(m, ref, xm, ym, np.array([dtc])) == 0
+    f.run(10, cfl=0.1)
+    err = max_rel_err(f.download(), ref)
+    print(case, shape, "mixed CG CFL rel err", err)
+    assert err[0] < 1e-6 and err[3] < 1e-6, err
+    f.close()
