@@ -196,6 +196,89 @@ module dim2d_hip_cabi
       integer(c_int) :: st
     end function dim2d_adapt_domain
 
+    ! batches of independent simulations (no counterpart in the reference:
+    ! a parameter sweep in one set of persistent launches instead of N runs)
+    function dim2d_batch_create(params, n, batch) bind(C, name="dim2d_batch_create") result(st)
+      import :: c_int, c_ptr, c_int32_t, dim2d_params
+      type(dim2d_params), intent(in) :: params(*)
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: batch
+      integer(c_int) :: st
+    end function dim2d_batch_create
+
+    function dim2d_batch_destroy(batch) bind(C, name="dim2d_batch_destroy") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int) :: st
+    end function dim2d_batch_destroy
+
+    !> nodes: the simulation's nodes(nx,ny,4), Fortran order
+    function dim2d_batch_upload(batch, sim, nodes) bind(C, name="dim2d_batch_upload") result(st)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: sim
+      real(c_double), intent(in) :: nodes(*)
+      integer(c_int) :: st
+    end function dim2d_batch_upload
+
+    function dim2d_batch_download(batch, sim, nodes) bind(C, name="dim2d_batch_download") result(st)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: sim
+      real(c_double), intent(out) :: nodes(*)
+      integer(c_int) :: st
+    end function dim2d_batch_download
+
+    !> dts(n): each simulation's fixed time step
+    function dim2d_batch_integrate(batch, dts, nsteps) bind(C, name="dim2d_batch_integrate") &
+        result(st)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: batch
+      real(c_double), intent(in) :: dts(*)
+      integer(c_int32_t), value :: nsteps
+      integer(c_int) :: st
+    end function dim2d_batch_integrate
+
+    function dim2d_batch_synchronize(batch) bind(C, name="dim2d_batch_synchronize") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int) :: st
+    end function dim2d_batch_synchronize
+
+    function dim2d_batch_set_tiles(batch, block_x, rows_per_block, workgroups) &
+        bind(C, name="dim2d_batch_set_tiles") result(st)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: block_x, rows_per_block, workgroups
+      integer(c_int) :: st
+    end function dim2d_batch_set_tiles
+
+    !> fuse: 0 separate stages, 1 stages 1 and 2 fused, 2 fused and pipelined
+    function dim2d_batch_set_stage_fusion(batch, fuse, rows_per_block, workgroups) &
+        bind(C, name="dim2d_batch_set_stage_fusion") result(st)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: fuse, rows_per_block, workgroups
+      integer(c_int) :: st
+    end function dim2d_batch_set_stage_fusion
+
+    function dim2d_batch_get_tiles(batch, out) bind(C, name="dim2d_batch_get_tiles") result(st)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), intent(out) :: out(6)
+      integer(c_int) :: st
+    end function dim2d_batch_get_tiles
+
+    function dim2d_batch_autotune(batch, dts, nsteps, out) bind(C, name="dim2d_batch_autotune") &
+        result(st)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: batch
+      real(c_double), intent(in) :: dts(*)
+      integer(c_int32_t), value :: nsteps
+      real(c_double), intent(out) :: out(3)
+      integer(c_int) :: st
+    end function dim2d_batch_autotune
+
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s

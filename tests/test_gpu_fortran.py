@@ -71,3 +71,58 @@ def test_fortran_adapt_domain_matches_python(tmp_path, mask, shrink):
     got = np.fromfile(fout, dtype=np.float64).reshape(4, ny, nx)
     assert np.array_equal(got, f.download())
     f.close()
+
+
+SWEEP = os.path.join(ROOT, "dim2d_finite_volume_amd", "fortran", "build", "sweep_dim2d_hip")
+
+
+@pytest.mark.parametrize("fuse", [-1, 0, 2])
+def test_fortran_sweep_matches_batch_and_oracle(orc, tmp_path, fuse):
+    """The Fortran sweep driver (dim2d_batch_* through ISO_C_BINDING) on three
+    simulations of different sizes and edges (reflection; periodic x / walls
+    y; periodic), each with its own dt, in the library's default mode, with
+    the stages apart, and pipelined: the final states equal the Python batch
+    of the same mode bit for bit, and the oracle to 1e-10."""
+    if not os.path.exists(SWEEP):
+        pytest.fail("Fortran sweep driver not built (make -C dim2d_finite_volume_amd/fortran)")
+    from test_gpu_parity import oracle_model, perturbed_state
+    import dim2d_finite_volume_amd as d
+    specs = [((0, 0, 0, 0), 44, 37), ((1, 1, 2, 2), 70, 45), ((1, 1, 1, 1), 53, 61)]
+    sims = []
+    for bc, nx, ny in specs:
+        p = d.ModelParams(nx=nx, ny=ny, bc=bc)
+        m = oracle_model(orc, p)
+        xm, ym = p.coordinates()
+        nodes = perturbed_state(p)
+        orc.apply_bc(m, nodes, xm, ym)
+        sims.append((p, m, xm, ym, nodes, orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]))
+    nt = 9
+    lst, fin, fout = tmp_path / "list.txt", tmp_path / "in.bin", tmp_path / "out.bin"
+    lst.write_text("".join(f"{p.nx} {p.ny} {' '.join(map(str, p.bc))} {float(dt):.17g}\n"
+                           for p, _, _, _, _, dt in sims))
+    with open(fin, "wb") as fh:
+        for s in sims:
+            s[4].tofile(fh)
+    r = subprocess.run([SWEEP, str(lst), str(fin), str(fout), str(nt), "0", str(fuse)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    out = np.fromfile(fout, dtype=np.float64)
+    b = d.Batch([s[0] for s in sims])
+    if fuse >= 0:
+        b.set_stage_fusion(fuse != 0, pipeline=fuse == 2)
+    for i, s in enumerate(sims):
+        b.upload(i, s[4])
+    b.integrate([s[5] for s in sims], nt)
+    off = 0
+    for i, (p, m, xm, ym, nodes, dt) in enumerate(sims):
+        n = 4 * p.nx * p.ny
+        got = out[off:off + n].reshape(4, p.ny, p.nx)
+        off += n
+        assert np.array_equal(got[:, 2:-2, 2:-2], b.download(i)[:, 2:-2, 2:-2]), i
+        ref = nodes.copy()
+        assert orc.run(m, ref, xm, ym, np.full(nt, dt)) == 0
+        err = max_rel_err(got, ref)
+        assert max(err) < 1e-10, (i, err)
+    assert off == out.size
+    b.close()
