@@ -581,6 +581,36 @@ def test_batch_matches_single(orc, fuse):
         assert np.array_equal(got[:, 2:-2, 2:-2], single[:, 2:-2, 2:-2]), i
 
 
+def test_pipelined_batch_edge_cases(orc):
+    """A pipelined batch of one simulation (its second group is empty: each
+    step is the stage pair, then stage 3, from the two mixed launches) equals
+    the single fused run bitwise, over calls of 1, 2 and 3 steps; fusion modes
+    other than 0, 1 and 2 are rejected."""
+    import ctypes
+    d = _lib()
+    _, p, m, xm, ym, nodes = setup_case(orc, BC_CASES["periodic_x_wall_y"], nx=61, ny=47)
+    dt = orc.cfl_dt(m, nodes, *p.spacing(), 0.1)[0]
+    b = d.Batch([p])
+    assert not b.tiles()["pipelined"]  # one simulation: not pipelined by default
+    b.set_stage_fusion(True, pipeline=True)
+    assert b.tiles()["pipelined"]
+    b.upload(0, nodes)
+    for k in (1, 2, 3):
+        b.integrate([dt], k)
+    f = d.Field(p, nodes.copy())
+    f.run(6, dt=dt)
+    single = f.download()
+    f.close()
+    got = b.download(0)
+    assert np.array_equal(got[:, 2:-2, 2:-2], single[:, 2:-2, 2:-2])
+    assert np.array_equal(got[:3, 1, 2:-2], single[:3, 1, 2:-2])
+    L = d.capi.load()
+    for bad in (3, -1):
+        assert L.dim2d_batch_set_stage_fusion(b._b, bad, 0, 0) == d.capi.EINVAL
+    assert b.tiles()["pipelined"]  # a rejected call changes nothing
+    b.close()
+
+
 @pytest.mark.parametrize("prec,mode", [(0, True), (1, True), (0, "pipe")])
 def test_batch_pair_tiles_of_one_to_three_rows(orc, prec, mode):
     """Batch stage-pair tiles of 1, 2 and 3 rows (the single-context API takes
