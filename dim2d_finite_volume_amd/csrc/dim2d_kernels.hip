@@ -2247,19 +2247,25 @@ __global__ void unpack_kernel(const DevParams p, const double* __restrict__ F,
 // dim2d_fluxes_module.f:103-1649 term by term (F = inv - visc/Re - cap/We).
 namespace cg {
 
-// kTX x 15 output cells (16 x 15 by default): the tile's (kTX+1) x 15 x-faces
-// and kTX x 16 y-faces each take one pass of the kTX * 16 threads.  A 32-wide
-// tile (512 threads, two tiles per CU) measured the same as the 16-wide one.
-#ifndef DIM2D_CG_TX
-#define DIM2D_CG_TX 16
-#endif
-constexpr int kTX = DIM2D_CG_TX, kTY = 15;  // output cells per tile along x and y
+// 16 x 15 output cells: the tile's 17 x 15 x-faces and 16 x 16 y-faces each
+// take one pass of the 256 threads.  A 32-wide tile (512 threads, two tiles
+// per CU) measured the same as the 16-wide one.
+constexpr int kTX = 16, kTY = 15;  // output cells per tile along x and y
 constexpr int kThreads = kTX * (kTY + 1);   // one y-face per thread
+static_assert(kTX == 16 && kTY == 15, "row_of and the face maps assume 16 x 15 tiles");
 constexpr int kH = 2;              // halo (bc_size)
 constexpr int kWX = kTX + 2 * kH;  // staged cells along x (20) and y (19)
 constexpr int kWY = kTY + 2 * kH;
 constexpr int kQ = 9;              // staged quantities besides the density
 enum { Q_QX = 0, Q_QY, Q_E, Q_U, Q_V, Q_P, Q_TCL, Q_TCAP, Q_W };
+
+// Face rows of the thread groups.  A group of 16 threads computes one row of
+// 16 faces, and the LDS serves FP32 reads 32 lanes at a time: with the row
+// stride of kWX = 20 words, the two rows of a half-wave are placed 4 rows
+// apart (4 * 20 = 16 mod 32 banks), so that their 16-word segments fall on
+// disjoint banks.  row_of(g) is a permutation of 0..15, and of 0..14 over
+// g < 15.
+__device__ __forceinline__ int row_of(int g) { return ((g >> 1) & 3) + 8 * (g >> 3) + 4 * (g & 1); }
 
 // one face buffer, for the x-faces (kTX+1 per row, kTY rows) and then the
 // y-faces (kTX per row, kTY+1 rows).  The density stays FP64 in both modes
@@ -2417,6 +2423,24 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
     pk[1] = peek(&a.ctl->rmin[a.parity ^ 1]);
     pk[2] = peek(&a.ctl->rmax[a.parity ^ 1]);
   }
+  // the output cell of this thread; in stages 2 and 3 its u^n (and, mixed,
+  // its stage increment) are loaded now, so that the loads' latency hides
+  // behind the tile's work instead of delaying the update at its end
+  const int ti = tid % kTX, tj = tid / kTX;
+  const int i = i0 + ti, j = j0 + tj;
+  const bool out_cell = tj < kTY && i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1;
+  const long long o = ((long long)j * p.pitch + i) * 4;
+  constexpr bool NEED_U = (STAGE == 2 || STAGE == 3) && !(MIXED && STAGE == 2);
+  constexpr bool NEED_S32 = MIXED && (STAGE == 2 || STAGE == 3);
+  double un[4] = {0.0, 0.0, 0.0, 0.0};
+  float s32v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (out_cell) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (NEED_U) un[k] = a.U[o + k];
+      if (NEED_S32) s32v[k] = a.S32[o + k];
+    }
+  }
   // 1. state of the tile and its halo (virtual ghosts through the index
   // maps), the per-cell quantities in FP64, stored in the flux precision
   for (int c = tid; c < kWX * kWY; c += kThreads) {
@@ -2445,8 +2469,12 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
   }
   __syncthreads();
   // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell halo
+  // (rows of 16 cells on consecutive threads, then the last two columns)
   for (int c = tid; c < (kWX - 2) * (kWY - 2); c += kThreads) {
-    const int x = 1 + c % (kWX - 2), y = 1 + c / (kWX - 2);
+    constexpr int kMain = kTX * (kWY - 2);
+    const int e = c - kMain;
+    const int x = c < kMain ? 1 + c % kTX : kTX + 1 + (e & 1);
+    const int y = c < kMain ? 1 + c / kTX : 1 + (e >> 1);
     const int k = y * kWX + x;
     const double gx = 0.5 * p.idx * (-t.rho[k - 1] + t.rho[k + 1]);
     const double gy = 0.5 * p.idy * (-t.rho[k - kWX] + t.rho[k + kWX]);
@@ -2456,15 +2484,16 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
   // 3. the faces of the tile, one of each kind per thread, and their
   // divergence (td_operators_class.f:181-194): x-faces i-1/2 for
   // i = i0..i0+kTX, then y-faces j-1/2 for j = j0..j0+kTY, through one buffer
-  const int ti = tid % kTX, tj = tid / kTX;
-  const int i = i0 + ti, j = j0 + tj;
-  const bool out_cell = tj < kTY && i <= p.nx - kBcSize - 1 && j <= p.ny - kBcSize - 1;
   if (tid < (kTX + 1) * kTY) {
+    // rows of kTX faces on consecutive threads, so that each group of 16
+    // lanes reads one contiguous row segment of the staged planes (no LDS
+    // bank conflicts); the last face column on the kTY threads after them
     R F[4];
-    const int fi = tid % (kTX + 1), fj = tid / (kTX + 1);
+    const bool last_col = tid >= kTX * kTY;
+    const int fi = last_col ? kTX : tid % kTX, fj = last_col ? tid - kTX * kTY : row_of(tid / kTX);
     cg::x_face<R>(p, t, kH + fi, kH + fj, F);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t.f[k][tid] = F[k];
+    for (int k = 0; k < 4; ++k) t.f[k][fj * (kTX + 1) + fi] = F[k];
   }
   __syncthreads();
   R L[4];
@@ -2477,10 +2506,10 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
   __syncthreads();
   {
     R G[4];
-    const int fi = tid % kTX, fj = tid / kTX;
+    const int fi = tid % kTX, fj = row_of(tid / kTX);
     cg::y_face<R>(p, t, kH + fi, kH + fj, G);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t.f[k][tid] = G[k];
+    for (int k = 0; k < 4; ++k) t.f[k][fj * kTX + fi] = G[k];
   }
   __syncthreads();
   // 4. body force and RK update
@@ -2495,7 +2524,6 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
       L[2] -= (R)(t.rho[c] * p.gravity);
       L[3] -= (R)((double)t.q[Q_QY][c] * p.gravity);
     }
-    const long long o = ((long long)j * p.pitch + i) * 4;
     double out[4];
     if (STAGE == 0) {
 #pragma unroll
@@ -2509,9 +2537,12 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double inc = S[k] + dt * (double)L[k];
-        if (STAGE == 1) out[k] = inc;
-        else if (STAGE == 2) out[k] = 0.75 * a.U[o + k] + (1.0 - 0.75) * inc;
-        else out[k] = (1.0 / 3.0) * a.U[o + k] + (1.0 - 1.0 / 3.0) * inc;
+        if (STAGE == 1) {
+          out[k] = inc;
+        } else {
+          if (STAGE == 2) out[k] = 0.75 * un[k] + (1.0 - 0.75) * inc;
+          else out[k] = (1.0 / 3.0) * un[k] + (1.0 - 1.0 / 3.0) * inc;
+        }
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
@@ -2520,14 +2551,14 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double dl = dt * (double)L[k];
-        a.O32[o + k] = STAGE == 1 ? (float)dl : (float)(0.25 * ((double)a.S32[o + k] + dl));
+        a.O32[o + k] = STAGE == 1 ? (float)dl : (float)(0.25 * ((double)s32v[k] + dl));
       }
     } else {
       // u^n + 2/3 (S32 + dt L) = (u^n + S32) + (2/3 dt L - 1/3 S32)
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double s32 = (double)a.S32[o + k];
-        out[k] = (a.U[o + k] + s32) + ((2.0 / 3.0) * (dt * (double)L[k]) - (1.0 / 3.0) * s32);
+        const double s32 = (double)s32v[k];
+        out[k] = (un[k] + s32) + ((2.0 / 3.0) * (dt * (double)L[k]) - (1.0 / 3.0) * s32);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
