@@ -37,6 +37,7 @@ STAGE_RE = re.compile(r"dim2d::stage_kernel<(\d+),\s*(\d+)>")
 PAIR_RE = re.compile(r"dim2d::stage_pair_kernel<(\d+)(?:,\s*(true|false))?(?:,\s*(true|false))?>")
 BSTAGE_RE = re.compile(r"dim2d::stage_batch_kernel<(\d+),\s*(\d+)>")
 BPAIR_RE = re.compile(r"dim2d::stage_pair_batch_kernel<(\d+)>")
+CG_RE = re.compile(r"dim2d::cg_stage_kernel<(\d+),\s*(\d+)>")
 BMIX_RE = re.compile(r"dim2d::stage_mix_batch_kernel<(\d+)>")
 
 
@@ -55,6 +56,9 @@ def kernel_key(name):
     m = STAGE_RE.search(name)
     if m:
         return f"stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
+    m = CG_RE.search(name)
+    if m:
+        return f"cg_stage{m.group(2)}_{'mixed' if m.group(1) == '1' else 'fp64'}"
     m = PAIR_RE.search(name)
     if m:
         return (f"pair_{'mixed' if m.group(1) == '1' else 'fp64'}"
@@ -63,6 +67,10 @@ def kernel_key(name):
 
 
 def short_name(name):
+    m = CG_RE.search(name)
+    if m:
+        prec = "mixed" if m.group(1) == "1" else "FP64"
+        return f"cg_stage_kernel<{m.group(1)}, {m.group(2)}> (Cockburn-Gau, {prec} stage {m.group(2)})"
     m = BMIX_RE.search(name)
     if m:
         prec = "mixed" if m.group(1) == "1" else "FP64"
