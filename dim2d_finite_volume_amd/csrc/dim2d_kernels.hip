@@ -2256,8 +2256,14 @@ static_assert(kTX == 16 && kTY == 15, "row_of and the face maps assume 16 x 15 t
 constexpr int kH = 2;              // halo (bc_size)
 constexpr int kWX = kTX + 2 * kH;  // staged cells along x (20) and y (19)
 constexpr int kWY = kTY + 2 * kH;
-constexpr int kQ = 9;              // staged quantities besides the density
-enum { Q_QX = 0, Q_QY, Q_E, Q_U, Q_V, Q_P, Q_TCL, Q_TCAP, Q_W };
+// Staged quantities besides the density.  Two planes carry sums that the
+// fluxes only use through linear stencils: H = E + P, since the energy flux
+// averages E u + P u; and the heat-flux potential
+// Q_T = -T_cl/(Re Pr) + T_cap/(We Re Pr), whose difference across a face is
+// the flux's two temperature-gradient terms together.  Eight planes keep the
+// FP64 tile at 32.5 KB: five tiles per CU.
+constexpr int kQ = 8;
+enum { Q_QX = 0, Q_QY, Q_H, Q_U, Q_V, Q_P, Q_T, Q_W };
 
 // Face rows of the thread groups.  A group of 16 threads computes one row of
 // 16 faces, and the LDS serves FP32 reads 32 lanes at a time: with the row
@@ -2270,8 +2276,8 @@ __device__ __forceinline__ int row_of(int g) { return ((g >> 1) & 3) + 8 * (g >>
 // one face buffer, for the x-faces (kTX+1 per row, kTY rows) and then the
 // y-faces (kTX per row, kTY+1 rows).  The density stays FP64 in both modes
 // (its differences feed the capillary second differences); the other
-// quantities and the faces are in the flux precision R: 38.6 KB per tile in
-// FP64 (four tiles per CU), 21.4 KB with FP32 fluxes
+// quantities and the faces are in the flux precision R: 32.5 KB per tile in
+// FP64 (five tiles per CU), 19.3 KB with FP32 fluxes
 constexpr int kFaces = (kTX + 1) * kTY > kTX * (kTY + 1) ? (kTX + 1) * kTY : kTX * (kTY + 1);
 template <typename R>
 struct Tile {
@@ -2333,12 +2339,10 @@ __device__ void x_face(const DevParams& p, const Tile<R>& t, int x, int y, R F[4
          PC(R, inv_we) * (PC(R, c15cv) * fx4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + fr * lap +
                           (R)0.5 * (-rx * rx + ry * ry));
   F[2] = fx4(t, Q_QY, Q_U, x, y) - PC(R, inv_re) * shear - PC(R, inv_we) * (-rx * ry);
-  const R visc_e = fu * visc + fv * shear + PC(R, inv_pr) * dfdx(Q_TCL);
+  const R visc_e = fu * visc + fv * shear;
   const R cap_e = PC(R, c15cv) * fx4(t, Q_W, Q_U, x, y) * (rx * rx + ry * ry) + fqx * lap +
-                  (R)0.5 * fu * (-rx * rx + ry * ry) - rx * ry * fv - fr * rx * (dux + dvy) -
-                  PC(R, inv_repr) * dfdx(Q_TCAP);
-  F[3] = fx4(t, Q_E, Q_U, x, y) + fx4(t, Q_P, Q_U, x, y) - PC(R, inv_re) * visc_e -
-         PC(R, inv_we) * cap_e;
+                  (R)0.5 * fu * (-rx * rx + ry * ry) - rx * ry * fv - fr * rx * (dux + dvy);
+  F[3] = fx4(t, Q_H, Q_U, x, y) - PC(R, inv_re) * visc_e - PC(R, inv_we) * cap_e + dfdx(Q_T);
 }
 
 // y-face (i, j-1/2) at tile-local (x, y) = cell j's position
@@ -2377,12 +2381,10 @@ __device__ void y_face(const DevParams& p, const Tile<R>& t, int x, int y, R G[4
   G[2] = gy4(t, Q_QY, Q_V, x, y) + gy4(t, Q_P, -1, x, y) - PC(R, inv_re) * visc -
          PC(R, inv_we) * (PC(R, c15cv) * gy4(t, Q_W, -1, x, y) * (rx * rx + ry * ry) + gr * lap +
                           (R)0.5 * (rx * rx - ry * ry));
-  const R visc_e = gu * shear + gv * visc + PC(R, inv_pr) * dgdy(Q_TCL);
+  const R visc_e = gu * shear + gv * visc;
   const R cap_e = PC(R, c15cv) * gy4(t, Q_W, Q_V, x, y) * (rx * rx + ry * ry) + gqy * lap +
-                  (R)0.5 * gv * (rx * rx - ry * ry) - rx * ry * gu - gr * ry * (dux + dvy) -
-                  PC(R, inv_repr) * dgdy(Q_TCAP);
-  G[3] = gy4(t, Q_E, Q_V, x, y) + gy4(t, Q_P, Q_V, x, y) - PC(R, inv_re) * visc_e -
-         PC(R, inv_we) * cap_e;
+                  (R)0.5 * gv * (rx * rx - ry * ry) - rx * ry * gu - gr * ry * (dux + dvy);
+  G[3] = gy4(t, Q_H, Q_V, x, y) - PC(R, inv_re) * visc_e - PC(R, inv_we) * cap_e + dgdy(Q_T);
 }
 
 #undef CG_AT
@@ -2432,9 +2434,13 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
   const long long o = ((long long)j * p.pitch + i) * 4;
   constexpr bool NEED_U = (STAGE == 2 || STAGE == 3) && !(MIXED && STAGE == 2);
   constexpr bool NEED_S32 = MIXED && (STAGE == 2 || STAGE == 3);
+  // FP64 stages: the total energy of the stage input (not staged on its own)
+  constexpr bool NEED_E = !MIXED && STAGE != 0;
   double un[4] = {0.0, 0.0, 0.0, 0.0};
   float s32v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  double e_in = 0.0;
   if (out_cell) {
+    if (NEED_E) e_in = a.S64[o + 3];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (NEED_U) un[k] = a.U[o + k];
@@ -2460,15 +2466,18 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
     t.rho[c] = r;
     t.q[Q_QX][c] = (R)qx;
     t.q[Q_QY][c] = (R)qy;
-    t.q[Q_E][c] = (R)E;
+    const double P = p.c3cv * w * Tn - 3.0 * r * r;  // classical_pressure_local
+    t.q[Q_H][c] = (R)(E + P);
     t.q[Q_U][c] = (R)u;
     t.q[Q_V][c] = (R)v;
-    t.q[Q_P][c] = (R)(p.c3cv * w * Tn - 3.0 * r * r);  // classical_pressure_local
-    t.q[Q_TCL][c] = (R)(Tn * ir);                      // classical_temperature_eff
+    t.q[Q_P][c] = (R)P;
+    t.q[Q_T][c] = (R)(Tn * ir);  // classical_temperature_eff, completed to Q_T in step 2
     t.q[Q_W][c] = (R)w;                                // capillarity_pressure
   }
   __syncthreads();
-  // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell halo
+  // 2. capillarity_temperature_eff (central gradients) on the tile and a 1-cell
+  // halo, combined with the classical temperature into the heat-flux potential
+  const double kt_cl = -p.inv_re * p.inv_pr, kt_cap = p.inv_we * p.inv_repr;
   // (rows of 16 cells on consecutive threads, then the last two columns)
   for (int c = tid; c < (kWX - 2) * (kWY - 2); c += kThreads) {
     constexpr int kMain = kTX * (kWY - 2);
@@ -2478,7 +2487,8 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
     const int k = y * kWX + x;
     const double gx = 0.5 * p.idx * (-t.rho[k - 1] + t.rho[k + 1]);
     const double gy = 0.5 * p.idy * (-t.rho[k - kWX] + t.rho[k + kWX]);
-    t.q[Q_TCAP][k] = (R)(0.5 * fast_rcp(t.rho[k]) * (gx * gx + gy * gy));
+    const double tcap = 0.5 * fast_rcp(t.rho[k]) * (gx * gx + gy * gy);
+    t.q[Q_T][k] = (R)(kt_cl * (double)t.q[Q_T][k] + kt_cap * tcap);
   }
   __syncthreads();
   // 3. the faces of the tile, one of each kind per thread, and their
@@ -2532,8 +2542,7 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
       for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
     } else if (!MIXED) {
       // the stage input at an output cell is interior: no map or sign applies
-      const double S[4] = {t.rho[c], (double)t.q[Q_QX][c], (double)t.q[Q_QY][c],
-                           (double)t.q[Q_E][c]};
+      const double S[4] = {t.rho[c], (double)t.q[Q_QX][c], (double)t.q[Q_QY][c], e_in};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double inc = S[k] + dt * (double)L[k];
