@@ -2260,10 +2260,16 @@ constexpr int kWY = kTY + 2 * kH;
 // fluxes only use through linear stencils: H = E + P, since the energy flux
 // averages E u + P u; and the heat-flux potential
 // Q_T = -T_cl/(Re Pr) + T_cap/(We Re Pr), whose difference across a face is
-// the flux's two temperature-gradient terms together.  Eight planes keep the
-// FP64 tile at 32.5 KB: five tiles per CU.
-constexpr int kQ = 8;
-enum { Q_QX = 0, Q_QY, Q_H, Q_U, Q_V, Q_P, Q_T, Q_W };
+// the flux's two temperature-gradient terms together.  In FP64 the momenta
+// are not staged either: they are rho u and rho v at the point where the
+// fluxes use them, from the density and velocity planes, which keeps the FP64
+// tile at 29.4 KB (five tiles per CU; the mixed tile, 19.3 KB with the
+// momenta, is bound by registers).
+enum { Q_H = 0, Q_U, Q_V, Q_P, Q_T, Q_W, Q_QX, Q_QY };
+template <typename R>
+constexpr bool kMomFromRho = std::is_same<R, double>::value;
+template <typename R>
+constexpr int kQ = kMomFromRho<R> ? 6 : 8;
 
 // Face rows of the thread groups.  A group of 16 threads computes one row of
 // 16 faces, and the LDS serves FP32 reads 32 lanes at a time: with the row
@@ -2282,7 +2288,7 @@ constexpr int kFaces = (kTX + 1) * kTY > kTX * (kTY + 1) ? (kTX + 1) * kTY : kTX
 template <typename R>
 struct Tile {
   double rho[kWX * kWY];
-  R q[kQ][kWX * kWY];
+  R q[kQ<R>][kWX * kWY];
   R f[4][kFaces];
 };
 
@@ -2290,15 +2296,24 @@ struct Tile {
 #define CG_AT(Q, x, y) (t.q[(Q)][(y) * kWX + (x)])
 #define CG_RHO(x, y) (t.rho[(y) * kWX + (x)])
 
+// staged quantity Q at (x, y); in FP64 the momenta from the density and the
+// velocity planes
+template <typename R>
+__device__ __forceinline__ R qv(const Tile<R>& t, int Q, int x, int y) {
+  if (kMomFromRho<R> && (Q == Q_QX || Q == Q_QY))
+    return (R)CG_RHO(x, y) * CG_AT(Q == Q_QX ? Q_U : Q_V, x, y);
+  return CG_AT(Q, x, y);
+}
+
 // the four-point face average of a product A*B (B = 1 when absent)
 template <typename R>
 __device__ __forceinline__ R fx4(const Tile<R>& t, int qa, int qb, int x, int y) {
-  auto v = [&](int xx) { return qb < 0 ? CG_AT(qa, xx, y) : CG_AT(qa, xx, y) * CG_AT(qb, xx, y); };
+  auto v = [&](int xx) { return qb < 0 ? qv(t, qa, xx, y) : qv(t, qa, xx, y) * qv(t, qb, xx, y); };
   return (R)(1.0 / 12.0) * (-v(x - 2) + (R)7 * v(x - 1) + (R)7 * v(x) - v(x + 1));
 }
 template <typename R>
 __device__ __forceinline__ R gy4(const Tile<R>& t, int qa, int qb, int x, int y) {
-  auto v = [&](int yy) { return qb < 0 ? CG_AT(qa, x, yy) : CG_AT(qa, x, yy) * CG_AT(qb, x, yy); };
+  auto v = [&](int yy) { return qb < 0 ? qv(t, qa, x, yy) : qv(t, qa, x, yy) * qv(t, qb, x, yy); };
   return (R)(1.0 / 12.0) * (-v(y - 2) + (R)7 * v(y - 1) + (R)7 * v(y) - v(y + 1));
 }
 
@@ -2392,13 +2407,20 @@ __device__ void y_face(const DevParams& p, const Tile<R>& t, int x, int y, R G[4
 
 }  // namespace cg
 
+// FP64: five tiles per CU fit the LDS, so the registers are held to five
+// waves per SIMD (stage 3 would otherwise take 103 VGPRs and four)
+#ifndef DIM2D_CG_WAVES_FP64
+#define DIM2D_CG_WAVES_FP64 5
+#endif
 // STAGE 0: time derivative into O64; 1, 2, 3: RK3-TVD stages, as stage_kernel's:
 // FP64 (MIXED = 0) from S64 (stage input) and U (u^n), or mixed precision
 // (FP32 fluxes and stage increments: stage 1 U -> O32, stage 2 U + S32 ->
 // O32, stage 3 U + S32 -> O64), stage 3 with the CFL reduction.  Grid: one
 // workgroup of kThreads threads per tile.
 template <int MIXED, int STAGE>
-__global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams p, const StageArgs a) {
+__global__ void __launch_bounds__(cg::kThreads)
+__attribute__((amdgpu_waves_per_eu(MIXED ? 1 : DIM2D_CG_WAVES_FP64, 8)))
+cg_stage_kernel(const DevParams p, const StageArgs a) {
   using namespace cg;
   using R = typename std::conditional<MIXED, float, double>::type;
   __shared__ Tile<R> t;
@@ -2434,13 +2456,14 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
   const long long o = ((long long)j * p.pitch + i) * 4;
   constexpr bool NEED_U = (STAGE == 2 || STAGE == 3) && !(MIXED && STAGE == 2);
   constexpr bool NEED_S32 = MIXED && (STAGE == 2 || STAGE == 3);
-  // FP64 stages: the total energy of the stage input (not staged on its own)
-  constexpr bool NEED_E = !MIXED && STAGE != 0;
+  // FP64 stages: the stage input of the output cell (its momenta and energy
+  // are not staged on their own)
+  constexpr bool NEED_S = !MIXED && STAGE != 0;
   double un[4] = {0.0, 0.0, 0.0, 0.0};
   float s32v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  double e_in = 0.0;
+  double s_in[4] = {0.0, 0.0, 0.0, 0.0};
   if (out_cell) {
-    if (NEED_E) e_in = a.S64[o + 3];
+    if (NEED_S) ld_cell(a.S64 + (long long)j * p.pitch * 4, (unsigned)i, s_in);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (NEED_U) un[k] = a.U[o + k];
@@ -2464,8 +2487,10 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
     const double u = qx * ir, v = qy * ir;
     const double Tn = E - 0.5 * r * (u * u + v * v) + 3.0 * r * r;
     t.rho[c] = r;
-    t.q[Q_QX][c] = (R)qx;
-    t.q[Q_QY][c] = (R)qy;
+    if (!kMomFromRho<R>) {
+      t.q[Q_QX][c] = (R)qx;
+      t.q[Q_QY][c] = (R)qy;
+    }
     const double P = p.c3cv * w * Tn - 3.0 * r * r;  // classical_pressure_local
     t.q[Q_H][c] = (R)(E + P);
     t.q[Q_U][c] = (R)u;
@@ -2532,7 +2557,7 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
     if (p.gravity_ac) {
       // compute_body_forces (pmodel_eq_class.f:1840-1918)
       L[2] -= (R)(t.rho[c] * p.gravity);
-      L[3] -= (R)((double)t.q[Q_QY][c] * p.gravity);
+      L[3] -= (R)((double)qv(t, Q_QY, kH + ti, kH + tj) * p.gravity);
     }
     double out[4];
     if (STAGE == 0) {
@@ -2542,7 +2567,7 @@ __global__ void __launch_bounds__(cg::kThreads) cg_stage_kernel(const DevParams 
       for (int k = 0; k < 4; ++k) a.O64[o + k] = out[k];
     } else if (!MIXED) {
       // the stage input at an output cell is interior: no map or sign applies
-      const double S[4] = {t.rho[c], (double)t.q[Q_QX][c], (double)t.q[Q_QY][c], e_in};
+      const double* S = s_in;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double inc = S[k] + dt * (double)L[k];
