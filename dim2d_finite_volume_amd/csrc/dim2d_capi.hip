@@ -131,6 +131,7 @@ DevParams make_dev_params(const dim2d_params& q) {
   p.inv_repr = 1.0 / (q.re * q.pr);
   p.c15cv = 1.5 / q.cv_r;
   p.c3cv = 3.0 / q.cv_r;
+  p.c3cv_3 = p.c3cv / 3.0;  // rounded once here, not divided per cell
   p.c38cv = 3.0 / (8.0 * q.cv_r);
   p.two_vr = 2.0 + q.viscous_r;
   p.two_inv_re = 2.0 / q.re;
@@ -149,7 +150,7 @@ DevParams make_dev_params(const dim2d_params& q) {
                        (float)p.inv_we,     (float)p.two_inv_re, (float)p.two_inv_we,
                        (float)p.c15cv,      (float)p.k_heat_x,   (float)p.k_heat_y,
                        (float)p.k_cap_x,    (float)p.k_cap_y,    (float)p.inv_pr,
-                       (float)p.inv_repr};
+                       (float)p.inv_repr,   (float)p.c3cv / 3.0f};
   return p;
 }
 

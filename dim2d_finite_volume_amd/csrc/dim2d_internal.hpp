@@ -41,6 +41,7 @@ struct DevParams {
   double re, we, pr, vr, cv;
   double inv_re, inv_we, inv_pr, inv_repr;
   double c15cv, c3cv, c38cv;  // 1.5/cv, 3/cv, 3/(8cv)
+  double c3cv_3;           // c3cv / 3 (the quotient, as the sound speed takes it)
   double two_vr;           // 2 + viscous_r
   double two_inv_re, two_inv_we;   // 2/Re, 2/We
   double k_heat_x, k_heat_y;       // 2/(Pr dx), 2/(Pr dy)
@@ -58,6 +59,7 @@ struct DevParams {
     float vr, two_vr, inv_re, inv_we, two_inv_re, two_inv_we, c15cv;
     float k_heat_x, k_heat_y, k_cap_x, k_cap_y;
     float inv_pr, inv_repr;
+    float c3cv_3;          // (float)c3cv / 3 (the mixed sound speed's quotient)
   } f;
 };
 

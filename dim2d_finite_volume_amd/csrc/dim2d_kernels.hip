@@ -370,7 +370,7 @@ __device__ __forceinline__ double sound_speed(const DevParams& p, const double v
   const double w = fast_rcp(3.0 - rho);
   const double P = p.c3cv * w * Tn - 3.0 * rho2;
   const double e = P + rho2 * (2.0 * rho - 3.0);
-  const double c2 = 3.0 * (e + (P + 3.0 * rho2) * (p.c3cv / 3.0)) * w * ir;
+  const double c2 = 3.0 * (e + (P + 3.0 * rho2) * p.c3cv_3) * w * ir;
   *speed = sqrt(u * u + vv * vv);
   return c2 > 0.0 ? sqrt(c2) : 0.0;
 }
@@ -392,7 +392,7 @@ __device__ __forceinline__ float sound_speed_f32(const DevParams& p, const doubl
   const float c3cv = (float)p.c3cv;
   const float P = c3cv * w * (float)Tn - 3.0f * r2;
   const float e = P + r2 * (2.0f * rf - 3.0f);
-  const float c2 = 3.0f * (e + (P + 3.0f * r2) * (c3cv / 3.0f)) * w * (float)ir;
+  const float c2 = 3.0f * (e + (P + 3.0f * r2) * p.f.c3cv_3) * w * (float)ir;
   *speed = __builtin_amdgcn_sqrtf(uf * uf + vf * vf);
   return c2 > 0.0f ? __builtin_amdgcn_sqrtf(c2) : 0.0f;
 }
